@@ -1,0 +1,603 @@
+// Device kernels of the encoding path.  Included once, by zgr.hip.
+//
+// Domain / data layout of one blob (N = 2^log_n = ROW * COL field elements):
+//   pts    N Fr (mont), column-major: pts[c * ROW + r] = raw element (r, c)
+//          (change_matrix_direction(LOG_COL, LOG_ROW), crates/amt/src/utils.rs:90-107)
+//   coeff  N Fr, iNTT(pts) in bit-reversed order (DIF output)
+//   y      N Fr, evaluations of one coset in natural order
+//   The MSM scalar vector of a coset is ri[i] = y[bitrev(i)] (index_reverse,
+//   crates/amt/src/utils.rs:70-88), gathered on the fly by k_msm_digits.
+#pragma once
+#include "curve.h"
+#include "keccak.h"
+
+namespace zgr {
+
+constexpr int MSM_C = 8;          // window width (bits)
+constexpr int MSM_W = 32;         // windows: 254-bit scalars with signed 8-bit digits
+constexpr int ACC_THREADS = 256;  // 32 windows x 8 lanes
+constexpr int LANES_PER_WIN = 8;
+
+__device__ __forceinline__ uint32_t bitrev_bits(uint32_t x, int bits) {
+    return bits ? (__builtin_bitreverse32(x) >> (32 - bits)) : 0u;
+}
+
+// Spread consecutive workgroup ids over the 8 XCDs so that each XCD works on a
+// contiguous range of logical ids (and so on one L2-resident set of chunks).
+__device__ __forceinline__ uint32_t xcd_remap(uint32_t g, uint32_t total) {
+    const uint32_t X = 8;
+    uint32_t per = total / X;
+    if (per == 0 || g >= per * X) return g;  // tail stays as is
+    return (g % X) * per + g / X;
+}
+
+__device__ __forceinline__ int wave_max(int v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o));
+    return v;
+}
+
+// ---------------------------------------------------------------------------
+// constants
+// ---------------------------------------------------------------------------
+// consts buffer (8 u32 per entry, Fr mont)
+enum {
+    K_OMEGA = 0,
+    K_OMEGA_INV = 1,
+    K_NINV = 2,
+    K_CF = 3,  // K_CF + c, c < 3
+    K_COUNT = 8
+};
+
+// TWO_ADIC_ROOT_OF_UNITY of Fr = 5^((r-1)/2^28), canonical
+__device__ __forceinline__ fr g28_canon() {
+    fr g;
+    const uint32_t v[8] = {0x725b19f0u, 0x9bd61b6eu, 0x41112ed4u, 0x402d111eu,
+                           0x8ef62abcu, 0x00e0a7ebu, 0xa58a7e85u, 0x2a3c09f0u};
+#pragma unroll
+    for (int i = 0; i < 8; i++) g.v[i] = v[i];
+    return g;
+}
+
+__device__ __forceinline__ fr fr_from_u64(uint64_t x) {
+    fr a = f_zero<FrP>();
+    a.v[0] = (uint32_t)x;
+    a.v[1] = (uint32_t)(x >> 32);
+    return f_to_mont(a);
+}
+
+// omega, omega^-1, N^-1 and the coset factors (generate.rs:128-137) of a 2^log_n domain
+__global__ void k_setup_consts(uint32_t* out, int log_n, int coset_n) {
+    if (threadIdx.x || blockIdx.x) return;
+    fr g = f_to_mont(g28_canon());
+    fr w = g;
+    for (int i = log_n; i < 28; i++) w = f_sqr(w);
+    f_store(out + 8 * K_OMEGA, w);
+    f_store(out + 8 * K_OMEGA_INV, f_inv(w));
+    f_store(out + 8 * K_NINV, f_inv(fr_from_u64(1ull << log_n)));
+    for (int c = 0; c < coset_n; c++) {
+        uint32_t pw = bitrev_bits((uint32_t)c, 28 - log_n);
+        f_store(out + 8 * (K_CF + c), f_pow_u64(g, pw));
+    }
+}
+
+// tw[k] = base^k for k < count (base = consts[idx])
+__global__ void k_powers(uint32_t* tw, const uint32_t* consts, int idx, uint32_t count) {
+    uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= count) return;
+    fr b = f_load<FrP>(consts + 8 * idx);
+    fr r = f_one<FrP>();
+    for (int i = 31; i >= 0; i--) {
+        r = f_sqr(r);
+        if ((k >> i) & 1) r = f_mul(r, b);
+    }
+    f_store(tw + 8 * (size_t)k, r);
+}
+
+// scale[p] = N^-1 * cf^bitrev(p): coefficient bitrev(p) sits at DIF position p
+__global__ void k_coset_scale_table(uint32_t* scale, const uint32_t* consts, int coset, int log_n) {
+    uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= (1u << log_n)) return;
+    fr cf = f_load<FrP>(consts + 8 * (K_CF + coset));
+    uint32_t e = bitrev_bits(p, log_n);
+    fr r = f_one<FrP>();
+    for (int i = log_n - 1; i >= 0; i--) {
+        r = f_sqr(r);
+        if ((e >> i) & 1) r = f_mul(r, cf);
+    }
+    f_store(scale + 8 * (size_t)p, f_mul(r, f_load<FrP>(consts + 8 * K_NINV)));
+}
+
+// ---------------------------------------------------------------------------
+// packing: RawData::try_from + RawBlob::from (raw_data.rs:55-73, raw_blob.rs:33-40,
+// utils.rs:10-15), written column-major (utils.rs:90-107)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint8_t raw_byte(const uint8_t* raw, size_t len, size_t max_raw, size_t q) {
+    if (q < len) return raw[q];
+    if (q >= max_raw) return (uint8_t)((uint32_t)len >> (8 * (q - max_raw)));
+    return 0;
+}
+
+__global__ void k_pack(const uint8_t* raw, size_t len, int log_col, int log_row, uint32_t* pts) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t n = 1u << (log_col + log_row);
+    if (i >= n) return;
+    const size_t max_raw = 31 * (size_t)n - 4;
+    size_t off = 31 * (size_t)i;
+    fr a;
+#pragma unroll
+    for (int l = 0; l < 8; l++) {
+        uint32_t v = 0;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            int q = 4 * l + k;
+            if (q < 31) v |= (uint32_t)raw_byte(raw, len, max_raw, off + q) << (8 * k);
+        }
+        a.v[l] = v;
+    }
+    uint32_t r = i >> log_col, c = i & ((1u << log_col) - 1);
+    f_store(pts + 8 * (size_t)((c << log_row) | r), f_to_mont(a));
+}
+
+// ---------------------------------------------------------------------------
+// radix-2 NTT stages (global memory).  tw holds omega^k of the context domain;
+// a stage of half-width 2^(s-1) uses tw[j << tw_shift].
+// ---------------------------------------------------------------------------
+// decimation in frequency: natural in, bit-reversed out
+__global__ void k_ntt_dif_stage(uint32_t* a, const uint32_t* tw, int s, int tw_shift, uint32_t half_n) {
+    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= half_n) return;
+    uint32_t half = 1u << (s - 1);
+    uint32_t j = t & (half - 1);
+    uint32_t i0 = ((t >> (s - 1)) << s) | j, i1 = i0 + half;
+    fr u = f_load<FrP>(a + 8 * (size_t)i0), v = f_load<FrP>(a + 8 * (size_t)i1);
+    fr w = f_load<FrP>(tw + 8 * (size_t)(j << tw_shift));
+    f_store(a + 8 * (size_t)i0, f_add(u, v));
+    f_store(a + 8 * (size_t)i1, f_mul(f_sub(u, v), w));
+}
+
+// decimation in time: bit-reversed in, natural out
+__global__ void k_ntt_dit_stage(uint32_t* a, const uint32_t* tw, int s, int tw_shift, uint32_t half_n) {
+    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= half_n) return;
+    uint32_t half = 1u << (s - 1);
+    uint32_t j = t & (half - 1);
+    uint32_t i0 = ((t >> (s - 1)) << s) | j, i1 = i0 + half;
+    fr u = f_load<FrP>(a + 8 * (size_t)i0);
+    fr v = f_mul(f_load<FrP>(a + 8 * (size_t)i1), f_load<FrP>(tw + 8 * (size_t)(j << tw_shift)));
+    f_store(a + 8 * (size_t)i0, f_add(u, v));
+    f_store(a + 8 * (size_t)i1, f_sub(u, v));
+}
+
+// out[p] = in[p] * scale[p]
+__global__ void k_mul_table(uint32_t* out, const uint32_t* in, const uint32_t* scale, uint32_t n) {
+    uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    f_store(out + 8 * (size_t)p, f_mul(f_load<FrP>(in + 8 * (size_t)p), f_load<FrP>(scale + 8 * (size_t)p)));
+}
+
+// out[p] = in[p] * c (c = consts[idx])
+__global__ void k_mul_const(uint32_t* out, const uint32_t* in, const uint32_t* consts, int idx, uint32_t n) {
+    uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    f_store(out + 8 * (size_t)p, f_mul(f_load<FrP>(in + 8 * (size_t)p), f_load<FrP>(consts + 8 * idx)));
+}
+
+// a[p] *= 1 / 2^(log_n_ctx - shift) = N_ctx^-1 * 2^shift (size_inv of a sub-domain)
+__global__ void k_mul_ninv_sub(uint32_t* a, const uint32_t* consts, int shift, uint32_t n) {
+    uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    fr f = f_load<FrP>(consts + 8 * K_NINV);
+    for (int i = 0; i < shift; i++) f = f_dbl(f);
+    f_store(a + 8 * (size_t)p, f_mul(f_load<FrP>(a + 8 * (size_t)p), f));
+}
+
+// out[p] = in[bitrev(p)]
+__global__ void k_bitrev_copy(uint32_t* out, const uint32_t* in, int log_n) {
+    uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= (1u << log_n)) return;
+    uint32_t q = bitrev_bits(p, log_n);
+    const uint4* s = (const uint4*)(in + 8 * (size_t)q);
+    uint4* d = (uint4*)(out + 8 * (size_t)p);
+    d[0] = s[0];
+    d[1] = s[1];
+}
+
+// ---------------------------------------------------------------------------
+// MSM (multiple_multiexp semantics, prove_gpu.rs:77-84)
+// ---------------------------------------------------------------------------
+// Signed 8-bit digits of every scalar: digit[(b * 32 + w) * bs + k] for scalar
+// ri[b * bs + k] = y[bitrev(b * bs + k)] (converted out of Montgomery form).
+// from_y != 0: gather from y through the bit reversal; else read scalars in order.
+__global__ void k_msm_digits(const uint32_t* y, int log_n, int log_bs, int8_t* digits, int from_y) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (1u << log_n)) return;
+    uint32_t src = from_y ? bitrev_bits(i, log_n) : i;
+    fr s = f_from_mont(f_load<FrP>(y + 8 * (size_t)src));
+    uint32_t b = i >> log_bs, k = i & ((1u << log_bs) - 1);
+    int8_t* out = digits + ((size_t)b * MSM_W << log_bs) + k;
+    int carry = 0;
+#pragma unroll
+    for (int w = 0; w < MSM_W; w++) {
+        int d = (int)((s.v[w >> 2] >> (8 * (w & 3))) & 0xffu) + carry;
+        carry = d >= 128;
+        d -= carry << 8;  // d in [-128, 127]
+        out[(size_t)w << log_bs] = (int8_t)d;
+    }
+}
+
+// Counting sort of one (chunk, window) by |digit| into buckets 1..128.
+// entries: k | sign << 15 | bucket << 16, ascending bucket; count = nonzero digits.
+__global__ void k_msm_sort(const int8_t* digits, int log_bs, uint32_t* entries, uint32_t* counts) {
+    __shared__ uint32_t hist[130];
+    __shared__ uint32_t start[130];
+    const uint32_t bw = blockIdx.x;  // b * 32 + w
+    const uint32_t bs = 1u << log_bs;
+    const int8_t* d = digits + ((size_t)bw << log_bs);
+    for (uint32_t k = threadIdx.x; k < 130; k += blockDim.x) hist[k] = 0;
+    __syncthreads();
+    for (uint32_t k = threadIdx.x; k < bs; k += blockDim.x) {
+        int v = d[k];
+        if (v) atomicAdd(&hist[v < 0 ? -v : v], 1u);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t acc = 0;
+        for (int k = 1; k <= 128; k++) {
+            start[k] = acc;
+            acc += hist[k];
+        }
+        counts[bw] = acc;
+    }
+    __syncthreads();
+    uint32_t* ent = entries + ((size_t)bw << log_bs);
+    for (uint32_t k = threadIdx.x; k < bs; k += blockDim.x) {
+        int v = d[k];
+        if (!v) continue;
+        uint32_t mag = v < 0 ? -v : v;
+        uint32_t pos = atomicAdd(&start[mag], 1u);
+        ent[pos] = k | ((uint32_t)(v < 0) << 15) | (mag << 16);
+    }
+}
+
+// Uniform small scalar multiple: k * p for k < 2^bits (double-and-add without
+// lane-dependent control flow).
+__device__ __forceinline__ g1_xyzz xyzz_mul_small_uniform(const g1_xyzz& p, uint32_t k, int bits) {
+    g1_xyzz r = xyzz_inf();
+    for (int b = bits - 1; b >= 0; b--) {
+        r = xyzz_dbl(r);
+        g1_xyzz s = xyzz_add(r, p);
+        r = xyzz_select((k >> b) & 1, s, r);
+    }
+    return r;
+}
+
+// Bucket accumulation of one (line L, chunk b): per window w the 8 lanes split the
+// window's sorted entries evenly and each computes sum_e bucket(e) * P_e with the
+// running-sum trick, one uniform XYZZ addition per step:
+//   consume an entry:   R += +-P_e
+//   close a bucket:     T += R
+// giving T = sum_{j=lo..hi} R_j, so the lane's share is T + (lo - 1) * R.
+// The per-window sums S_w land in heap leaf nb + b of tree (L, w).
+__global__ __launch_bounds__(ACC_THREADS) void k_msm_accum(const uint32_t* bases, const uint32_t* entries,
+                                                           const uint32_t* counts, uint32_t* heap,
+                                                           int log_n, int h, int nlines) {
+    extern __shared__ uint32_t lds[];
+    const uint32_t nb = 1u << h;
+    const int log_bs = log_n - h;
+    const uint32_t bs = 1u << log_bs;
+    const uint32_t logical = xcd_remap(blockIdx.x, gridDim.x);
+    const uint32_t b = logical / nlines, L = logical % nlines;
+    const int tid = threadIdx.x;
+    const int w = tid >> 3, s = tid & 7;
+    uint32_t* leaf = heap + ((((size_t)L * MSM_W + w) * 2 * nb) + nb + b) * 32;
+
+    // zero chunk: every window empty -> all sums are infinity
+    const int nz = __syncthreads_or(tid < MSM_W && counts[b * MSM_W + tid] != 0);
+    if (!nz) {
+        if (s == 0) xyzz_store(leaf, xyzz_inf());
+        return;
+    }
+
+    // stage the chunk's bases (bs points, 64 B each) in LDS
+    const uint4* src = (const uint4*)(bases + (((size_t)L << log_n) + ((size_t)b << log_bs)) * 16);
+    uint4* dst = (uint4*)lds;
+    for (uint32_t k = tid; k < bs * 4; k += ACC_THREADS) dst[k] = src[k];
+    __syncthreads();
+
+    const int cnt = (int)counts[b * MSM_W + w];
+    const uint32_t* ent = entries + (((size_t)b * MSM_W + w) << log_bs);
+    const int begin = (s * cnt) >> 3, end = ((s + 1) * cnt) >> 3;
+    int hi = 0, lo = 1, steps = 0;
+    if (end > begin) {
+        hi = (int)(ent[end - 1] >> 16);
+        lo = (int)(ent[begin] >> 16);
+        steps = (end - begin) + (hi - lo + 1);
+    }
+    const int maxsteps = wave_max(steps);
+    g1_xyzz R = xyzz_inf(), T = xyzz_inf();
+    int e = end - 1, kc = hi;
+    for (int st = 0; st < maxsteps; st++) {
+        const bool active = st < steps;
+        const bool have = e >= begin;
+        const uint32_t ev = have ? ent[e] : 0u;
+        const bool consume = active && have && (int)(ev >> 16) == kc;
+        const bool trans = active && !consume;
+        g1_aff P = aff_load(lds + (ev & 0x7fffu) * 16);
+        if (ev & 0x8000u) P.y = f_neg(P.y);
+        g1_xyzz A = xyzz_select(trans, T, R);
+        g1_xyzz B = xyzz_select(trans, R, xyzz_from_aff(P));
+        g1_xyzz C = xyzz_add(A, B);
+        R = xyzz_select(consume, C, R);
+        T = xyzz_select(trans, C, T);
+        e -= consume ? 1 : 0;
+        kc -= trans ? 1 : 0;
+    }
+    g1_xyzz W = xyzz_add(T, xyzz_mul_small_uniform(R, (uint32_t)(lo - 1), 7));
+
+    // sum the 8 lanes of each window
+    __syncthreads();
+    xyzz_store(lds + tid * 32, W);
+    __syncthreads();
+    for (int off = 1; off < LANES_PER_WIN; off <<= 1) {
+        if ((s & (2 * off - 1)) == 0) {
+            W = xyzz_add(W, xyzz_load(lds + (tid + off) * 32));
+            xyzz_store(lds + tid * 32, W);
+        }
+        __syncthreads();
+    }
+    if (s == 0) xyzz_store(leaf, W);
+}
+
+// One level of the per-(line, window) binary trees over the chunks:
+// heap[i] = heap[2i] + heap[2i+1] for i in [2^d, 2^(d+1)).
+__global__ __launch_bounds__(64) void k_msm_tree_level(uint32_t* heap, int h, int d, uint32_t total) {
+    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= total) return;
+    const uint32_t nb = 1u << h;
+    uint32_t lw = t >> d;
+    uint32_t i = (1u << d) + (t & ((1u << d) - 1));
+    uint32_t* base = heap + (size_t)lw * 2 * nb * 32;
+    g1_xyzz a = xyzz_load(base + (size_t)(2 * i) * 32);
+    g1_xyzz c = xyzz_load(base + (size_t)(2 * i + 1) * 32);
+    xyzz_store(base + (size_t)i * 32, xyzz_add(a, c));
+}
+
+// Window recombination sum_w 2^(8w) S_w for every output node, then canonical affine.
+// Output slots (T = 2nb - 2):  [0, T) proof_tree  (line d-1, node i = slot + 2, d = log2 i)
+//                              [T, 2T) comm_tree  (line h, node i = slot - T + 2)
+//                              2T commitment (line h, node 1), 2T + 1 high (line h + 1, node 1)
+__global__ __launch_bounds__(64) void k_msm_horner(const uint32_t* heap, int h, uint32_t* out) {
+    const uint32_t nb = 1u << h;
+    const uint32_t T = 2 * nb - 2;
+    uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
+    if (slot >= 2 * T + 2) return;
+    uint32_t line, node;
+    if (slot < T) {
+        node = slot + 2;
+        line = 31 - __builtin_clz(node) - 1;
+    } else if (slot < 2 * T) {
+        node = slot - T + 2;
+        line = h;
+    } else if (slot == 2 * T) {
+        node = 1;
+        line = h;
+    } else {
+        node = 1;
+        line = h + 1;
+    }
+    g1_xyzz acc = xyzz_inf();
+    for (int w = MSM_W - 1; w >= 0; w--) {
+        for (int k = 0; k < MSM_C; k++) acc = xyzz_dbl(acc);
+        acc = xyzz_add(acc, xyzz_load(heap + (((size_t)line * MSM_W + w) * 2 * nb + node) * 32));
+    }
+    out_store_canon(out + (size_t)slot * 16, acc);
+}
+
+// multiple_multiexp output: out[L * nb + b] = sum_w 2^(8w) heap leaf (L, w, nb + b)
+__global__ __launch_bounds__(64) void k_msm_leaves_out(const uint32_t* heap, int h, int nlines, uint32_t* out) {
+    const uint32_t nb = 1u << h;
+    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nlines * nb) return;
+    uint32_t L = t >> h, b = t & (nb - 1);
+    g1_xyzz acc = xyzz_inf();
+    for (int w = MSM_W - 1; w >= 0; w--) {
+        for (int k = 0; k < MSM_C; k++) acc = xyzz_dbl(acc);
+        acc = xyzz_add(acc, xyzz_load(heap + (((size_t)L * MSM_W + w) * 2 * nb + nb + b) * 32));
+    }
+    out_store_canon(out + (size_t)t * 16, acc);
+}
+
+// ---------------------------------------------------------------------------
+// Merkle (EncodedBlobMerkle::build, crates/encoder/src/merkle/blob.rs:15-34)
+// ---------------------------------------------------------------------------
+// One leaf per thread: 8 consecutive row-major elements of coset c, converted out of
+// Montgomery form (scalar_to_h256, crates/encoder/src/utils.rs:17-20), written to the
+// encoded-data output and hashed (keccak_chunked(.., 8), utils.rs:44-57).
+__global__ void k_merkle_leaves(const uint32_t* y, int log_col, int log_row, int coset, uint32_t* data_out,
+                                uint32_t* leaves) {
+    const uint32_t gpr = 1u << (log_col - 3);  // leaf groups per row
+    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t rows = 1u << log_row;
+    if (t >= rows * gpr) return;
+    uint32_t r = t & (rows - 1), g = t >> log_row;  // consecutive threads walk down a column
+    const size_t n = (size_t)1 << (log_col + log_row);
+    uint64_t lanes[32];
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+        uint32_t col = 8 * g + k;
+        fr v = f_from_mont(f_load<FrP>(y + 8 * (((size_t)col << log_row) | r)));
+#pragma unroll
+        for (int q = 0; q < 4; q++) lanes[4 * k + q] = (uint64_t)v.v[2 * q] | ((uint64_t)v.v[2 * q + 1] << 32);
+    }
+    size_t elem = (size_t)coset * n + ((size_t)r << log_col) + 8 * g;
+    if (data_out) {
+        uint4* dst = (uint4*)(data_out + 8 * elem);
+#pragma unroll
+        for (int q = 0; q < 16; q++)
+            dst[q] = make_uint4((uint32_t)lanes[2 * q], (uint32_t)(lanes[2 * q] >> 32), (uint32_t)lanes[2 * q + 1],
+                                (uint32_t)(lanes[2 * q + 1] >> 32));
+    }
+    uint64_t hsh[4];
+    keccak256_256(lanes, hsh);
+    uint64_t* lo = (uint64_t*)(leaves + 8 * (elem >> 3));
+#pragma unroll
+    for (int q = 0; q < 4; q++) lo[q] = hsh[q];
+}
+
+// leaves from 32-byte elements already in memory (primitive)
+__global__ void k_merkle_leaves_flat(const uint32_t* data, uint32_t* leaves, uint32_t count) {
+    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= count) return;
+    uint64_t lanes[32];
+    const uint64_t* src = (const uint64_t*)(data + 64 * (size_t)t);
+#pragma unroll
+    for (int q = 0; q < 32; q++) lanes[q] = src[q];
+    uint64_t hsh[4];
+    keccak256_256(lanes, hsh);
+    uint64_t* lo = (uint64_t*)(leaves + 8 * (size_t)t);
+#pragma unroll
+    for (int q = 0; q < 4; q++) lo[q] = hsh[q];
+}
+
+// parent[i] = keccak256(child[2i] || child[2i+1])  (keccak_chunked(.., 2))
+__global__ void k_merkle_level(const uint32_t* child, uint32_t* parent, uint32_t count) {
+    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= count) return;
+    uint64_t in[8];
+    const uint64_t* src = (const uint64_t*)(child + 16 * (size_t)t);
+#pragma unroll
+    for (int q = 0; q < 8; q++) in[q] = src[q];
+    uint64_t hsh[4];
+    keccak256_64(in, hsh);
+    uint64_t* dst = (uint64_t*)(parent + 8 * (size_t)t);
+#pragma unroll
+    for (int q = 0; q < 4; q++) dst[q] = hsh[q];
+}
+
+// compute_file_root (crates/encoder/src/encoder/blob.rs:64-72) over the top layer
+__global__ void k_file_root(const uint32_t* roots, int count, uint32_t* out) {
+    if (threadIdx.x || blockIdx.x) return;
+    const uint64_t* r = (const uint64_t*)roots;
+    uint64_t res[4];
+    if (count == 1) {
+        for (int q = 0; q < 4; q++) res[q] = r[q];
+    } else {
+        uint64_t in[8];
+        for (int q = 0; q < 8; q++) in[q] = r[q];
+        keccak256_64(in, res);
+        if (count == 3) {
+            for (int q = 0; q < 4; q++) {
+                in[q] = res[q];
+                in[4 + q] = r[8 + q];
+            }
+            keccak256_64(in, res);
+        }
+    }
+    uint64_t* o = (uint64_t*)out;
+    for (int q = 0; q < 4; q++) o[q] = res[q];
+}
+
+// ---------------------------------------------------------------------------
+// params from tau: closed form of AMTParams::from_pp (generate.rs:139-232)
+// ---------------------------------------------------------------------------
+// fixed-base table: table[w * 256 + j] = [j * 2^(8w)] G1 (affine mont), j > 0
+__global__ __launch_bounds__(64) void k_fixed_base_table(uint32_t* table) {
+    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 32 * 256) return;
+    uint32_t w = t >> 8, j = t & 255;
+    g1_aff G;
+    G.x = f_one<FqP>();  // generator (1, 2)
+    G.y = f_dbl(G.x);
+    g1_xyzz acc = xyzz_inf();
+    for (int b = 7; b >= 0; b--) {
+        acc = xyzz_dbl(acc);
+        if ((j >> b) & 1) acc = xyzz_madd(acc, G);
+    }
+    for (uint32_t k = 0; k < 8 * w; k++) acc = xyzz_dbl(acc);
+    bool inf;
+    g1_aff a = xyzz_to_aff(acc, &inf);
+    if (inf) {
+        a.x = f_zero<FqP>();
+        a.y = f_zero<FqP>();
+    }
+    aff_store(table + 16 * (size_t)t, a);
+}
+
+// [s] G1 for a canonical scalar s, via the fixed-base table
+__device__ __forceinline__ g1_xyzz fixed_base_mul(const uint32_t* table, const fr& s) {
+    g1_xyzz acc = xyzz_inf();
+    for (int w = 0; w < 32; w++) {
+        uint32_t d = (s.v[w >> 2] >> (8 * (w & 3))) & 0xffu;
+        g1_aff P = aff_load(table + 16 * (size_t)(w * 256 + (d ? d : 1)));
+        g1_xyzz nxt = xyzz_madd(acc, P);
+        acc = xyzz_select(d != 0, nxt, acc);
+    }
+    return acc;
+}
+
+// per-coset scalars: pc[0] = tau' = tau / cf, pc[1] = tau'^N - 1, pc[2] = tau^(3N),
+// pc[3 + d - 1] = tau'^(N / 2^d) for d = 1..h
+__global__ void k_params_consts(const uint32_t* tau_canon, const uint32_t* consts, int coset, int log_n, int h,
+                                uint32_t* pc) {
+    if (threadIdx.x || blockIdx.x) return;
+    fr tau = f_to_mont(f_load<FrP>(tau_canon));
+    fr cf = f_load<FrP>(consts + 8 * (K_CF + coset));
+    fr tp = f_mul(tau, f_inv(cf));
+    f_store(pc, tp);
+    fr tpN = tp;
+    for (int i = 0; i < log_n; i++) tpN = f_sqr(tpN);
+    f_store(pc + 8, f_sub(tpN, f_one<FrP>()));
+    // tau^(3N) (power_tau.rs:66-71: high_start = 2^(depth+2) - 2^depth)
+    fr tN = tau;
+    for (int i = 0; i < log_n; i++) tN = f_sqr(tN);
+    f_store(pc + 16, f_mul(f_sqr(tN), tN));
+    for (int d = 1; d <= h; d++) {
+        fr x = tp;
+        for (int i = 0; i < log_n - d; i++) x = f_sqr(x);
+        f_store(pc + 8 * (3 + d - 1), x);
+    }
+}
+
+// One thread per domain index t: the 2^h + ... = h + 2 line scalars
+//   A_t = w^t / (N (tau' - w^t)),  basis = A_t (tau'^N - 1),  high = tau^(3N) basis,
+//   quot_d = A_t (tau'^m - w^(t m)), m = N / 2^d,
+// each turned into [s] G1 and stored at the bit-reversed position (enact, generate.rs:122-126).
+__global__ __launch_bounds__(64) void k_params_gen(uint32_t* bases, const uint32_t* consts, const uint32_t* pc, const uint32_t* table,
+                             int log_n, int h, int* bad) {
+    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t n = 1u << log_n;
+    if (t >= n) return;
+    fr w = f_load<FrP>(consts + 8 * K_OMEGA);
+    fr wt = f_one<FrP>();
+    for (int i = log_n - 1; i >= 0; i--) {
+        wt = f_sqr(wt);
+        if ((t >> i) & 1) wt = f_mul(wt, w);
+    }
+    fr tp = f_load<FrP>(pc);
+    fr den = f_sub(tp, wt);
+    if (f_is_zero(den)) *bad = 1;
+    fr A = f_mul(f_mul(f_inv(den), wt), f_load<FrP>(consts + 8 * K_NINV));
+    const uint32_t pos = bitrev_bits(t, log_n);
+    fr sb = f_mul(A, f_load<FrP>(pc + 8));
+    for (int line = 0; line < h + 2; line++) {
+        fr sc;
+        if (line == h) {
+            sc = sb;
+        } else if (line == h + 1) {
+            sc = f_mul(sb, f_load<FrP>(pc + 16));
+        } else {
+            int d = line + 1;
+            fr wtm = wt;  // w^(t m) = (w^t)^(2^(log_n - d))
+            for (int i = 0; i < log_n - d; i++) wtm = f_sqr(wtm);
+            sc = f_mul(A, f_sub(f_load<FrP>(pc + 8 * (3 + d - 1)), wtm));
+        }
+        g1_xyzz P = fixed_base_mul(table, f_from_mont(sc));
+        bool inf;
+        g1_aff a = xyzz_to_aff(P, &inf);
+        if (inf) *bad = 1;
+        aff_store(bases + (((size_t)line << log_n) + pos) * 16, a);
+    }
+}
+
+}  // namespace zgr

@@ -1,0 +1,356 @@
+"""ctypes binding of lib/libzgr.so (include/zgr.h) plus a thin Python mirror of the
+reference encode API (crates/encoder: RawData, RawBlob, EncodedBlob; grpc:
+EncoderService::process_data).
+
+The HIP library is mandatory: importing works without it, but every call that
+needs the GPU raises if lib/libzgr.so is missing or no device is present.  There
+is no CPU fallback.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "libzgr.so")
+
+R_MOD = 21888242871839275222246405745257275088548364400416034343698204186575808495617
+
+OK = 0
+ERR_INVALID_ARG = -1
+ERR_TOO_LARGE = -2
+ERR_OOM = -3
+ERR_DEVICE = -4
+ERR_NOT_LOADED = -5
+ERR_INCONSISTENT = -6
+
+# exported symbols and their ctypes signatures (kept in step with include/zgr.h)
+P = ctypes.c_void_p
+I = ctypes.c_int
+S = ctypes.c_size_t
+SIGNATURES = {
+    "zgr_last_error": (ctypes.c_char_p, []),
+    "zgr_ctx_create": (I, [I, I, I, I, ctypes.POINTER(P)]),
+    "zgr_ctx_destroy": (None, [P]),
+    "zgr_params_load": (I, [P, I, P, S]),
+    "zgr_params_generate": (I, [P, P]),
+    "zgr_params_download": (I, [P, I, P, S]),
+    "zgr_amt_prove": (I, [P, I, P, S, P, P, P, P]),
+    "zgr_multiple_multiexp": (I, [P, I, P, S, P]),
+    "zgr_encode": (I, [P, P, S, P]),
+    "zgr_encode_device": (I, [P, P, S, P]),
+    "zgr_encode_batch_device": (I, [P, I, P, P, P]),
+    "zgr_ntt": (I, [P, P, I, I, I]),
+    "zgr_coset_extend": (I, [P, P, I, P]),
+    "zgr_msm": (I, [P, P, P, S, P]),
+    "zgr_ecfft": (I, [P, P, I, I, P]),
+    "zgr_merkle": (I, [P, P, S, I, P, P]),
+    "zgr_kernel_time_ms": (ctypes.c_double, [P, ctypes.c_char_p, ctypes.POINTER(I)]),
+    "zgr_kernel_time_reset": (None, [P]),
+    "zgr_set_profiling": (None, [P, I]),
+}
+
+_lib = None
+
+
+class ZgrError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("zgr error %d: %s" % (code, msg))
+        self.code = code
+
+
+class TooLargeBlob(ValueError):
+    """EncoderError::TooLargeBlob (crates/encoder/src/encoder/error.rs:3-6)."""
+
+    def __init__(self, actual, expected_max):
+        super().__init__("TooLargeBlob { actual: %d, expected_max: %d }" % (actual, expected_max))
+        self.actual = actual
+        self.expected_max = expected_max
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("HIP library %s is not built (run 0g-da-encoder_amd/build.py)" % LIB_PATH)
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return ctypes.c_void_p(a.ctypes.data) if a is not None else None
+
+
+def _check(rc):
+    if rc != OK:
+        raise ZgrError(rc, lib().zgr_last_error().decode(errors="replace"))
+    return rc
+
+
+def tau_from_seed(seed):
+    """Deterministic insecure-setup tau: 64 bytes of numpy PCG64(seed) reduced mod r."""
+    b = np.random.default_rng(seed).integers(0, 256, 64, dtype=np.uint8).tobytes()
+    return int.from_bytes(b, "little") % R_MOD
+
+
+def seeded_bytes(seed, length):
+    """Seeded uniform random bytes (numpy PCG64)."""
+    return np.random.default_rng(seed).integers(0, 256, length, dtype=np.uint8).tobytes()
+
+
+class Geometry:
+    """Blob geometry: crates/encoder/src/constants.rs:11-28 as runtime values."""
+
+    def __init__(self, log_col, log_row, coset_n=3):
+        self.log_col, self.log_row, self.coset_n = log_col, log_row, coset_n
+        self.log_n = log_col + log_row
+        self.h = log_row
+        self.n = 1 << self.log_n
+        self.row_n = 1 << log_row
+        self.col_n = 1 << log_col
+        self.row_encoded = self.row_n * coset_n
+        self.max_blob_size = 31 * self.n
+        self.max_raw_data_size = self.max_blob_size - 4
+        self.tree_pts = (1 << (log_row + 1)) - 2
+        self.merkle_nodes = coset_n * ((1 << (log_row + 1)) - 1)
+
+    def params_bytes(self):
+        return (self.h + 2) * self.n * 64
+
+
+PRODUCTION = Geometry(10, 10, 3)  # BLOB_COL_LOG = BLOB_ROW_LOG = 10
+TEST_SETTING = Geometry(5, 6, 3)   # BLOB_COL_LOG = 5, BLOB_ROW_LOG = 6 (TEST_SETTING)
+
+
+class EncodedBlobOutputs:
+    """Host buffers matching zgr_encoded."""
+
+    class _C(ctypes.Structure):
+        _fields_ = [(k, ctypes.c_void_p) for k in
+                    ("data", "commitment", "comm_tree", "proof_tree", "high", "merkle", "file_root")]
+
+    def __init__(self, geo, want_data=True):
+        g = geo
+        self.data = np.zeros(g.coset_n * g.n * 32, dtype=np.uint8) if want_data else None
+        self.commitment = np.zeros(64, dtype=np.uint8)
+        self.comm_tree = np.zeros(g.coset_n * g.tree_pts * 64, dtype=np.uint8)
+        self.proof_tree = np.zeros(g.coset_n * g.tree_pts * 64, dtype=np.uint8)
+        self.high = np.zeros(g.coset_n * 64, dtype=np.uint8)
+        self.merkle = np.zeros(g.merkle_nodes * 32, dtype=np.uint8)
+        self.file_root = np.zeros(32, dtype=np.uint8)
+
+    def cstruct(self):
+        c = self._C()
+        for k, _ in self._C._fields_:
+            a = getattr(self, k)
+            setattr(c, k, a.ctypes.data if a is not None else None)
+        return c
+
+
+class Context:
+    """A zgr_ctx: one device, one blob geometry, resident params."""
+
+    def __init__(self, geo=PRODUCTION, device=0):
+        self.geo = geo
+        h = ctypes.c_void_p()
+        _check(lib().zgr_ctx_create(device, geo.log_col, geo.log_row, geo.coset_n, ctypes.byref(h)))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            lib().zgr_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- params ----
+    def params_generate(self, tau):
+        t = np.frombuffer((tau % R_MOD).to_bytes(32, "little"), dtype=np.uint8).copy()
+        _check(lib().zgr_params_generate(self.h, _p(t)))
+
+    def params_load(self, coset, bases):
+        b = np.ascontiguousarray(bases)
+        _check(lib().zgr_params_load(self.h, coset, _p(b), b.nbytes))
+
+    def params_download(self, coset):
+        out = np.zeros(self.geo.params_bytes() // 8, dtype=np.uint64)
+        _check(lib().zgr_params_download(self.h, coset, _p(out), out.nbytes))
+        return out
+
+    # ---- prover seam ----
+    def amt_prove(self, coset, ri_mont):
+        g = self.geo
+        ri = np.ascontiguousarray(ri_mont, dtype=np.uint64)
+        com = np.zeros(64, dtype=np.uint8)
+        ct = np.zeros(g.tree_pts * 64, dtype=np.uint8)
+        pt = np.zeros(g.tree_pts * 64, dtype=np.uint8)
+        hi = np.zeros(64, dtype=np.uint8)
+        _check(lib().zgr_amt_prove(self.h, coset, _p(ri), ri.size // 4, _p(com), _p(ct), _p(pt), _p(hi)))
+        return com, ct.reshape(-1, 64), pt.reshape(-1, 64), hi
+
+    def multiple_multiexp(self, coset, ri_mont):
+        g = self.geo
+        ri = np.ascontiguousarray(ri_mont, dtype=np.uint64)
+        out = np.zeros((g.h + 2) * (1 << g.h) * 64, dtype=np.uint8)
+        _check(lib().zgr_multiple_multiexp(self.h, coset, _p(ri), ri.size // 4, _p(out)))
+        return out.reshape(-1, 64)
+
+    # ---- encode ----
+    def encode(self, raw, want_data=True):
+        if len(raw) > self.geo.max_raw_data_size:
+            raise TooLargeBlob(len(raw), self.geo.max_raw_data_size)
+        out = EncodedBlobOutputs(self.geo, want_data)
+        buf = np.frombuffer(bytes(raw), dtype=np.uint8) if len(raw) else np.zeros(1, dtype=np.uint8)
+        cs = out.cstruct()
+        _check(lib().zgr_encode(self.h, _p(buf), len(raw), ctypes.byref(cs)))
+        return out
+
+    def encode_device(self, raw_dev_ptr, length, want_data=True, out=None):
+        out = out or EncodedBlobOutputs(self.geo, want_data)
+        cs = out.cstruct()
+        _check(lib().zgr_encode_device(self.h, ctypes.c_void_p(raw_dev_ptr), length, ctypes.byref(cs)))
+        return out
+
+    def encode_batch_device(self, ptrs, lengths, outs):
+        n = len(ptrs)
+        parr = (ctypes.c_void_p * n)(*ptrs)
+        larr = (ctypes.c_size_t * n)(*lengths)
+        carr = (EncodedBlobOutputs._C * n)(*[o.cstruct() for o in outs])
+        _check(lib().zgr_encode_batch_device(self.h, n, ctypes.cast(parr, P), ctypes.cast(larr, P),
+                                             ctypes.cast(carr, P)))
+        return outs
+
+    # ---- primitives ----
+    def ntt(self, rows_mont, log_n, inverse=False):
+        a = np.ascontiguousarray(rows_mont, dtype=np.uint64).copy()
+        batch = a.size // (4 << log_n)
+        _check(lib().zgr_ntt(self.h, _p(a), log_n, batch, int(inverse)))
+        return a
+
+    def coset_extend(self, data_mont, coset):
+        d = np.ascontiguousarray(data_mont, dtype=np.uint64)
+        out = np.zeros_like(d)
+        _check(lib().zgr_coset_extend(self.h, _p(d), coset, _p(out)))
+        return out
+
+    def msm(self, bases_mont, scalars_mont):
+        b = np.ascontiguousarray(bases_mont, dtype=np.uint64)
+        s = np.ascontiguousarray(scalars_mont, dtype=np.uint64)
+        out = np.zeros(64, dtype=np.uint8)
+        _check(lib().zgr_msm(self.h, _p(b), _p(s), s.size // 4, _p(out)))
+        return out
+
+    def ecfft(self, points, log_n, inverse=False):
+        p = np.ascontiguousarray(points, dtype=np.uint8).reshape(-1)
+        out = np.zeros_like(p)
+        _check(lib().zgr_ecfft(self.h, _p(p), log_n, int(inverse), _p(out)))
+        return out.reshape(-1, 64)
+
+    def merkle(self, data, levels):
+        d = np.ascontiguousarray(data, dtype=np.uint8).reshape(-1)
+        top = sum(self.geo.coset_n << k for k in range(levels + 1))
+        layers = np.zeros(top * 32, dtype=np.uint8)
+        root = np.zeros(32, dtype=np.uint8)
+        _check(lib().zgr_merkle(self.h, _p(d), d.size // 32, levels, _p(layers), _p(root)))
+        return layers.reshape(-1, 32), bytes(root)
+
+    # ---- profiling ----
+    def set_profiling(self, on):
+        lib().zgr_set_profiling(self.h, int(on))
+
+    def kernel_time_ms(self, name):
+        cnt = ctypes.c_int(0)
+        ms = lib().zgr_kernel_time_ms(self.h, name.encode(), ctypes.byref(cnt))
+        return ms, cnt.value
+
+    def kernel_time_reset(self):
+        lib().zgr_kernel_time_reset(self.h)
+
+
+# ---------------------------------------------------------------------------
+# mirror of the reference encode API
+# ---------------------------------------------------------------------------
+class EncodedBlob:
+    """EncodedBlob (crates/encoder/src/encoder/blob.rs:26-62) backed by GPU outputs."""
+
+    def __init__(self, geo, out):
+        self.geo = geo
+        self.out = out
+
+    @classmethod
+    def build(cls, ctx, raw):
+        return cls(ctx.geo, ctx.encode(raw))
+
+    def get_affine_commitment(self):
+        return bytes(self.out.commitment)
+
+    def get_commitment(self):
+        return self.get_affine_commitment()
+
+    def get_file_root(self):
+        return bytes(self.out.file_root)
+
+    def get_roots(self):
+        return [bytes(self.out.merkle[32 * i:32 * i + 32]) for i in range(self.geo.coset_n)]
+
+    def get_data(self):
+        return self.out.data
+
+    def merkle_layer(self, d):
+        """EncodedBlobMerkle.tree[d] for d <= log_row."""
+        g = self.geo
+        off = sum(g.coset_n << k for k in range(d))
+        cnt = g.coset_n << d
+        return self.out.merkle[32 * off:32 * (off + cnt)].reshape(cnt, 32)
+
+    def _tree_level(self, tree, coset, d):
+        g = self.geo
+        base = coset * g.tree_pts + (1 << (d + 1)) - 2
+        return tree.reshape(-1, 64)[base:base + (1 << (d + 1))]
+
+    def get_row(self, index):
+        """EncodedBlob::get_row (blob.rs:44-49): (row data, AMT proof, Merkle proof)."""
+        g = self.geo
+        assert index < g.row_encoded
+        coset, local = divmod(index, g.row_n)
+        row = self.out.data[32 * (coset * g.n + local * g.col_n):32 * (coset * g.n + (local + 1) * g.col_n)]
+        # AllProofs::get_proof with reversed_index = bitrev(local) (proofs.rs:16-31)
+        rev = int(format(local, "0%db" % g.log_row)[::-1], 2)
+        proof = []
+        for d in range(g.h):
+            lv = rev >> (g.h - 1 - d)
+            proof.append((bytes(self._tree_level(self.out.comm_tree, coset, d)[lv ^ 1]),
+                          bytes(self._tree_level(self.out.proof_tree, coset, d)[lv ^ 1])))
+        high = bytes(self.out.high[64 * coset:64 * coset + 64])
+        # EncodedBlobMerkle::get_row (merkle/blob.rs:44-57)
+        mproof = [bytes(self.merkle_layer(d)[(index >> (g.log_row - d)) ^ 1]) for d in range(g.log_row, 0, -1)]
+        return {"index": index, "row": bytes(row), "amt_proof": proof, "high_commitment": high,
+                "commitment": self.get_affine_commitment(), "merkle_root": self.get_roots(),
+                "merkle_proof": mproof, "row_root": bytes(self.merkle_layer(g.log_row)[index])}
+
+
+class EncoderService:
+    """EncoderService::process_data (grpc/src/service.rs:67-106) minus the slice wire bytes."""
+
+    def __init__(self, ctx):
+        self.ctx = ctx
+
+    def process_data(self, data, require_data):
+        blob = EncodedBlob.build(self.ctx, data)
+        c = blob.get_affine_commitment()
+        return {
+            "version": 0,
+            "erasure_commitment": c,  # x || y, 32 B LE each (service.rs:77-83)
+            "storage_root": blob.get_file_root(),
+            "encoded_data": bytes(blob.get_data()) if require_data else b"",
+            "blob": blob,
+        }

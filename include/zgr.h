@@ -1,0 +1,139 @@
+/*
+ * zgr.h -- C ABI of the MI355X (gfx950) 0G DA encoder.
+ *
+ * One shared library, lib/libzgr.so inside 0g-da-encoder_amd/, implements the
+ * encoding path RawBlob -> EncodedBlob of 0glabs/0g-da-encoder on the GPU.  Every
+ * entry point below names the reference interface it replaces (paths relative to
+ * the reference repository root).
+ *
+ * Conventions
+ *   - Plain pointers and sizes only.  Inputs are borrowed for the duration of the
+ *     call; outputs are caller-allocated.  Nothing is freed across the boundary.
+ *   - Fr / Fq elements are 32 bytes little-endian.  "mont" = Montgomery form with
+ *     R = 2^256 (arkworks' internal representation).
+ *   - Prove-base buffers (params) hold affine G1 points as x_mont || y_mont
+ *     (64 B each, the bamt layout of crates/amt/src/prove_params/
+ *     fast_serde_bn254.rs:139-147), line-major in the order read_gpu_bases uploads
+ *     them (crates/amt/src/prove_params/prove_gpu.rs:42-47):
+ *         quotients[0] .. quotients[h-1] || basis || high_basis,
+ *     each line 2^log_n points, so one coset's buffer is (h + 2) * 2^log_n * 64 B.
+ *   - Output points are canonical affine x || y (64 B).  The point at infinity is
+ *     64 zero bytes ((0, 0) is not on y^2 = x^3 + 3).
+ *   - Return value: ZGR_OK or a negative ZGR_ERR_*; zgr_last_error() gives a
+ *     thread-local message.  Never unwinds.
+ *   - A context serialises the calls made on it; use one context per host thread
+ *     for concurrency, or the batch entry points.
+ */
+#ifndef ZGR_H
+#define ZGR_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    ZGR_OK = 0,
+    ZGR_ERR_INVALID_ARG = -1,
+    ZGR_ERR_TOO_LARGE = -2,      /* EncoderError::TooLargeBlob (crates/encoder/src/encoder/error.rs:3-6) */
+    ZGR_ERR_OOM = -3,            /* device allocation / admission failure */
+    ZGR_ERR_DEVICE = -4,         /* HIP runtime error */
+    ZGR_ERR_NOT_LOADED = -5,     /* params for a coset are not resident */
+    ZGR_ERR_INCONSISTENT = -6    /* coset commitments differ (crates/encoder/src/amt/blob.rs:42-47) */
+};
+
+typedef struct zgr_ctx zgr_ctx;
+
+/* Outputs of one encoded blob, caller-allocated (NULL = not wanted).
+ * h = log_row, nb = 2^h, T = 2^(h+1) - 2 (points of one AllProofs tree). */
+typedef struct {
+    uint8_t *data;        /* coset_n * 2^log_n * 32: scalar_to_h256 of every encoded element,
+                             coset-major then row-major (EncodedBlob::get_data,
+                             crates/encoder/src/encoder/blob.rs:61) */
+    uint8_t *commitment;  /* 64: EncodedBlob::get_affine_commitment (blob.rs:51) */
+    uint8_t *comm_tree;   /* coset_n * T * 64: AllProofs.commitments, level d (2^(d+1) points)
+                             after levels 0..d-1 (crates/amt/src/proofs.rs:8-14) */
+    uint8_t *proof_tree;  /* coset_n * T * 64: AllProofs.proofs, same layout */
+    uint8_t *high;        /* coset_n * 64: AllProofs.high_commitment */
+    uint8_t *merkle;      /* coset_n * (2^(h+1) - 1) * 32: EncodedBlobMerkle.tree[0..=h]
+                             (crates/encoder/src/merkle/blob.rs:15-34) */
+    uint8_t *file_root;   /* 32: EncodedBlob::get_file_root (blob.rs:57-59) */
+} zgr_encoded;
+
+const char *zgr_last_error(void);
+
+/* Context on one device for one blob geometry (log_col, log_row, coset_n).
+ * Replaces the per-thread workspace of ag_cuda_ec::init_local_workspace
+ * (crates/amt/src/prove_params/interfaces.rs:41-44).  coset_n <= 3. */
+int zgr_ctx_create(int device, int log_col, int log_row, int coset_n, zgr_ctx **out);
+void zgr_ctx_destroy(zgr_ctx *ctx);
+
+/* ---- resident public parameters (EncoderParams::warmup, crates/amt/src/blob/encode.rs:85-89;
+ *      read_gpu_bases, prove_gpu.rs:31-53) ---- */
+/* Upload one coset's prove bases from host memory (layout above). */
+int zgr_params_load(zgr_ctx *ctx, int coset, const void *bases_mont, size_t nbytes);
+/* Insecure test setup: generate every coset's prove bases on the device from a known
+ * tau (32 B canonical LE).  Same points as PowerTau::setup_inner with that tau
+ * (crates/amt/src/power_tau.rs:54-80) followed by AMTParams::from_pp
+ * (crates/amt/src/prove_params/generate.rs:139-195). */
+int zgr_params_generate(zgr_ctx *ctx, const uint8_t tau[32]);
+/* Copy one coset's resident bases back to host memory (layout above). */
+int zgr_params_download(zgr_ctx *ctx, int coset, void *out, size_t nbytes);
+
+/* ---- AMT prover seam: AMTProofs::gen_amt_proofs
+ *      (crates/amt/src/prove_params/interfaces.rs:7-45) ---- */
+/* ri_mont: 2^log_n Fr (mont), already bit-reversed (HalfBlob::generate,
+ * crates/amt/src/blob/encode.rs:171-183).  Outputs: commitment (64 B),
+ * comm_tree / proof_tree (T points each), high (64 B). */
+int zgr_amt_prove(zgr_ctx *ctx, int coset, const uint8_t *ri_mont, size_t n,
+                  uint8_t *commitment, uint8_t *comm_tree, uint8_t *proof_tree, uint8_t *high);
+/* ag_cuda_ec::multiple_multiexp_mt as called by gen_all_proofs_gpu
+ * (prove_gpu.rs:77-84): out[L*nb + b] = sum_{i<bs} bases[L*N + b*bs + i] * s[b*bs + i]
+ * for the h + 2 lines L.  out: (h + 2) * nb canonical points. */
+int zgr_multiple_multiexp(zgr_ctx *ctx, int coset, const uint8_t *ri_mont, size_t n,
+                          uint8_t *out_points);
+
+/* ---- full encode: EncodedBlob::build over RawData::try_from + RawBlob::from
+ *      (crates/encoder/src/encoder/blob.rs:32-42, raw_data.rs:55-73, raw_blob.rs:33-40),
+ *      the body of EncoderService::process_data (grpc/src/service.rs:67-106) ---- */
+int zgr_encode(zgr_ctx *ctx, const uint8_t *raw, size_t len, const zgr_encoded *out);
+/* Same, with the raw bytes already in device memory (pointer from hipMalloc /
+ * torch), outputs to host. */
+int zgr_encode_device(zgr_ctx *ctx, const void *raw_dev, size_t len, const zgr_encoded *out);
+/* Batched encode of nblobs blobs whose raw bytes are already in device memory; the
+ * blobs share each pass over the resident bases.  Outputs to host. */
+int zgr_encode_batch_device(zgr_ctx *ctx, int nblobs, const void *const *raw_dev,
+                            const size_t *lens, const zgr_encoded *outs);
+
+/* ---- primitives (reference call sites named per function) ---- */
+/* Radix-2 DFT over the 2^log_n domain of ark-poly Radix2EvaluationDomain
+ * (to_coset_blob, crates/amt/src/blob/encode.rs:142-157).  inverse != 0 applies
+ * the domain's size_inv.  batch rows of 2^log_n Fr (mont), in place, host memory. */
+int zgr_ntt(zgr_ctx *ctx, uint8_t *data_mont, int log_n, int batch, int inverse);
+/* to_coset_blob (encode.rs:135-158) for coset 1..coset_n-1 of ctx's geometry:
+ * data_mont (2^log_n, column-major) -> out_mont evaluations on that coset. */
+int zgr_coset_extend(zgr_ctx *ctx, const uint8_t *data_mont, int coset, uint8_t *out_mont);
+/* Plain MSM sum_i k_i * P_i: bases in param layout (mont), scalars mont. */
+int zgr_msm(zgr_ctx *ctx, const uint8_t *bases_mont, const uint8_t *scalars_mont, size_t n,
+            uint8_t *out_point);
+/* G1 EC-FFT: Pairing::fast_fft / fast_ifft (crates/amt/src/ec_algebra.rs:34-95) over
+ * 2^log_n points (canonical affine in and out); inverse applies size_inv. */
+int zgr_ecfft(zgr_ctx *ctx, const uint8_t *points, int log_n, int inverse, uint8_t *out);
+/* EncodedBlobMerkle::build over n_elems 32-byte elements (host memory): writes
+ * tree[0..=levels] and the file root (merkle/blob.rs:15-34, encoder/blob.rs:64-72). */
+int zgr_merkle(zgr_ctx *ctx, const uint8_t *data, size_t n_elems, int levels,
+               uint8_t *out_layers, uint8_t *file_root);
+
+/* ---- timing hooks for the benchmark (HIP events on the library's own stream) ---- */
+/* Average device time in ms of the named kernel over the calls since the last reset
+ * (name: "msm_accum", "ntt", "merkle", ...); count receives the launch count. */
+double zgr_kernel_time_ms(zgr_ctx *ctx, const char *name, int *count);
+void zgr_kernel_time_reset(zgr_ctx *ctx);
+void zgr_set_profiling(zgr_ctx *ctx, int enabled);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

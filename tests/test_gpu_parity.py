@@ -1,0 +1,122 @@
+"""HIP path vs the CPU oracle at the reference's test geometry (2^6 rows x 2^5
+columns, 3 cosets; crates/encoder/src/constants.rs:11-15).  Bit-exact."""
+import numpy as np
+import pytest
+
+import oracle as O
+import zg_rocm as Z
+from helpers import DATA_SEED, oracle_params, random_fr_mont, ri_from_y, test_tau
+
+pytestmark = pytest.mark.gpu
+
+GEO = Z.TEST_SETTING
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = Z.Context(GEO)
+    c.params_generate(test_tau())
+    yield c
+    c.close()
+
+
+@pytest.fixture(scope="module")
+def params():
+    return oracle_params(GEO.log_col, GEO.log_row, GEO.coset_n)
+
+
+def test_params_generate_matches_oracle(ctx, params):
+    # generate.rs:139-232 (closed form in both; the oracle's closed form is pinned
+    # against its EC-FFT restatement in test_oracle_params.py)
+    for c in range(GEO.coset_n):
+        got = ctx.params_download(c)
+        assert np.array_equal(got, params[c]), "coset %d" % c
+
+
+def test_coset_extend_matches_oracle(ctx):
+    rng = np.random.default_rng(11)
+    data = random_fr_mont(rng, GEO.n)
+    for c in range(GEO.coset_n):
+        exp = O.to_coset_blob(data, GEO.log_n, c)
+        got = ctx.coset_extend(data, c)
+        assert np.array_equal(got.reshape(-1), exp.reshape(-1)), "coset %d" % c
+
+
+@pytest.mark.parametrize("inverse", [False, True])
+@pytest.mark.parametrize("log_n", [1, 4, 10, 11])
+def test_ntt_matches_oracle(ctx, log_n, inverse):
+    rng = np.random.default_rng(100 + log_n)
+    rows = np.stack([random_fr_mont(rng, 1 << log_n) for _ in range(2)])
+    got = ctx.ntt(rows, log_n, inverse)
+    for r in range(2):
+        exp = (O.ifft if inverse else O.fft)(rows[r], log_n)
+        assert np.array_equal(got[r].reshape(-1), exp.reshape(-1))
+
+
+@pytest.mark.parametrize("kind", ["random", "zeros", "small", "sparse"])
+def test_multiple_multiexp_matches_oracle(ctx, params, kind):
+    rng = np.random.default_rng(5)
+    if kind == "random":
+        ri = random_fr_mont(rng, GEO.n)
+    elif kind == "zeros":
+        ri = np.zeros((GEO.n, 4), dtype=np.uint64)
+    elif kind == "small":
+        ri = O.fr_to_mont_array([int(x) for x in rng.integers(0, 300, GEO.n)])
+    else:
+        vals = [0] * GEO.n
+        for i in rng.choice(GEO.n, 40, replace=False):
+            vals[int(i)] = int.from_bytes(rng.bytes(32), "little") % O.R_MOD
+        vals[3] = O.R_MOD - 1  # -1: every digit at its extreme
+        ri = O.fr_to_mont_array(vals)
+    for c in range(GEO.coset_n):
+        exp = O.multiple_multiexp(params[c], GEO.log_n, GEO.h, ri)
+        got = ctx.multiple_multiexp(c, ri)
+        assert np.array_equal(got, exp), "coset %d" % c
+
+
+def test_amt_prove_matches_oracle(ctx, params):
+    rng = np.random.default_rng(9)
+    ri = random_fr_mont(rng, GEO.n)
+    for c in range(GEO.coset_n):
+        exp = O.amt_prove(params[c], GEO.log_n, GEO.h, ri)
+        got = ctx.amt_prove(c, ri)
+        for a, b, name in zip(got, exp, ("commitment", "comm_tree", "proof_tree", "high")):
+            assert np.array_equal(np.asarray(a).reshape(-1), np.asarray(b).reshape(-1)), (c, name)
+
+
+# encoder/blob.rs:282-286 test sizes: 0, 1, 1234, MAX_RAW_DATA_SIZE
+@pytest.mark.parametrize("length", [0, 1, 1234, GEO.max_raw_data_size])
+def test_encode_matches_oracle(ctx, params, length):
+    raw = Z.seeded_bytes(DATA_SEED, length)
+    exp = O.encode(list(params), GEO.log_col, GEO.log_row, raw)
+    got = ctx.encode(raw)
+    assert np.array_equal(got.commitment, exp.commitment)
+    assert np.array_equal(got.file_root, exp.root)
+    assert np.array_equal(got.high, exp.high)
+    assert np.array_equal(got.comm_tree, exp.comm_tree)
+    assert np.array_equal(got.proof_tree, exp.proof_tree)
+    assert np.array_equal(got.merkle, exp.merkle)
+    assert np.array_equal(got.data, exp.data)
+
+
+def test_encode_too_large(ctx):
+    # encoder/blob.rs:286: MAX_RAW_DATA_SIZE + 1 -> TooLargeBlob
+    with pytest.raises(Z.TooLargeBlob):
+        ctx.encode(b"\x00" * (GEO.max_raw_data_size + 1))
+    # the C ABI itself rejects the size too (no Python pre-check)
+    out = Z.EncodedBlobOutputs(GEO)
+    buf = np.zeros(GEO.max_raw_data_size + 1, dtype=np.uint8)
+    cs = out.cstruct()
+    import ctypes
+    rc = Z.lib().zgr_encode(ctx.h, ctypes.c_void_p(buf.ctypes.data), buf.size, ctypes.byref(cs))
+    assert rc == Z.ERR_TOO_LARGE
+    assert "TooLargeBlob" in Z.lib().zgr_last_error().decode()
+
+
+def test_merkle_primitive_matches_oracle(ctx):
+    rng = np.random.default_rng(3)
+    data = rng.integers(0, 256, GEO.coset_n * GEO.n * 32, dtype=np.uint8)
+    nl, exp_layers, exp_root = O.merkle(data, GEO.coset_n, GEO.log_row)
+    layers, root = ctx.merkle(data, GEO.log_row)
+    assert root == exp_root
+    assert np.array_equal(layers.reshape(-1), exp_layers.reshape(-1))
