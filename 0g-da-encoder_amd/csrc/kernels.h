@@ -408,6 +408,78 @@ __global__ __launch_bounds__(64) void k_msm_leaves_out(const uint32_t* heap, int
     out_store_canon(out + (size_t)t * 16, acc);
 }
 
+// single MSM result: Horner over the windows of tree node 1 of line 0
+__global__ __launch_bounds__(64) void k_msm_root_out(const uint32_t* heap, int h, uint32_t* out) {
+    if (threadIdx.x || blockIdx.x) return;
+    const uint32_t nb = 1u << h;
+    g1_xyzz acc = xyzz_inf();
+    for (int w = MSM_W - 1; w >= 0; w--) {
+        for (int k = 0; k < MSM_C; k++) acc = xyzz_dbl(acc);
+        acc = xyzz_add(acc, xyzz_load(heap + ((size_t)w * 2 * nb + 1) * 32));
+    }
+    out_store_canon(out, acc);
+}
+
+// ---------------------------------------------------------------------------
+// G1 EC-FFT (Pairing::fast_fft / fast_ifft, crates/amt/src/ec_algebra.rs:34-95):
+// radix-2 DIT over XYZZ points, twiddle products by 254-bit scalar multiplication.
+// ---------------------------------------------------------------------------
+// canonical affine (zeros = infinity) -> XYZZ mont, at bit-reversed position
+__global__ void k_ec_load_bitrev(const uint32_t* in, uint32_t* pts, int log_n) {
+    uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= (1u << log_n)) return;
+    const uint32_t* s = in + 16 * (size_t)bitrev_bits(p, log_n);
+    fq x = f_load<FqP>(s), y = f_load<FqP>(s + 8);
+    bool inf = f_is_zero(x) && f_is_zero(y);
+    g1_aff a;
+    a.x = f_to_mont(x);
+    a.y = f_to_mont(y);
+    g1_xyzz r = xyzz_select(inf, xyzz_inf(), xyzz_from_aff(a));
+    xyzz_store(pts + 32 * (size_t)p, r);
+}
+
+// k * P for a full-width canonical scalar (MSB-first, uniform per lane)
+__device__ __forceinline__ g1_xyzz xyzz_mul_scalar(const g1_xyzz& p, const fr& k) {
+    g1_xyzz r = xyzz_inf();
+    for (int i = 253; i >= 0; i--) {
+        r = xyzz_dbl(r);
+        g1_xyzz s = xyzz_add(r, p);
+        r = xyzz_select((k.v[i >> 5] >> (i & 31)) & 1, s, r);
+    }
+    return r;
+}
+
+__global__ __launch_bounds__(64) void k_ec_dit_stage(uint32_t* pts, const uint32_t* tw, int s, int tw_shift,
+                                                     uint32_t half_n) {
+    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= half_n) return;
+    uint32_t half = 1u << (s - 1);
+    uint32_t j = t & (half - 1);
+    uint32_t i0 = ((t >> (s - 1)) << s) | j, i1 = i0 + half;
+    g1_xyzz u = xyzz_load(pts + 32 * (size_t)i0);
+    g1_xyzz v = xyzz_load(pts + 32 * (size_t)i1);
+    fr w = f_from_mont(f_load<FrP>(tw + 8 * (size_t)(j << tw_shift)));
+    v = xyzz_mul_scalar(v, w);
+    g1_xyzz nv = v;
+    nv.Y = f_neg(v.Y);
+    xyzz_store(pts + 32 * (size_t)i0, xyzz_add(u, v));
+    xyzz_store(pts + 32 * (size_t)i1, xyzz_add(u, nv));
+}
+
+// size_inv scaling of the inverse transform (N_ctx^-1 * 2^shift), then canonical affine out
+__global__ __launch_bounds__(64) void k_ec_store(const uint32_t* pts, const uint32_t* consts, int scale_shift,
+                                                 int scale, uint32_t* out, uint32_t n) {
+    uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    g1_xyzz v = xyzz_load(pts + 32 * (size_t)p);
+    if (scale) {
+        fr f = f_load<FrP>(consts + 8 * K_NINV);
+        for (int i = 0; i < scale_shift; i++) f = f_dbl(f);
+        v = xyzz_mul_scalar(v, f_from_mont(f));
+    }
+    out_store_canon(out + 16 * (size_t)p, v);
+}
+
 // ---------------------------------------------------------------------------
 // Merkle (EncodedBlobMerkle::build, crates/encoder/src/merkle/blob.rs:15-34)
 // ---------------------------------------------------------------------------

@@ -546,20 +546,76 @@ int zgr_coset_extend(zgr_ctx* c, const uint8_t* data_mont, int coset, uint8_t* o
 }
 
 int zgr_msm(zgr_ctx* c, const uint8_t* bases_mont, const uint8_t* scalars_mont, size_t n, uint8_t* out_point) {
-    (void)bases_mont;
-    (void)scalars_mont;
-    (void)n;
-    (void)out_point;
-    return fail(ZGR_ERR_INVALID_ARG, "zgr_msm: not implemented yet");
+    if (!c || !bases_mont || !scalars_mont || !out_point) return fail(ZGR_ERR_INVALID_ARG, "null argument");
+    if (n == 0 || n > c->n) return fail(ZGR_ERR_INVALID_ARG, "msm size %zu outside [1, %zu]", n, c->n);
+    std::lock_guard<std::mutex> lk(c->mu);
+    HIPCHK(hipSetDevice(c->device));
+    int log_np = 0;
+    while (((size_t)1 << log_np) < n) log_np++;
+    const size_t np = (size_t)1 << log_np;
+    const int log_bs = log_np < 10 ? log_np : 10;
+    const int h = log_np - log_bs;
+    uint32_t* dbases = nullptr;
+    DALLOC(dbases, np * 64);
+    auto body = [&]() -> int {
+        // zero-padded scalars: padding points are never read by the bucket kernel
+        HIPCHK(hipMemsetAsync(c->y, 0, np * 32, c->stream));
+        HIPCHK(hipMemsetAsync(dbases, 0, np * 64, c->stream));
+        HIPCHK(hipMemcpyAsync(c->y, scalars_mont, n * 32, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(hipMemcpyAsync(dbases, bases_mont, n * 64, hipMemcpyHostToDevice, c->stream));
+        LAUNCH(c, "msm_digits", k_msm_digits, blocks(np, 256), 256, 0, c->y, log_np, log_bs, c->digits, 0);
+        uint32_t bs = 1u << log_bs, nb = 1u << h;
+        LAUNCH(c, "msm_sort", k_msm_sort, nb * MSM_W, bs < 256 ? bs : 256u, 0, c->digits, log_bs, c->entries,
+               c->counts);
+        size_t shm = (size_t)bs * 64;
+        if (shm < (size_t)ACC_THREADS * 128) shm = (size_t)ACC_THREADS * 128;
+        LAUNCH(c, "msm_accum", k_msm_accum, nb, ACC_THREADS, shm, dbases, c->entries, c->counts, c->heap, log_np,
+               h, 1);
+        for (int d = h - 1; d >= 0; d--) {
+            uint32_t total = (uint32_t)MSM_W << d;
+            LAUNCH(c, "msm_tree", k_msm_tree_level, blocks(total, 64), 64, 0, c->heap, h, d, total);
+        }
+        LAUNCH(c, "msm_root", k_msm_root_out, 1, 64, 0, c->heap, h, c->outpts);
+        HIPCHK(hipMemcpyAsync(out_point, c->outpts, 64, hipMemcpyDeviceToHost, c->stream));
+        return sync_and_collect(c);
+    };
+    int rc = body();
+    hipStreamSynchronize(c->stream);
+    hipFree(dbases);
+    return rc;
 }
 
 int zgr_ecfft(zgr_ctx* c, const uint8_t* points, int log_n, int inverse, uint8_t* out) {
-    (void)c;
-    (void)points;
-    (void)log_n;
-    (void)inverse;
-    (void)out;
-    return fail(ZGR_ERR_INVALID_ARG, "zgr_ecfft: not implemented yet");
+    if (!c || !points || !out) return fail(ZGR_ERR_INVALID_ARG, "null argument");
+    if (log_n < 1 || log_n > c->log_n)
+        return fail(ZGR_ERR_INVALID_ARG, "log_n %d outside [1, %d] of this context", log_n, c->log_n);
+    std::lock_guard<std::mutex> lk(c->mu);
+    HIPCHK(hipSetDevice(c->device));
+    const size_t n = (size_t)1 << log_n;
+    uint32_t *din = nullptr, *dpts = nullptr;
+    DALLOC(din, n * 64);
+    int rc = dmalloc((void**)&dpts, n * 128);
+    if (rc) {
+        hipFree(din);
+        return rc;
+    }
+    auto body = [&]() -> int {
+        HIPCHK(hipMemcpyAsync(din, points, n * 64, hipMemcpyHostToDevice, c->stream));
+        LAUNCH(c, "ecfft", k_ec_load_bitrev, blocks(n, 256), 256, 0, din, dpts, log_n);
+        const uint32_t* tw = inverse ? c->tw_inv : c->tw;
+        uint32_t half_n = (uint32_t)(n / 2);
+        for (int s = 1; s <= log_n; s++)
+            LAUNCH(c, "ecfft", k_ec_dit_stage, blocks(half_n, 64), 64, 0, dpts, tw, s, c->log_n - s, half_n);
+        LAUNCH(c, "ecfft", k_ec_store, blocks(n, 64), 64, 0, dpts, c->consts, c->log_n - log_n, inverse ? 1 : 0,
+               din, (uint32_t)n);
+        HIPCHK(hipMemcpyAsync(out, din, n * 64, hipMemcpyDeviceToHost, c->stream));
+        return sync_and_collect(c);
+    };
+    rc = body();
+    hipStreamSynchronize(c->stream);
+    hipFree(din);
+    hipFree(dpts);
+    return rc;
 }
 
 int zgr_merkle(zgr_ctx* c, const uint8_t* data, size_t n_elems, int levels, uint8_t* out_layers,

@@ -12,14 +12,14 @@ DATA_SEED = 222  # crates/encoder/src/encoder/blob.rs:257 uses seed 222 for its 
 CPU_THREADS = max(1, min(16, os.cpu_count() or 1))
 
 
-def test_tau():
+def seeded_tau():
     return Z.tau_from_seed(TEST_TAU_SEED)
 
 
 @functools.lru_cache(maxsize=None)
 def oracle_params(log_col, log_row, coset_n=3, tau=None):
     """Prove bases from the oracle's closed form, one uint64 array per coset."""
-    tau = test_tau() if tau is None else tau
+    tau = seeded_tau() if tau is None else tau
     O.set_threads(CPU_THREADS)
     log_n = log_col + log_row
     return tuple(O.params_from_tau(tau, log_n, log_row, c, 0) for c in range(coset_n))

@@ -5,7 +5,7 @@ import pytest
 
 import oracle as O
 import zg_rocm as Z
-from helpers import DATA_SEED, oracle_params, random_fr_mont, ri_from_y, test_tau
+from helpers import DATA_SEED, oracle_params, random_fr_mont, ri_from_y, seeded_tau
 
 pytestmark = pytest.mark.gpu
 
@@ -15,7 +15,7 @@ GEO = Z.TEST_SETTING
 @pytest.fixture(scope="module")
 def ctx():
     c = Z.Context(GEO)
-    c.params_generate(test_tau())
+    c.params_generate(seeded_tau())
     yield c
     c.close()
 
@@ -120,3 +120,46 @@ def test_merkle_primitive_matches_oracle(ctx):
     layers, root = ctx.merkle(data, GEO.log_row)
     assert root == exp_root
     assert np.array_equal(layers.reshape(-1), exp_layers.reshape(-1))
+
+
+def _random_points(rng, n, with_inf=True):
+    pts = np.zeros((n, 64), dtype=np.uint8)
+    for i in range(n):
+        pts[i] = O.g1_mul_gen(int.from_bytes(rng.bytes(32), "little"))
+    if with_inf:
+        pts[n // 3] = 0  # the point at infinity
+    return pts
+
+
+def _canon_to_mont_bases(pts):
+    out = np.zeros((len(pts), 8), dtype=np.uint64)
+    for i, p in enumerate(pts):
+        xy = O.point_to_ints(p)
+        out[i, :4] = O.int_to_limbs(O.to_mont(1, xy[0]))
+        out[i, 4:] = O.int_to_limbs(O.to_mont(1, xy[1]))
+    return out
+
+
+@pytest.mark.parametrize("log_n", [1, 3, 6])
+@pytest.mark.parametrize("inverse", [False, True])
+def test_ecfft_matches_oracle(ctx, log_n, inverse):
+    # Pairing::fast_fft / fast_ifft (crates/amt/src/ec_algebra.rs:34-95)
+    rng = np.random.default_rng(40 + log_n)
+    pts = _random_points(rng, 1 << log_n)
+    exp = O.ecfft(pts, log_n, inverse)
+    got = ctx.ecfft(pts, log_n, inverse)
+    assert np.array_equal(got, exp)
+
+
+@pytest.mark.parametrize("n", [1, 5, 1000, 2048])
+def test_msm_matches_oracle(ctx, n):
+    rng = np.random.default_rng(n)
+    pts = _random_points(rng, n, with_inf=False)
+    bases = _canon_to_mont_bases(pts)
+    scal = random_fr_mont(rng, n)
+    if n > 4:
+        scal[1] = 0
+        scal[2] = scal[3]
+    exp = O.msm(bases, np.array([O.int_to_limbs(v) for v in O.fr_from_mont_array(scal)]))
+    got = ctx.msm(bases, scal)
+    assert np.array_equal(got, exp)

@@ -13,7 +13,7 @@ import pytest
 
 import oracle as O
 import zg_rocm as Z
-from helpers import CPU_THREADS, DATA_SEED, test_tau
+from helpers import CPU_THREADS, DATA_SEED, seeded_tau
 
 pytestmark = pytest.mark.gpu
 
@@ -28,7 +28,7 @@ def log(*a):
 def ctx():
     c = Z.Context(GEO)
     t = time.time()
-    c.params_generate(test_tau())
+    c.params_generate(seeded_tau())
     log("production params generated in %.2f s" % (time.time() - t))
     yield c
     c.close()
@@ -46,7 +46,7 @@ def blob(ctx):
 
 def test_params_sampled_points(ctx):
     rng = np.random.default_rng(17)
-    tau = test_tau()
+    tau = seeded_tau()
     for c in range(GEO.coset_n):
         got = ctx.params_download(c).reshape(GEO.h + 2, GEO.n, 8)
         for _ in range(6):
@@ -59,7 +59,7 @@ def test_params_sampled_points(ctx):
 def test_encode_closed_form(blob):
     raw, out = blob
     O.set_threads(CPU_THREADS)
-    tau = test_tau()
+    tau = seeded_tau()
     rowmaj = O.raw_to_blob(raw, GEO.log_n).reshape(GEO.row_n, GEO.col_n, 4)
     colmaj = np.ascontiguousarray(rowmaj.transpose(1, 0, 2)).reshape(-1, 4)
     ftau = O.eval_interp(colmaj, GEO.log_n, tau)
@@ -87,3 +87,31 @@ def test_encode_bit_exact_vs_oracle(ctx, blob):
     assert np.array_equal(out.proof_tree, exp.proof_tree)
     assert np.array_equal(out.merkle, exp.merkle)
     assert np.array_equal(out.data, exp.data)
+
+
+def test_ecfft_2pow10(ctx):
+    # config 3: G1 EC-FFT of size 2^10 over seeded multiples of G1, both directions
+    rng = np.random.default_rng(1010)
+    pts = np.zeros((1 << 10, 64), dtype=np.uint8)
+    for i in range(1 << 10):
+        pts[i] = O.g1_mul_gen(int.from_bytes(rng.bytes(32), "little"))
+    for inverse in (False, True):
+        t = time.time()
+        got = ctx.ecfft(pts, 10, inverse)
+        log("ecfft 2^10 inverse=%d: %.3f s" % (inverse, time.time() - t))
+        assert np.array_equal(got, O.ecfft(pts, 10, inverse))
+
+
+def test_msm_2pow20(ctx):
+    # config 3: one MSM of 2^20 resident-style bases (the basis line of coset 0)
+    # with uniform random scalars, bit-exact against the oracle's Pippenger
+    rng = np.random.default_rng(2020)
+    bases = ctx.params_download(0).reshape(GEO.h + 2, GEO.n, 8)[GEO.h].copy()
+    scal = rng.integers(0, 2**63, (GEO.n, 4), dtype=np.uint64)
+    scal[:, 3] &= np.uint64(0x0FFFFFFFFFFFFFFF)  # < 2^252 < r: a valid Fr value
+    t = time.time()
+    got = ctx.msm(bases, scal)
+    log("msm 2^20: %.3f s" % (time.time() - t))
+    canon = [O.from_mont(0, O.limbs_to_int(scal[i])) for i in range(GEO.n)]
+    exp = O.msm(bases, O.ints_to_limbs(canon))
+    assert np.array_equal(got, exp)
