@@ -1,0 +1,116 @@
+// G1 XYZZ arithmetic over the reduced-radix base field (field29.h) for the MSM
+// bucket loop.  Same formulas and exceptional-case handling as curve.h
+// (add-2008-s, dbl-2008-s-1, mdbl-2008-s-1); ZZ == 0 (mod p) means infinity.
+#pragma once
+#include "curve.h"
+#include "field29.h"
+
+namespace zgr {
+
+struct a29 {  // affine
+    fq29 x, y;
+};
+
+struct x29 {  // XYZZ
+    fq29 X, Y, ZZ, ZZZ;
+};
+
+__device__ __forceinline__ x29 x29_inf() {
+    x29 r;
+    r.X = q_zero();
+    r.Y = q_zero();
+    r.ZZ = q_zero();
+    r.ZZZ = q_zero();
+    return r;
+}
+
+__device__ __forceinline__ bool x29_is_inf(const x29& p) { return q_is_zero(p.ZZ); }
+
+__device__ __forceinline__ x29 x29_from_aff(const a29& a) {
+    x29 r;
+    r.X = a.x;
+    r.Y = a.y;
+    r.ZZ = q_one();
+    r.ZZZ = q_one();
+    return r;
+}
+
+__device__ __forceinline__ x29 x29_select(bool c, const x29& a, const x29& b) {
+    x29 r;
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+        r.X.l[k] = c ? a.X.l[k] : b.X.l[k];
+        r.Y.l[k] = c ? a.Y.l[k] : b.Y.l[k];
+        r.ZZ.l[k] = c ? a.ZZ.l[k] : b.ZZ.l[k];
+        r.ZZZ.l[k] = c ? a.ZZZ.l[k] : b.ZZZ.l[k];
+    }
+    return r;
+}
+
+__device__ __forceinline__ x29 x29_dbl(const x29& p) {
+    fq29 U = q_dbl(p.Y);
+    fq29 V = q_sqr(U);
+    fq29 W = q_mul(U, V);
+    fq29 S = q_mul(p.X, V);
+    fq29 X2 = q_sqr(p.X);
+    fq29 M = q_add(q_dbl(X2), X2);
+    x29 r;
+    r.X = q_sub(q_sqr(M), q_dbl(S));
+    r.Y = q_sub(q_mul(M, q_sub(S, r.X)), q_mul(W, p.Y));
+    r.ZZ = q_mul(V, p.ZZ);
+    r.ZZZ = q_mul(W, p.ZZZ);
+    return r;
+}
+
+__device__ __forceinline__ x29 x29_add(const x29& p, const x29& q) {
+    const bool pinf = x29_is_inf(p), qinf = x29_is_inf(q);
+    fq29 U1 = q_mul(p.X, q.ZZ);
+    fq29 P = q_sub(q_mul(q.X, p.ZZ), U1);
+    fq29 S1 = q_mul(p.Y, q.ZZZ);
+    fq29 R = q_sub(q_mul(q.Y, p.ZZZ), S1);
+    if (!pinf && !qinf && q_is_zero(P) && q_is_zero(R)) return x29_dbl(p);
+    x29 r;
+    // Z coordinates first so the inputs' ZZ / ZZZ die early
+    fq29 PP = q_sqr(P);
+    r.ZZ = q_mul(q_mul(p.ZZ, q.ZZ), PP);
+    fq29 PPP = q_mul(P, PP);
+    r.ZZZ = q_mul(q_mul(p.ZZZ, q.ZZZ), PPP);
+    fq29 Q = q_mul(U1, PP);
+    r.X = q_sub(q_sub(q_sqr(R), PPP), q_dbl(Q));
+    r.Y = q_sub(q_mul(R, q_sub(Q, r.X)), q_mul(S1, PPP));
+    r = x29_select(pinf, q, r);
+    r = x29_select(qinf, p, r);
+    return r;
+}
+
+// 2^256-Montgomery XYZZ <-> reduced-radix XYZZ
+__device__ __forceinline__ x29 x29_from_xyzz(const g1_xyzz& p) {
+    x29 r;
+    r.X = q_from_fq(p.X);
+    r.Y = q_from_fq(p.Y);
+    r.ZZ = q_from_fq(p.ZZ);
+    r.ZZZ = q_from_fq(p.ZZZ);
+    return r;
+}
+
+__device__ __forceinline__ g1_xyzz x29_to_xyzz(const x29& p) {
+    g1_xyzz r;
+    r.X = q_to_fq(p.X);
+    r.Y = q_to_fq(p.Y);
+    r.ZZ = q_to_fq(p.ZZ);
+    r.ZZZ = q_to_fq(p.ZZZ);
+    return r;
+}
+
+// k * p for k < 2^bits, uniform per lane
+__device__ __forceinline__ x29 x29_mul_small_uniform(const x29& p, uint32_t k, int bits) {
+    x29 r = x29_inf();
+    for (int b = bits - 1; b >= 0; b--) {
+        r = x29_dbl(r);
+        x29 s = x29_add(r, p);
+        r = x29_select((k >> b) & 1, s, r);
+    }
+    return r;
+}
+
+}  // namespace zgr

@@ -8,7 +8,7 @@
 //   The MSM scalar vector of a coset is ri[i] = y[bitrev(i)] (index_reverse,
 //   crates/amt/src/utils.rs:70-88), gathered on the fly by k_msm_digits.
 #pragma once
-#include "curve.h"
+#include "curve29.h"
 #include "keccak.h"
 
 namespace zgr {
@@ -278,8 +278,13 @@ __device__ __forceinline__ g1_xyzz xyzz_mul_small_uniform(const g1_xyzz& p, uint
 //   consume an entry:   R += +-P_e
 //   close a bucket:     T += R
 // giving T = sum_{j=lo..hi} R_j, so the lane's share is T + (lo - 1) * R.
+// Point arithmetic runs in the reduced-radix field (field29.h); the chunk's bases
+// are converted once while being staged in LDS (72 B per point).
 // The per-window sums S_w land in heap leaf nb + b of tree (L, w).
-__global__ __launch_bounds__(ACC_THREADS) void k_msm_accum(const uint32_t* bases, const uint32_t* entries,
+constexpr int LDS_PT = 18;  // u32 per staged affine point (2 x 9 limbs)
+constexpr int LDS_XP = 36;  // u32 per XYZZ point in the lane reduction
+
+__global__ __launch_bounds__(ACC_THREADS, 2) void k_msm_accum(const uint32_t* bases, const uint32_t* entries,
                                                            const uint32_t* counts, uint32_t* heap,
                                                            int log_n, int h, int nlines) {
     extern __shared__ uint32_t lds[];
@@ -299,10 +304,17 @@ __global__ __launch_bounds__(ACC_THREADS) void k_msm_accum(const uint32_t* bases
         return;
     }
 
-    // stage the chunk's bases (bs points, 64 B each) in LDS
-    const uint4* src = (const uint4*)(bases + (((size_t)L << log_n) + ((size_t)b << log_bs)) * 16);
-    uint4* dst = (uint4*)lds;
-    for (uint32_t k = tid; k < bs * 4; k += ACC_THREADS) dst[k] = src[k];
+    // stage the chunk's bases in LDS, converted to the reduced-radix form
+    const uint32_t* src = bases + (((size_t)L << log_n) + ((size_t)b << log_bs)) * 16;
+    for (uint32_t k = tid; k < bs; k += ACC_THREADS) {
+        fq x = f_load<FqP>(src + 16 * (size_t)k), y = f_load<FqP>(src + 16 * (size_t)k + 8);
+        fq29 qx = q_from_fq(x), qy = q_from_fq(y);
+#pragma unroll
+        for (int i = 0; i < 9; i++) {
+            lds[k * LDS_PT + i] = qx.l[i];
+            lds[k * LDS_PT + 9 + i] = qy.l[i];
+        }
+    }
     __syncthreads();
 
     const int cnt = (int)counts[b * MSM_W + w];
@@ -315,7 +327,7 @@ __global__ __launch_bounds__(ACC_THREADS) void k_msm_accum(const uint32_t* bases
         steps = (end - begin) + (hi - lo + 1);
     }
     const int maxsteps = wave_max(steps);
-    g1_xyzz R = xyzz_inf(), T = xyzz_inf();
+    x29 R = x29_inf(), T = x29_inf();
     int e = end - 1, kc = hi;
     for (int st = 0; st < maxsteps; st++) {
         const bool active = st < steps;
@@ -323,30 +335,60 @@ __global__ __launch_bounds__(ACC_THREADS) void k_msm_accum(const uint32_t* bases
         const uint32_t ev = have ? ent[e] : 0u;
         const bool consume = active && have && (int)(ev >> 16) == kc;
         const bool trans = active && !consume;
-        g1_aff P = aff_load(lds + (ev & 0x7fffu) * 16);
-        if (ev & 0x8000u) P.y = f_neg(P.y);
-        g1_xyzz A = xyzz_select(trans, T, R);
-        g1_xyzz B = xyzz_select(trans, R, xyzz_from_aff(P));
-        g1_xyzz C = xyzz_add(A, B);
-        R = xyzz_select(consume, C, R);
-        T = xyzz_select(trans, C, T);
+        a29 P;
+        const uint32_t* lp = lds + (ev & 0x7fffu) * LDS_PT;
+#pragma unroll
+        for (int i = 0; i < 9; i++) {
+            P.x.l[i] = lp[i];
+            P.y.l[i] = lp[9 + i];
+        }
+        if (ev & 0x8000u) P.y = q_neg(P.y);
+        // R is always one of the two operands (A on consume or idle, B on a bucket
+        // close), so only A, B and T stay live across the addition
+        x29 A = x29_select(trans, T, R);
+        x29 B = x29_select(trans, R, x29_from_aff(P));
+        x29 C = x29_add(A, B);
+        R = x29_select(consume, C, x29_select(trans, B, A));
+        T = x29_select(trans, C, T);
         e -= consume ? 1 : 0;
         kc -= trans ? 1 : 0;
     }
-    g1_xyzz W = xyzz_add(T, xyzz_mul_small_uniform(R, (uint32_t)(lo - 1), 7));
+    x29 W = x29_add(T, x29_mul_small_uniform(R, (uint32_t)(lo - 1), 7));
 
     // sum the 8 lanes of each window
     __syncthreads();
-    xyzz_store(lds + tid * 32, W);
+    auto st29 = [&](int slot, const x29& v) {
+        uint32_t* d = lds + slot * LDS_XP;
+#pragma unroll
+        for (int i = 0; i < 9; i++) {
+            d[i] = v.X.l[i];
+            d[9 + i] = v.Y.l[i];
+            d[18 + i] = v.ZZ.l[i];
+            d[27 + i] = v.ZZZ.l[i];
+        }
+    };
+    auto ld29 = [&](int slot) {
+        const uint32_t* d = lds + slot * LDS_XP;
+        x29 v;
+#pragma unroll
+        for (int i = 0; i < 9; i++) {
+            v.X.l[i] = d[i];
+            v.Y.l[i] = d[9 + i];
+            v.ZZ.l[i] = d[18 + i];
+            v.ZZZ.l[i] = d[27 + i];
+        }
+        return v;
+    };
+    st29(tid, W);
     __syncthreads();
     for (int off = 1; off < LANES_PER_WIN; off <<= 1) {
         if ((s & (2 * off - 1)) == 0) {
-            W = xyzz_add(W, xyzz_load(lds + (tid + off) * 32));
-            xyzz_store(lds + tid * 32, W);
+            W = x29_add(W, ld29(tid + off));
+            st29(tid, W);
         }
         __syncthreads();
     }
-    if (s == 0) xyzz_store(leaf, W);
+    if (s == 0) xyzz_store(leaf, x29_to_xyzz(W));
 }
 
 // One level of the per-(line, window) binary trees over the chunks:

@@ -40,6 +40,12 @@ int fail(int code, const char* fmt, ...) {
 
 inline uint32_t blocks(size_t n, uint32_t t) { return (uint32_t)((n + t - 1) / t); }
 
+// dynamic LDS of the bucket kernel: staged bases, reused for the lane reduction
+inline size_t accum_lds_bytes(size_t bs) {
+    size_t a = bs * LDS_PT * 4, r = (size_t)ACC_THREADS * LDS_XP * 4;
+    return a > r ? a : r;
+}
+
 struct Timer {
     hipEvent_t a, b;
     std::string name;
@@ -169,8 +175,7 @@ int msm_lines(zgr_ctx* c, int coset, const uint32_t* src, int from_y) {
     uint32_t sort_threads = c->bs < 256 ? (uint32_t)c->bs : 256u;
     LAUNCH(c, "msm_sort", k_msm_sort, (uint32_t)(c->nb * MSM_W), sort_threads, 0, c->digits, log_bs, c->entries,
            c->counts);
-    size_t shm = c->bs * 64;
-    if (shm < (size_t)ACC_THREADS * 128) shm = (size_t)ACC_THREADS * 128;
+    size_t shm = accum_lds_bytes(c->bs);
     LAUNCH(c, "msm_accum", k_msm_accum, (uint32_t)(c->nb * c->nlines), ACC_THREADS, shm, c->bases[coset],
            c->entries, c->counts, c->heap, c->log_n, c->h, c->nlines);
     return ZGR_OK;
@@ -310,7 +315,7 @@ int zgr_ctx_create(int device, int log_col, int log_row, int coset_n, zgr_ctx** 
         {
             int max_shm = 0;
             HIPCHK(hipDeviceGetAttribute(&max_shm, hipDeviceAttributeMaxSharedMemoryPerBlock, device));
-            size_t need = c->bs * 64 > (size_t)ACC_THREADS * 128 ? c->bs * 64 : (size_t)ACC_THREADS * 128;
+            size_t need = accum_lds_bytes(c->bs);
             if (need + 1024 > (size_t)max_shm)
                 return fail(ZGR_ERR_INVALID_ARG, "bucket kernel needs %zu B of LDS, device allows %d", need, max_shm);
             HIPCHK(hipFuncSetAttribute((const void*)k_msm_accum, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -567,8 +572,7 @@ int zgr_msm(zgr_ctx* c, const uint8_t* bases_mont, const uint8_t* scalars_mont, 
         uint32_t bs = 1u << log_bs, nb = 1u << h;
         LAUNCH(c, "msm_sort", k_msm_sort, nb * MSM_W, bs < 256 ? bs : 256u, 0, c->digits, log_bs, c->entries,
                c->counts);
-        size_t shm = (size_t)bs * 64;
-        if (shm < (size_t)ACC_THREADS * 128) shm = (size_t)ACC_THREADS * 128;
+        size_t shm = accum_lds_bytes(bs);
         LAUNCH(c, "msm_accum", k_msm_accum, nb, ACC_THREADS, shm, dbases, c->entries, c->counts, c->heap, log_np,
                h, 1);
         for (int d = h - 1; d >= 0; d--) {
