@@ -83,6 +83,31 @@ __device__ __forceinline__ x29 x29_add(const x29& p, const x29& q) {
     return r;
 }
 
+// memory layout of a stored XYZZ point: X, Y, ZZ, ZZZ, 9 limbs each (36 u32)
+constexpr int X29_WORDS = 36;
+
+__device__ __forceinline__ void x29_store(uint32_t* d, const x29& v) {
+#pragma unroll
+    for (int i = 0; i < 9; i++) {
+        d[i] = v.X.l[i];
+        d[9 + i] = v.Y.l[i];
+        d[18 + i] = v.ZZ.l[i];
+        d[27 + i] = v.ZZZ.l[i];
+    }
+}
+
+__device__ __forceinline__ x29 x29_load(const uint32_t* d) {
+    x29 v;
+#pragma unroll
+    for (int i = 0; i < 9; i++) {
+        v.X.l[i] = d[i];
+        v.Y.l[i] = d[9 + i];
+        v.ZZ.l[i] = d[18 + i];
+        v.ZZZ.l[i] = d[27 + i];
+    }
+    return v;
+}
+
 // 2^256-Montgomery XYZZ <-> reduced-radix XYZZ
 __device__ __forceinline__ x29 x29_from_xyzz(const g1_xyzz& p) {
     x29 r;

@@ -295,12 +295,12 @@ __global__ __launch_bounds__(ACC_THREADS, 2) void k_msm_accum(const uint32_t* ba
     const uint32_t b = logical / nlines, L = logical % nlines;
     const int tid = threadIdx.x;
     const int w = tid >> 3, s = tid & 7;
-    uint32_t* leaf = heap + ((((size_t)L * MSM_W + w) * 2 * nb) + nb + b) * 32;
+    uint32_t* leaf = heap + ((((size_t)L * MSM_W + w) * 2 * nb) + nb + b) * X29_WORDS;
 
     // zero chunk: every window empty -> all sums are infinity
     const int nz = __syncthreads_or(tid < MSM_W && counts[b * MSM_W + tid] != 0);
     if (!nz) {
-        if (s == 0) xyzz_store(leaf, xyzz_inf());
+        if (s == 0) x29_store(leaf, x29_inf());
         return;
     }
 
@@ -357,38 +357,16 @@ __global__ __launch_bounds__(ACC_THREADS, 2) void k_msm_accum(const uint32_t* ba
 
     // sum the 8 lanes of each window
     __syncthreads();
-    auto st29 = [&](int slot, const x29& v) {
-        uint32_t* d = lds + slot * LDS_XP;
-#pragma unroll
-        for (int i = 0; i < 9; i++) {
-            d[i] = v.X.l[i];
-            d[9 + i] = v.Y.l[i];
-            d[18 + i] = v.ZZ.l[i];
-            d[27 + i] = v.ZZZ.l[i];
-        }
-    };
-    auto ld29 = [&](int slot) {
-        const uint32_t* d = lds + slot * LDS_XP;
-        x29 v;
-#pragma unroll
-        for (int i = 0; i < 9; i++) {
-            v.X.l[i] = d[i];
-            v.Y.l[i] = d[9 + i];
-            v.ZZ.l[i] = d[18 + i];
-            v.ZZZ.l[i] = d[27 + i];
-        }
-        return v;
-    };
-    st29(tid, W);
+    x29_store(lds + tid * LDS_XP, W);
     __syncthreads();
     for (int off = 1; off < LANES_PER_WIN; off <<= 1) {
         if ((s & (2 * off - 1)) == 0) {
-            W = x29_add(W, ld29(tid + off));
-            st29(tid, W);
+            W = x29_add(W, x29_load(lds + (tid + off) * LDS_XP));
+            x29_store(lds + tid * LDS_XP, W);
         }
         __syncthreads();
     }
-    if (s == 0) xyzz_store(leaf, x29_to_xyzz(W));
+    if (s == 0) x29_store(leaf, W);
 }
 
 // One level of the per-(line, window) binary trees over the chunks:
@@ -399,10 +377,20 @@ __global__ __launch_bounds__(64) void k_msm_tree_level(uint32_t* heap, int h, in
     const uint32_t nb = 1u << h;
     uint32_t lw = t >> d;
     uint32_t i = (1u << d) + (t & ((1u << d) - 1));
-    uint32_t* base = heap + (size_t)lw * 2 * nb * 32;
-    g1_xyzz a = xyzz_load(base + (size_t)(2 * i) * 32);
-    g1_xyzz c = xyzz_load(base + (size_t)(2 * i + 1) * 32);
-    xyzz_store(base + (size_t)i * 32, xyzz_add(a, c));
+    uint32_t* base = heap + (size_t)lw * 2 * nb * X29_WORDS;
+    x29 a = x29_load(base + (size_t)(2 * i) * X29_WORDS);
+    x29 c = x29_load(base + (size_t)(2 * i + 1) * X29_WORDS);
+    x29_store(base + (size_t)i * X29_WORDS, x29_add(a, c));
+}
+
+// sum_w 2^(8w) S_w over the windows of heap node `node` of line `line`
+__device__ __forceinline__ x29 horner_windows(const uint32_t* heap, uint32_t nb, uint32_t line, uint32_t node) {
+    x29 acc = x29_inf();
+    for (int w = MSM_W - 1; w >= 0; w--) {
+        for (int k = 0; k < MSM_C; k++) acc = x29_dbl(acc);
+        acc = x29_add(acc, x29_load(heap + (((size_t)line * MSM_W + w) * 2 * nb + node) * X29_WORDS));
+    }
+    return acc;
 }
 
 // Window recombination sum_w 2^(8w) S_w for every output node, then canonical affine.
@@ -428,12 +416,7 @@ __global__ __launch_bounds__(64) void k_msm_horner(const uint32_t* heap, int h, 
         node = 1;
         line = h + 1;
     }
-    g1_xyzz acc = xyzz_inf();
-    for (int w = MSM_W - 1; w >= 0; w--) {
-        for (int k = 0; k < MSM_C; k++) acc = xyzz_dbl(acc);
-        acc = xyzz_add(acc, xyzz_load(heap + (((size_t)line * MSM_W + w) * 2 * nb + node) * 32));
-    }
-    out_store_canon(out + (size_t)slot * 16, acc);
+    out_store_canon(out + (size_t)slot * 16, x29_to_xyzz(horner_windows(heap, nb, line, node)));
 }
 
 // multiple_multiexp output: out[L * nb + b] = sum_w 2^(8w) heap leaf (L, w, nb + b)
@@ -442,24 +425,13 @@ __global__ __launch_bounds__(64) void k_msm_leaves_out(const uint32_t* heap, int
     uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nlines * nb) return;
     uint32_t L = t >> h, b = t & (nb - 1);
-    g1_xyzz acc = xyzz_inf();
-    for (int w = MSM_W - 1; w >= 0; w--) {
-        for (int k = 0; k < MSM_C; k++) acc = xyzz_dbl(acc);
-        acc = xyzz_add(acc, xyzz_load(heap + (((size_t)L * MSM_W + w) * 2 * nb + nb + b) * 32));
-    }
-    out_store_canon(out + (size_t)t * 16, acc);
+    out_store_canon(out + (size_t)t * 16, x29_to_xyzz(horner_windows(heap, nb, L, nb + b)));
 }
 
 // single MSM result: Horner over the windows of tree node 1 of line 0
 __global__ __launch_bounds__(64) void k_msm_root_out(const uint32_t* heap, int h, uint32_t* out) {
     if (threadIdx.x || blockIdx.x) return;
-    const uint32_t nb = 1u << h;
-    g1_xyzz acc = xyzz_inf();
-    for (int w = MSM_W - 1; w >= 0; w--) {
-        for (int k = 0; k < MSM_C; k++) acc = xyzz_dbl(acc);
-        acc = xyzz_add(acc, xyzz_load(heap + ((size_t)w * 2 * nb + 1) * 32));
-    }
-    out_store_canon(out, acc);
+    out_store_canon(out, x29_to_xyzz(horner_windows(heap, 1u << h, 0, 1)));
 }
 
 // ---------------------------------------------------------------------------

@@ -82,6 +82,65 @@ def lib():
     return _lib
 
 
+_hip = None
+
+
+def hip():
+    """The HIP runtime instance libzgr.so is linked against (found through the
+    process maps after the library is loaded), so device buffers made here and the
+    encoder share one runtime."""
+    global _hip
+    if _hip is None:
+        lib()
+        path = None
+        with open("/proc/self/maps") as f:
+            for line in f:
+                if "libamdhip64.so" in line:
+                    path = line.split()[-1]
+                    break
+        H = ctypes.CDLL(path or "libamdhip64.so")
+        H.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
+        H.hipFree.argtypes = [ctypes.c_void_p]
+        H.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+        H.hipDeviceSynchronize.argtypes = []
+        for f in (H.hipMalloc, H.hipFree, H.hipMemcpy, H.hipDeviceSynchronize):
+            f.restype = ctypes.c_int
+        _hip = H
+    return _hip
+
+
+class DeviceBuffer:
+    """A hipMalloc'd byte buffer holding a copy of `data` (inputs resident in HBM)."""
+
+    def __init__(self, data):
+        data = bytes(data)
+        self.size = len(data)
+        self.ptr = ctypes.c_void_p()
+        if self.size:
+            rc = hip().hipMalloc(ctypes.byref(self.ptr), self.size)
+            if rc:
+                raise ZgrError(ERR_OOM, "hipMalloc(%d) failed: %d" % (self.size, rc))
+            src = ctypes.create_string_buffer(data, self.size)
+            rc = hip().hipMemcpy(self.ptr, src, self.size, 1)  # hipMemcpyHostToDevice
+            if rc:
+                raise ZgrError(ERR_DEVICE, "hipMemcpy failed: %d" % rc)
+
+    @property
+    def address(self):
+        return self.ptr.value or 0
+
+    def free(self):
+        if self.ptr.value:
+            hip().hipFree(self.ptr)
+            self.ptr = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
 def _p(a):
     return ctypes.c_void_p(a.ctypes.data) if a is not None else None
 

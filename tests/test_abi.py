@@ -30,14 +30,12 @@ def test_library_exports_every_declared_symbol():
 
 
 def test_error_paths_without_device():
-    try:
-        import torch
-
-        if torch.cuda.is_available():
-            pytest.skip("a device is present")
-    except ImportError:
-        pass
     out = ctypes.c_void_p()
+    # geometry is validated before any device call, so this never touches a GPU
+    rc = Z.lib().zgr_ctx_create(0, 10, 10, 4, ctypes.byref(out))
+    assert rc == Z.ERR_INVALID_ARG
+    if os.path.exists("/dev/kfd") and os.access("/dev/kfd", os.R_OK | os.W_OK):
+        pytest.skip("a device may be present; the no-device path is not exercised")
     rc = Z.lib().zgr_ctx_create(0, 10, 10, 3, ctypes.byref(out))
     assert rc in (Z.ERR_DEVICE, Z.ERR_INVALID_ARG)
     assert Z.lib().zgr_last_error()

@@ -163,3 +163,22 @@ def test_msm_matches_oracle(ctx, n):
     exp = O.msm(bases, np.array([O.int_to_limbs(v) for v in O.fr_from_mont_array(scal)]))
     got = ctx.msm(bases, scal)
     assert np.array_equal(got, exp)
+
+
+def test_encode_batch_matches_oracle(ctx, params):
+    # the batched entry point pipelines blobs over two slots and two coset lanes;
+    # every blob must come out exactly as the oracle encodes it alone
+    lengths = [0, 1, 1234, GEO.max_raw_data_size, 777, 31 * 64]
+    raws = [Z.seeded_bytes(DATA_SEED + 1 + i, L) for i, L in enumerate(lengths)]
+    dev = [Z.DeviceBuffer(r) for r in raws]
+    outs = [Z.EncodedBlobOutputs(GEO) for _ in raws]
+    ctx.encode_batch_device([d.address for d in dev], [len(r) for r in raws], outs)
+    for r, got in zip(raws, outs):
+        exp = O.encode(list(params), GEO.log_col, GEO.log_row, r)
+        assert np.array_equal(got.commitment, exp.commitment)
+        assert np.array_equal(got.file_root, exp.root)
+        assert np.array_equal(got.proof_tree, exp.proof_tree)
+        assert np.array_equal(got.comm_tree, exp.comm_tree)
+        assert np.array_equal(got.high, exp.high)
+        assert np.array_equal(got.merkle, exp.merkle)
+        assert np.array_equal(got.data, exp.data)

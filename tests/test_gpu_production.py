@@ -115,3 +115,21 @@ def test_msm_2pow20(ctx):
     canon = [O.from_mont(0, O.limbs_to_int(scal[i])) for i in range(GEO.n)]
     exp = O.msm(bases, O.ints_to_limbs(canon))
     assert np.array_equal(got, exp)
+
+
+def test_batch_equals_single(ctx, blob):
+    # config 4 shape: uneven full-geometry blobs in one batched call, each bit-equal
+    # to the same blob encoded alone
+    raw0, out0 = blob
+    rng = np.random.default_rng(44)
+    lengths = [GEO.max_raw_data_size, 1024, int(rng.integers(1, GEO.max_raw_data_size)), 0]
+    raws = [raw0] + [Z.seeded_bytes(300 + i, L) for i, L in enumerate(lengths[1:])]
+    dev = [Z.DeviceBuffer(r) for r in raws]
+    outs = [Z.EncodedBlobOutputs(GEO) for _ in raws]
+    t = time.time()
+    ctx.encode_batch_device([d.address for d in dev], [len(r) for r in raws], outs)
+    log("batch of %d blobs: %.3f s" % (len(raws), time.time() - t))
+    singles = [out0] + [ctx.encode(r) for r in raws[1:]]
+    for a, b in zip(outs, singles):
+        for f in ("commitment", "file_root", "proof_tree", "comm_tree", "high", "merkle", "data"):
+            assert np.array_equal(getattr(a, f), getattr(b, f)), f
