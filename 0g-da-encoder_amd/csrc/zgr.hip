@@ -16,12 +16,17 @@
 // mutex; the primitives run synchronously on lane 0.
 #include <hip/hip_runtime.h>
 
+#include <chrono>
+#include <condition_variable>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <deque>
 #include <map>
 #include <mutex>
 #include <string>
+#include <thread>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/zgr.h"
@@ -345,6 +350,169 @@ int encode_pipeline(zgr_ctx* c, int nblobs, const uint8_t* const* raws, const si
     return first_err;
 }
 
+// ---------------------------------------------------------------------------
+// serving: continuous batching over the two blob slots
+// ---------------------------------------------------------------------------
+
+// first-fit allocator over one device buffer (the raw-input staging arena)
+struct Arena {
+    uint8_t* base = nullptr;
+    size_t size = 0;
+    std::map<size_t, size_t> free_;  // offset -> length, coalesced
+
+    static size_t round(size_t n) { return (n + 255) & ~(size_t)255; }
+    bool alloc(size_t n, size_t* off) {
+        n = round(n ? n : 1);
+        for (auto it = free_.begin(); it != free_.end(); ++it) {
+            if (it->second < n) continue;
+            *off = it->first;
+            size_t rest = it->second - n, at = it->first + n;
+            free_.erase(it);
+            if (rest) free_[at] = rest;
+            return true;
+        }
+        return false;
+    }
+    void release(size_t off, size_t n) {
+        n = round(n ? n : 1);
+        auto it = free_.emplace(off, n).first;
+        auto nx = std::next(it);
+        if (nx != free_.end() && it->first + it->second == nx->first) {
+            it->second += nx->second;
+            free_.erase(nx);
+        }
+        if (it != free_.begin()) {
+            auto pv = std::prev(it);
+            if (pv->first + pv->second == it->first) {
+                pv->second += it->second;
+                free_.erase(it);
+            }
+        }
+    }
+};
+
+struct Request {
+    uint64_t id = 0;
+    const uint8_t* raw = nullptr;
+    size_t len = 0;
+    const zgr_encoded* out = nullptr;
+    size_t off = 0;
+    hipEvent_t uploaded = nullptr;
+};
+
+}  // namespace
+
+struct zgr_server {
+    zgr_ctx* c = nullptr;
+    Arena arena;
+    int max_inflight = 1;
+    hipStream_t upload = nullptr;
+    std::mutex m;
+    std::condition_variable cv_done, cv_work;
+    std::deque<Request*> queue;   // submitted, waiting for admission
+    std::deque<Request*> staged;  // admitted and uploading / uploaded, waiting for a slot
+    Request* in_slot[2] = {nullptr, nullptr};
+    std::unordered_map<uint64_t, int> done;  // ticket -> final status, until polled
+    std::unordered_map<uint64_t, bool> pending;  // submitted tickets not yet completed
+    uint64_t next_id = 1, n_completed = 0;
+    int n_admitted = 0;
+    bool stop = false;
+    std::thread worker;
+};
+
+namespace {
+
+void server_loop(zgr_server* sv) {
+    zgr_ctx* c = sv->c;
+    std::lock_guard<std::mutex> own(c->mu);  // the server owns its context
+    (void)hipSetDevice(c->device);
+    while (true) {
+        bool progressed = false;
+        std::vector<Request*> to_upload;
+        {
+            std::unique_lock<std::mutex> lk(sv->m);
+            // admission: by staging-arena footprint and in-flight count
+            while (!sv->queue.empty() && sv->n_admitted < sv->max_inflight) {
+                Request* r = sv->queue.front();
+                if (!sv->arena.alloc(r->len, &r->off)) break;
+                sv->queue.pop_front();
+                sv->n_admitted++;
+                to_upload.push_back(r);
+            }
+            bool busy = sv->in_slot[0] || sv->in_slot[1] || !sv->staged.empty() || !to_upload.empty();
+            if (sv->stop && sv->queue.empty() && !busy) break;
+            if (!busy && sv->queue.empty()) {
+                sv->cv_work.wait_for(lk, std::chrono::milliseconds(5));
+                continue;
+            }
+        }
+        for (Request* r : to_upload) {
+            (void)hipEventCreateWithFlags(&r->uploaded, hipEventDisableTiming);
+            if (r->len)
+                (void)hipMemcpyAsync(sv->arena.base + r->off, r->raw, r->len, hipMemcpyHostToDevice, sv->upload);
+            (void)hipEventRecord(r->uploaded, sv->upload);
+            std::lock_guard<std::mutex> lk(sv->m);
+            sv->staged.push_back(r);
+            progressed = true;
+        }
+        // retire finished slots
+        for (int k = 0; k < 2; k++) {
+            Request* r = sv->in_slot[k];
+            if (!r || hipEventQuery(c->slots[k].ev_done) != hipSuccess) continue;
+            int st = finish_blob(c, c->slots[k]);
+            (void)hipEventDestroy(r->uploaded);
+            {
+                std::lock_guard<std::mutex> lk(sv->m);
+                sv->arena.release(r->off, r->len);
+                sv->in_slot[k] = nullptr;
+                sv->n_admitted--;
+                sv->n_completed++;
+                sv->done[r->id] = st;
+                sv->pending.erase(r->id);
+            }
+            sv->cv_done.notify_all();
+            delete r;
+            progressed = true;
+        }
+        // issue staged requests into free slots (upload ordered before the blob)
+        for (int k = 0; k < 2; k++) {
+            if (sv->in_slot[k]) continue;
+            Request* r = nullptr;
+            {
+                std::lock_guard<std::mutex> lk(sv->m);
+                if (sv->staged.empty()) break;
+                r = sv->staged.front();
+                sv->staged.pop_front();
+            }
+            Slot& s = c->slots[k];
+            int st = ZGR_OK;
+            if (hipStreamWaitEvent(s.st, r->uploaded, 0) != hipSuccess)
+                st = fail(ZGR_ERR_DEVICE, "hipStreamWaitEvent failed");
+            if (!st) st = issue_blob(c, s, sv->arena.base + r->off, r->len, r->out);
+            if (st) {  // could not be issued: complete it with the error
+                s.busy = false;
+                (void)hipStreamSynchronize(s.st);
+                (void)hipEventDestroy(r->uploaded);
+                {
+                    std::lock_guard<std::mutex> lk(sv->m);
+                    sv->arena.release(r->off, r->len);
+                    sv->n_admitted--;
+                    sv->n_completed++;
+                    sv->done[r->id] = st;
+                sv->pending.erase(r->id);
+                }
+                sv->cv_done.notify_all();
+                delete r;
+            } else {
+                sv->in_slot[k] = r;
+            }
+            progressed = true;
+        }
+        if (!progressed) std::this_thread::sleep_for(std::chrono::microseconds(100));
+    }
+    (void)sync_all(c);
+}
+
 void destroy_streams(zgr_ctx* c) {
     for (auto& l : c->lanes)
         if (l.st) (void)hipStreamDestroy(l.st);
@@ -622,6 +790,93 @@ int zgr_encode_batch_device(zgr_ctx* c, int nblobs, const void* const* raw_dev, 
     std::vector<const uint8_t*> r(nblobs);
     for (int i = 0; i < nblobs; i++) r[i] = raw_dev[i] ? (const uint8_t*)raw_dev[i] : c->slots[0].raw;
     return encode_pipeline(c, nblobs, r.data(), lens, outs);
+}
+
+int zgr_server_create(zgr_ctx* c, size_t hbm_budget, int max_inflight, zgr_server** out) {
+    if (!c || !out || max_inflight < 1) return fail(ZGR_ERR_INVALID_ARG, "bad argument");
+    *out = nullptr;
+    for (int cs = 0; cs < c->coset_n; cs++) CHECK(ensure_params(c, cs));
+    HIPCHK(hipSetDevice(c->device));
+    zgr_server* sv = new zgr_server();
+    sv->c = c;
+    sv->max_inflight = max_inflight;
+    sv->arena.size = Arena::round(hbm_budget ? hbm_budget : 1);
+    int rc = dmalloc((void**)&sv->arena.base, sv->arena.size);
+    if (!rc && hipStreamCreateWithFlags(&sv->upload, hipStreamNonBlocking) != hipSuccess)
+        rc = fail(ZGR_ERR_DEVICE, "hipStreamCreateWithFlags failed");
+    if (rc) {
+        if (sv->arena.base) (void)hipFree(sv->arena.base);
+        delete sv;
+        return rc;
+    }
+    sv->arena.free_[0] = sv->arena.size;
+    sv->worker = std::thread(server_loop, sv);
+    *out = sv;
+    return ZGR_OK;
+}
+
+void zgr_server_destroy(zgr_server* sv) {
+    if (!sv) return;
+    {
+        std::lock_guard<std::mutex> lk(sv->m);
+        sv->stop = true;
+    }
+    sv->cv_work.notify_all();
+    if (sv->worker.joinable()) sv->worker.join();
+    (void)hipSetDevice(sv->c->device);
+    if (sv->upload) {
+        (void)hipStreamSynchronize(sv->upload);
+        (void)hipStreamDestroy(sv->upload);
+    }
+    if (sv->arena.base) (void)hipFree(sv->arena.base);
+    delete sv;
+}
+
+int zgr_server_submit(zgr_server* sv, const uint8_t* raw, size_t len, const zgr_encoded* out, uint64_t* ticket) {
+    if (!sv || !out || !ticket || (!raw && len)) return fail(ZGR_ERR_INVALID_ARG, "null argument");
+    CHECK(check_len(sv->c, len));
+    if (Arena::round(len ? len : 1) > sv->arena.size)
+        return fail(ZGR_ERR_OOM, "request of %zu B can never fit the %zu B staging arena", len, sv->arena.size);
+    Request* r = new Request();
+    r->raw = raw;
+    r->len = len;
+    r->out = out;
+    {
+        std::lock_guard<std::mutex> lk(sv->m);
+        if (sv->stop) {
+            delete r;
+            return fail(ZGR_ERR_INVALID_ARG, "server is stopping");
+        }
+        r->id = sv->next_id++;
+        sv->pending[r->id] = true;
+        sv->queue.push_back(r);
+        *ticket = r->id;
+    }
+    sv->cv_work.notify_all();
+    return ZGR_OK;
+}
+
+int zgr_server_poll(zgr_server* sv, uint64_t ticket, int timeout_ms) {
+    if (!sv) return fail(ZGR_ERR_INVALID_ARG, "null argument");
+    std::unique_lock<std::mutex> lk(sv->m);
+    if (!sv->pending.count(ticket) && !sv->done.count(ticket))
+        return fail(ZGR_ERR_INVALID_ARG, "unknown or already collected ticket %llu", (unsigned long long)ticket);
+    auto ready = [&] { return sv->done.count(ticket) != 0; };
+    if (timeout_ms < 0)
+        sv->cv_done.wait(lk, ready);
+    else if (!sv->cv_done.wait_for(lk, std::chrono::milliseconds(timeout_ms), ready))
+        return 1;
+    int st = sv->done[ticket];
+    sv->done.erase(ticket);
+    return st;
+}
+
+void zgr_server_stats(zgr_server* sv, uint64_t* queued, uint64_t* admitted, uint64_t* completed) {
+    if (!sv) return;
+    std::lock_guard<std::mutex> lk(sv->m);
+    if (queued) *queued = sv->queue.size();
+    if (admitted) *admitted = (uint64_t)sv->n_admitted;
+    if (completed) *completed = sv->n_completed;
 }
 
 int zgr_ntt(zgr_ctx* c, uint8_t* data_mont, int log_n, int batch, int inverse) {

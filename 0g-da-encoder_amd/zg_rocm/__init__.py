@@ -45,6 +45,12 @@ SIGNATURES = {
     "zgr_msm": (I, [P, P, P, S, P]),
     "zgr_ecfft": (I, [P, P, I, I, P]),
     "zgr_merkle": (I, [P, P, S, I, P, P]),
+    "zgr_server_create": (I, [P, S, I, ctypes.POINTER(P)]),
+    "zgr_server_destroy": (None, [P]),
+    "zgr_server_submit": (I, [P, P, S, P, ctypes.POINTER(ctypes.c_uint64)]),
+    "zgr_server_poll": (I, [P, ctypes.c_uint64, I]),
+    "zgr_server_stats": (None, [P, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64),
+                                ctypes.POINTER(ctypes.c_uint64)]),
     "zgr_kernel_time_ms": (ctypes.c_double, [P, ctypes.c_char_p, ctypes.POINTER(I)]),
     "zgr_kernel_time_reset": (None, [P]),
     "zgr_set_profiling": (None, [P, I]),
@@ -333,6 +339,56 @@ class Context:
 
     def kernel_time_reset(self):
         lib().zgr_kernel_time_reset(self.h)
+
+
+class Server:
+    """zgr_server: continuous-batching encode service over one Context.
+
+    submit() returns immediately with a ticket; poll() waits for it.  While the
+    server exists it owns the context; close it before using the context again."""
+
+    PENDING = 1
+
+    def __init__(self, ctx, hbm_budget=1 << 30, max_inflight=64):
+        self.ctx = ctx
+        h = ctypes.c_void_p()
+        _check(lib().zgr_server_create(ctx.h, hbm_budget, max_inflight, ctypes.byref(h)))
+        self.h = h
+        self._live = {}
+
+    def submit(self, raw, want_data=True, out=None):
+        out = out or EncodedBlobOutputs(self.ctx.geo, want_data)
+        buf = ctypes.create_string_buffer(bytes(raw), max(1, len(raw)))
+        cs = out.cstruct()
+        t = ctypes.c_uint64()
+        _check(lib().zgr_server_submit(self.h, buf, len(raw), ctypes.byref(cs), ctypes.byref(t)))
+        self._live[t.value] = (buf, cs, out)
+        return t.value
+
+    def poll(self, ticket, timeout_ms=-1):
+        """Returns the EncodedBlobOutputs once done, None while pending; raises on error."""
+        rc = lib().zgr_server_poll(self.h, ticket, timeout_ms)
+        if rc == self.PENDING:
+            return None
+        _, _, out = self._live.pop(ticket)
+        _check(rc)
+        return out
+
+    def stats(self):
+        q, a, c = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()
+        lib().zgr_server_stats(self.h, ctypes.byref(q), ctypes.byref(a), ctypes.byref(c))
+        return {"queued": q.value, "admitted": a.value, "completed": c.value}
+
+    def close(self):
+        if self.h:
+            lib().zgr_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 # ---------------------------------------------------------------------------

@@ -80,6 +80,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--batch", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--served-requests", type=int, default=48)
+    ap.add_argument("--qps", type=float, default=0.0)
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     args = ap.parse_args()
     if args.gpus != 1:
@@ -154,6 +156,50 @@ def main():
     mixed = {"blobs": 32, "value": round(32 / m_el, 4), "unit": "blobs/s",
              "raw_MB_total": round(float(mlens.sum()) / 1e6, 1)}
     del mkeep, mouts
+
+    # served: Poisson arrivals at a fixed QPS with the same size mix, through the
+    # continuous-batching server (host input, <= 64 in flight), against the plain
+    # per-call path fed the same arrivals one at a time
+    n_req = args.served_requests
+    qps = args.qps if args.qps > 0 else 0.8 * mixed["value"]
+    arr = np.cumsum(np.random.default_rng(DATA_SEED + 7).exponential(1.0 / qps, n_req))
+    sraws = [mraws[i % len(mraws)] for i in range(n_req)]
+
+    def served():
+        srv = Z.Server(ctx, hbm_budget=4 << 30, max_inflight=64)
+        lat_s, pend, i = [], {}, 0
+        t0 = time.perf_counter()
+        while i < n_req or pend:
+            now = time.perf_counter() - t0
+            while i < n_req and arr[i] <= now:
+                pend[srv.submit(sraws[i])] = arr[i]
+                i += 1
+            for t in list(pend):
+                if srv.poll(t, 0) is not None:
+                    lat_s.append(time.perf_counter() - t0 - pend.pop(t))
+            time.sleep(0.0005)
+        wall = time.perf_counter() - t0
+        srv.close()
+        return wall, lat_s
+
+    def plain_fifo():
+        lat_p = []
+        t0 = time.perf_counter()
+        for i in range(n_req):
+            now = time.perf_counter() - t0
+            if arr[i] > now:
+                time.sleep(arr[i] - now)
+            ctx.encode(sraws[i])
+            lat_p.append(time.perf_counter() - t0 - arr[i])
+        return time.perf_counter() - t0, lat_p
+
+    s_wall, s_lat = served()
+    p_wall, p_lat = plain_fifo()
+    pct = lambda v, q: round(float(np.percentile(np.array(v) * 1e3, q)), 1)  # noqa: E731
+    serving = {"requests": n_req, "qps": round(qps, 3), "value": round(n_req / s_wall, 4), "unit": "blobs/s",
+               "p50_ms": pct(s_lat, 50), "p99_ms": pct(s_lat, 99),
+               "plain_per_call_same_arrivals": {"value": round(n_req / p_wall, 4), "p50_ms": pct(p_lat, 50),
+                                                "p99_ms": pct(p_lat, 99)}}
     # The encode pipeline overlaps kernels of different cosets on two streams, which
     # stretches event-timed durations; the roofline uses isolated launches of the same
     # bucket kernel (multiple_multiexp of one coset: 12 lines x 2^20 dense scalars).
@@ -193,6 +239,7 @@ def main():
         "latency_ms": {"p50": round(p50, 3), "p99": round(p99, 3)},
         "batched": batched,
         "uneven_mix_32": mixed,
+        "served": serving,
         "params_generate_s": round(t_params, 3),
         "roofline": {
             "bound": "valu",

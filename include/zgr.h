@@ -107,6 +107,31 @@ int zgr_encode_device(zgr_ctx *ctx, const void *raw_dev, size_t len, const zgr_e
 int zgr_encode_batch_device(zgr_ctx *ctx, int nblobs, const void *const *raw_dev,
                             const size_t *lens, const zgr_encoded *outs);
 
+/* ---- serving: continuous batching over one context ----
+ * The reference serves one blob per gRPC request, synchronously on a tokio worker
+ * (Encoder::encode_blob -> process_data, grpc/src/service.rs:48-106).  A server keeps
+ * the context's params resident and runs a worker thread that admits queued requests
+ * while the device staging arena (hbm_budget bytes of raw input) has room and at most
+ * max_inflight requests are admitted, uploads them, and issues each into the next
+ * free pipeline slot as soon as one frees up, so requests that arrive while others
+ * are on the GPU overlap with them.  Each request's outputs are exactly those of
+ * zgr_encode on the same bytes. */
+typedef struct zgr_server zgr_server;
+int zgr_server_create(zgr_ctx *ctx, size_t hbm_budget, int max_inflight, zgr_server **out);
+/* Completes every submitted request, then stops the worker.  While a server exists
+ * it owns its context: other calls on that context wait until it is destroyed. */
+void zgr_server_destroy(zgr_server *srv);
+/* Non-blocking.  raw (host memory) and out must stay valid until the ticket
+ * completes.  Returns ZGR_ERR_TOO_LARGE for oversize input, or ZGR_ERR_OOM when a
+ * single request can never fit the staging arena. */
+int zgr_server_submit(zgr_server *srv, const uint8_t *raw, size_t len, const zgr_encoded *out,
+                      uint64_t *ticket);
+/* Wait up to timeout_ms (< 0: forever) for a ticket.  Returns 1 while pending, else
+ * the request's final status (ZGR_OK or an error), once. */
+int zgr_server_poll(zgr_server *srv, uint64_t ticket, int timeout_ms);
+/* queued (not yet admitted), admitted (staged or on the GPU), completed counts */
+void zgr_server_stats(zgr_server *srv, uint64_t *queued, uint64_t *admitted, uint64_t *completed);
+
 /* ---- primitives (reference call sites named per function) ---- */
 /* Radix-2 DFT over the 2^log_n domain of ark-poly Radix2EvaluationDomain
  * (to_coset_blob, crates/amt/src/blob/encode.rs:142-157).  inverse != 0 applies

@@ -133,3 +133,23 @@ def test_batch_equals_single(ctx, blob):
     for a, b in zip(outs, singles):
         for f in ("commitment", "file_root", "proof_tree", "comm_tree", "high", "merkle", "data"):
             assert np.array_equal(getattr(a, f), getattr(b, f)), f
+
+
+def test_served_equals_single(ctx, blob):
+    # config 5 shape: requests of mixed sizes submitted together to the serving
+    # layer; each result equals the same blob encoded alone
+    raw0, out0 = blob
+    lengths = [GEO.max_raw_data_size, 5000, 3_000_000, 1, 20_000_000]
+    raws = [raw0] + [Z.seeded_bytes(400 + i, L) for i, L in enumerate(lengths[1:])]
+    srv = Z.Server(ctx, hbm_budget=1 << 30, max_inflight=64)
+    try:
+        t = time.time()
+        tickets = [srv.submit(r) for r in raws]
+        outs = [srv.poll(tk) for tk in tickets]
+        log("served %d mixed blobs: %.3f s" % (len(raws), time.time() - t))
+    finally:
+        srv.close()
+    singles = [out0] + [ctx.encode(r) for r in raws[1:]]
+    for a, b in zip(outs, singles):
+        for f in ("commitment", "file_root", "proof_tree", "comm_tree", "high", "merkle", "data"):
+            assert np.array_equal(getattr(a, f), getattr(b, f)), f
