@@ -56,7 +56,7 @@ __device__ __forceinline__ x29 x29_dbl(const x29& p) {
     fq29 M = q_add(q_dbl(X2), X2);
     x29 r;
     r.X = q_sub(q_sqr(M), q_dbl(S));
-    r.Y = q_sub(q_mul(M, q_sub(S, r.X)), q_mul(W, p.Y));
+    r.Y = q_sub(q_mul(M, q_sub_lazy(S, r.X)), q_mul(W, p.Y));
     r.ZZ = q_mul(V, p.ZZ);
     r.ZZZ = q_mul(W, p.ZZZ);
     return r;
@@ -65,10 +65,10 @@ __device__ __forceinline__ x29 x29_dbl(const x29& p) {
 __device__ __forceinline__ x29 x29_add(const x29& p, const x29& q) {
     const bool pinf = x29_is_inf(p), qinf = x29_is_inf(q);
     fq29 U1 = q_mul(p.X, q.ZZ);
-    fq29 P = q_sub(q_mul(q.X, p.ZZ), U1);
+    fq29 P = q_sub_lazy(q_mul(q.X, p.ZZ), U1);  // < 4p, feeds multiplications only
     fq29 S1 = q_mul(p.Y, q.ZZZ);
-    fq29 R = q_sub(q_mul(q.Y, p.ZZZ), S1);
-    if (!pinf && !qinf && q_is_zero(P) && q_is_zero(R)) return x29_dbl(p);
+    fq29 R = q_sub_lazy(q_mul(q.Y, p.ZZZ), S1);
+    if (!pinf && !qinf && q_is_zero4(P) && q_is_zero4(R)) return x29_dbl(p);
     x29 r;
     // Z coordinates first so the inputs' ZZ / ZZZ die early
     fq29 PP = q_sqr(P);
@@ -77,7 +77,7 @@ __device__ __forceinline__ x29 x29_add(const x29& p, const x29& q) {
     r.ZZZ = q_mul(q_mul(p.ZZZ, q.ZZZ), PPP);
     fq29 Q = q_mul(U1, PP);
     r.X = q_sub(q_sub(q_sqr(R), PPP), q_dbl(Q));
-    r.Y = q_sub(q_mul(R, q_sub(Q, r.X)), q_mul(S1, PPP));
+    r.Y = q_sub(q_mul(R, q_sub_lazy(Q, r.X)), q_mul(S1, PPP));
     r = x29_select(pinf, q, r);
     r = x29_select(qinf, p, r);
     return r;

@@ -81,6 +81,8 @@ __device__ __forceinline__ fq29 q_mul(const fq29& a, const fq29& b) {
     return r;
 }
 
+// (a dedicated squaring with doubled cross products raised register pressure in the
+// bucket kernel and measured no faster on gfx950)
 __device__ __forceinline__ fq29 q_sqr(const fq29& a) { return q_mul(a, a); }
 
 // s in [0, 4p) with normalised limbs -> [0, 2p)
@@ -128,6 +130,20 @@ __device__ __forceinline__ fq29 q_sub(const fq29& a, const fq29& b) {
 
 __device__ __forceinline__ fq29 q_neg(const fq29& a) { return q_sub(q_zero(), a); }
 
+// a - b + 2p in (0, 4p) without the final reduction: only for values that feed a
+// multiplication (which accepts inputs below 4p) or q_is_zero4
+__device__ __forceinline__ fq29 q_sub_lazy(const fq29& a, const fq29& b) {
+    fq29 s;
+    int32_t c = 0;
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+        int32_t x = (int32_t)a.l[k] + (int32_t)Q29::P2[k] - (int32_t)b.l[k] + c;
+        s.l[k] = (uint32_t)x & M29;
+        c = x >> 29;
+    }
+    return s;
+}
+
 // x in [0, 2p): x == 0 mod p iff x == 0 or x == p
 __device__ __forceinline__ bool q_is_zero(const fq29& a) {
     uint32_t z = 0, e = 0;
@@ -138,6 +154,9 @@ __device__ __forceinline__ bool q_is_zero(const fq29& a) {
     }
     return z == 0 || e == 0;
 }
+
+// x in [0, 4p): x == 0 mod p iff x is 0, p, 2p or 3p
+__device__ __forceinline__ bool q_is_zero4(const fq29& a) { return q_is_zero(q_reduce_2p(a)); }
 
 // 8 x 32-bit limbs (any value < 2^256) -> 9 x 29-bit limbs of the same integer
 __device__ __forceinline__ fq29 q_unpack(const uint32_t w[8]) {

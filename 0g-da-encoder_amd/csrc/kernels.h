@@ -397,11 +397,11 @@ __device__ __forceinline__ x29 horner_windows(const uint32_t* heap, uint32_t nb,
 // Output slots (T = 2nb - 2):  [0, T) proof_tree  (line d-1, node i = slot + 2, d = log2 i)
 //                              [T, 2T) comm_tree  (line h, node i = slot - T + 2)
 //                              2T commitment (line h, node 1), 2T + 1 high (line h + 1, node 1)
-__global__ __launch_bounds__(64) void k_msm_horner(const uint32_t* heap, int h, uint32_t* out) {
+__global__ __launch_bounds__(64) void k_msm_horner(const uint32_t* heap, int h, uint32_t* out, uint32_t nslots) {
     const uint32_t nb = 1u << h;
     const uint32_t T = 2 * nb - 2;
     uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
-    if (slot >= 2 * T + 2) return;
+    if (slot >= nslots || slot >= 2 * T + 2) return;
     uint32_t line, node;
     if (slot < T) {
         node = slot + 2;

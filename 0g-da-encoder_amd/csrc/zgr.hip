@@ -211,27 +211,30 @@ int ntt_stages(zgr_ctx* c, hipStream_t st, uint32_t* a, int k, bool dif, bool in
     return ZGR_OK;
 }
 
-// multi-line MSM of one coset on a lane: digits of `src` -> window sums in lane heap
-int msm_lines(zgr_ctx* c, Lane& ln, const uint32_t* bases, const uint32_t* src, int from_y) {
+// multi-line MSM of one coset on a lane: digits of `src` -> window sums in lane heap,
+// for the first `lines` lines (quotients[0..h), basis, high_basis)
+int msm_lines(zgr_ctx* c, Lane& ln, const uint32_t* bases, const uint32_t* src, int from_y, int lines) {
     const int log_bs = c->log_n - c->h;
     LAUNCH(c, ln.st, "msm_digits", k_msm_digits, blocks(c->n, 256), 256, 0, src, c->log_n, log_bs, ln.digits,
            from_y);
     uint32_t sort_threads = c->bs < 256 ? (uint32_t)c->bs : 256u;
     LAUNCH(c, ln.st, "msm_sort", k_msm_sort, (uint32_t)(c->nb * MSM_W), sort_threads, 0, ln.digits, log_bs,
            ln.entries, ln.counts);
-    LAUNCH(c, ln.st, "msm_accum", k_msm_accum, (uint32_t)(c->nb * c->nlines), ACC_THREADS,
-           accum_lds_bytes(c->bs), bases, ln.entries, ln.counts, ln.heap, c->log_n, c->h, c->nlines);
+    LAUNCH(c, ln.st, "msm_accum", k_msm_accum, (uint32_t)(c->nb * lines), ACC_THREADS, accum_lds_bytes(c->bs),
+           bases, ln.entries, ln.counts, ln.heap, c->log_n, c->h, lines);
     return ZGR_OK;
 }
 
-// window trees + Horner: AMT outputs of one coset into `out` (out_slots points)
-int amt_outputs(zgr_ctx* c, Lane& ln, uint32_t* out) {
+// window trees + Horner: AMT outputs of one coset into `out` (out_slots points).
+// With lines == h + 1 the high-basis line was not computed and its output slot (the
+// last one) is left for the caller to fill.
+int amt_outputs(zgr_ctx* c, Lane& ln, uint32_t* out, int lines) {
     for (int d = c->h - 1; d >= 0; d--) {
-        uint32_t total = (uint32_t)c->nlines * MSM_W << d;
+        uint32_t total = (uint32_t)lines * MSM_W << d;
         LAUNCH(c, ln.st, "msm_tree", k_msm_tree_level, blocks(total, 64), 64, 0, ln.heap, c->h, d, total);
     }
-    uint32_t slots = (uint32_t)out_slots(c);
-    LAUNCH(c, ln.st, "msm_horner", k_msm_horner, blocks(slots, 64), 64, 0, ln.heap, c->h, out);
+    uint32_t slots = (uint32_t)out_slots(c) - (lines == c->nlines ? 0u : 1u);
+    LAUNCH(c, ln.st, "msm_horner", k_msm_horner, blocks(slots, 64), 64, 0, ln.heap, c->h, out, slots);
     return ZGR_OK;
 }
 
@@ -274,8 +277,12 @@ int issue_blob(zgr_ctx* c, Slot& s, const uint8_t* raw_dev, size_t len, const zg
             CHECK(ntt_stages(c, ln.st, ln.y, c->log_n, false, false));
             y = ln.y;
         }
-        CHECK(msm_lines(c, ln, c->bases[cs], y, 1));
-        CHECK(amt_outputs(c, ln, s.outpts + (size_t)cs * slots * 16));
+        // high_commitment = [tau^(3N) f(tau)] G1 is the same group element for every
+        // coset (coset c's evaluations interpolate f(cf_c X)), so the high-basis line
+        // runs for coset 0 only and finish_blob copies its result to the others
+        const int lines = cs == 0 ? c->nlines : c->nlines - 1;
+        CHECK(msm_lines(c, ln, c->bases[cs], y, 1, lines));
+        CHECK(amt_outputs(c, ln, s.outpts + (size_t)cs * slots * 16, lines));
         LAUNCH(c, ln.st, "merkle_leaves", k_merkle_leaves, blocks(n / 8, 128), 128, 0, y, c->log_col, c->log_row,
                cs, s.data, s.mlayers[0]);
         HIPCHK(hipEventRecord(s.ev_coset[cs], ln.st));
@@ -318,7 +325,7 @@ int finish_blob(zgr_ctx* c, Slot& s) {
         const uint8_t* base = pts + (size_t)cs * slots * 64;
         if (out->proof_tree) memcpy(out->proof_tree + cs * T * 64, base, T * 64);
         if (out->comm_tree) memcpy(out->comm_tree + cs * T * 64, base + T * 64, T * 64);
-        if (out->high) memcpy(out->high + cs * 64, base + (2 * T + 1) * 64, 64);
+        if (out->high) memcpy(out->high + cs * 64, pts + (2 * T + 1) * 64, 64);  // coset 0 (see issue_blob)
     }
     if (out->commitment) memcpy(out->commitment, pts + 2 * T * 64, 64);
     if (out->data) memcpy(out->data, s.h_data, (size_t)c->coset_n * c->n * 32);
@@ -727,7 +734,7 @@ int zgr_multiple_multiexp(zgr_ctx* c, int coset, const uint8_t* ri_mont, size_t 
     Lane& ln = c->lanes[0];
     uint32_t* opts = c->slots[0].outpts;
     HIPCHK(hipMemcpyAsync(ln.y, ri_mont, n * 32, hipMemcpyHostToDevice, ln.st));
-    CHECK(msm_lines(c, ln, c->bases[coset], ln.y, 0));
+    CHECK(msm_lines(c, ln, c->bases[coset], ln.y, 0, c->nlines));
     uint32_t total = (uint32_t)(c->nlines * c->nb);
     LAUNCH(c, ln.st, "msm_leaves_out", k_msm_leaves_out, blocks(total, 64), 64, 0, ln.heap, c->h, c->nlines, opts);
     HIPCHK(hipMemcpyAsync(out_points, opts, (size_t)total * 64, hipMemcpyDeviceToHost, ln.st));
@@ -744,8 +751,8 @@ int zgr_amt_prove(zgr_ctx* c, int coset, const uint8_t* ri_mont, size_t n, uint8
     Lane& ln = c->lanes[0];
     uint32_t* opts = c->slots[0].outpts;
     HIPCHK(hipMemcpyAsync(ln.y, ri_mont, n * 32, hipMemcpyHostToDevice, ln.st));
-    CHECK(msm_lines(c, ln, c->bases[coset], ln.y, 0));
-    CHECK(amt_outputs(c, ln, opts));
+    CHECK(msm_lines(c, ln, c->bases[coset], ln.y, 0, c->nlines));
+    CHECK(amt_outputs(c, ln, opts, c->nlines));
     const size_t slots = out_slots(c), T = tree_pts(c);
     std::vector<uint8_t> pts(slots * 64);
     HIPCHK(hipMemcpyAsync(pts.data(), opts, pts.size(), hipMemcpyDeviceToHost, ln.st));
