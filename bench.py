@@ -82,6 +82,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--served-requests", type=int, default=48)
     ap.add_argument("--qps", type=float, default=0.0)
+    ap.add_argument("--roofline-only", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     args = ap.parse_args()
     if args.gpus != 1:
@@ -96,6 +97,26 @@ def main():
     ctx.params_generate(Z.tau_from_seed(TAU_SEED))
     t_params = time.time() - t0
     log("params (3 cosets x 12 x 2^20 bases) generated on device in %.2f s" % t_params)
+
+    if args.roofline_only:
+        # workload for the counter passes: three isolated bucket-kernel launches (one
+        # coset of multiple_multiexp each) and one 1 GiB device-to-device copy whose
+        # FETCH_SIZE / WRITE_SIZE calibrate the counters
+        scal = np.random.default_rng(5).integers(0, 2**63, (geo.n, 4), dtype=np.uint64)
+        scal[:, 3] &= np.uint64(0x2FFFFFFFFFFFFFFF)
+        for _ in range(3):
+            ctx.multiple_multiexp(1, scal)
+        import ctypes
+        pa, pb = ctypes.c_void_p(), ctypes.c_void_p()
+        Z.hip().hipMalloc(ctypes.byref(pa), 1 << 30)
+        Z.hip().hipMalloc(ctypes.byref(pb), 1 << 30)
+        Z.hip().hipMemcpy(pb, pa, 1 << 30, 3)  # hipMemcpyDeviceToDevice
+        Z.hip().hipDeviceSynchronize()
+        Z.hip().hipFree(pa)
+        Z.hip().hipFree(pb)
+        ctx.close()
+        print(json.dumps({"roofline_only": True, "msm_launches": 3, "copy_bytes": 1 << 30}))
+        return
 
     def resident(raws):
         dev = [Z.DeviceBuffer(r) for r in raws]
