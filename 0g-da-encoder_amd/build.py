@@ -44,7 +44,26 @@ def build(verbose=False):
     if verbose and out.strip():
         print(out)
     os.replace(tmp, LIB)
+    build_host_tools(verbose)
     return LIB
+
+
+HOST = os.path.join(HERE, "host")
+ENCODE_FILE = os.path.join(LIBDIR, "zg_encode_file")
+
+
+def build_host_tools(verbose=False):
+    """The C++ host mirror's example program (host code only, linked to libzgr.so)."""
+    tmp = ENCODE_FILE + ".tmp"
+    cmd = [os.environ.get("CXX", "g++"), "-O2", "-std=c++17", "-Wall",
+           os.path.join(HOST, "zg_encode_file.cpp"), "-L" + LIBDIR, "-lzgr", "-Wl,-rpath,$ORIGIN", "-o", tmp]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    res = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    if res.returncode != 0:
+        raise RuntimeError("g++ failed:\n" + res.stdout)
+    os.replace(tmp, ENCODE_FILE)
+    return ENCODE_FILE
 
 
 if __name__ == "__main__":

@@ -1,0 +1,285 @@
+// zg_encoder.hpp -- C++ host mirror of the reference encode API over include/zgr.h.
+//
+// Mirrors the types a caller of zg_encoder / amt uses on the encoding path:
+//   RawData          crates/encoder/src/raw_data.rs:55-73      (TooLargeBlob)
+//   EncoderParams    crates/amt/src/blob/encode.rs:21-133      (from_dir_mont, warmup)
+//   EncodedBlob      crates/encoder/src/encoder/blob.rs:26-72  (build, get_row, commitment, roots)
+//   EncodedSlice     crates/encoder/src/encoder/slice.rs        (row + AMT proof + Merkle proof)
+//   EncoderService   grpc/src/service.rs:67-106                 (process_data reply fields)
+// The compute runs in lib/libzgr.so on the GPU; this header holds no arithmetic.
+// Errors follow the reference: an oversize payload throws TooLargeBlob, device and
+// format failures throw std::runtime_error.
+#pragma once
+
+#include <array>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/zgr.h"
+
+namespace zg {
+
+using Bytes32 = std::array<uint8_t, 32>;
+
+struct TooLargeBlob : std::runtime_error {  // EncoderError::TooLargeBlob
+    size_t actual, expected_max;
+    TooLargeBlob(size_t a, size_t m)
+        : std::runtime_error("TooLargeBlob { actual: " + std::to_string(a) + ", expected_max: " + std::to_string(m) +
+                             " }"),
+          actual(a),
+          expected_max(m) {}
+};
+
+inline void check(int rc, const char* what) {
+    if (rc == ZGR_OK) return;
+    throw std::runtime_error(std::string(what) + ": " + zgr_last_error() + " (code " + std::to_string(rc) + ")");
+}
+
+// Blob geometry (crates/encoder/src/constants.rs:14-28) as runtime values.
+struct Geometry {
+    int log_col = 10, log_row = 10, coset_n = 3;
+    size_t n() const { return (size_t)1 << (log_col + log_row); }
+    size_t row_n() const { return (size_t)1 << log_row; }
+    size_t col_n() const { return (size_t)1 << log_col; }
+    size_t row_encoded() const { return row_n() * coset_n; }
+    size_t max_blob_size() const { return 31 * n(); }
+    size_t max_raw_data_size() const { return max_blob_size() - 4; }
+    size_t tree_pts() const { return ((size_t)1 << (log_row + 1)) - 2; }
+    size_t merkle_nodes() const { return (size_t)coset_n * (((size_t)1 << (log_row + 1)) - 1); }
+    size_t params_bytes() const { return (size_t)(log_row + 2) * n() * 64; }
+    static Geometry production() { return Geometry{10, 10, 3}; }
+    static Geometry test_setting() { return Geometry{5, 6, 3}; }
+};
+
+// RawData::try_from: only the size rule lives on the host; padding and packing run
+// on the device (the 31-byte units and the u32 LE length footer).
+struct RawData {
+    const uint8_t* ptr = nullptr;
+    size_t len = 0;
+    static RawData try_from(const Geometry& g, const uint8_t* p, size_t n) {
+        if (n > g.max_raw_data_size()) throw TooLargeBlob(n, g.max_raw_data_size());
+        return RawData{p, n};
+    }
+};
+
+// Prove parameters resident on one GPU for one geometry (EncoderParams<Bn254, COSET_N,
+// LOG_COL, LOG_ROW>); owns the zgr context.
+class EncoderParams {
+  public:
+    EncoderParams(const Geometry& g, int device = 0) : geo_(g) {
+        zgr_ctx* c = nullptr;
+        check(zgr_ctx_create(device, g.log_col, g.log_row, g.coset_n, &c), "zgr_ctx_create");
+        ctx_.reset(c);
+    }
+
+    // insecure test setup from a known tau (PowerTau::setup_inner + AMTParams::from_pp)
+    static EncoderParams from_tau(const Geometry& g, const uint8_t tau_le[32], int device = 0) {
+        EncoderParams p(g, device);
+        check(zgr_params_generate(p.ctx(), tau_le), "zgr_params_generate");
+        return p;
+    }
+
+    // AMTParams::from_dir_mont (crates/amt/src/prove_params/generate.rs:27-65): read the
+    // bamt files amt-prove-coset{c}-mont-{hash}-{h:02}-{n:02}.bin and keep the prove
+    // bases resident.  The G2 sections (g2, high_g2, vanishes) belong to the verifier
+    // and are skipped.  bamt layout: fast_serde_bn254.rs:20-88.
+    static EncoderParams from_dir_mont(const Geometry& g, const std::string& dir,
+                                       const std::string& type_hash = "5DWgDV", int device = 0) {
+        EncoderParams p(g, device);
+        const int depth = g.log_col + g.log_row, h = g.log_row;
+        for (int c = 0; c < g.coset_n; c++) {
+            char name[128];
+            snprintf(name, sizeof(name), "amt-prove-coset%d-mont-%s-%02d-%02d.bin", c, type_hash.c_str(), h, depth);
+            p.load_bamt(c, dir + "/" + name);
+        }
+        return p;
+    }
+
+    // read one bamt file and upload its prove bases in the line-major order the
+    // device uses (quotients[0..h) || basis || high_basis, as read_gpu_bases uploads)
+    void load_bamt(int coset, const std::string& path) {
+        std::unique_ptr<FILE, int (*)(FILE*)> f(fopen(path.c_str(), "rb"), fclose);
+        if (!f) throw std::runtime_error("cannot open " + path);
+        uint8_t head[6];
+        if (fread(head, 1, 6, f.get()) != 6 || memcmp(head, "bamt", 4))
+            throw std::runtime_error(path + ": Incorrect format");
+        const int degree = head[4], sub = head[5];
+        if (degree != geo_.log_col + geo_.log_row || sub != geo_.log_row)
+            throw std::runtime_error(path + ": degree / sub-degree do not match the geometry");
+        const size_t n = geo_.n(), line = n * 64;
+        // g2 and high_g2 are ark-serialize uncompressed G2 affine points; their exact size
+        // (128 B with the flags in y's spare bits) is taken from the file length so that
+        // the loader does not depend on it: total = 6 + 2 G + (sub + 2) lines + vanishes
+        size_t vanish = 0;
+        for (int d = 0; d < sub; d++) vanish += ((size_t)2 << d) * 128;  // write_g2: 4 x 32 B mont
+        if (fseek(f.get(), 0, SEEK_END)) throw std::runtime_error(path + ": seek");
+        const size_t total = (size_t)ftell(f.get());
+        const size_t fixed = 6 + (size_t)(sub + 2) * line + vanish;
+        if (total < fixed || (total - fixed) % 2) throw std::runtime_error(path + ": unexpected length");
+        const size_t g2_uncompressed = (total - fixed) / 2;
+        if (fseek(f.get(), 6, SEEK_SET)) throw std::runtime_error(path + ": seek");
+        std::vector<uint8_t> buf(geo_.params_bytes());
+        auto rd = [&](uint8_t* dst, size_t bytes) {
+            if (fread(dst, 1, bytes, f.get()) != bytes) throw std::runtime_error(path + ": truncated");
+        };
+        auto skip = [&](size_t bytes) {
+            if (fseek(f.get(), (long)bytes, SEEK_CUR)) throw std::runtime_error(path + ": truncated");
+        };
+        skip(2 * g2_uncompressed);                             // g2, high_g2
+        rd(buf.data() + (size_t)h() * line, line);             // basis
+        for (int d = 0; d < sub; d++) rd(buf.data() + (size_t)d * line, line);  // quotients
+        for (int d = 0; d < sub; d++) skip(((size_t)2 << d) * 128);  // vanishes (G2, 4 x 32 B)
+        rd(buf.data() + (size_t)(h() + 1) * line, line);       // high_basis
+        check(zgr_params_load(ctx(), coset, buf.data(), buf.size()), "zgr_params_load");
+    }
+
+    void warmup() const {}  // bases are uploaded when loaded (EncoderParams::warmup)
+
+    zgr_ctx* ctx() const { return ctx_.get(); }
+    const Geometry& geometry() const { return geo_; }
+
+  private:
+    int h() const { return geo_.log_row; }
+    struct Del {
+        void operator()(zgr_ctx* c) const { zgr_ctx_destroy(c); }
+    };
+    Geometry geo_;
+    std::unique_ptr<zgr_ctx, Del> ctx_;
+};
+
+// One encoded row as EncodedSlice carries it (encoder/slice.rs, amt/slice.rs,
+// merkle/slice.rs): index, row data, the AMT proof (commitment, proof) per level,
+// the high commitment, the authoritative commitment, the Merkle proof.
+struct EncodedSlice {
+    size_t index = 0;
+    std::vector<Bytes32> row;
+    std::vector<std::pair<std::array<uint8_t, 64>, std::array<uint8_t, 64>>> amt_proof;
+    std::array<uint8_t, 64> high_commitment{}, commitment{};
+    std::vector<Bytes32> merkle_root;  // COSET_N roots
+    std::vector<Bytes32> merkle_proof;
+    Bytes32 row_root{};
+};
+
+class EncodedBlob {
+  public:
+    // EncodedBlob::build over RawData (RawBlob packing happens on the device)
+    static EncodedBlob build(const RawData& raw, const EncoderParams& params) {
+        EncodedBlob b(params.geometry());
+        zgr_encoded o = b.outputs();
+        check(zgr_encode(params.ctx(), raw.ptr, raw.len, &o), "zgr_encode");
+        return b;
+    }
+
+    std::array<uint8_t, 64> get_affine_commitment() const { return point(commitment_.data()); }
+    Bytes32 get_file_root() const {
+        Bytes32 r;
+        memcpy(r.data(), root_.data(), 32);
+        return r;
+    }
+    std::vector<Bytes32> get_roots() const { return layer(0); }
+    const std::vector<uint8_t>& get_data() const { return data_; }
+
+    // EncodedBlob::get_row (encoder/blob.rs:44-49)
+    EncodedSlice get_row(size_t index) const {
+        const Geometry& g = geo_;
+        if (index >= g.row_encoded()) throw std::out_of_range("row index");
+        EncodedSlice s;
+        s.index = index;
+        const size_t coset = index / g.row_n(), local = index % g.row_n();
+        for (size_t c = 0; c < g.col_n(); c++) {
+            Bytes32 e;
+            memcpy(e.data(), data_.data() + 32 * (coset * g.n() + local * g.col_n() + c), 32);
+            s.row.push_back(e);
+        }
+        // HalfBlob::get_row + AllProofs::get_proof with reversed_index = bitrev(local)
+        size_t rev = 0;
+        for (int b = 0; b < g.log_row; b++) rev |= ((local >> b) & 1) << (g.log_row - 1 - b);
+        for (int d = 0; d < g.log_row; d++) {
+            size_t lv = rev >> (g.log_row - 1 - d);
+            size_t at = coset * g.tree_pts() + ((size_t)2 << d) - 2 + (lv ^ 1);
+            s.amt_proof.push_back({point(comm_tree_.data() + 64 * at), point(proof_tree_.data() + 64 * at)});
+        }
+        s.high_commitment = point(high_.data() + 64 * coset);
+        s.commitment = get_affine_commitment();
+        // EncodedBlobMerkle::get_row (merkle/blob.rs:44-57)
+        s.merkle_root = layer(0);
+        for (int d = g.log_row; d >= 1; d--) s.merkle_proof.push_back(layer(d)[(index >> (g.log_row - d)) ^ 1]);
+        s.row_root = layer(g.log_row)[index];
+        return s;
+    }
+
+    // raw output buffers (zgr_encoded layout), for callers that copy them out
+    const std::vector<uint8_t>& comm_tree() const { return comm_tree_; }
+    const std::vector<uint8_t>& proof_tree() const { return proof_tree_; }
+    const std::vector<uint8_t>& high() const { return high_; }
+    const std::vector<uint8_t>& merkle() const { return merkle_; }
+
+  private:
+    explicit EncodedBlob(const Geometry& g)
+        : geo_(g),
+          data_((size_t)g.coset_n * g.n() * 32),
+          commitment_(64),
+          comm_tree_((size_t)g.coset_n * g.tree_pts() * 64),
+          proof_tree_((size_t)g.coset_n * g.tree_pts() * 64),
+          high_((size_t)g.coset_n * 64),
+          merkle_(g.merkle_nodes() * 32),
+          root_(32) {}
+
+    zgr_encoded outputs() {
+        return zgr_encoded{data_.data(), commitment_.data(), comm_tree_.data(), proof_tree_.data(),
+                           high_.data(),  merkle_.data(),     root_.data()};
+    }
+    static std::array<uint8_t, 64> point(const uint8_t* p) {
+        std::array<uint8_t, 64> r;
+        memcpy(r.data(), p, 64);
+        return r;
+    }
+    std::vector<Bytes32> layer(int d) const {  // EncodedBlobMerkle.tree[d]
+        size_t off = 0;
+        for (int k = 0; k < d; k++) off += (size_t)geo_.coset_n << k;
+        std::vector<Bytes32> v((size_t)geo_.coset_n << d);
+        for (size_t i = 0; i < v.size(); i++) memcpy(v[i].data(), merkle_.data() + 32 * (off + i), 32);
+        return v;
+    }
+
+    Geometry geo_;
+    std::vector<uint8_t> data_, commitment_, comm_tree_, proof_tree_, high_, merkle_, root_;
+};
+
+// EncodeBlobReply fields produced by EncoderService::process_data (grpc/src/service.rs:67-106);
+// the per-row encoded_slice wire bytes are not produced here.
+struct EncodeBlobReply {
+    uint32_t version = 0;
+    std::vector<uint8_t> erasure_commitment;  // x || y, 32 B canonical LE each (service.rs:77-83)
+    std::vector<uint8_t> storage_root;        // file root (service.rs:84)
+    std::vector<uint8_t> encoded_data;        // 32 B per encoded element when require_data (service.rs:85-91)
+};
+
+class EncoderService {
+  public:
+    explicit EncoderService(EncoderParams params) : params_(std::move(params)) {}
+
+    EncodeBlobReply process_data(const uint8_t* data, size_t len, bool require_data) const {
+        RawData raw = RawData::try_from(params_.geometry(), data, len);
+        EncodedBlob blob = EncodedBlob::build(raw, params_);
+        EncodeBlobReply r;
+        auto c = blob.get_affine_commitment();
+        r.erasure_commitment.assign(c.begin(), c.end());
+        auto root = blob.get_file_root();
+        r.storage_root.assign(root.begin(), root.end());
+        if (require_data) r.encoded_data = blob.get_data();
+        return r;
+    }
+
+    const EncoderParams& params() const { return params_; }
+
+  private:
+    EncoderParams params_;
+};
+
+}  // namespace zg

@@ -31,6 +31,10 @@ MAC32_PER_PAIR_WINDOW = 10 * 136
 # i.e. one v_mad_u64_u32 (32x32->64 MAC) per lane per clock.  (A dependent-chain
 # microbenchmark of v_mad_u64_u32 alone reaches 25.0 T/s on the GPU.)
 PEAK_MAC32 = 256 * 4 * 16 * 2.4e9
+# HBM bytes per bucket-kernel launch from two rocprofv3 counter passes over
+# `bench.py --roofline-only`: FETCH_SIZE 523,413 KiB x 2.0 (a 1 GiB device copy reads
+# as 524,342 KiB) + WRITE_SIZE 213,733 KiB x 1.0 (the copy writes as 1,048,576 KiB).
+MSM_TRAFFIC_BYTES = (523413 * 2.0 + 213733) * 1024
 
 
 def log(*a):
@@ -268,7 +272,8 @@ def main():
             "peak": round(peak, 3),
             "unit": "TOP/s (32x32->64 integer MAC)",
             "frac": round(achieved / peak, 4) if achieved else None,
-            "traffic": None,
+            "traffic": round(MSM_TRAFFIC_BYTES),
+            "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, calibrated)",
             "kernel": "k_msm_accum",
             "avg_ms": round(acc_ms, 3),
             "launches": acc_cnt,
