@@ -144,6 +144,22 @@ __device__ __forceinline__ fq29 q_sub_lazy(const fq29& a, const fq29& b) {
     return s;
 }
 
+// [0, 2p) -> [0, p)
+__device__ __forceinline__ fq29 q_canon(const fq29& s) {
+    fq29 d;
+    int32_t br = 0;
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+        int32_t x = (int32_t)s.l[k] - (int32_t)Q29::P[k] - br;
+        br = (x >> 31) & 1;
+        d.l[k] = (uint32_t)x & M29;
+    }
+    fq29 r;
+#pragma unroll
+    for (int k = 0; k < 9; k++) r.l[k] = br ? s.l[k] : d.l[k];
+    return r;
+}
+
 // x in [0, 2p): x == 0 mod p iff x == 0 or x == p
 __device__ __forceinline__ bool q_is_zero(const fq29& a) {
     uint32_t z = 0, e = 0;

@@ -383,12 +383,14 @@ __global__ __launch_bounds__(64) void k_msm_tree_level(uint32_t* heap, int h, in
     x29_store(base + (size_t)i * X29_WORDS, x29_add(a, c));
 }
 
-// sum_w 2^(8w) S_w over the windows of heap node `node` of line `line`
-__device__ __forceinline__ x29 horner_windows(const uint32_t* heap, uint32_t nb, uint32_t line, uint32_t node) {
-    x29 acc = x29_inf();
-    for (int w = MSM_W - 1; w >= 0; w--) {
+// sum_w 2^(8w) S_w over the nw windows of heap node `node` of line `line` (heap trees
+// (line, w) of 2nb nodes each); nw == 1: the node itself
+__device__ __forceinline__ x29 horner_windows(const uint32_t* heap, uint32_t nb, int nw, uint32_t line,
+                                              uint32_t node) {
+    x29 acc = x29_load(heap + (((size_t)line * nw + nw - 1) * 2 * nb + node) * X29_WORDS);
+    for (int w = nw - 2; w >= 0; w--) {
         for (int k = 0; k < MSM_C; k++) acc = x29_dbl(acc);
-        acc = x29_add(acc, x29_load(heap + (((size_t)line * MSM_W + w) * 2 * nb + node) * X29_WORDS));
+        acc = x29_add(acc, x29_load(heap + (((size_t)line * nw + w) * 2 * nb + node) * X29_WORDS));
     }
     return acc;
 }
@@ -397,7 +399,8 @@ __device__ __forceinline__ x29 horner_windows(const uint32_t* heap, uint32_t nb,
 // Output slots (T = 2nb - 2):  [0, T) proof_tree  (line d-1, node i = slot + 2, d = log2 i)
 //                              [T, 2T) comm_tree  (line h, node i = slot - T + 2)
 //                              2T commitment (line h, node 1), 2T + 1 high (line h + 1, node 1)
-__global__ __launch_bounds__(64) void k_msm_horner(const uint32_t* heap, int h, uint32_t* out, uint32_t nslots) {
+__global__ __launch_bounds__(64) void k_msm_horner(const uint32_t* heap, int h, int nw, uint32_t* out,
+                                                   uint32_t nslots) {
     const uint32_t nb = 1u << h;
     const uint32_t T = 2 * nb - 2;
     uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
@@ -416,22 +419,215 @@ __global__ __launch_bounds__(64) void k_msm_horner(const uint32_t* heap, int h, 
         node = 1;
         line = h + 1;
     }
-    out_store_canon(out + (size_t)slot * 16, x29_to_xyzz(horner_windows(heap, nb, line, node)));
+    out_store_canon(out + (size_t)slot * 16, x29_to_xyzz(horner_windows(heap, nb, nw, line, node)));
 }
 
 // multiple_multiexp output: out[L * nb + b] = sum_w 2^(8w) heap leaf (L, w, nb + b)
-__global__ __launch_bounds__(64) void k_msm_leaves_out(const uint32_t* heap, int h, int nlines, uint32_t* out) {
+__global__ __launch_bounds__(64) void k_msm_leaves_out(const uint32_t* heap, int h, int nw, int nlines,
+                                                       uint32_t* out) {
     const uint32_t nb = 1u << h;
     uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nlines * nb) return;
     uint32_t L = t >> h, b = t & (nb - 1);
-    out_store_canon(out + (size_t)t * 16, x29_to_xyzz(horner_windows(heap, nb, L, nb + b)));
+    out_store_canon(out + (size_t)t * 16, x29_to_xyzz(horner_windows(heap, nb, nw, L, nb + b)));
 }
 
 // single MSM result: Horner over the windows of tree node 1 of line 0
 __global__ __launch_bounds__(64) void k_msm_root_out(const uint32_t* heap, int h, uint32_t* out) {
     if (threadIdx.x || blockIdx.x) return;
-    out_store_canon(out, x29_to_xyzz(horner_windows(heap, 1u << h, 0, 1)));
+    out_store_canon(out, x29_to_xyzz(horner_windows(heap, 1u << h, MSM_W, 0, 1)));
+}
+
+// ---------------------------------------------------------------------------
+// Fixed-base MSM over the resident prove bases
+// ---------------------------------------------------------------------------
+// The prove bases do not change from blob to blob, so every base P is kept with its
+// shifted copies P_j = 2^(12 j) P, j < FB_W.  A scalar with signed 12-bit digits
+// s = sum_j d_j 2^(12 j) contributes sum_j |d_j| (sign(d_j) P_j): every chunk MSM
+// (line L, chunk b) becomes ONE bucket sum over FB_W * bs (point, digit) pairs into
+// 2048 buckets -- about 24.6k additions per 1024-point chunk instead of 32 windows x
+// (1024 + 128) = 36.9k, and no window recombination (no doublings, one tree per line).
+// The digits, and so the sorted entries, are shared by every line of a chunk.
+//
+// Table layout (16 u32 per point: x, y as 256-bit integers holding the 2^261-
+// Montgomery value, canonical): point (L, b, j, k) at ((L * nb + b) * FB_W + j) * bs + k,
+// i.e. a chunk's FB_W * bs points are contiguous.
+constexpr int FB_C = 12;
+constexpr int FB_W = 22;  // ceil(255 / 12): the top digit (bits 252..253 + carry) never carries out
+constexpr int FB_BUCKETS = 1 << (FB_C - 1);  // |digit| in 1..2048
+constexpr int FB_LANES = 64;                 // one wave per (line, chunk)
+constexpr int FB_MAX_LOG_BS = 10;            // entry index j * bs + k in 15 bits
+
+__device__ __forceinline__ void fb_store_coord(uint32_t* d, const fq& v) {
+    uint32_t w[8];
+    q_pack(q_canon(q_from_fq(v)), w);
+    uint4* o = (uint4*)d;
+    o[0] = make_uint4(w[0], w[1], w[2], w[3]);
+    o[1] = make_uint4(w[4], w[5], w[6], w[7]);
+}
+
+__device__ __forceinline__ fq29 fb_load_coord(const uint32_t* s) {
+    const uint4* p = (const uint4*)s;
+    uint4 a = p[0], b = p[1];
+    const uint32_t w[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+    return q_unpack(w);
+}
+
+// Build the shifted copies of `count` consecutive bases of lines [line0, ...), one
+// thread per base: P_0 = P, P_(j+1) = 2^12 P_j, each stored affine.
+__global__ __launch_bounds__(64) void k_fb_build(const uint32_t* bases, uint32_t* table, int log_n, int h,
+                                                 uint32_t first, uint32_t count) {
+    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= count) return;
+    const uint32_t p = first + t;  // global base index L * n + i
+    const int log_bs = log_n - h;
+    const uint32_t L = p >> log_n, i = p & ((1u << log_n) - 1);
+    const uint32_t b = i >> log_bs, k = i & ((1u << log_bs) - 1);
+    uint32_t* dst = table + ((((((size_t)L << h) + b) * FB_W) << log_bs) + k) * 16;
+    const size_t stride = (size_t)16 << log_bs;  // next copy of the same point
+    g1_aff a = aff_load(bases + (size_t)p * 16);
+    g1_xyzz q = xyzz_from_aff(a);
+    fb_store_coord(dst, a.x);
+    fb_store_coord(dst + 8, a.y);
+    for (int j = 1; j < FB_W; j++) {
+        for (int s = 0; s < FB_C; s++) q = xyzz_dbl(q);
+        bool inf;
+        a = xyzz_to_aff(q, &inf);  // never infinity: 2^(12j) is not a multiple of the group order
+        fb_store_coord(dst + j * stride, a.x);
+        fb_store_coord(dst + j * stride + 8, a.y);
+    }
+}
+
+// Signed 12-bit digits of the chunk's scalars (ri[b * bs + k] = y[bitrev(b * bs + k)]
+// with from_y, else scalars in order), counting-sorted by |digit| into one entry list
+// per chunk: entries j * bs + k | sign << 15 | |digit| << 16, ascending |digit|.
+// One thread per scalar (blockDim == bs <= 1024).
+__global__ __launch_bounds__(1024) void k_fb_sort(const uint32_t* y, int log_n, int log_bs, int from_y,
+                                                  uint32_t* entries, uint32_t* counts) {
+    __shared__ uint32_t hist[FB_BUCKETS + 1];  // by |digit|; [0] unused
+    __shared__ uint32_t wsum[16];
+    const uint32_t b = blockIdx.x, k = threadIdx.x, T = blockDim.x;
+    for (uint32_t q = k; q <= FB_BUCKETS; q += T) hist[q] = 0;
+    __syncthreads();
+    const uint32_t i = (b << log_bs) + k;
+    fr s = f_from_mont(f_load<FrP>(y + 8 * (size_t)(from_y ? bitrev_bits(i, log_n) : i)));
+    int dig[FB_W];
+    int carry = 0;
+#pragma unroll
+    for (int w = 0; w < FB_W; w++) {
+        const int bit = FB_C * w, wi = bit >> 5, sh = bit & 31;
+        uint32_t v = s.v[wi] >> sh;
+        if (sh > 32 - FB_C && wi + 1 < 8) v |= s.v[wi + 1] << (32 - sh);
+        int d = (int)(v & ((1u << FB_C) - 1)) + carry;  // [0, 4096]
+        carry = d > FB_BUCKETS;
+        d -= carry << FB_C;  // [-2047, 2048]
+        dig[w] = d;
+        if (d) atomicAdd(&hist[d < 0 ? -d : d], 1u);
+    }
+    __syncthreads();
+    // exclusive prefix over the buckets: each thread owns FB_BUCKETS / T consecutive bins
+    const uint32_t per = FB_BUCKETS / T;
+    uint32_t loc = 0;
+    for (uint32_t q = 0; q < per; q++) loc += hist[1 + k * per + q];
+    uint32_t inc = loc;
+    const int lane = (int)(k & 63);
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        uint32_t t = __shfl_up(inc, o);
+        if (lane >= o) inc += t;
+    }
+    if (lane == 63 || k == T - 1) wsum[k >> 6] = inc;
+    __syncthreads();
+    if (k == 0) {
+        uint32_t acc = 0;
+        for (uint32_t w = 0; w < (T + 63) / 64; w++) {
+            uint32_t t = wsum[w];
+            wsum[w] = acc;
+            acc += t;
+        }
+        counts[b] = acc;
+    }
+    __syncthreads();
+    uint32_t run = wsum[k >> 6] + inc - loc;
+    for (uint32_t q = 0; q < per; q++) {
+        uint32_t hc = hist[1 + k * per + q];
+        hist[1 + k * per + q] = run;
+        run += hc;
+    }
+    __syncthreads();
+    uint32_t* ent = entries + (((size_t)b * FB_W) << log_bs);
+#pragma unroll
+    for (int w = 0; w < FB_W; w++) {
+        const int d = dig[w];
+        if (!d) continue;
+        const uint32_t mag = d < 0 ? -d : d;
+        const uint32_t pos = atomicAdd(&hist[mag], 1u);
+        ent[pos] = (((uint32_t)w << log_bs) | k) | ((uint32_t)(d < 0) << 15) | (mag << 16);
+    }
+}
+
+// Bucket sum of one (line L, chunk b) over its shifted copies: the 64 lanes split the
+// chunk's sorted entries evenly and run the same running-sum steps as k_msm_accum
+// (consume: R += +-P_e; close: T += R), each lane's share being T + (lo - 1) R; the
+// lane shares are summed through LDS into heap leaf nb + b of line L.
+__global__ __launch_bounds__(FB_LANES, 2) void k_fb_accum(const uint32_t* table, const uint32_t* entries,
+                                                          const uint32_t* counts, uint32_t* heap, int log_n,
+                                                          int h, int lines) {
+    __shared__ uint32_t red[FB_LANES * X29_WORDS];
+    const uint32_t nb = 1u << h;
+    const int log_bs = log_n - h;
+    const uint32_t logical = xcd_remap(blockIdx.x, gridDim.x);
+    const uint32_t b = logical / lines, L = logical % lines;
+    const int s = threadIdx.x;
+    uint32_t* leaf = heap + ((size_t)L * 2 * nb + nb + b) * X29_WORDS;
+    const int cnt = (int)counts[b];
+    if (cnt == 0) {  // zero chunk
+        if (s == 0) x29_store(leaf, x29_inf());
+        return;
+    }
+    const uint32_t* ent = entries + (((size_t)b * FB_W) << log_bs);
+    const uint32_t* tab = table + (((((size_t)L << h) + b) * FB_W) << log_bs) * 16;
+    const int begin = (s * cnt) / FB_LANES, end = ((s + 1) * cnt) / FB_LANES;
+    int hi = 0, lo = 1, steps = 0;
+    if (end > begin) {
+        hi = (int)(ent[end - 1] >> 16);
+        lo = (int)(ent[begin] >> 16);
+        steps = (end - begin) + (hi - lo + 1);
+    }
+    const int maxsteps = wave_max(steps);
+    x29 R = x29_inf(), T = x29_inf();
+    int e = end - 1, kc = hi;
+    for (int st = 0; st < maxsteps; st++) {
+        const bool active = st < steps;
+        const bool have = e >= begin;
+        const uint32_t ev = have ? ent[e] : 0u;
+        const bool consume = active && have && (int)(ev >> 16) == kc;
+        const bool trans = active && !consume;
+        const uint32_t* pp = tab + (size_t)(ev & 0x7fffu) * 16;
+        a29 P;
+        P.x = fb_load_coord(pp);
+        P.y = fb_load_coord(pp + 8);
+        if (ev & 0x8000u) P.y = q_neg(P.y);
+        x29 A = x29_select(trans, T, R);
+        x29 B = x29_select(trans, R, x29_from_aff(P));
+        x29 C = x29_add(A, B);
+        R = x29_select(consume, C, x29_select(trans, B, A));
+        T = x29_select(trans, C, T);
+        e -= consume ? 1 : 0;
+        kc -= trans ? 1 : 0;
+    }
+    x29 W = x29_add(T, x29_mul_small_uniform(R, (uint32_t)(lo - 1), FB_C - 1));
+
+    x29_store(red + s * X29_WORDS, W);
+    __syncthreads();
+    for (int off = 1; off < FB_LANES; off <<= 1) {
+        if ((s & (2 * off - 1)) == 0) {
+            W = x29_add(W, x29_load(red + (s + off) * X29_WORDS));
+            x29_store(red + s * X29_WORDS, W);
+        }
+        __syncthreads();
+    }
+    if (s == 0) x29_store(leaf, W);
 }
 
 // ---------------------------------------------------------------------------

@@ -20,6 +20,7 @@
 #include <condition_variable>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <deque>
 #include <map>
@@ -137,6 +138,9 @@ struct zgr_ctx {
     // resident params
     uint32_t* bases[3] = {nullptr, nullptr, nullptr};
     bool loaded[3] = {false, false, false};
+    // fixed-base tables (shifted copies of the bases); fb == false: windowed MSM
+    bool fb = false;
+    uint32_t* fbtab[3] = {nullptr, nullptr, nullptr};
 
     Lane lanes[2];
     Slot slots[2];
@@ -211,30 +215,54 @@ int ntt_stages(zgr_ctx* c, hipStream_t st, uint32_t* a, int k, bool dif, bool in
     return ZGR_OK;
 }
 
-// multi-line MSM of one coset on a lane: digits of `src` -> window sums in lane heap,
-// for the first `lines` lines (quotients[0..h), basis, high_basis)
-int msm_lines(zgr_ctx* c, Lane& ln, const uint32_t* bases, const uint32_t* src, int from_y, int lines) {
+// windows per heap tree set: 1 with the fixed-base tables, else MSM_W
+int heap_windows(const zgr_ctx* c) { return c->fb ? 1 : MSM_W; }
+
+// multi-line MSM of one coset on a lane: scalars `src` -> per-chunk sums in the lane
+// heap (one tree per line, or one per (line, window)), for the first `lines` lines
+// (quotients[0..h), basis, high_basis)
+int msm_lines(zgr_ctx* c, Lane& ln, int coset, const uint32_t* src, int from_y, int lines) {
     const int log_bs = c->log_n - c->h;
+    if (c->fb) {
+        LAUNCH(c, ln.st, "msm_sort", k_fb_sort, (uint32_t)c->nb, (uint32_t)c->bs, 0, src, c->log_n, log_bs, from_y,
+               ln.entries, ln.counts);
+        LAUNCH(c, ln.st, "msm_accum", k_fb_accum, (uint32_t)(c->nb * lines), FB_LANES, 0, c->fbtab[coset],
+               ln.entries, ln.counts, ln.heap, c->log_n, c->h, lines);
+        return ZGR_OK;
+    }
     LAUNCH(c, ln.st, "msm_digits", k_msm_digits, blocks(c->n, 256), 256, 0, src, c->log_n, log_bs, ln.digits,
            from_y);
     uint32_t sort_threads = c->bs < 256 ? (uint32_t)c->bs : 256u;
     LAUNCH(c, ln.st, "msm_sort", k_msm_sort, (uint32_t)(c->nb * MSM_W), sort_threads, 0, ln.digits, log_bs,
            ln.entries, ln.counts);
     LAUNCH(c, ln.st, "msm_accum", k_msm_accum, (uint32_t)(c->nb * lines), ACC_THREADS, accum_lds_bytes(c->bs),
-           bases, ln.entries, ln.counts, ln.heap, c->log_n, c->h, lines);
+           c->bases[coset], ln.entries, ln.counts, ln.heap, c->log_n, c->h, lines);
     return ZGR_OK;
 }
 
-// window trees + Horner: AMT outputs of one coset into `out` (out_slots points).
-// With lines == h + 1 the high-basis line was not computed and its output slot (the
-// last one) is left for the caller to fill.
+// trees over the chunks (+ Horner over the windows): AMT outputs of one coset into
+// `out` (out_slots points).  With lines == h + 1 the high-basis line was not computed
+// and its output slot (the last one) is left for the caller to fill.
 int amt_outputs(zgr_ctx* c, Lane& ln, uint32_t* out, int lines) {
+    const int nw = heap_windows(c);
     for (int d = c->h - 1; d >= 0; d--) {
-        uint32_t total = (uint32_t)lines * MSM_W << d;
+        uint32_t total = (uint32_t)(lines * nw) << d;
         LAUNCH(c, ln.st, "msm_tree", k_msm_tree_level, blocks(total, 64), 64, 0, ln.heap, c->h, d, total);
     }
     uint32_t slots = (uint32_t)out_slots(c) - (lines == c->nlines ? 0u : 1u);
-    LAUNCH(c, ln.st, "msm_horner", k_msm_horner, blocks(slots, 64), 64, 0, ln.heap, c->h, out, slots);
+    LAUNCH(c, ln.st, "msm_horner", k_msm_horner, blocks(slots, 64), 64, 0, ln.heap, c->h, nw, out, slots);
+    return ZGR_OK;
+}
+
+// shifted-copy table of one coset from its resident bases (after every params upload)
+int build_fb(zgr_ctx* c, int coset) {
+    if (!c->fb) return ZGR_OK;
+    if (!c->fbtab[coset]) DALLOC(c->fbtab[coset], (size_t)c->nlines * c->n * FB_W * 64);
+    hipStream_t st = c->lanes[0].st;
+    const uint32_t per = (uint32_t)c->n;  // one launch per line
+    for (int L = 0; L < c->nlines; L++)
+        LAUNCH(c, st, "fb_build", k_fb_build, blocks(per, 64), 64, 0, c->bases[coset], c->fbtab[coset], c->log_n,
+               c->h, (uint32_t)L * per, per);
     return ZGR_OK;
 }
 
@@ -281,7 +309,7 @@ int issue_blob(zgr_ctx* c, Slot& s, const uint8_t* raw_dev, size_t len, const zg
         // coset (coset c's evaluations interpolate f(cf_c X)), so the high-basis line
         // runs for coset 0 only and finish_blob copies its result to the others
         const int lines = cs == 0 ? c->nlines : c->nlines - 1;
-        CHECK(msm_lines(c, ln, c->bases[cs], y, 1, lines));
+        CHECK(msm_lines(c, ln, cs, y, 1, lines));
         CHECK(amt_outputs(c, ln, s.outpts + (size_t)cs * slots * 16, lines));
         LAUNCH(c, ln.st, "merkle_leaves", k_merkle_leaves, blocks(n / 8, 128), 128, 0, y, c->log_col, c->log_row,
                cs, s.data, s.mlayers[0]);
@@ -559,6 +587,10 @@ int zgr_ctx_create(int device, int log_col, int log_row, int coset_n, zgr_ctx** 
     c->n = (size_t)1 << c->log_n;
     c->nb = (size_t)1 << c->h;
     c->bs = c->n / c->nb;
+    // fixed-base tables unless the chunk is too large for their entry format, or
+    // ZGR_FIXED_BASE=0 asks for the windowed MSM (same results, far less memory)
+    const char* fbenv = getenv("ZGR_FIXED_BASE");
+    c->fb = (c->log_n - c->h) <= FB_MAX_LOG_BS && !(fbenv && fbenv[0] == '0');
     auto init = [&]() -> int {
         int max_shm = 0;
         HIPCHK(hipDeviceGetAttribute(&max_shm, hipDeviceAttributeMaxSharedMemoryPerBlock, device));
@@ -644,6 +676,7 @@ void zgr_ctx_destroy(zgr_ctx* c) {
     for (int i = 0; i < 3; i++) {
         if (c->scale[i]) (void)hipFree(c->scale[i]);
         if (c->bases[i]) (void)hipFree(c->bases[i]);
+        if (c->fbtab[i]) (void)hipFree(c->fbtab[i]);
     }
     for (auto& l : c->lanes) {
         void* lp[] = {l.y, l.digits, l.entries, l.counts, l.heap};
@@ -675,7 +708,9 @@ int zgr_params_load(zgr_ctx* c, int coset, const void* bases_mont, size_t nbytes
     if (nbytes != want) return fail(ZGR_ERR_INVALID_ARG, "params size %zu, expected %zu", nbytes, want);
     HIPCHK(hipSetDevice(c->device));
     CHECK(alloc_bases(c, coset));
+    c->loaded[coset] = false;
     HIPCHK(hipMemcpyAsync(c->bases[coset], bases_mont, want, hipMemcpyHostToDevice, c->lanes[0].st));
+    CHECK(build_fb(c, coset));
     CHECK(sync_all(c));
     c->loaded[coset] = true;
     return ZGR_OK;
@@ -704,6 +739,8 @@ int zgr_params_generate(zgr_ctx* c, const uint8_t tau[32]) {
         HIPCHK(hipMemcpyAsync(&bad, c->flag, sizeof(int), hipMemcpyDeviceToHost, st));
         CHECK(sync_all(c));
         if (bad) return fail(ZGR_ERR_INVALID_ARG, "tau hits the evaluation domain (or a base is infinity)");
+        for (int cs = 0; cs < c->coset_n; cs++) CHECK(build_fb(c, cs));
+        CHECK(sync_all(c));
         for (int cs = 0; cs < c->coset_n; cs++) c->loaded[cs] = true;
         return ZGR_OK;
     };
@@ -734,9 +771,10 @@ int zgr_multiple_multiexp(zgr_ctx* c, int coset, const uint8_t* ri_mont, size_t 
     Lane& ln = c->lanes[0];
     uint32_t* opts = c->slots[0].outpts;
     HIPCHK(hipMemcpyAsync(ln.y, ri_mont, n * 32, hipMemcpyHostToDevice, ln.st));
-    CHECK(msm_lines(c, ln, c->bases[coset], ln.y, 0, c->nlines));
+    CHECK(msm_lines(c, ln, coset, ln.y, 0, c->nlines));
     uint32_t total = (uint32_t)(c->nlines * c->nb);
-    LAUNCH(c, ln.st, "msm_leaves_out", k_msm_leaves_out, blocks(total, 64), 64, 0, ln.heap, c->h, c->nlines, opts);
+    LAUNCH(c, ln.st, "msm_leaves_out", k_msm_leaves_out, blocks(total, 64), 64, 0, ln.heap, c->h, heap_windows(c),
+           c->nlines, opts);
     HIPCHK(hipMemcpyAsync(out_points, opts, (size_t)total * 64, hipMemcpyDeviceToHost, ln.st));
     return sync_all(c);
 }
@@ -751,7 +789,7 @@ int zgr_amt_prove(zgr_ctx* c, int coset, const uint8_t* ri_mont, size_t n, uint8
     Lane& ln = c->lanes[0];
     uint32_t* opts = c->slots[0].outpts;
     HIPCHK(hipMemcpyAsync(ln.y, ri_mont, n * 32, hipMemcpyHostToDevice, ln.st));
-    CHECK(msm_lines(c, ln, c->bases[coset], ln.y, 0, c->nlines));
+    CHECK(msm_lines(c, ln, coset, ln.y, 0, c->nlines));
     CHECK(amt_outputs(c, ln, opts, c->nlines));
     const size_t slots = out_slots(c), T = tree_pts(c);
     std::vector<uint8_t> pts(slots * 64);

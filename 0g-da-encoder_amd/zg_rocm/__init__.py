@@ -341,6 +341,22 @@ class Context:
         lib().zgr_kernel_time_reset(self.h)
 
 
+def make_context(geo=PRODUCTION, fixed_base=True, device=0):
+    """Context with its MSM path chosen explicitly.  zgr_ctx_create reads
+    ZGR_FIXED_BASE: the default keeps shifted copies of the resident bases (one bucket
+    set per chunk, FB_W = 22 copies: 22 x the params' HBM); "0" selects the windowed
+    kernels.  Both give identical results."""
+    old = os.environ.get("ZGR_FIXED_BASE")
+    os.environ["ZGR_FIXED_BASE"] = "1" if fixed_base else "0"
+    try:
+        return Context(geo, device)
+    finally:
+        if old is None:
+            del os.environ["ZGR_FIXED_BASE"]
+        else:
+            os.environ["ZGR_FIXED_BASE"] = old
+
+
 class Server:
     """zgr_server: continuous-batching encode service over one Context.
 

@@ -12,10 +12,17 @@ pytestmark = pytest.mark.gpu
 GEO = Z.TEST_SETTING
 
 
-@pytest.fixture(scope="module")
-def ctx():
-    c = Z.Context(GEO)
+@pytest.fixture(scope="module", params=["fixed_base", "windowed"])
+def ctx(request):
+    """Both MSM paths over the resident bases: the shifted-copy tables (default) and
+    the windowed kernels (ZGR_FIXED_BASE=0, read at context creation)."""
+    c = Z.make_context(GEO, fixed_base=request.param == "fixed_base")
+    c.set_profiling(True)
     c.params_generate(seeded_tau())
+    # the table build runs only on the fixed-base path
+    assert (c.kernel_time_ms("fb_build")[1] > 0) == (request.param == "fixed_base")
+    c.set_profiling(False)
+    c.kernel_time_reset()
     yield c
     c.close()
 

@@ -153,3 +153,22 @@ def test_served_equals_single(ctx, blob):
     for a, b in zip(outs, singles):
         for f in ("commitment", "file_root", "proof_tree", "comm_tree", "high", "merkle", "data"):
             assert np.array_equal(getattr(a, f), getattr(b, f)), f
+
+
+def test_windowed_path_equals_fixed_base(ctx, blob):
+    # the windowed MSM kernels (ZGR_FIXED_BASE=0) over the same params give the same
+    # multiple_multiexp points and the same encoded blob as the shifted-copy tables
+    raw0, out0 = blob
+    w = Z.make_context(GEO, fixed_base=False)
+    try:
+        w.params_generate(seeded_tau())
+        rng = np.random.default_rng(77)
+        ri = rng.integers(0, 2**63, (GEO.n, 4), dtype=np.uint64)
+        ri[:, 3] &= np.uint64(0x0FFFFFFFFFFFFFFF)
+        for c in (0, 2):
+            assert np.array_equal(w.multiple_multiexp(c, ri), ctx.multiple_multiexp(c, ri)), "coset %d" % c
+        got = w.encode(raw0)
+        for f in ("commitment", "file_root", "proof_tree", "comm_tree", "high", "merkle", "data"):
+            assert np.array_equal(getattr(got, f), getattr(out0, f)), f
+    finally:
+        w.close()
