@@ -83,6 +83,24 @@ __device__ __forceinline__ x29 x29_add(const x29& p, const x29& q) {
     return r;
 }
 
+// p + q for affine q (madd-2008-s, 8M + 2S); q is never infinity.  Lanes whose
+// result is discarded pass live = false so that they never enter the doubling branch.
+__device__ __forceinline__ x29 x29_madd(const x29& p, const a29& q, bool live = true) {
+    const bool pinf = x29_is_inf(p);
+    fq29 P = q_sub_lazy(q_mul(q.x, p.ZZ), p.X);
+    fq29 R = q_sub_lazy(q_mul(q.y, p.ZZZ), p.Y);
+    if (live && !pinf && q_is_zero4(P) && q_is_zero4(R)) return x29_dbl(p);
+    x29 r;
+    fq29 PP = q_sqr(P);
+    r.ZZ = q_mul(p.ZZ, PP);
+    fq29 PPP = q_mul(P, PP);
+    r.ZZZ = q_mul(p.ZZZ, PPP);
+    fq29 Q = q_mul(p.X, PP);
+    r.X = q_sub(q_sub(q_sqr(R), PPP), q_dbl(Q));
+    r.Y = q_sub(q_mul(R, q_sub_lazy(Q, r.X)), q_mul(p.Y, PPP));
+    return x29_select(pinf, x29_from_aff(q), r);
+}
+
 // memory layout of a stored XYZZ point: X, Y, ZZ, ZZZ, 9 limbs each (36 u32)
 constexpr int X29_WORDS = 36;
 

@@ -501,9 +501,13 @@ __global__ __launch_bounds__(64) void k_fb_build(const uint32_t* bases, uint32_t
 // Signed 12-bit digits of the chunk's scalars (ri[b * bs + k] = y[bitrev(b * bs + k)]
 // with from_y, else scalars in order), counting-sorted by |digit| into one entry list
 // per chunk: entries j * bs + k | sign << 15 | |digit| << 16, ascending |digit|.
+// bstart[b * FB_BSTART + m] = first entry of bucket m (m = 1..FB_BUCKETS), and
+// bstart[b * FB_BSTART + FB_BUCKETS + 1] = count.
 // One thread per scalar (blockDim == bs <= 1024).
+constexpr int FB_BSTART = FB_BUCKETS + 2;
+
 __global__ __launch_bounds__(1024) void k_fb_sort(const uint32_t* y, int log_n, int log_bs, int from_y,
-                                                  uint32_t* entries, uint32_t* counts) {
+                                                  uint32_t* entries, uint32_t* counts, uint32_t* bstart) {
     __shared__ uint32_t hist[FB_BUCKETS + 1];  // by |digit|; [0] unused
     __shared__ uint32_t wsum[16];
     const uint32_t b = blockIdx.x, k = threadIdx.x, T = blockDim.x;
@@ -546,12 +550,14 @@ __global__ __launch_bounds__(1024) void k_fb_sort(const uint32_t* y, int log_n, 
             acc += t;
         }
         counts[b] = acc;
+        bstart[(size_t)b * FB_BSTART + FB_BUCKETS + 1] = acc;
     }
     __syncthreads();
     uint32_t run = wsum[k >> 6] + inc - loc;
     for (uint32_t q = 0; q < per; q++) {
         uint32_t hc = hist[1 + k * per + q];
         hist[1 + k * per + q] = run;
+        bstart[(size_t)b * FB_BSTART + 1 + k * per + q] = run;
         run += hc;
     }
     __syncthreads();
@@ -566,14 +572,27 @@ __global__ __launch_bounds__(1024) void k_fb_sort(const uint32_t* y, int log_n, 
     }
 }
 
-// Bucket sum of one (line L, chunk b) over its shifted copies: the 64 lanes split the
-// chunk's sorted entries evenly and run the same running-sum steps as k_msm_accum
-// (consume: R += +-P_e; close: T += R), each lane's share being T + (lo - 1) R; the
-// lane shares are summed through LDS into heap leaf nb + b of line L.
+// Bucket sum of one (line L, chunk b) over its shifted copies, one wave, two phases:
+//  1. bucket sums: lane s takes an even share of the sorted entries and adds each
+//     bucket's points with mixed additions (R += +-P_e).  A bucket belongs to the lane
+//     holding its first entry, which writes the bucket sum B_m to `bkt` (the chunk's
+//     FB_BUCKETS slots; empty buckets stay unwritten); a lane whose share starts inside
+//     a bucket hands that partial sum to the owner through LDS;
+//  2. bucket reduction: lane s folds buckets m in (32 s, 32 s + 32] from the top with
+//     the running sum (R += B_m; T += R), its share being T + 32 s R; the shares are
+//     summed through LDS into heap leaf nb + b of line L.
+// Splitting the work this way keeps every step of a phase the same addition on every
+// lane: phase 1 never adds two projective points, phase 2 never touches a base.
+constexpr int FB_PER_LANE = FB_BUCKETS / FB_LANES;  // buckets per lane in phase 2 (32)
+constexpr uint32_t FB_NO_SHARE = 0xffffffffu;        // lane with no entries
+
 __global__ __launch_bounds__(FB_LANES, 2) void k_fb_accum(const uint32_t* table, const uint32_t* entries,
-                                                          const uint32_t* counts, uint32_t* heap, int log_n,
-                                                          int h, int lines) {
-    __shared__ uint32_t red[FB_LANES * X29_WORDS];
+                                                          const uint32_t* counts, const uint32_t* bstart,
+                                                          uint32_t* bkt, uint32_t* heap, int log_n, int h,
+                                                          int lines) {
+    __shared__ uint32_t red[2 * FB_LANES * X29_WORDS];  // phase 1: first / last partials; then the reduction
+    __shared__ uint32_t head_k[FB_LANES];               // bucket continued from an earlier lane, 0 if none
+    __shared__ uint32_t one_bucket[FB_LANES];           // the share lies inside one bucket
     const uint32_t nb = 1u << h;
     const int log_bs = log_n - h;
     const uint32_t logical = xcd_remap(blockIdx.x, gridDim.x);
@@ -587,36 +606,75 @@ __global__ __launch_bounds__(FB_LANES, 2) void k_fb_accum(const uint32_t* table,
     }
     const uint32_t* ent = entries + (((size_t)b * FB_W) << log_bs);
     const uint32_t* tab = table + (((((size_t)L << h) + b) * FB_W) << log_bs) * 16;
+    const uint32_t* bst = bstart + (size_t)b * FB_BSTART;
+    uint32_t* bk = bkt + ((size_t)L * nb + b) * FB_BUCKETS * X29_WORDS;  // slot m - 1 for bucket m
+
+    // phase 1: even share [begin, end) of the sorted entries
+    uint32_t* first_p = red;                      // partial of a continued first bucket
+    uint32_t* last_p = red + FB_LANES * X29_WORDS;  // partial of the share's last bucket
     const int begin = (s * cnt) / FB_LANES, end = ((s + 1) * cnt) / FB_LANES;
-    int hi = 0, lo = 1, steps = 0;
-    if (end > begin) {
-        hi = (int)(ent[end - 1] >> 16);
-        lo = (int)(ent[begin] >> 16);
-        steps = (end - begin) + (hi - lo + 1);
-    }
+    const int steps = end - begin;
+    const uint32_t k_first = steps > 0 ? ent[begin] >> 16 : 0u;
+    const uint32_t k_last = steps > 0 ? ent[end - 1] >> 16 : 0u;
+    const bool cont = steps > 0 && begin > 0 && (ent[begin - 1] >> 16) == k_first;
     const int maxsteps = wave_max(steps);
-    x29 R = x29_inf(), T = x29_inf();
-    int e = end - 1, kc = hi;
+    x29 R = x29_inf();
+    uint32_t cur = 0;   // bucket of R (0: none yet)
+    bool head = true;   // R still holds the share's first bucket
     for (int st = 0; st < maxsteps; st++) {
         const bool active = st < steps;
-        const bool have = e >= begin;
-        const uint32_t ev = have ? ent[e] : 0u;
-        const bool consume = active && have && (int)(ev >> 16) == kc;
-        const bool trans = active && !consume;
+        const uint32_t ev = active ? ent[begin + st] : 0u;
+        const uint32_t m = ev >> 16;
         const uint32_t* pp = tab + (size_t)(ev & 0x7fffu) * 16;
         a29 P;
         P.x = fb_load_coord(pp);
         P.y = fb_load_coord(pp + 8);
         if (ev & 0x8000u) P.y = q_neg(P.y);
-        x29 A = x29_select(trans, T, R);
-        x29 B = x29_select(trans, R, x29_from_aff(P));
-        x29 C = x29_add(A, B);
-        R = x29_select(consume, C, x29_select(trans, B, A));
-        T = x29_select(trans, C, T);
-        e -= consume ? 1 : 0;
-        kc -= trans ? 1 : 0;
+        const bool same = m == cur;
+        x29 M = x29_madd(R, P, active);
+        if (active && !same && cur != 0) {  // bucket cur is complete in this share
+            if (head && cont)
+                x29_store(first_p + s * X29_WORDS, R);
+            else
+                x29_store(bk + (size_t)(cur - 1) * X29_WORDS, R);
+            head = false;
+        }
+        R = x29_select(same, M, x29_from_aff(P));
+        cur = m;
+        if (active && st == steps - 1) x29_store(last_p + s * X29_WORDS, R);
     }
-    x29 W = x29_add(T, x29_mul_small_uniform(R, (uint32_t)(lo - 1), FB_C - 1));
+    const bool single = steps > 0 && head;  // never left the first bucket
+    head_k[s] = steps == 0 ? FB_NO_SHARE : (cont ? k_first : 0u);
+    one_bucket[s] = single;
+    __syncthreads();
+    // the owner of the share's last bucket adds the partials of the lanes that continue it
+    if (steps > 0 && !(single && cont)) {
+        x29 W = x29_load(last_p + s * X29_WORDS);
+        for (int t = s + 1; t < FB_LANES; t++) {
+            const uint32_t hk = head_k[t];
+            if (hk == FB_NO_SHARE) continue;
+            if (hk != k_last) break;
+            W = x29_add(W, x29_load((one_bucket[t] ? last_p : first_p) + t * X29_WORDS));
+            if (!one_bucket[t]) break;
+        }
+        x29_store(bk + (size_t)(k_last - 1) * X29_WORDS, W);
+    }
+    __syncthreads();  // phase-1 bucket sums visible to every lane
+
+    // phase 2: buckets (32 s, 32 s + 32], top down
+    x29 Rr = x29_inf(), T = x29_inf();
+    for (int q = FB_PER_LANE; q >= 1; q--) {
+        const uint32_t m = (uint32_t)(s * FB_PER_LANE + q);
+        const bool filled = bst[m + 1] > bst[m];
+        x29 B = x29_inf();
+        if (filled) B = x29_load(bk + (size_t)(m - 1) * X29_WORDS);
+        Rr = x29_add(Rr, B);
+        T = x29_add(T, Rr);
+    }
+    // T + 32 s Rr = (s Rr) doubled 5 times, plus T
+    x29 S = x29_mul_small_uniform(Rr, (uint32_t)s, 6);
+    for (int k = 0; k < 5; k++) S = x29_dbl(S);
+    x29 W = x29_add(T, S);
 
     x29_store(red + s * X29_WORDS, W);
     __syncthreads();

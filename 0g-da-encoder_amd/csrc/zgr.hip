@@ -98,6 +98,8 @@ struct Lane {
     uint32_t* entries = nullptr;  // 32 n u32
     uint32_t* counts = nullptr;   // 32 nb
     uint32_t* heap = nullptr;     // nlines * 32 * 2nb XYZZ (36 u32 each)
+    uint32_t* bstart = nullptr;   // fixed-base: nb * FB_BSTART bucket starts
+    uint32_t* bkt = nullptr;      // fixed-base: nlines * nb * FB_BUCKETS bucket sums (XYZZ)
 };
 
 struct Slot {
@@ -225,9 +227,9 @@ int msm_lines(zgr_ctx* c, Lane& ln, int coset, const uint32_t* src, int from_y, 
     const int log_bs = c->log_n - c->h;
     if (c->fb) {
         LAUNCH(c, ln.st, "msm_sort", k_fb_sort, (uint32_t)c->nb, (uint32_t)c->bs, 0, src, c->log_n, log_bs, from_y,
-               ln.entries, ln.counts);
+               ln.entries, ln.counts, ln.bstart);
         LAUNCH(c, ln.st, "msm_accum", k_fb_accum, (uint32_t)(c->nb * lines), FB_LANES, 0, c->fbtab[coset],
-               ln.entries, ln.counts, ln.heap, c->log_n, c->h, lines);
+               ln.entries, ln.counts, ln.bstart, ln.bkt, ln.heap, c->log_n, c->h, lines);
         return ZGR_OK;
     }
     LAUNCH(c, ln.st, "msm_digits", k_msm_digits, blocks(c->n, 256), 256, 0, src, c->log_n, log_bs, ln.digits,
@@ -615,6 +617,10 @@ int zgr_ctx_create(int device, int log_col, int log_row, int coset_n, zgr_ctx** 
             DALLOC(l.entries, c->n * MSM_W * 4);
             DALLOC(l.counts, c->nb * MSM_W * 4);
             DALLOC(l.heap, (size_t)c->nlines * MSM_W * 2 * c->nb * X29_WORDS * 4);
+            if (c->fb) {
+                DALLOC(l.bstart, c->nb * FB_BSTART * 4);
+                DALLOC(l.bkt, (size_t)c->nlines * c->nb * FB_BUCKETS * X29_WORDS * 4);
+            }
         }
         // lane 0's point buffer also serves multiple_multiexp (nlines * nb points)
         size_t opts = out_slots(c) * coset_n;
@@ -679,7 +685,7 @@ void zgr_ctx_destroy(zgr_ctx* c) {
         if (c->fbtab[i]) (void)hipFree(c->fbtab[i]);
     }
     for (auto& l : c->lanes) {
-        void* lp[] = {l.y, l.digits, l.entries, l.counts, l.heap};
+        void* lp[] = {l.y, l.digits, l.entries, l.counts, l.heap, l.bstart, l.bkt};
         for (void* p : lp)
             if (p) (void)hipFree(p);
     }

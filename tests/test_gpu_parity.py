@@ -60,7 +60,7 @@ def test_ntt_matches_oracle(ctx, log_n, inverse):
         assert np.array_equal(got[r].reshape(-1), exp.reshape(-1))
 
 
-@pytest.mark.parametrize("kind", ["random", "zeros", "small", "sparse"])
+@pytest.mark.parametrize("kind", ["random", "zeros", "small", "sparse", "equal", "negative"])
 def test_multiple_multiexp_matches_oracle(ctx, params, kind):
     rng = np.random.default_rng(5)
     if kind == "random":
@@ -69,6 +69,13 @@ def test_multiple_multiexp_matches_oracle(ctx, params, kind):
         ri = np.zeros((GEO.n, 4), dtype=np.uint64)
     elif kind == "small":
         ri = O.fr_to_mont_array([int(x) for x in rng.integers(0, 300, GEO.n)])
+    elif kind == "equal":
+        # one scalar everywhere: every chunk's digits land in at most 22 buckets, each
+        # far larger than a lane's even share of the entries
+        ri = O.fr_to_mont_array([int.from_bytes(rng.bytes(32), "little") % O.R_MOD] * GEO.n)
+    elif kind == "negative":
+        # -k for small k: top digits at their extremes with carries through every window
+        ri = O.fr_to_mont_array([O.R_MOD - int(x) for x in rng.integers(1, 5000, GEO.n)])
     else:
         vals = [0] * GEO.n
         for i in rng.choice(GEO.n, 40, replace=False):
