@@ -56,7 +56,7 @@ __device__ __forceinline__ x29 x29_dbl(const x29& p) {
     fq29 M = q_add(q_dbl(X2), X2);
     x29 r;
     r.X = q_sub(q_sqr(M), q_dbl(S));
-    r.Y = q_sub(q_mul(M, q_sub_lazy(S, r.X)), q_mul(W, p.Y));
+    r.Y = q_mul2(M, q_sub_lazy(S, r.X), W, q_sub_lazy(q_zero(), p.Y));  // M (S - X3) - W Y
     r.ZZ = q_mul(V, p.ZZ);
     r.ZZZ = q_mul(W, p.ZZZ);
     return r;
@@ -68,7 +68,8 @@ __device__ __forceinline__ x29 x29_add(const x29& p, const x29& q) {
     fq29 P = q_sub_lazy(q_mul(q.X, p.ZZ), U1);  // < 4p, feeds multiplications only
     fq29 S1 = q_mul(p.Y, q.ZZZ);
     fq29 R = q_sub_lazy(q_mul(q.Y, p.ZZZ), S1);
-    if (!pinf && !qinf && q_is_zero4(P) && q_is_zero4(R)) return x29_dbl(p);
+    // equal x: p == q (double) or p == -q (infinity); one test on the common path
+    if (!pinf && !qinf && q_is_zero4(P)) return q_is_zero4(R) ? x29_dbl(p) : x29_inf();
     x29 r;
     // Z coordinates first so the inputs' ZZ / ZZZ die early
     fq29 PP = q_sqr(P);
@@ -77,7 +78,7 @@ __device__ __forceinline__ x29 x29_add(const x29& p, const x29& q) {
     r.ZZZ = q_mul(q_mul(p.ZZZ, q.ZZZ), PPP);
     fq29 Q = q_mul(U1, PP);
     r.X = q_sub(q_sub(q_sqr(R), PPP), q_dbl(Q));
-    r.Y = q_sub(q_mul(R, q_sub_lazy(Q, r.X)), q_mul(S1, PPP));
+    r.Y = q_mul2(R, q_sub_lazy(Q, r.X), q_sub_lazy(q_zero(), S1), PPP);  // R (Q - X3) - S1 PPP
     r = x29_select(pinf, q, r);
     r = x29_select(qinf, p, r);
     return r;
@@ -89,7 +90,8 @@ __device__ __forceinline__ x29 x29_madd(const x29& p, const a29& q, bool live = 
     const bool pinf = x29_is_inf(p);
     fq29 P = q_sub_lazy(q_mul(q.x, p.ZZ), p.X);
     fq29 R = q_sub_lazy(q_mul(q.y, p.ZZZ), p.Y);
-    if (live && !pinf && q_is_zero4(P) && q_is_zero4(R)) return x29_dbl(p);
+    // equal x: p == q (double) or p == -q (infinity); one test on the common path
+    if (live && !pinf && q_is_zero4(P)) return q_is_zero4(R) ? x29_dbl(p) : x29_inf();
     x29 r;
     fq29 PP = q_sqr(P);
     r.ZZ = q_mul(p.ZZ, PP);
@@ -97,7 +99,7 @@ __device__ __forceinline__ x29 x29_madd(const x29& p, const a29& q, bool live = 
     r.ZZZ = q_mul(p.ZZZ, PPP);
     fq29 Q = q_mul(p.X, PP);
     r.X = q_sub(q_sub(q_sqr(R), PPP), q_dbl(Q));
-    r.Y = q_sub(q_mul(R, q_sub_lazy(Q, r.X)), q_mul(p.Y, PPP));
+    r.Y = q_mul2(R, q_sub_lazy(Q, r.X), q_sub_lazy(q_zero(), p.Y), PPP);  // R (Q - X3) - Y1 PPP
     return x29_select(pinf, x29_from_aff(q), r);
 }
 

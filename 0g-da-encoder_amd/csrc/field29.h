@@ -81,9 +81,65 @@ __device__ __forceinline__ fq29 q_mul(const fq29& a, const fq29& b) {
     return r;
 }
 
-// (a dedicated squaring with doubled cross products raised register pressure in the
-// bucket kernel and measured no faster on gfx950)
-__device__ __forceinline__ fq29 q_sqr(const fq29& a) { return q_mul(a, a); }
+// a^2 / 2^261: 45 products (cross terms with the doubled operand 2 a_i < 2^30, at most
+// 4 of them plus one square per column before the reduction)
+__device__ __forceinline__ fq29 q_sqr(const fq29& a) {
+    uint64_t T[18];
+#pragma unroll
+    for (int k = 0; k < 18; k++) T[k] = 0;
+#pragma unroll
+    for (int i = 0; i < 9; i++) {
+        T[2 * i] += (uint64_t)a.l[i] * a.l[i];
+        const uint32_t d = a.l[i] << 1;
+#pragma unroll
+        for (int j = i + 1; j < 9; j++) T[i + j] += (uint64_t)d * a.l[j];
+    }
+#pragma unroll
+    for (int i = 0; i < 9; i++) {
+        uint32_t m = ((uint32_t)T[i] * Q29::INV) & M29;
+#pragma unroll
+        for (int j = 0; j < 9; j++) T[i + j] += (uint64_t)m * Q29::P[j];
+        T[i + 1] += T[i] >> 29;
+    }
+    fq29 r;
+    uint64_t c = 0;
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+        uint64_t x = T[9 + k] + c;
+        r.l[k] = (uint32_t)x & M29;
+        c = x >> 29;
+    }
+    return r;
+}
+
+// (a * b + c * d) / 2^261 with one Montgomery reduction, for a, b, c, d in [0, 4p):
+// the sum is below 32 p^2, so the result is below 32 p^2 / 2^261 + p < 1.25 p; a
+// column holds at most 27 products below 2^58 (< 2^63).
+__device__ __forceinline__ fq29 q_mul2(const fq29& a, const fq29& b, const fq29& c, const fq29& d) {
+    uint64_t T[18];
+#pragma unroll
+    for (int k = 0; k < 18; k++) T[k] = 0;
+#pragma unroll
+    for (int i = 0; i < 9; i++)
+#pragma unroll
+        for (int j = 0; j < 9; j++) T[i + j] += (uint64_t)a.l[i] * b.l[j] + (uint64_t)c.l[i] * d.l[j];
+#pragma unroll
+    for (int i = 0; i < 9; i++) {
+        uint32_t m = ((uint32_t)T[i] * Q29::INV) & M29;
+#pragma unroll
+        for (int j = 0; j < 9; j++) T[i + j] += (uint64_t)m * Q29::P[j];
+        T[i + 1] += T[i] >> 29;
+    }
+    fq29 r;
+    uint64_t cy = 0;
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+        uint64_t x = T[9 + k] + cy;
+        r.l[k] = (uint32_t)x & M29;
+        cy = x >> 29;
+    }
+    return r;
+}
 
 // s in [0, 4p) with normalised limbs -> [0, 2p)
 __device__ __forceinline__ fq29 q_reduce_2p(const fq29& s) {

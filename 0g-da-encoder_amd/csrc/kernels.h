@@ -585,12 +585,15 @@ __global__ __launch_bounds__(1024) void k_fb_sort(const uint32_t* y, int log_n, 
 // lane: phase 1 never adds two projective points, phase 2 never touches a base.
 constexpr int FB_PER_LANE = FB_BUCKETS / FB_LANES;  // buckets per lane in phase 2 (32)
 constexpr uint32_t FB_NO_SHARE = 0xffffffffu;        // lane with no entries
+// XYZZ slots per (line, chunk) in `bkt`: the buckets, then each lane's first- and
+// last-bucket partials (kept in global memory so that LDS does not bound occupancy)
+constexpr int FB_REGION = FB_BUCKETS + 2 * FB_LANES;
 
-__global__ __launch_bounds__(FB_LANES, 2) void k_fb_accum(const uint32_t* table, const uint32_t* entries,
-                                                          const uint32_t* counts, const uint32_t* bstart,
-                                                          uint32_t* bkt, uint32_t* heap, int log_n, int h,
-                                                          int lines) {
-    __shared__ uint32_t red[2 * FB_LANES * X29_WORDS];  // phase 1: first / last partials; then the reduction
+// The two phases are separate kernels (k_fb_sums, k_fb_reduce) so that the waves on a
+// CU run one loop body at a time: the two bodies together overflow the instruction cache.
+__global__ __launch_bounds__(FB_LANES, 2) void k_fb_sums(const uint32_t* table, const uint32_t* entries,
+                                                         const uint32_t* counts, uint32_t* bkt, int log_n, int h,
+                                                         int lines) {
     __shared__ uint32_t head_k[FB_LANES];               // bucket continued from an earlier lane, 0 if none
     __shared__ uint32_t one_bucket[FB_LANES];           // the share lies inside one bucket
     const uint32_t nb = 1u << h;
@@ -598,20 +601,15 @@ __global__ __launch_bounds__(FB_LANES, 2) void k_fb_accum(const uint32_t* table,
     const uint32_t logical = xcd_remap(blockIdx.x, gridDim.x);
     const uint32_t b = logical / lines, L = logical % lines;
     const int s = threadIdx.x;
-    uint32_t* leaf = heap + ((size_t)L * 2 * nb + nb + b) * X29_WORDS;
     const int cnt = (int)counts[b];
-    if (cnt == 0) {  // zero chunk
-        if (s == 0) x29_store(leaf, x29_inf());
-        return;
-    }
+    if (cnt == 0) return;  // zero chunk: k_fb_reduce writes infinity
     const uint32_t* ent = entries + (((size_t)b * FB_W) << log_bs);
     const uint32_t* tab = table + (((((size_t)L << h) + b) * FB_W) << log_bs) * 16;
-    const uint32_t* bst = bstart + (size_t)b * FB_BSTART;
-    uint32_t* bk = bkt + ((size_t)L * nb + b) * FB_BUCKETS * X29_WORDS;  // slot m - 1 for bucket m
+    uint32_t* bk = bkt + ((size_t)L * nb + b) * FB_REGION * X29_WORDS;  // slot m - 1 for bucket m
 
     // phase 1: even share [begin, end) of the sorted entries
-    uint32_t* first_p = red;                      // partial of a continued first bucket
-    uint32_t* last_p = red + FB_LANES * X29_WORDS;  // partial of the share's last bucket
+    uint32_t* first_p = bk + FB_BUCKETS * X29_WORDS;  // partial of a continued first bucket
+    uint32_t* last_p = first_p + FB_LANES * X29_WORDS;  // partial of the share's last bucket
     const int begin = (s * cnt) / FB_LANES, end = ((s + 1) * cnt) / FB_LANES;
     const int steps = end - begin;
     const uint32_t k_first = steps > 0 ? ent[begin] >> 16 : 0u;
@@ -659,7 +657,22 @@ __global__ __launch_bounds__(FB_LANES, 2) void k_fb_accum(const uint32_t* table,
         }
         x29_store(bk + (size_t)(k_last - 1) * X29_WORDS, W);
     }
-    __syncthreads();  // phase-1 bucket sums visible to every lane
+}
+
+__global__ __launch_bounds__(FB_LANES, 2) void k_fb_reduce(const uint32_t* counts, const uint32_t* bstart,
+                                                           const uint32_t* bkt, uint32_t* heap, int h, int lines) {
+    __shared__ uint32_t red[FB_LANES * X29_WORDS];
+    const uint32_t nb = 1u << h;
+    const uint32_t logical = xcd_remap(blockIdx.x, gridDim.x);
+    const uint32_t b = logical / lines, L = logical % lines;
+    const int s = threadIdx.x;
+    uint32_t* leaf = heap + ((size_t)L * 2 * nb + nb + b) * X29_WORDS;
+    if (counts[b] == 0) {  // zero chunk
+        if (s == 0) x29_store(leaf, x29_inf());
+        return;
+    }
+    const uint32_t* bst = bstart + (size_t)b * FB_BSTART;
+    const uint32_t* bk = bkt + ((size_t)L * nb + b) * FB_REGION * X29_WORDS;
 
     // phase 2: buckets (32 s, 32 s + 32], top down
     x29 Rr = x29_inf(), T = x29_inf();

@@ -228,8 +228,10 @@ int msm_lines(zgr_ctx* c, Lane& ln, int coset, const uint32_t* src, int from_y, 
     if (c->fb) {
         LAUNCH(c, ln.st, "msm_sort", k_fb_sort, (uint32_t)c->nb, (uint32_t)c->bs, 0, src, c->log_n, log_bs, from_y,
                ln.entries, ln.counts, ln.bstart);
-        LAUNCH(c, ln.st, "msm_accum", k_fb_accum, (uint32_t)(c->nb * lines), FB_LANES, 0, c->fbtab[coset],
-               ln.entries, ln.counts, ln.bstart, ln.bkt, ln.heap, c->log_n, c->h, lines);
+        LAUNCH(c, ln.st, "msm_accum", k_fb_sums, (uint32_t)(c->nb * lines), FB_LANES, 0, c->fbtab[coset],
+               ln.entries, ln.counts, ln.bkt, c->log_n, c->h, lines);
+        LAUNCH(c, ln.st, "msm_reduce", k_fb_reduce, (uint32_t)(c->nb * lines), FB_LANES, 0, ln.counts, ln.bstart,
+               ln.bkt, ln.heap, c->h, lines);
         return ZGR_OK;
     }
     LAUNCH(c, ln.st, "msm_digits", k_msm_digits, blocks(c->n, 256), 256, 0, src, c->log_n, log_bs, ln.digits,
@@ -619,7 +621,7 @@ int zgr_ctx_create(int device, int log_col, int log_row, int coset_n, zgr_ctx** 
             DALLOC(l.heap, (size_t)c->nlines * MSM_W * 2 * c->nb * X29_WORDS * 4);
             if (c->fb) {
                 DALLOC(l.bstart, c->nb * FB_BSTART * 4);
-                DALLOC(l.bkt, (size_t)c->nlines * c->nb * FB_BUCKETS * X29_WORDS * 4);
+                DALLOC(l.bkt, (size_t)c->nlines * c->nb * FB_REGION * X29_WORDS * 4);
             }
         }
         // lane 0's point buffer also serves multiple_multiexp (nlines * nb points)
