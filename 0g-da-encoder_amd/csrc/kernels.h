@@ -18,6 +18,17 @@ constexpr int MSM_W = 32;         // windows: 254-bit scalars with signed 8-bit 
 constexpr int ACC_THREADS = 256;  // 32 windows x 8 lanes
 constexpr int LANES_PER_WIN = 8;
 
+// Fixed-base MSM: at most 2^FB_GL_MAX chunks share one bucket set.
+constexpr int FB_GL_MAX = 2;  // (groups of 8 measured slower: their waves outlast the rest)
+
+// log2 of the chunks that share one bucket set on line L of the AMT outputs: quotient
+// line L < h is read only at tree level L + 1 (sums over groups of 2^(h-1-L) chunks),
+// the basis line h at every level down to the chunks, the high line h + 1 at the root.
+__host__ __device__ inline int fb_group_log(int L, int h) {
+    int g = L < h ? h - 1 - L : (L == h ? 0 : h);
+    return g < FB_GL_MAX ? g : FB_GL_MAX;
+}
+
 __device__ __forceinline__ uint32_t bitrev_bits(uint32_t x, int bits) {
     return bits ? (__builtin_bitreverse32(x) >> (32 - bits)) : 0u;
 }
@@ -370,12 +381,16 @@ __global__ __launch_bounds__(ACC_THREADS, 2) void k_msm_accum(const uint32_t* ba
 }
 
 // One level of the per-(line, window) binary trees over the chunks:
-// heap[i] = heap[2i] + heap[2i+1] for i in [2^d, 2^(d+1)).
-__global__ __launch_bounds__(64) void k_msm_tree_level(uint32_t* heap, int h, int d, uint32_t total) {
+// heap[i] = heap[2i] + heap[2i+1] for i in [2^d, 2^(d+1)).  plan: 2 bits per line, the
+// log2 chunk-group size gl of its bucket sets, which already produced its level h - gl
+// (left as it is); 0 = every line from the chunks.
+__global__ __launch_bounds__(64) void k_msm_tree_level(uint32_t* heap, int h, int d, uint32_t total, int nw,
+                                                       uint64_t plan) {
     uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= total) return;
     const uint32_t nb = 1u << h;
     uint32_t lw = t >> d;
+    if (d >= h - (int)((plan >> (2 * (lw / (uint32_t)nw))) & 3u)) return;
     uint32_t i = (1u << d) + (t & ((1u << d) - 1));
     uint32_t* base = heap + (size_t)lw * 2 * nb * X29_WORDS;
     x29 a = x29_load(base + (size_t)(2 * i) * X29_WORDS);
@@ -455,8 +470,44 @@ __global__ __launch_bounds__(64) void k_msm_root_out(const uint32_t* heap, int h
 constexpr int FB_C = 12;
 constexpr int FB_W = 22;  // ceil(255 / 12): the top digit (bits 252..253 + carry) never carries out
 constexpr int FB_BUCKETS = 1 << (FB_C - 1);  // |digit| in 1..2048
-constexpr int FB_LANES = 64;                 // one wave per (line, chunk)
-constexpr int FB_MAX_LOG_BS = 10;            // entry index j * bs + k in 15 bits
+constexpr int FB_LANES = 64;                 // one wave per (line, chunk group)
+constexpr int FB_MAX_LOG_BS = 10;
+// Sorted entry: point index within the chunk group's table region (g * FB_W + j) * bs + k
+// (< 4 * 22 * 1024 < 2^18) | sign << 18 | |digit| << 19.
+constexpr int FB_SIGN_SHIFT = 18;
+constexpr int FB_MAG_SHIFT = 19;
+constexpr uint32_t FB_IDX_MASK = (1u << FB_SIGN_SHIFT) - 1;
+constexpr int FB_BSTART = FB_BUCKETS + 2;  // bucket starts 1..FB_BUCKETS, then the count
+
+// per-level sorted entries: level gl groups 2^gl consecutive chunks
+struct FbLevels {
+    const uint32_t* ent[FB_GL_MAX + 1];  // (nb >> gl) groups x 2^gl * FB_W * bs entries
+    const uint32_t* cnt[FB_GL_MAX + 1];  // entries per group
+    const uint32_t* bst[FB_GL_MAX + 1];  // bucket starts per group (FB_BSTART each)
+};
+
+// Wave id: without a work list every wave has the same work, and consecutive ids (the
+// lines of one chunk, which read the same sorted entries) are kept on one XCD; a work
+// list is sorted longest first, so its entries are dealt round-robin to the XCDs as
+// dispatched, giving each XCD the same mix of wave lengths.
+__device__ __forceinline__ uint32_t fb_wave_id(const uint32_t* work) {
+    return work ? blockIdx.x : xcd_remap(blockIdx.x, gridDim.x);
+}
+
+// wave -> (line, level, group): from the work list, or (no list) every line at level 0
+__device__ __forceinline__ void fb_work(const uint32_t* work, uint32_t wid, int lines, uint32_t* L, uint32_t* gl,
+                                        uint32_t* S) {
+    if (work) {
+        const uint32_t u = work[wid];
+        *L = u & 0xffu;
+        *gl = (u >> 8) & 3u;
+        *S = u >> 10;
+    } else {
+        *L = wid % (uint32_t)lines;
+        *gl = 0;
+        *S = wid / (uint32_t)lines;
+    }
+}
 
 __device__ __forceinline__ void fb_store_coord(uint32_t* d, const fq& v) {
     uint32_t w[8];
@@ -498,24 +549,10 @@ __global__ __launch_bounds__(64) void k_fb_build(const uint32_t* bases, uint32_t
     }
 }
 
-// Signed 12-bit digits of the chunk's scalars (ri[b * bs + k] = y[bitrev(b * bs + k)]
-// with from_y, else scalars in order), counting-sorted by |digit| into one entry list
-// per chunk: entries j * bs + k | sign << 15 | |digit| << 16, ascending |digit|.
-// bstart[b * FB_BSTART + m] = first entry of bucket m (m = 1..FB_BUCKETS), and
-// bstart[b * FB_BSTART + FB_BUCKETS + 1] = count.
-// One thread per scalar (blockDim == bs <= 1024).
-constexpr int FB_BSTART = FB_BUCKETS + 2;
-
-__global__ __launch_bounds__(1024) void k_fb_sort(const uint32_t* y, int log_n, int log_bs, int from_y,
-                                                  uint32_t* entries, uint32_t* counts, uint32_t* bstart) {
-    __shared__ uint32_t hist[FB_BUCKETS + 1];  // by |digit|; [0] unused
-    __shared__ uint32_t wsum[16];
-    const uint32_t b = blockIdx.x, k = threadIdx.x, T = blockDim.x;
-    for (uint32_t q = k; q <= FB_BUCKETS; q += T) hist[q] = 0;
-    __syncthreads();
-    const uint32_t i = (b << log_bs) + k;
+// Signed 12-bit digits of scalar i (ri[i] = y[bitrev(i)] with from_y, else y[i]),
+// converted out of Montgomery form: s = sum_w d_w 2^(12 w), d_w in [-2047, 2048].
+__device__ __forceinline__ void fb_digits(const uint32_t* y, int log_n, int from_y, uint32_t i, int dig[FB_W]) {
     fr s = f_from_mont(f_load<FrP>(y + 8 * (size_t)(from_y ? bitrev_bits(i, log_n) : i)));
-    int dig[FB_W];
     int carry = 0;
 #pragma unroll
     for (int w = 0; w < FB_W; w++) {
@@ -524,10 +561,31 @@ __global__ __launch_bounds__(1024) void k_fb_sort(const uint32_t* y, int log_n, 
         if (sh > 32 - FB_C && wi + 1 < 8) v |= s.v[wi + 1] << (32 - sh);
         int d = (int)(v & ((1u << FB_C) - 1)) + carry;  // [0, 4096]
         carry = d > FB_BUCKETS;
-        d -= carry << FB_C;  // [-2047, 2048]
-        dig[w] = d;
-        if (d) atomicAdd(&hist[d < 0 ? -d : d], 1u);
+        dig[w] = d - (carry << FB_C);
     }
+}
+
+// Counting sort of the digits by |digit|, in two passes over each chunk b (one thread
+// per scalar, blockDim == bs <= 1024):
+//   k_fb_count    bucket sizes -> bstart[b * FB_BSTART + m] = first entry of bucket m
+//                 (m = 1..FB_BUCKETS), [.. + FB_BUCKETS + 1] = count, counts[b] = count;
+//   k_fb_scatter  entries (point index j * bs + k, sign, |digit|) at level 0 (the
+//                 chunk's list, ascending |digit|) and at every chunk-group level
+//                 gl <= gl_used: group S = b >> gl lists bucket m of its chunks one after
+//                 the other (chunk g's indices offset by g * FB_W * bs, its table region),
+//                 so the group's bucket start is the sum of its chunks' bucket starts.
+__global__ __launch_bounds__(1024) void k_fb_count(const uint32_t* y, int log_n, int log_bs, int from_y,
+                                                   uint32_t* counts, uint32_t* bstart) {
+    __shared__ uint32_t hist[FB_BUCKETS + 1];  // by |digit|; [0] unused
+    __shared__ uint32_t wsum[16];
+    const uint32_t b = blockIdx.x, k = threadIdx.x, T = blockDim.x;
+    for (uint32_t q = k; q <= FB_BUCKETS; q += T) hist[q] = 0;
+    __syncthreads();
+    int dig[FB_W];
+    fb_digits(y, log_n, from_y, (b << log_bs) + k, dig);
+#pragma unroll
+    for (int w = 0; w < FB_W; w++)
+        if (dig[w]) atomicAdd(&hist[dig[w] < 0 ? -dig[w] : dig[w]], 1u);
     __syncthreads();
     // exclusive prefix over the buckets: each thread owns FB_BUCKETS / T consecutive bins
     const uint32_t per = FB_BUCKETS / T;
@@ -555,24 +613,67 @@ __global__ __launch_bounds__(1024) void k_fb_sort(const uint32_t* y, int log_n, 
     __syncthreads();
     uint32_t run = wsum[k >> 6] + inc - loc;
     for (uint32_t q = 0; q < per; q++) {
-        uint32_t hc = hist[1 + k * per + q];
-        hist[1 + k * per + q] = run;
         bstart[(size_t)b * FB_BSTART + 1 + k * per + q] = run;
-        run += hc;
+        run += hist[1 + k * per + q];
+    }
+}
+
+// level-gl outputs of k_fb_scatter (level 0: ent only; cnt / bst come from k_fb_count)
+struct FbOut {
+    uint32_t* ent[FB_GL_MAX + 1];
+    uint32_t* cnt[FB_GL_MAX + 1];
+    uint32_t* bst[FB_GL_MAX + 1];
+};
+
+__global__ __launch_bounds__(1024) void k_fb_scatter(const uint32_t* y, int log_n, int log_bs, int from_y,
+                                                     const uint32_t* bstart, int gl_used, FbOut out) {
+    __shared__ uint32_t run[FB_BUCKETS + 1];                // next level-0 slot of bucket m
+    __shared__ uint32_t base[FB_GL_MAX][FB_BUCKETS + 1];    // level-gl slot - level-0 slot
+    const uint32_t b = blockIdx.x, k = threadIdx.x, T = blockDim.x;
+    const size_t cap = (size_t)FB_W << log_bs;  // entries (and table points) of one chunk
+    const uint32_t* b0 = bstart + (size_t)b * FB_BSTART;
+    for (uint32_t m = 1 + k; m <= FB_BUCKETS; m += T) {
+        run[m] = b0[m];
+        for (int gl = 1; gl <= gl_used; gl++) {
+            const uint32_t G = 1u << gl, S = b >> gl, g = b & (G - 1);
+            uint32_t start = 0, before = 0;  // group's bucket start; earlier chunks' part of it
+            for (uint32_t q = 0; q < G; q++) {
+                const uint32_t* bq = bstart + (size_t)(S * G + q) * FB_BSTART;
+                start += bq[m];
+                if (q < g) before += bq[m + 1] - bq[m];
+            }
+            base[gl - 1][m] = start + before - b0[m];
+            if (g == 0) {
+                out.bst[gl][(size_t)S * FB_BSTART + m] = start;
+                if (m == FB_BUCKETS) {
+                    uint32_t total = 0;
+                    for (uint32_t q = 0; q < G; q++) total += bstart[(size_t)(S * G + q) * FB_BSTART + FB_BUCKETS + 1];
+                    out.bst[gl][(size_t)S * FB_BSTART + FB_BUCKETS + 1] = total;
+                    out.cnt[gl][S] = total;
+                }
+            }
+        }
     }
     __syncthreads();
-    uint32_t* ent = entries + (((size_t)b * FB_W) << log_bs);
+    int dig[FB_W];
+    fb_digits(y, log_n, from_y, (b << log_bs) + k, dig);
 #pragma unroll
     for (int w = 0; w < FB_W; w++) {
         const int d = dig[w];
         if (!d) continue;
         const uint32_t mag = d < 0 ? -d : d;
-        const uint32_t pos = atomicAdd(&hist[mag], 1u);
-        ent[pos] = (((uint32_t)w << log_bs) | k) | ((uint32_t)(d < 0) << 15) | (mag << 16);
+        const uint32_t pos = atomicAdd(&run[mag], 1u);
+        const uint32_t e = (((uint32_t)w << log_bs) | k) | ((uint32_t)(d < 0) << FB_SIGN_SHIFT) | (mag << FB_MAG_SHIFT);
+        out.ent[0][(size_t)b * cap + pos] = e;
+        for (int gl = 1; gl <= gl_used; gl++) {
+            const uint32_t S = b >> gl, g = b & ((1u << gl) - 1);
+            // the index field absorbs g * cap: j * bs + k + g * cap < 2^gl * cap < 2^18
+            out.ent[gl][((size_t)S << gl) * cap + base[gl - 1][mag] + pos] = e + g * (uint32_t)cap;
+        }
     }
 }
 
-// Bucket sum of one (line L, chunk b) over its shifted copies, one wave, two phases:
+// Bucket sum of one (line L, chunk group) over its shifted copies, one wave, two phases:
 //  1. bucket sums: lane s takes an even share of the sorted entries and adds each
 //     bucket's points with mixed additions (R += +-P_e).  A bucket belongs to the lane
 //     holding its first entry, which writes the bucket sum B_m to `bkt` (the chunk's
@@ -580,7 +681,7 @@ __global__ __launch_bounds__(1024) void k_fb_sort(const uint32_t* y, int log_n, 
 //     a bucket hands that partial sum to the owner through LDS;
 //  2. bucket reduction: lane s folds buckets m in (32 s, 32 s + 32] from the top with
 //     the running sum (R += B_m; T += R), its share being T + 32 s R; the shares are
-//     summed through LDS into heap leaf nb + b of line L.
+//     summed through LDS into the group's heap node of line L.
 // Splitting the work this way keeps every step of a phase the same addition on every
 // lane: phase 1 never adds two projective points, phase 2 never touches a base.
 constexpr int FB_PER_LANE = FB_BUCKETS / FB_LANES;  // buckets per lane in phase 2 (32)
@@ -591,30 +692,32 @@ constexpr int FB_REGION = FB_BUCKETS + 2 * FB_LANES;
 
 // The two phases are separate kernels (k_fb_sums, k_fb_reduce) so that the waves on a
 // CU run one loop body at a time: the two bodies together overflow the instruction cache.
-__global__ __launch_bounds__(FB_LANES, 2) void k_fb_sums(const uint32_t* table, const uint32_t* entries,
-                                                         const uint32_t* counts, uint32_t* bkt, int log_n, int h,
-                                                         int lines) {
+// One wave per work item (line L, chunk group S of level gl): the group's 2^gl chunks
+// share one bucket set, whose result lands in heap node (nb >> gl) + S of line L.
+__global__ __launch_bounds__(FB_LANES, 2) void k_fb_sums(const uint32_t* table, FbLevels lv, const uint32_t* work,
+                                                         uint32_t* bkt, int log_n, int h, int lines) {
     __shared__ uint32_t head_k[FB_LANES];               // bucket continued from an earlier lane, 0 if none
     __shared__ uint32_t one_bucket[FB_LANES];           // the share lies inside one bucket
-    const uint32_t nb = 1u << h;
     const int log_bs = log_n - h;
-    const uint32_t logical = xcd_remap(blockIdx.x, gridDim.x);
-    const uint32_t b = logical / lines, L = logical % lines;
+    const uint32_t wid = fb_wave_id(work);
+    uint32_t L, gl, S;
+    fb_work(work, wid, lines, &L, &gl, &S);
     const int s = threadIdx.x;
-    const int cnt = (int)counts[b];
-    if (cnt == 0) return;  // zero chunk: k_fb_reduce writes infinity
-    const uint32_t* ent = entries + (((size_t)b * FB_W) << log_bs);
-    const uint32_t* tab = table + (((((size_t)L << h) + b) * FB_W) << log_bs) * 16;
-    uint32_t* bk = bkt + ((size_t)L * nb + b) * FB_REGION * X29_WORDS;  // slot m - 1 for bucket m
+    const int cnt = (int)lv.cnt[gl][S];
+    if (cnt == 0) return;  // zero group: k_fb_reduce writes infinity
+    const size_t cap = (size_t)FB_W << log_bs;  // entries (and table points) of one chunk
+    const uint32_t* ent = lv.ent[gl] + ((size_t)S << gl) * cap;
+    const uint32_t* tab = table + ((((size_t)L << h) + ((size_t)S << gl)) * cap) * 16;
+    uint32_t* bk = bkt + (size_t)wid * FB_REGION * X29_WORDS;  // slot m - 1 for bucket m
 
     // phase 1: even share [begin, end) of the sorted entries
     uint32_t* first_p = bk + FB_BUCKETS * X29_WORDS;  // partial of a continued first bucket
     uint32_t* last_p = first_p + FB_LANES * X29_WORDS;  // partial of the share's last bucket
     const int begin = (s * cnt) / FB_LANES, end = ((s + 1) * cnt) / FB_LANES;
     const int steps = end - begin;
-    const uint32_t k_first = steps > 0 ? ent[begin] >> 16 : 0u;
-    const uint32_t k_last = steps > 0 ? ent[end - 1] >> 16 : 0u;
-    const bool cont = steps > 0 && begin > 0 && (ent[begin - 1] >> 16) == k_first;
+    const uint32_t k_first = steps > 0 ? ent[begin] >> FB_MAG_SHIFT : 0u;
+    const uint32_t k_last = steps > 0 ? ent[end - 1] >> FB_MAG_SHIFT : 0u;
+    const bool cont = steps > 0 && begin > 0 && (ent[begin - 1] >> FB_MAG_SHIFT) == k_first;
     const int maxsteps = wave_max(steps);
     x29 R = x29_inf();
     uint32_t cur = 0;   // bucket of R (0: none yet)
@@ -622,12 +725,12 @@ __global__ __launch_bounds__(FB_LANES, 2) void k_fb_sums(const uint32_t* table, 
     for (int st = 0; st < maxsteps; st++) {
         const bool active = st < steps;
         const uint32_t ev = active ? ent[begin + st] : 0u;
-        const uint32_t m = ev >> 16;
-        const uint32_t* pp = tab + (size_t)(ev & 0x7fffu) * 16;
+        const uint32_t m = ev >> FB_MAG_SHIFT;
+        const uint32_t* pp = tab + (size_t)(ev & FB_IDX_MASK) * 16;
         a29 P;
         P.x = fb_load_coord(pp);
         P.y = fb_load_coord(pp + 8);
-        if (ev & 0x8000u) P.y = q_neg(P.y);
+        if ((ev >> FB_SIGN_SHIFT) & 1u) P.y = q_neg(P.y);
         const bool same = m == cur;
         x29 M = x29_madd(R, P, active);
         if (active && !same && cur != 0) {  // bucket cur is complete in this share
@@ -659,20 +762,21 @@ __global__ __launch_bounds__(FB_LANES, 2) void k_fb_sums(const uint32_t* table, 
     }
 }
 
-__global__ __launch_bounds__(FB_LANES, 2) void k_fb_reduce(const uint32_t* counts, const uint32_t* bstart,
-                                                           const uint32_t* bkt, uint32_t* heap, int h, int lines) {
+__global__ __launch_bounds__(FB_LANES, 2) void k_fb_reduce(FbLevels lv, const uint32_t* work, const uint32_t* bkt,
+                                                           uint32_t* heap, int h, int lines) {
     __shared__ uint32_t red[FB_LANES * X29_WORDS];
     const uint32_t nb = 1u << h;
-    const uint32_t logical = xcd_remap(blockIdx.x, gridDim.x);
-    const uint32_t b = logical / lines, L = logical % lines;
+    const uint32_t wid = fb_wave_id(work);
+    uint32_t L, gl, S;
+    fb_work(work, wid, lines, &L, &gl, &S);
     const int s = threadIdx.x;
-    uint32_t* leaf = heap + ((size_t)L * 2 * nb + nb + b) * X29_WORDS;
-    if (counts[b] == 0) {  // zero chunk
+    uint32_t* leaf = heap + ((size_t)L * 2 * nb + (nb >> gl) + S) * X29_WORDS;
+    if (lv.cnt[gl][S] == 0) {  // zero group
         if (s == 0) x29_store(leaf, x29_inf());
         return;
     }
-    const uint32_t* bst = bstart + (size_t)b * FB_BSTART;
-    const uint32_t* bk = bkt + ((size_t)L * nb + b) * FB_REGION * X29_WORDS;
+    const uint32_t* bst = lv.bst[gl] + (size_t)S * FB_BSTART;
+    const uint32_t* bk = bkt + (size_t)wid * FB_REGION * X29_WORDS;
 
     // phase 2: buckets (32 s, 32 s + 32], top down
     x29 Rr = x29_inf(), T = x29_inf();
@@ -685,9 +789,9 @@ __global__ __launch_bounds__(FB_LANES, 2) void k_fb_reduce(const uint32_t* count
         T = x29_add(T, Rr);
     }
     // T + 32 s Rr = (s Rr) doubled 5 times, plus T
-    x29 S = x29_mul_small_uniform(Rr, (uint32_t)s, 6);
-    for (int k = 0; k < 5; k++) S = x29_dbl(S);
-    x29 W = x29_add(T, S);
+    x29 off = x29_mul_small_uniform(Rr, (uint32_t)s, 6);
+    for (int k = 0; k < 5; k++) off = x29_dbl(off);
+    x29 W = x29_add(T, off);
 
     x29_store(red + s * X29_WORDS, W);
     __syncthreads();

@@ -23,6 +23,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <deque>
+#include <algorithm>
 #include <map>
 #include <mutex>
 #include <string>
@@ -99,7 +100,11 @@ struct Lane {
     uint32_t* counts = nullptr;   // 32 nb
     uint32_t* heap = nullptr;     // nlines * 32 * 2nb XYZZ (36 u32 each)
     uint32_t* bstart = nullptr;   // fixed-base: nb * FB_BSTART bucket starts
-    uint32_t* bkt = nullptr;      // fixed-base: nlines * nb * FB_BUCKETS bucket sums (XYZZ)
+    uint32_t* bkt = nullptr;      // fixed-base: nlines * nb * FB_REGION bucket sums (XYZZ)
+    // fixed-base chunk groups, levels 1..FB_GL_MAX (level 0: entries / counts / bstart)
+    uint32_t* gent[FB_GL_MAX + 1] = {};
+    uint32_t* gcnt[FB_GL_MAX + 1] = {};
+    uint32_t* gbst[FB_GL_MAX + 1] = {};
 };
 
 struct Slot {
@@ -143,6 +148,11 @@ struct zgr_ctx {
     // fixed-base tables (shifted copies of the bases); fb == false: windowed MSM
     bool fb = false;
     uint32_t* fbtab[3] = {nullptr, nullptr, nullptr};
+    // work lists of the grouped bucket kernels: [0] every line, [1] without the high line
+    uint32_t* work[2] = {nullptr, nullptr};
+    uint32_t nwork[2] = {0, 0};
+    uint64_t plan[2] = {0, 0};  // chunk-group level per line, 2 bits each
+    int gl_used = 0;            // deepest chunk-group level any line uses
 
     Lane lanes[2];
     Slot slots[2];
@@ -220,18 +230,39 @@ int ntt_stages(zgr_ctx* c, hipStream_t st, uint32_t* a, int k, bool dif, bool in
 // windows per heap tree set: 1 with the fixed-base tables, else MSM_W
 int heap_windows(const zgr_ctx* c) { return c->fb ? 1 : MSM_W; }
 
-// multi-line MSM of one coset on a lane: scalars `src` -> per-chunk sums in the lane
-// heap (one tree per line, or one per (line, window)), for the first `lines` lines
-// (quotients[0..h), basis, high_basis)
-int msm_lines(zgr_ctx* c, Lane& ln, int coset, const uint32_t* src, int from_y, int lines) {
+// multi-line MSM of one coset on a lane: scalars `src` -> chunk sums in the lane heap
+// (one tree per line, or one per (line, window)), for the first `lines` lines
+// (quotients[0..h), basis, high_basis).  grouped (fixed-base only, lines == nlines or
+// nlines - 1): each line gets only the chunk-group sums its AMT outputs read
+// (fb_group_log), at heap level h - fb_group_log(L, h); else every chunk of every line.
+int msm_lines(zgr_ctx* c, Lane& ln, int coset, const uint32_t* src, int from_y, int lines, bool grouped) {
     const int log_bs = c->log_n - c->h;
     if (c->fb) {
-        LAUNCH(c, ln.st, "msm_sort", k_fb_sort, (uint32_t)c->nb, (uint32_t)c->bs, 0, src, c->log_n, log_bs, from_y,
-               ln.entries, ln.counts, ln.bstart);
-        LAUNCH(c, ln.st, "msm_accum", k_fb_sums, (uint32_t)(c->nb * lines), FB_LANES, 0, c->fbtab[coset],
-               ln.entries, ln.counts, ln.bkt, c->log_n, c->h, lines);
-        LAUNCH(c, ln.st, "msm_reduce", k_fb_reduce, (uint32_t)(c->nb * lines), FB_LANES, 0, ln.counts, ln.bstart,
-               ln.bkt, ln.heap, c->h, lines);
+        const int gl_used = grouped ? c->gl_used : 0;
+        FbLevels lv{};
+        FbOut fo{};
+        lv.ent[0] = fo.ent[0] = ln.entries;
+        lv.cnt[0] = ln.counts;
+        lv.bst[0] = ln.bstart;
+        for (int gl = 1; gl <= gl_used; gl++) {
+            lv.ent[gl] = fo.ent[gl] = ln.gent[gl];
+            lv.cnt[gl] = fo.cnt[gl] = ln.gcnt[gl];
+            lv.bst[gl] = fo.bst[gl] = ln.gbst[gl];
+        }
+        LAUNCH(c, ln.st, "msm_sort", k_fb_count, (uint32_t)c->nb, (uint32_t)c->bs, 0, src, c->log_n, log_bs, from_y,
+               ln.counts, ln.bstart);
+        LAUNCH(c, ln.st, gl_used ? "msm_group_scatter" : "msm_scatter", k_fb_scatter, (uint32_t)c->nb,
+               (uint32_t)c->bs, 0, src, c->log_n, log_bs, from_y, ln.bstart, gl_used, fo);
+        const uint32_t* work = nullptr;
+        uint32_t waves = (uint32_t)(c->nb * lines);
+        if (grouped) {
+            const int v = lines == c->nlines ? 0 : 1;
+            work = c->work[v];
+            waves = c->nwork[v];
+        }
+        LAUNCH(c, ln.st, "msm_accum", k_fb_sums, waves, FB_LANES, 0, c->fbtab[coset], lv, work, ln.bkt, c->log_n,
+               c->h, lines);
+        LAUNCH(c, ln.st, "msm_reduce", k_fb_reduce, waves, FB_LANES, 0, lv, work, ln.bkt, ln.heap, c->h, lines);
         return ZGR_OK;
     }
     LAUNCH(c, ln.st, "msm_digits", k_msm_digits, blocks(c->n, 256), 256, 0, src, c->log_n, log_bs, ln.digits,
@@ -247,14 +278,47 @@ int msm_lines(zgr_ctx* c, Lane& ln, int coset, const uint32_t* src, int from_y, 
 // trees over the chunks (+ Horner over the windows): AMT outputs of one coset into
 // `out` (out_slots points).  With lines == h + 1 the high-basis line was not computed
 // and its output slot (the last one) is left for the caller to fill.
-int amt_outputs(zgr_ctx* c, Lane& ln, uint32_t* out, int lines) {
+int amt_outputs(zgr_ctx* c, Lane& ln, uint32_t* out, int lines, bool grouped) {
     const int nw = heap_windows(c);
+    const uint64_t plan = grouped ? c->plan[lines == c->nlines ? 0 : 1] : 0;
     for (int d = c->h - 1; d >= 0; d--) {
         uint32_t total = (uint32_t)(lines * nw) << d;
-        LAUNCH(c, ln.st, "msm_tree", k_msm_tree_level, blocks(total, 64), 64, 0, ln.heap, c->h, d, total);
+        LAUNCH(c, ln.st, "msm_tree", k_msm_tree_level, blocks(total, 64), 64, 0, ln.heap, c->h, d, total, nw, plan);
     }
     uint32_t slots = (uint32_t)out_slots(c) - (lines == c->nlines ? 0u : 1u);
     LAUNCH(c, ln.st, "msm_horner", k_msm_horner, blocks(slots, 64), 64, 0, ln.heap, c->h, nw, out, slots);
+    return ZGR_OK;
+}
+
+// Work lists of the grouped bucket kernels (one u32 per wave: L | gl << 8 | S << 10),
+// deepest levels first so that the longest waves start first; within a level the lines
+// of one chunk group are adjacent (they read the same sorted entries).
+// Every line sits at the largest level its outputs allow (fb_group_log): on MI355X this
+// measured faster than plans that keep some lines at smaller groups to shorten the
+// longest waves.  ZGR_GROUP_PLAN=none keeps every line at level 0 (one bucket set per
+// chunk, as multiple_multiexp needs).
+int build_work_lists(zgr_ctx* c) {
+    const char* mode = getenv("ZGR_GROUP_PLAN");
+    const bool none = mode && !strcmp(mode, "none");
+    for (int v = 0; v < 2; v++) {
+        const int lines = c->nlines - v;
+        std::vector<int> gl(lines, 0);
+        if (!none)
+            for (int L = 0; L < lines; L++) gl[L] = fb_group_log(L, c->h);
+        std::vector<uint32_t> w;
+        c->plan[v] = 0;
+        for (int L = 0; L < lines; L++) {
+            c->plan[v] |= (uint64_t)gl[L] << (2 * L);
+            c->gl_used = std::max(c->gl_used, gl[L]);
+        }
+        for (int g = FB_GL_MAX; g >= 0; g--)
+            for (uint32_t S = 0; S < (uint32_t)(c->nb >> g); S++)
+                for (int L = 0; L < lines; L++)
+                    if (gl[L] == g) w.push_back((uint32_t)L | (uint32_t)g << 8 | S << 10);
+        DALLOC(c->work[v], w.size() * 4);
+        HIPCHK(hipMemcpy(c->work[v], w.data(), w.size() * 4, hipMemcpyHostToDevice));
+        c->nwork[v] = (uint32_t)w.size();
+    }
     return ZGR_OK;
 }
 
@@ -313,8 +377,8 @@ int issue_blob(zgr_ctx* c, Slot& s, const uint8_t* raw_dev, size_t len, const zg
         // coset (coset c's evaluations interpolate f(cf_c X)), so the high-basis line
         // runs for coset 0 only and finish_blob copies its result to the others
         const int lines = cs == 0 ? c->nlines : c->nlines - 1;
-        CHECK(msm_lines(c, ln, cs, y, 1, lines));
-        CHECK(amt_outputs(c, ln, s.outpts + (size_t)cs * slots * 16, lines));
+        CHECK(msm_lines(c, ln, cs, y, 1, lines, c->fb));
+        CHECK(amt_outputs(c, ln, s.outpts + (size_t)cs * slots * 16, lines, c->fb));
         LAUNCH(c, ln.st, "merkle_leaves", k_merkle_leaves, blocks(n / 8, 128), 128, 0, y, c->log_col, c->log_row,
                cs, s.data, s.mlayers[0]);
         HIPCHK(hipEventRecord(s.ev_coset[cs], ln.st));
@@ -622,8 +686,14 @@ int zgr_ctx_create(int device, int log_col, int log_row, int coset_n, zgr_ctx** 
             if (c->fb) {
                 DALLOC(l.bstart, c->nb * FB_BSTART * 4);
                 DALLOC(l.bkt, (size_t)c->nlines * c->nb * FB_REGION * X29_WORDS * 4);
+                for (int gl = 1; gl <= FB_GL_MAX && gl <= c->h; gl++) {
+                    DALLOC(l.gent[gl], c->n * FB_W * 4);
+                    DALLOC(l.gcnt[gl], (c->nb >> gl) * 4);
+                    DALLOC(l.gbst[gl], (c->nb >> gl) * FB_BSTART * 4);
+                }
             }
         }
+        if (c->fb) CHECK(build_work_lists(c));
         // lane 0's point buffer also serves multiple_multiexp (nlines * nb points)
         size_t opts = out_slots(c) * coset_n;
         if (opts < (size_t)c->nlines * c->nb) opts = (size_t)c->nlines * c->nb;
@@ -678,7 +748,7 @@ void zgr_ctx_destroy(zgr_ctx* c) {
         if (l.st) (void)hipStreamSynchronize(l.st);
     for (auto& s : c->slots)
         if (s.st) (void)hipStreamSynchronize(s.st);
-    void* ptrs[] = {c->consts, c->tw, c->tw_inv, c->flag};
+    void* ptrs[] = {c->consts, c->tw, c->tw_inv, c->flag, c->work[0], c->work[1]};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     for (int i = 0; i < 3; i++) {
@@ -690,6 +760,9 @@ void zgr_ctx_destroy(zgr_ctx* c) {
         void* lp[] = {l.y, l.digits, l.entries, l.counts, l.heap, l.bstart, l.bkt};
         for (void* p : lp)
             if (p) (void)hipFree(p);
+        for (int gl = 0; gl <= FB_GL_MAX; gl++)
+            for (void* p : {(void*)l.gent[gl], (void*)l.gcnt[gl], (void*)l.gbst[gl]})
+                if (p) (void)hipFree(p);
     }
     for (auto& s : c->slots) {
         void* sp[] = {s.raw, s.pts, s.coeff, s.outpts, s.data, s.root};
@@ -779,7 +852,7 @@ int zgr_multiple_multiexp(zgr_ctx* c, int coset, const uint8_t* ri_mont, size_t 
     Lane& ln = c->lanes[0];
     uint32_t* opts = c->slots[0].outpts;
     HIPCHK(hipMemcpyAsync(ln.y, ri_mont, n * 32, hipMemcpyHostToDevice, ln.st));
-    CHECK(msm_lines(c, ln, coset, ln.y, 0, c->nlines));
+    CHECK(msm_lines(c, ln, coset, ln.y, 0, c->nlines, false));  // every chunk of every line
     uint32_t total = (uint32_t)(c->nlines * c->nb);
     LAUNCH(c, ln.st, "msm_leaves_out", k_msm_leaves_out, blocks(total, 64), 64, 0, ln.heap, c->h, heap_windows(c),
            c->nlines, opts);
@@ -797,8 +870,8 @@ int zgr_amt_prove(zgr_ctx* c, int coset, const uint8_t* ri_mont, size_t n, uint8
     Lane& ln = c->lanes[0];
     uint32_t* opts = c->slots[0].outpts;
     HIPCHK(hipMemcpyAsync(ln.y, ri_mont, n * 32, hipMemcpyHostToDevice, ln.st));
-    CHECK(msm_lines(c, ln, coset, ln.y, 0, c->nlines));
-    CHECK(amt_outputs(c, ln, opts, c->nlines));
+    CHECK(msm_lines(c, ln, coset, ln.y, 0, c->nlines, c->fb));
+    CHECK(amt_outputs(c, ln, opts, c->nlines, c->fb));
     const size_t slots = out_slots(c), T = tree_pts(c);
     std::vector<uint8_t> pts(slots * 64);
     HIPCHK(hipMemcpyAsync(pts.data(), opts, pts.size(), hipMemcpyDeviceToHost, ln.st));
@@ -1000,7 +1073,8 @@ int zgr_msm(zgr_ctx* c, const uint8_t* bases_mont, const uint8_t* scalars_mont, 
                ln.counts, ln.heap, log_np, h, 1);
         for (int d = h - 1; d >= 0; d--) {
             uint32_t total = (uint32_t)MSM_W << d;
-            LAUNCH(c, ln.st, "msm_tree", k_msm_tree_level, blocks(total, 64), 64, 0, ln.heap, h, d, total);
+            LAUNCH(c, ln.st, "msm_tree", k_msm_tree_level, blocks(total, 64), 64, 0, ln.heap, h, d, total, MSM_W,
+                   (uint64_t)0);
         }
         uint32_t* opts = c->slots[0].outpts;
         LAUNCH(c, ln.st, "msm_root", k_msm_root_out, 1, 64, 0, ln.heap, h, opts);

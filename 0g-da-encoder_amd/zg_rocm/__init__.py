@@ -341,20 +341,28 @@ class Context:
         lib().zgr_kernel_time_reset(self.h)
 
 
-def make_context(geo=PRODUCTION, fixed_base=True, device=0):
+def make_context(geo=PRODUCTION, fixed_base=True, group_plan=None, device=0):
     """Context with its MSM path chosen explicitly.  zgr_ctx_create reads
     ZGR_FIXED_BASE: the default keeps shifted copies of the resident bases (one bucket
     set per chunk, FB_W = 22 copies: 22 x the params' HBM); "0" selects the windowed
-    kernels.  Both give identical results."""
-    old = os.environ.get("ZGR_FIXED_BASE")
-    os.environ["ZGR_FIXED_BASE"] = "1" if fixed_base else "0"
+    kernels.  ZGR_GROUP_PLAN="none" gives every AMT line one bucket set per chunk
+    (default: each line shares bucket sets over the largest chunk groups its outputs
+    allow, up to 4 chunks).  All give identical results."""
+    want = {"ZGR_FIXED_BASE": "1" if fixed_base else "0", "ZGR_GROUP_PLAN": group_plan}
+    old = {k: os.environ.get(k) for k in want}
     try:
+        for k, v in want.items():
+            if v is None:
+                os.environ.pop(k, None)
+            else:
+                os.environ[k] = v
         return Context(geo, device)
     finally:
-        if old is None:
-            del os.environ["ZGR_FIXED_BASE"]
-        else:
-            os.environ["ZGR_FIXED_BASE"] = old
+        for k, v in old.items():
+            if v is None:
+                os.environ.pop(k, None)
+            else:
+                os.environ[k] = v
 
 
 class Server:

@@ -31,14 +31,25 @@ MAC32_PER_PAIR_WINDOW = 10 * 136
 # i.e. one v_mad_u64_u32 (32x32->64 MAC) per lane per clock.  (A dependent-chain
 # microbenchmark of v_mad_u64_u32 alone reaches 25.0 T/s on the GPU.)
 PEAK_MAC32 = 256 * 4 * 16 * 2.4e9
-# HBM bytes per bucket-kernel launch from two rocprofv3 counter passes over
+# HBM bytes per bucket-kernel launch pair from two rocprofv3 counter passes over
 # `bench.py --roofline-only`: FETCH_SIZE 523,413 KiB x 2.0 (a 1 GiB device copy reads
 # as 524,342 KiB) + WRITE_SIZE 213,733 KiB x 1.0 (the copy writes as 1,048,576 KiB).
 MSM_TRAFFIC_BYTES = (523413 * 2.0 + 213733) * 1024
+# VALU lane-instructions per launch pair: rocprofv3 SQ_INSTS_VALU (wave instructions)
+# of k_fb_sums + k_fb_reduce over `bench.py --roofline-only`, x 64 lanes.
+VALU_LANE_OPS = 0.0
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
+
+
+def roofline_scalars(geo):
+    """Dense scalars for the isolated bucket-kernel launches: uniform Montgomery
+    residues below 3 * 2^252 < r (seeded)."""
+    scal = np.random.default_rng(5).integers(0, 2**63, (geo.n, 4), dtype=np.uint64)
+    scal[:, 3] &= np.uint64(0x2FFFFFFFFFFFFFFF)
+    return scal
 
 
 def cpu_baseline(ctx, geo, raw, threads):
@@ -103,13 +114,12 @@ def main():
     log("params (3 cosets x 12 x 2^20 bases) generated on device in %.2f s" % t_params)
 
     if args.roofline_only:
-        # workload for the counter passes: three isolated bucket-kernel launches (one
-        # coset of multiple_multiexp each) and one 1 GiB device-to-device copy whose
+        # workload for the counter passes: three isolated launches of the bucket kernels
+        # (one coset of amt_prove each) and one 1 GiB device-to-device copy whose
         # FETCH_SIZE / WRITE_SIZE calibrate the counters
-        scal = np.random.default_rng(5).integers(0, 2**63, (geo.n, 4), dtype=np.uint64)
-        scal[:, 3] &= np.uint64(0x2FFFFFFFFFFFFFFF)
+        scal = roofline_scalars(geo)
         for _ in range(3):
-            ctx.multiple_multiexp(1, scal)
+            ctx.amt_prove(1, scal)
         import ctypes
         pa, pb = ctypes.c_void_p(), ctypes.c_void_p()
         Z.hip().hipMalloc(ctypes.byref(pa), 1 << 30)
@@ -154,8 +164,8 @@ def main():
     value = args.steps / elapsed
     # per-kernel event times inside the plain pipeline (overlapping streams)
     kernels = {}
-    for name in ("pack", "ntt", "coset_scale", "msm_digits", "msm_sort", "msm_accum", "msm_tree", "msm_horner",
-                 "merkle_leaves", "merkle_level", "file_root"):
+    for name in ("pack", "ntt", "coset_scale", "msm_sort", "msm_scatter", "msm_group_scatter", "msm_accum",
+                 "msm_reduce", "msm_tree", "msm_horner", "merkle_leaves", "merkle_level", "file_root"):
         ms, cnt = ctx.kernel_time_ms(name)
         if cnt:
             kernels[name] = {"avg_ms": round(ms, 4), "launches": cnt}
@@ -227,18 +237,19 @@ def main():
                                                 "p99_ms": pct(p_lat, 99)}}
     # The encode pipeline overlaps kernels of different cosets on two streams, which
     # stretches event-timed durations; the roofline uses isolated launches of the same
-    # bucket kernel (multiple_multiexp of one coset: 12 lines x 2^20 dense scalars).
-    srng = np.random.default_rng(5)
-    scal = srng.integers(0, 2**63, (geo.n, 4), dtype=np.uint64)
-    scal[:, 3] &= np.uint64(0x2FFFFFFFFFFFFFFF)  # < r: a valid Montgomery residue
-    ctx.multiple_multiexp(1, scal)
+    # bucket kernels (amt_prove of one coset: the encode path's multi-MSM over 12 lines
+    # x 2^20 dense scalars).
+    scal = roofline_scalars(geo)
+    ctx.amt_prove(1, scal)
     ctx.set_profiling(True)
     ctx.kernel_time_reset()
     for _ in range(3):
-        ctx.multiple_multiexp(1, scal)
+        ctx.amt_prove(1, scal)
     ctx.set_profiling(False)
-    acc_ms, acc_cnt = ctx.kernel_time_ms("msm_accum")
-    work = 12 * geo.n * 32 * MAC32_PER_PAIR_WINDOW  # per launch (one coset)
+    sums_ms, acc_cnt = ctx.kernel_time_ms("msm_accum")
+    red_ms, _ = ctx.kernel_time_ms("msm_reduce")
+    acc_ms = sums_ms + red_ms
+    work = 12 * geo.n * 32 * MAC32_PER_PAIR_WINDOW  # per launch pair (one coset)
     achieved = work / (acc_ms * 1e-3) / 1e12 if acc_ms else None
     peak = PEAK_MAC32 / 1e12
     p50 = float(np.percentile(np.array(lat) * 1e3, 50))
@@ -273,10 +284,14 @@ def main():
             "unit": "TOP/s (32x32->64 integer MAC)",
             "frac": round(achieved / peak, 4) if achieved else None,
             "traffic": round(MSM_TRAFFIC_BYTES),
-            "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, calibrated)",
-            "kernel": "k_msm_accum",
+            "traffic_unit": "bytes per launch pair (rocprofv3 FETCH_SIZE + WRITE_SIZE, calibrated)",
+            "kernel": "k_fb_sums + k_fb_reduce (bucket sums, bucket reduction: one coset's AMT multi-MSM)",
             "avg_ms": round(acc_ms, 3),
+            "kernel_ms": {"k_fb_sums": round(sums_ms, 3), "k_fb_reduce": round(red_ms, 3)},
             "launches": acc_cnt,
+            # VALU lane-instructions the pair actually issues (rocprofv3 SQ_INSTS_VALU x 64)
+            # over the time, against the same 39.3 T lane-op/s issue peak
+            "valu_issue_frac": round(VALU_LANE_OPS / (acc_ms * 1e-3) / PEAK_MAC32, 4) if acc_ms else None,
         },
         "kernels": kernels,
     }

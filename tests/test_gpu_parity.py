@@ -12,19 +12,34 @@ pytestmark = pytest.mark.gpu
 GEO = Z.TEST_SETTING
 
 
-@pytest.fixture(scope="module", params=["fixed_base", "windowed"])
+@pytest.fixture(scope="module", params=["fixed_base", "fixed_base_per_chunk", "windowed"])
 def ctx(request):
-    """Both MSM paths over the resident bases: the shifted-copy tables (default) and
-    the windowed kernels (ZGR_FIXED_BASE=0, read at context creation)."""
-    c = Z.make_context(GEO, fixed_base=request.param == "fixed_base")
+    """Every MSM path over the resident bases: the shifted-copy tables (default: the
+    AMT lines share bucket sets over chunk groups), the same with one bucket set per
+    chunk on every line (ZGR_GROUP_PLAN=none), and the windowed kernels
+    (ZGR_FIXED_BASE=0); both variables are read at context creation."""
+    fixed = request.param != "windowed"
+    c = Z.make_context(GEO, fixed_base=fixed, group_plan="none" if request.param == "fixed_base_per_chunk" else None)
     c.set_profiling(True)
     c.params_generate(seeded_tau())
     # the table build runs only on the fixed-base path
-    assert (c.kernel_time_ms("fb_build")[1] > 0) == (request.param == "fixed_base")
+    assert (c.kernel_time_ms("fb_build")[1] > 0) == fixed
     c.set_profiling(False)
     c.kernel_time_reset()
+    c.mode = request.param
     yield c
     c.close()
+
+
+def test_msm_path_in_use(ctx):
+    # chunk-group entry lists are written only when some AMT line shares bucket sets
+    ctx.set_profiling(True)
+    ctx.kernel_time_reset()
+    ctx.encode(Z.seeded_bytes(DATA_SEED, 1000))
+    merges = ctx.kernel_time_ms("msm_group_scatter")[1]
+    ctx.set_profiling(False)
+    ctx.kernel_time_reset()
+    assert (merges > 0) == (ctx.mode == "fixed_base")
 
 
 @pytest.fixture(scope="module")
