@@ -722,14 +722,31 @@ __global__ __launch_bounds__(FB_LANES, 2) void k_fb_sums(const uint32_t* table, 
     x29 R = x29_inf();
     uint32_t cur = 0;   // bucket of R (0: none yet)
     bool head = true;   // R still holds the share's first bucket
+    // software pipeline: the entry two steps ahead and the point one step ahead are in
+    // flight while a step computes.  Past the end of its share a lane re-reads the
+    // list's last entry (an index clamp, not a select on the loaded value, so the load
+    // need not complete before the next one is issued); its results are discarded.
+    auto entry_at = [&](int st) -> uint32_t { return ent[min(begin + st, cnt - 1)]; };
+    uint32_t ev_next = entry_at(0), ev_after = entry_at(1);
+    const uint4* pn = (const uint4*)(tab + (size_t)(ev_next & FB_IDX_MASK) * 16);
+    uint4 p0 = pn[0], p1 = pn[1], p2 = pn[2], p3 = pn[3];
     for (int st = 0; st < maxsteps; st++) {
         const bool active = st < steps;
-        const uint32_t ev = active ? ent[begin + st] : 0u;
+        const uint32_t ev = ev_next;
         const uint32_t m = ev >> FB_MAG_SHIFT;
-        const uint32_t* pp = tab + (size_t)(ev & FB_IDX_MASK) * 16;
+        const uint32_t wx[8] = {p0.x, p0.y, p0.z, p0.w, p1.x, p1.y, p1.z, p1.w};
+        const uint32_t wy[8] = {p2.x, p2.y, p2.z, p2.w, p3.x, p3.y, p3.z, p3.w};
         a29 P;
-        P.x = fb_load_coord(pp);
-        P.y = fb_load_coord(pp + 8);
+        P.x = q_unpack(wx);
+        P.y = q_unpack(wy);
+        // issue the next step's loads before this step's arithmetic
+        ev_next = ev_after;
+        ev_after = entry_at(st + 2);
+        pn = (const uint4*)(tab + (size_t)(ev_next & FB_IDX_MASK) * 16);
+        p0 = pn[0];
+        p1 = pn[1];
+        p2 = pn[2];
+        p3 = pn[3];
         if ((ev >> FB_SIGN_SHIFT) & 1u) P.y = q_neg(P.y);
         const bool same = m == cur;
         x29 M = x29_madd(R, P, active);
