@@ -797,6 +797,45 @@ int zgr_params_load(zgr_ctx* c, int coset, const void* bases_mont, size_t nbytes
     return ZGR_OK;
 }
 
+int zgr_params_load_bamt(zgr_ctx* c, int coset, const uint8_t* bamt, size_t nbytes) {
+    if (!c || !bamt) return fail(ZGR_ERR_INVALID_ARG, "null argument");
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (coset < 0 || coset >= c->coset_n) return fail(ZGR_ERR_INVALID_ARG, "coset %d out of range", coset);
+    // write_amt_params layout: "bamt", u8 degree, u8 sub_degree, g2, high_g2 (ark
+    // uncompressed G2), basis, quotients[0..sub), vanishes[d] (2^(d+1) G2 of 128 B),
+    // high_basis; G1 points are x_mont || y_mont (64 B)
+    if (nbytes < 6 || memcmp(bamt, "bamt", 4)) return fail(ZGR_ERR_INVALID_ARG, "Incorrect format");
+    const int degree = bamt[4], sub = bamt[5];
+    if (degree != c->log_n || sub != c->h)
+        return fail(ZGR_ERR_INVALID_ARG, "bamt degree %d / sub-degree %d, context has %d / %d", degree, sub,
+                    c->log_n, c->h);
+    const size_t line = c->n * 64;
+    size_t vanish = 0;
+    for (int d = 0; d < sub; d++) vanish += ((size_t)2 << d) * 128;
+    // g2 / high_g2: ark uncompressed G2 affine = x.c0 x.c1 y.c0 y.c1, 32 B each, the
+    // infinity / sign flags in the spare top bits of the last byte
+    const size_t g2 = 128;
+    const size_t want = 6 + 2 * g2 + (size_t)(sub + 2) * line + vanish;
+    if (nbytes != want)
+        return fail(ZGR_ERR_INVALID_ARG, "bamt length %zu, expected %zu for degree %d", nbytes, want, degree);
+    HIPCHK(hipSetDevice(c->device));
+    CHECK(alloc_bases(c, coset));
+    c->loaded[coset] = false;
+    hipStream_t st = c->lanes[0].st;
+    uint8_t* dev = (uint8_t*)c->bases[coset];
+    const uint8_t* p = bamt + 6 + 2 * g2;
+    HIPCHK(hipMemcpyAsync(dev + (size_t)c->h * line, p, line, hipMemcpyHostToDevice, st));  // basis
+    p += line;
+    for (int d = 0; d < sub; d++, p += line)  // quotients
+        HIPCHK(hipMemcpyAsync(dev + (size_t)d * line, p, line, hipMemcpyHostToDevice, st));
+    p += vanish;
+    HIPCHK(hipMemcpyAsync(dev + (size_t)(c->h + 1) * line, p, line, hipMemcpyHostToDevice, st));  // high_basis
+    CHECK(build_fb(c, coset));
+    CHECK(sync_all(c));
+    c->loaded[coset] = true;
+    return ZGR_OK;
+}
+
 int zgr_params_generate(zgr_ctx* c, const uint8_t tau[32]) {
     if (!c || !tau) return fail(ZGR_ERR_INVALID_ARG, "null argument");
     std::lock_guard<std::mutex> lk(c->mu);

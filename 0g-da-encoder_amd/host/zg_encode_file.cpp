@@ -5,7 +5,8 @@
 //             dir:<path>                                bamt prove-param files
 //                                                       (amt-prove-coset{c}-mont-5DWgDV-{h}-{n}.bin)
 //   For RAW number k it writes OUTDIR/k.reply = erasure_commitment (64) || storage_root (32)
-//   || encoded_data (EncoderService::process_data with require_data), and, with --rows,
+//   || encoded_data (EncoderService::process_data with require_data), OUTDIR/k.slices = the
+//   reply's encoded_slice bytes of every row, concatenated in row order, and, with --rows,
 //   OUTDIR/k.rows = for each index: row || per level (commitment, proof) || high_commitment
 //   || commitment || merkle roots || merkle proof (leaf level first) || row root
 //   (EncodedBlob::get_row).
@@ -94,6 +95,9 @@ int main(int argc, char** argv) {
             put(reply, r.storage_root);
             put(reply, r.encoded_data);
             write_file(stem + ".reply", reply);
+            std::vector<uint8_t> slices;  // every encoded_slice, in row order
+            for (const auto& sl : r.encoded_slice) put(slices, sl);
+            write_file(stem + ".slices", slices);
             if (rows.empty()) continue;
             zg::EncodedBlob b = zg::EncodedBlob::build(zg::RawData::try_from(g, raw.data(), raw.size()), svc.params());
             std::vector<uint8_t> out;

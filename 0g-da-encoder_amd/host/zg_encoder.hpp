@@ -100,42 +100,18 @@ class EncoderParams {
         return p;
     }
 
-    // read one bamt file and upload its prove bases in the line-major order the
-    // device uses (quotients[0..h) || basis || high_basis, as read_gpu_bases uploads)
+    // read one bamt file and upload its prove bases (zgr_params_load_bamt parses the
+    // layout and skips the G2 sections)
     void load_bamt(int coset, const std::string& path) {
         std::unique_ptr<FILE, int (*)(FILE*)> f(fopen(path.c_str(), "rb"), fclose);
         if (!f) throw std::runtime_error("cannot open " + path);
-        uint8_t head[6];
-        if (fread(head, 1, 6, f.get()) != 6 || memcmp(head, "bamt", 4))
-            throw std::runtime_error(path + ": Incorrect format");
-        const int degree = head[4], sub = head[5];
-        if (degree != geo_.log_col + geo_.log_row || sub != geo_.log_row)
-            throw std::runtime_error(path + ": degree / sub-degree do not match the geometry");
-        const size_t n = geo_.n(), line = n * 64;
-        // g2 and high_g2 are ark-serialize uncompressed G2 affine points; their exact size
-        // (128 B with the flags in y's spare bits) is taken from the file length so that
-        // the loader does not depend on it: total = 6 + 2 G + (sub + 2) lines + vanishes
-        size_t vanish = 0;
-        for (int d = 0; d < sub; d++) vanish += ((size_t)2 << d) * 128;  // write_g2: 4 x 32 B mont
         if (fseek(f.get(), 0, SEEK_END)) throw std::runtime_error(path + ": seek");
-        const size_t total = (size_t)ftell(f.get());
-        const size_t fixed = 6 + (size_t)(sub + 2) * line + vanish;
-        if (total < fixed || (total - fixed) % 2) throw std::runtime_error(path + ": unexpected length");
-        const size_t g2_uncompressed = (total - fixed) / 2;
-        if (fseek(f.get(), 6, SEEK_SET)) throw std::runtime_error(path + ": seek");
-        std::vector<uint8_t> buf(geo_.params_bytes());
-        auto rd = [&](uint8_t* dst, size_t bytes) {
-            if (fread(dst, 1, bytes, f.get()) != bytes) throw std::runtime_error(path + ": truncated");
-        };
-        auto skip = [&](size_t bytes) {
-            if (fseek(f.get(), (long)bytes, SEEK_CUR)) throw std::runtime_error(path + ": truncated");
-        };
-        skip(2 * g2_uncompressed);                             // g2, high_g2
-        rd(buf.data() + (size_t)h() * line, line);             // basis
-        for (int d = 0; d < sub; d++) rd(buf.data() + (size_t)d * line, line);  // quotients
-        for (int d = 0; d < sub; d++) skip(((size_t)2 << d) * 128);  // vanishes (G2, 4 x 32 B)
-        rd(buf.data() + (size_t)(h() + 1) * line, line);       // high_basis
-        check(zgr_params_load(ctx(), coset, buf.data(), buf.size()), "zgr_params_load");
+        const long total = ftell(f.get());
+        if (total < 0 || fseek(f.get(), 0, SEEK_SET)) throw std::runtime_error(path + ": seek");
+        std::vector<uint8_t> image((size_t)total);
+        if (fread(image.data(), 1, image.size(), f.get()) != image.size())
+            throw std::runtime_error(path + ": truncated");
+        check(zgr_params_load_bamt(ctx(), coset, image.data(), image.size()), ("zgr_params_load_bamt " + path).c_str());
     }
 
     void warmup() const {}  // bases are uploaded when loaded (EncoderParams::warmup)
@@ -144,7 +120,6 @@ class EncoderParams {
     const Geometry& geometry() const { return geo_; }
 
   private:
-    int h() const { return geo_.log_row; }
     struct Del {
         void operator()(zgr_ctx* c) const { zgr_ctx_destroy(c); }
     };
@@ -152,17 +127,77 @@ class EncoderParams {
     std::unique_ptr<zgr_ctx, Del> ctx_;
 };
 
+// ark-serialize 0.4 uncompressed G1 affine from canonical x || y (64 zero bytes =
+// infinity): y carries the SW flags in its spare top bits, 0x40 in the last byte for
+// infinity, 0x80 when y > -y (y > (p - 1) / 2).
+inline std::array<uint8_t, 64> ark_g1_uncompressed(const std::array<uint8_t, 64>& p) {
+    static const uint64_t half[4] = {0x9e10460b6c3e7ea3ull, 0xcbc0b548b438e546ull, 0xdc2822db40c0ac2eull,
+                                     0x183227397098d014ull};  // (p - 1) / 2, little-endian limbs
+    std::array<uint8_t, 64> r = p;
+    bool zero = true;
+    for (uint8_t b : p) zero = zero && b == 0;
+    if (zero) {
+        r[63] = 0x40;
+        return r;
+    }
+    uint64_t y[4];
+    memcpy(y, p.data() + 32, 32);
+    for (int i = 3; i >= 0; i--) {
+        if (y[i] == half[i]) continue;
+        if (y[i] > half[i]) r[63] |= 0x80;
+        break;
+    }
+    return r;
+}
+
 // One encoded row as EncodedSlice carries it (encoder/slice.rs, amt/slice.rs,
 // merkle/slice.rs): index, row data, the AMT proof (commitment, proof) per level,
 // the high commitment, the authoritative commitment, the Merkle proof.
 struct EncodedSlice {
     size_t index = 0;
+    size_t row_n = 0;  // rows per coset (BlobRow.index = index % row_n)
     std::vector<Bytes32> row;
     std::vector<std::pair<std::array<uint8_t, 64>, std::array<uint8_t, 64>>> amt_proof;
     std::array<uint8_t, 64> high_commitment{}, commitment{};
     std::vector<Bytes32> merkle_root;  // COSET_N roots
     std::vector<Bytes32> merkle_proof;
     Bytes32 row_root{};
+
+    // serialize_uncompressed (grpc/src/service.rs:162-166) of EncodedSlice { index,
+    // amt: EncodedSliceAMT { index, commitment, row: BlobRow { index, row, proof,
+    // high_commitment } }, merkle: EncodedSliceMerkle { root, proof, leaf_index, leaf } }:
+    // fields in declaration order, usize as u64 LE, Vec with a u64 LE length, arrays
+    // without one, Fr as 32 B canonical LE.
+    std::vector<uint8_t> serialize() const {
+        std::vector<uint8_t> o;
+        auto u64 = [&](uint64_t v) {
+            for (int i = 0; i < 8; i++) o.push_back((uint8_t)(v >> (8 * i)));
+        };
+        auto put = [&](const uint8_t* p, size_t n) { o.insert(o.end(), p, p + n); };
+        auto g1 = [&](const std::array<uint8_t, 64>& p) {
+            auto a = ark_g1_uncompressed(p);
+            put(a.data(), 64);
+        };
+        o.reserve(160 + 40 * row.size() + 136 * amt_proof.size() + 32 * (merkle_root.size() + merkle_proof.size()));
+        u64(index);                     // EncodedSlice.index
+        u64(index);                     // EncodedSliceAMT.index
+        g1(commitment);
+        u64(index % row_n);             // BlobRow.index
+        u64(row.size());                // BlobRow.row
+        for (const auto& e : row) put(e.data(), 32);
+        u64(amt_proof.size());          // BlobRow.proof
+        for (const auto& cp : amt_proof) {
+            g1(cp.first);
+            g1(cp.second);
+        }
+        g1(high_commitment);
+        for (const auto& m : merkle_root) put(m.data(), 32);  // [Bytes32; COSET_N]
+        u64(merkle_proof.size());
+        for (const auto& m : merkle_proof) put(m.data(), 32);
+        u64(index);                     // leaf_index
+        put(row_root.data(), 32);       // leaf
+        return o;
+    }
 };
 
 class EncodedBlob {
@@ -190,6 +225,7 @@ class EncodedBlob {
         if (index >= g.row_encoded()) throw std::out_of_range("row index");
         EncodedSlice s;
         s.index = index;
+        s.row_n = g.row_n();
         const size_t coset = index / g.row_n(), local = index % g.row_n();
         for (size_t c = 0; c < g.col_n(); c++) {
             Bytes32 e;
@@ -251,13 +287,14 @@ class EncodedBlob {
     std::vector<uint8_t> data_, commitment_, comm_tree_, proof_tree_, high_, merkle_, root_;
 };
 
-// EncodeBlobReply fields produced by EncoderService::process_data (grpc/src/service.rs:67-106);
-// the per-row encoded_slice wire bytes are not produced here.
+// EncodeBlobReply (grpc/proto/encoder.proto:16-22) as EncoderService::process_data
+// fills it (grpc/src/service.rs:67-106).
 struct EncodeBlobReply {
     uint32_t version = 0;
     std::vector<uint8_t> erasure_commitment;  // x || y, 32 B canonical LE each (service.rs:77-83)
     std::vector<uint8_t> storage_root;        // file root (service.rs:84)
     std::vector<uint8_t> encoded_data;        // 32 B per encoded element when require_data (service.rs:85-91)
+    std::vector<std::vector<uint8_t>> encoded_slice;  // serialized get_row(i) for every row (service.rs:93-95)
 };
 
 class EncoderService {
@@ -273,6 +310,8 @@ class EncoderService {
         auto root = blob.get_file_root();
         r.storage_root.assign(root.begin(), root.end());
         if (require_data) r.encoded_data = blob.get_data();
+        r.encoded_slice.resize(params_.geometry().row_encoded());
+        for (size_t i = 0; i < r.encoded_slice.size(); i++) r.encoded_slice[i] = blob.get_row(i).serialize();
         return r;
     }
 

@@ -15,6 +15,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "libzgr.so")
 
 R_MOD = 21888242871839275222246405745257275088548364400416034343698204186575808495617
+P_MOD = 21888242871839275222246405745257275088696311157297823662689037894645226208583  # BN254 Fq
 
 OK = 0
 ERR_INVALID_ARG = -1
@@ -33,6 +34,7 @@ SIGNATURES = {
     "zgr_ctx_create": (I, [I, I, I, I, ctypes.POINTER(P)]),
     "zgr_ctx_destroy": (None, [P]),
     "zgr_params_load": (I, [P, I, P, S]),
+    "zgr_params_load_bamt": (I, [P, I, P, S]),
     "zgr_params_generate": (I, [P, P]),
     "zgr_params_download": (I, [P, I, P, S]),
     "zgr_amt_prove": (I, [P, I, P, S, P, P, P, P]),
@@ -245,6 +247,11 @@ class Context:
     def params_load(self, coset, bases):
         b = np.ascontiguousarray(bases)
         _check(lib().zgr_params_load(self.h, coset, _p(b), b.nbytes))
+
+    def params_load_bamt(self, coset, image):
+        """One coset's bamt file image (bytes / uint8 array), as AMTParams::from_dir_mont reads it."""
+        b = np.frombuffer(image, dtype=np.uint8) if isinstance(image, (bytes, bytearray)) else np.ascontiguousarray(image)
+        _check(lib().zgr_params_load_bamt(self.h, coset, _p(b), b.nbytes))
 
     def params_download(self, coset):
         out = np.zeros(self.geo.params_bytes() // 8, dtype=np.uint64)
@@ -476,9 +483,45 @@ class EncodedBlob:
                 "commitment": self.get_affine_commitment(), "merkle_root": self.get_roots(),
                 "merkle_proof": mproof, "row_root": bytes(self.merkle_layer(g.log_row)[index])}
 
+    def encoded_slice(self, index):
+        """serialize_uncompressed(get_row(index)) (grpc/src/service.rs:93-95, 162-166):
+        EncodedSlice { index, amt: EncodedSliceAMT { index, commitment, row: BlobRow {
+        index, row, proof, high_commitment } }, merkle: EncodedSliceMerkle { root, proof,
+        leaf_index, leaf } } (encoder/slice.rs:15-20, amt/slice.rs:13-18,
+        amt/src/blob/encode.rs:203-209, merkle/slice.rs:12-20), fields in declaration
+        order; usize as u64 LE, Vec with a u64 LE length, arrays without one, Fr as 32 B
+        canonical LE, G1 affine as ark_g1_uncompressed."""
+        g = self.geo
+        r = self.get_row(index)
+        u64 = lambda v: int(v).to_bytes(8, "little")  # noqa: E731
+        parts = [u64(index),                                   # EncodedSlice.index
+                 u64(index), ark_g1_uncompressed(r["commitment"]),  # EncodedSliceAMT.index, commitment
+                 u64(index % g.row_n),                         # BlobRow.index (row within the coset)
+                 u64(g.col_n), r["row"],                       # BlobRow.row
+                 u64(len(r["amt_proof"]))]                     # BlobRow.proof
+        for com, prf in r["amt_proof"]:
+            parts += [ark_g1_uncompressed(com), ark_g1_uncompressed(prf)]
+        parts.append(ark_g1_uncompressed(r["high_commitment"]))
+        parts += list(r["merkle_root"])                        # [Bytes32; COSET_N]
+        parts += [u64(len(r["merkle_proof"]))] + list(r["merkle_proof"])
+        parts += [u64(index), r["row_root"]]                   # leaf_index, leaf
+        return b"".join(parts)
+
+
+def ark_g1_uncompressed(p64):
+    """ark-serialize 0.4 uncompressed G1 affine from a canonical x || y (64 zero bytes =
+    infinity): x, then y with the SW flags in its spare top bits -- 0x40 in the last
+    byte for infinity (x = y = 0), 0x80 when y > -y, i.e. y > (p - 1) / 2."""
+    b = bytearray(p64)
+    if not any(b):
+        b[63] = 0x40
+    elif int.from_bytes(bytes(b[32:]), "little") > (P_MOD - 1) // 2:
+        b[63] |= 0x80
+    return bytes(b)
+
 
 class EncoderService:
-    """EncoderService::process_data (grpc/src/service.rs:67-106) minus the slice wire bytes."""
+    """EncoderService::process_data (grpc/src/service.rs:67-106)."""
 
     def __init__(self, ctx):
         self.ctx = ctx
@@ -491,5 +534,6 @@ class EncoderService:
             "erasure_commitment": c,  # x || y, 32 B LE each (service.rs:77-83)
             "storage_root": blob.get_file_root(),
             "encoded_data": bytes(blob.get_data()) if require_data else b"",
+            "encoded_slice": [blob.encoded_slice(i) for i in range(self.ctx.geo.row_encoded)],
             "blob": blob,
         }

@@ -74,6 +74,14 @@ void zgr_ctx_destroy(zgr_ctx *ctx);
  *      read_gpu_bases, prove_gpu.rs:31-53) ---- */
 /* Upload one coset's prove bases from host memory (layout above). */
 int zgr_params_load(zgr_ctx *ctx, int coset, const void *bases_mont, size_t nbytes);
+/* Upload one coset's prove bases from the image of its bamt file
+ * amt-prove-coset{c}-mont-{hash}-{h:02}-{n:02}.bin (AMTParams::from_dir_mont,
+ * crates/amt/src/prove_params/generate.rs:27-65; read_amt_params,
+ * crates/amt/src/prove_params/fast_serde_bn254.rs:58-88).  The G2 sections (g2,
+ * high_g2, vanishes) belong to the verifier and are skipped.  ZGR_ERR_INVALID_ARG for a
+ * wrong header ("Incorrect format"), degrees that differ from the context, or a length
+ * the layout does not account for. */
+int zgr_params_load_bamt(zgr_ctx *ctx, int coset, const uint8_t *bamt, size_t nbytes);
 /* Insecure test setup: generate every coset's prove bases on the device from a known
  * tau (32 B canonical LE).  Same points as PowerTau::setup_inner with that tau
  * (crates/amt/src/power_tau.rs:54-80) followed by AMTParams::from_pp

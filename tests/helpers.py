@@ -36,3 +36,29 @@ def random_fr_mont(rng, n):
     """n uniform Fr elements in Montgomery form (via the oracle's to_mont)."""
     vals = [int.from_bytes(rng.bytes(32), "little") % O.R_MOD for _ in range(n)]
     return O.fr_to_mont_array(vals)
+
+
+G2_BYTES = 128  # ark-serialize uncompressed G2 affine
+
+
+def bamt_image(bases_u64, log_n, h):
+    """write_amt_params layout (crates/amt/src/prove_params/fast_serde_bn254.rs:20-53):
+    header, degree, sub-degree, g2, high_g2, basis, quotients[0..h), vanishes (2^(d+1)
+    G2 points at level d), high_basis.  The G2 sections are verifier data and are zero
+    placeholders here; `bases_u64` is the device layout quotients || basis || high."""
+    line = (1 << log_n) * 64
+    b = np.ascontiguousarray(bases_u64).view(np.uint8)
+    assert b.size == (h + 2) * line
+    parts = [b"bamt" + bytes([log_n, h]), bytes(2 * G2_BYTES), b[h * line:(h + 1) * line].tobytes()]
+    parts += [b[d * line:(d + 1) * line].tobytes() for d in range(h)]
+    parts += [bytes((2 << d) * G2_BYTES) for d in range(h)]
+    parts.append(b[(h + 1) * line:(h + 2) * line].tobytes())
+    return b"".join(parts)
+
+
+class OracleOutputs:
+    """zgr_encoded-shaped view of the oracle's Encoded (for zg_rocm.EncodedBlob)."""
+
+    def __init__(self, e):
+        self.data, self.commitment, self.comm_tree = e.data, e.commitment, e.comm_tree
+        self.proof_tree, self.high, self.merkle, self.file_root = e.proof_tree, e.high, e.merkle, e.root

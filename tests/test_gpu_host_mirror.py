@@ -14,13 +14,12 @@ import pytest
 
 import oracle as O
 import zg_rocm as Z
-from helpers import DATA_SEED, oracle_params, seeded_tau
+from helpers import DATA_SEED, OracleOutputs, bamt_image, oracle_params, seeded_tau
 from pyref import keccak256_ref
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 BIN = os.path.join(HERE, "..", "0g-da-encoder_amd", "lib", "zg_encode_file")
 GEO = Z.TEST_SETTING
-G2_BYTES = 128  # ark-serialize uncompressed G2 affine
 
 
 def _run(args, timeout=120):
@@ -35,21 +34,8 @@ def _geo_args():
 
 
 def write_bamt(path, bases_u64, log_n, h):
-    """write_amt_params layout: header, degree, sub-degree, g2, high_g2, basis,
-    quotients[0..h), vanishes (2^(d+1) G2 points at level d), high_basis.  The G2
-    sections are verifier data and are written as zero placeholders here."""
-    line = (1 << log_n) * 64
-    b = np.ascontiguousarray(bases_u64).view(np.uint8)
-    assert b.size == (h + 2) * line
     with open(path, "wb") as f:
-        f.write(b"bamt" + bytes([log_n, h]))
-        f.write(bytes(2 * G2_BYTES))
-        f.write(b[h * line:(h + 1) * line].tobytes())  # basis
-        for d in range(h):
-            f.write(b[d * line:(d + 1) * line].tobytes())  # quotients[d]
-        for d in range(h):
-            f.write(bytes((2 << d) * G2_BYTES))  # vanishes[d]
-        f.write(b[(h + 1) * line:(h + 2) * line].tobytes())  # high_basis
+        f.write(bamt_image(bases_u64, log_n, h))
 
 
 def expected_row(e, index):
@@ -121,6 +107,10 @@ def _check_outputs(outdir, data):
         assert reply[:64] == exp.commitment.tobytes()
         assert reply[64:96] == exp.root.tobytes()
         assert reply[96:] == exp.data.tobytes()
+        # encoded_slice: the C++ serializer against the Python one over the oracle's outputs
+        blob = Z.EncodedBlob(GEO, OracleOutputs(exp))
+        want = b"".join(blob.encoded_slice(i) for i in range(GEO.row_encoded))
+        assert (outdir / ("%d.slices" % k)).read_bytes() == want
         rows = (outdir / ("%d.rows" % k)).read_bytes()
         per = len(rows) // len(ROWS)
         for j, idx in enumerate(ROWS):

@@ -5,7 +5,7 @@ import pytest
 
 import oracle as O
 import zg_rocm as Z
-from helpers import DATA_SEED, oracle_params, random_fr_mont, ri_from_y, seeded_tau
+from helpers import DATA_SEED, bamt_image, oracle_params, random_fr_mont, ri_from_y, seeded_tau
 
 pytestmark = pytest.mark.gpu
 
@@ -53,6 +53,26 @@ def test_params_generate_matches_oracle(ctx, params):
     for c in range(GEO.coset_n):
         got = ctx.params_download(c)
         assert np.array_equal(got, params[c]), "coset %d" % c
+
+
+def test_params_load_bamt(ctx, params):
+    # read_amt_params layout (fast_serde_bn254.rs:58-88): the prove lines land in the
+    # device order quotients || basis || high, G2 sections skipped
+    for c in range(GEO.coset_n):
+        img = bamt_image(params[c], GEO.log_n, GEO.h)
+        ctx.params_load_bamt(c, img)
+        assert np.array_equal(ctx.params_download(c), params[c]), "coset %d" % c
+    img = bamt_image(params[0], GEO.log_n, GEO.h)
+    bad = [b"bamx" + img[4:],                                    # header
+           img[:4] + bytes([GEO.log_n + 1]) + img[5:],           # degree
+           img[:5] + bytes([GEO.h - 1]) + img[6:],               # sub-degree
+           img[:-64],                                            # truncated
+           img + bytes(1)]                                       # trailing byte
+    for b in bad:
+        with pytest.raises(Z.ZgrError) as e:
+            ctx.params_load_bamt(0, b)
+        assert e.value.code == Z.ERR_INVALID_ARG
+    assert np.array_equal(ctx.params_download(0), params[0])  # still resident
 
 
 def test_coset_extend_matches_oracle(ctx):
