@@ -66,6 +66,24 @@ int fail(int code, const char* fmt, ...) {
 
 inline uint32_t blocks(size_t n, uint32_t t) { return (uint32_t)((n + t - 1) / t); }
 
+// host memcpy of large staged outputs, split over a few threads (one thread moves about
+// 10 GB/s; the encoded data is 32 MB per coset)
+void host_copy(uint8_t* dst, const uint8_t* src, size_t n) {
+    constexpr size_t PARTS = 4, MIN_PART = 4u << 20;
+    if (n < PARTS * MIN_PART) {
+        memcpy(dst, src, n);
+        return;
+    }
+    std::thread th[PARTS - 1];
+    const size_t part = (n / PARTS + 63) & ~(size_t)63;
+    for (size_t k = 1; k < PARTS; k++) {
+        const size_t off = k * part, len = off < n ? std::min(part, n - off) : 0;
+        th[k - 1] = std::thread([=] { memcpy(dst + off, src + off, len); });
+    }
+    memcpy(dst, src, std::min(part, n));
+    for (auto& t : th) t.join();
+}
+
 // dynamic LDS of the bucket kernel: staged bases, reused for the lane reduction
 inline size_t accum_lds_bytes(size_t bs) {
     size_t a = bs * LDS_PT * 4, r = (size_t)ACC_THREADS * LDS_XP * 4;
@@ -111,6 +129,7 @@ struct Slot {
     hipStream_t st = nullptr;
     hipEvent_t ev_pre = nullptr, ev_done = nullptr;
     hipEvent_t ev_coset[3] = {nullptr, nullptr, nullptr};
+    hipEvent_t ev_data[3] = {nullptr, nullptr, nullptr};  // coset's encoded data staged
     uint8_t* raw = nullptr;       // staging for host input (31 n bytes)
     uint32_t* pts = nullptr;      // n Fr, column-major blob
     uint32_t* coeff = nullptr;    // n Fr, coefficients (bit-reversed)
@@ -383,15 +402,21 @@ int issue_blob(zgr_ctx* c, Slot& s, const uint8_t* raw_dev, size_t len, const zg
                cs, s.data, s.mlayers[0]);
         HIPCHK(hipEventRecord(s.ev_coset[cs], ln.st));
     }
-    for (int cs = 0; cs < c->coset_n; cs++) HIPCHK(hipStreamWaitEvent(s.st, s.ev_coset[cs], 0));
+    // each coset's encoded data goes to the pinned staging as soon as its leaves are
+    // hashed, so finish_blob copies it out while later cosets still compute
+    for (int cs = 0; cs < c->coset_n; cs++) {
+        HIPCHK(hipStreamWaitEvent(s.st, s.ev_coset[cs], 0));
+        if (out->data)
+            HIPCHK(hipMemcpyAsync(s.h_data + (size_t)cs * c->n * 32, (const uint8_t*)s.data + (size_t)cs * c->n * 32,
+                                  c->n * 32, hipMemcpyDeviceToHost, s.st));
+        HIPCHK(hipEventRecord(s.ev_data[cs], s.st));
+    }
     for (size_t l = 1; l < s.mlayers.size(); l++)
         LAUNCH(c, s.st, "merkle_level", k_merkle_level, blocks(c->mlens[l], 128), 128, 0, s.mlayers[l - 1],
                s.mlayers[l], (uint32_t)c->mlens[l]);
     LAUNCH(c, s.st, "file_root", k_file_root, 1, 1, 0, s.mlayers.back(), (int)c->mlens.back(), s.root);
     // outputs into pinned staging
     HIPCHK(hipMemcpyAsync(s.h_pts, s.outpts, (size_t)c->coset_n * slots * 64, hipMemcpyDeviceToHost, s.st));
-    if (out->data)
-        HIPCHK(hipMemcpyAsync(s.h_data, s.data, (size_t)c->coset_n * c->n * 32, hipMemcpyDeviceToHost, s.st));
     if (out->merkle) {
         size_t off = 0;
         const int L = (int)s.mlayers.size();
@@ -413,8 +438,13 @@ int issue_blob(zgr_ctx* c, Slot& s, const uint8_t* raw_dev, size_t len, const zg
 int finish_blob(zgr_ctx* c, Slot& s) {
     if (!s.busy) return ZGR_OK;
     s.busy = false;
-    HIPCHK(hipEventSynchronize(s.ev_done));
     const zgr_encoded* out = s.out;
+    if (out->data)
+        for (int cs = 0; cs < c->coset_n; cs++) {  // coset by coset, as the staging fills
+            HIPCHK(hipEventSynchronize(s.ev_data[cs]));
+            host_copy(out->data + (size_t)cs * c->n * 32, s.h_data + (size_t)cs * c->n * 32, c->n * 32);
+        }
+    HIPCHK(hipEventSynchronize(s.ev_done));
     const size_t slots = out_slots(c), T = tree_pts(c);
     const uint8_t* pts = s.h_pts;
     for (int cs = 0; cs < c->coset_n; cs++) {
@@ -424,7 +454,6 @@ int finish_blob(zgr_ctx* c, Slot& s) {
         if (out->high) memcpy(out->high + cs * 64, pts + (2 * T + 1) * 64, 64);  // coset 0 (see issue_blob)
     }
     if (out->commitment) memcpy(out->commitment, pts + 2 * T * 64, 64);
-    if (out->data) memcpy(out->data, s.h_data, (size_t)c->coset_n * c->n * 32);
     if (out->merkle) memcpy(out->merkle, s.h_merkle, merkle_top_nodes(c) * 32);
     if (out->file_root) memcpy(out->file_root, s.h_root, 32);
     for (int cs = 1; cs < c->coset_n; cs++)
@@ -621,7 +650,8 @@ void destroy_streams(zgr_ctx* c) {
         if (l.st) (void)hipStreamDestroy(l.st);
     for (auto& s : c->slots) {
         if (s.st) (void)hipStreamDestroy(s.st);
-        for (hipEvent_t e : {s.ev_pre, s.ev_done, s.ev_coset[0], s.ev_coset[1], s.ev_coset[2]})
+        for (hipEvent_t e : {s.ev_pre, s.ev_done, s.ev_coset[0], s.ev_coset[1], s.ev_coset[2], s.ev_data[0],
+                             s.ev_data[1], s.ev_data[2]})
             if (e) (void)hipEventDestroy(e);
     }
 }
@@ -702,6 +732,7 @@ int zgr_ctx_create(int device, int log_col, int log_row, int coset_n, zgr_ctx** 
             HIPCHK(hipEventCreateWithFlags(&s.ev_pre, hipEventDisableTiming));
             HIPCHK(hipEventCreateWithFlags(&s.ev_done, hipEventDisableTiming));
             for (auto& e : s.ev_coset) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+            for (auto& e : s.ev_data) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
             DALLOC(s.raw, 31 * c->n);
             DALLOC(s.pts, c->n * 32);
             DALLOC(s.coeff, c->n * 32);
