@@ -230,6 +230,26 @@ __device__ __forceinline__ bool q_is_zero(const fq29& a) {
 // x in [0, 4p): x == 0 mod p iff x is 0, p, 2p or 3p
 __device__ __forceinline__ bool q_is_zero4(const fq29& a) { return q_is_zero(q_reduce_2p(a)); }
 
+// a^(p-2) = a^-1 (Fermat; 0 maps to 0) for a in [0, 4p), result in [0, 2p).  The
+// exponent bits are the same on every lane, so the branch is uniform.
+__device__ __noinline__ fq29 q_inv(const fq29& a) {
+    uint32_t e[8];
+    uint32_t br = 2;
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+        uint64_t x = (uint64_t)FqP::M[j] - br;
+        e[j] = (uint32_t)x;
+        br = (uint32_t)(x >> 63);
+    }
+    fq29 r = q_const(Q29::ONE);
+    for (int j = 7; j >= 0; j--)
+        for (int b = 31; b >= 0; b--) {
+            r = q_sqr(r);
+            if ((e[j] >> b) & 1) r = q_mul(r, a);
+        }
+    return r;
+}
+
 // 8 x 32-bit limbs (any value < 2^256) -> 9 x 29-bit limbs of the same integer
 __device__ __forceinline__ fq29 q_unpack(const uint32_t w[8]) {
     fq29 r;

@@ -509,14 +509,6 @@ __device__ __forceinline__ void fb_work(const uint32_t* work, uint32_t wid, int 
     }
 }
 
-__device__ __forceinline__ void fb_store_coord(uint32_t* d, const fq& v) {
-    uint32_t w[8];
-    q_pack(q_canon(q_from_fq(v)), w);
-    uint4* o = (uint4*)d;
-    o[0] = make_uint4(w[0], w[1], w[2], w[3]);
-    o[1] = make_uint4(w[4], w[5], w[6], w[7]);
-}
-
 __device__ __forceinline__ fq29 fb_load_coord(const uint32_t* s) {
     const uint4* p = (const uint4*)s;
     uint4 a = p[0], b = p[1];
@@ -524,28 +516,60 @@ __device__ __forceinline__ fq29 fb_load_coord(const uint32_t* s) {
     return q_unpack(w);
 }
 
-// Build the shifted copies of `count` consecutive bases of lines [line0, ...), one
-// thread per base: P_0 = P, P_(j+1) = 2^12 P_j, each stored affine.
-__global__ __launch_bounds__(64) void k_fb_build(const uint32_t* bases, uint32_t* table, int log_n, int h,
-                                                 uint32_t first, uint32_t count) {
+__device__ __forceinline__ void fb_store29(uint32_t* d, const fq29& v) {  // 256-bit integer, 8 u32
+    uint32_t w[8];
+    q_pack(v, w);
+    uint4* o = (uint4*)d;
+    o[0] = make_uint4(w[0], w[1], w[2], w[3]);
+    o[1] = make_uint4(w[4], w[5], w[6], w[7]);
+}
+
+// Build the shifted copies of `count` consecutive bases starting at global base index
+// `first` (= L * n + i), one thread per base: P_0 = P, P_(j+1) = 2^12 P_j, each stored
+// affine.  The chain runs in XYZZ; the 21 projective copies and the running products of
+// their ZZZ go to `scratch` (count * (FB_W - 1) * 40 u32), so that one inversion per base
+// (Montgomery's trick over its copies) converts all of them.  ZZ^3 = ZZZ^2 holds along
+// the chain (ZZ = Z^2, ZZZ = Z^3), so 1/ZZ = (ZZ / ZZZ)^2.
+__global__ __launch_bounds__(64) void k_fb_build(const uint32_t* bases, uint32_t* table, uint32_t* scratch,
+                                                 int log_n, int h, uint32_t first, uint32_t count) {
     uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= count) return;
-    const uint32_t p = first + t;  // global base index L * n + i
+    const uint32_t p = first + t;
     const int log_bs = log_n - h;
     const uint32_t L = p >> log_n, i = p & ((1u << log_n) - 1);
     const uint32_t b = i >> log_bs, k = i & ((1u << log_bs) - 1);
     uint32_t* dst = table + ((((((size_t)L << h) + b) * FB_W) << log_bs) + k) * 16;
     const size_t stride = (size_t)16 << log_bs;  // next copy of the same point
-    g1_aff a = aff_load(bases + (size_t)p * 16);
-    g1_xyzz q = xyzz_from_aff(a);
-    fb_store_coord(dst, a.x);
-    fb_store_coord(dst + 8, a.y);
+    auto slot = [&](int j) { return scratch + ((size_t)(j - 1) * count + t) * 40; };  // X, Y, ZZ, ZZZ, prefix
+    const g1_aff a = aff_load(bases + (size_t)p * 16);
+    a29 P;
+    P.x = q_canon(q_from_fq(a.x));
+    P.y = q_canon(q_from_fq(a.y));
+    fb_store29(dst, P.x);
+    fb_store29(dst + 8, P.y);
+    x29 q = x29_from_aff(P);
+    fq29 acc = q_const(Q29::ONE);
     for (int j = 1; j < FB_W; j++) {
-        for (int s = 0; s < FB_C; s++) q = xyzz_dbl(q);
-        bool inf;
-        a = xyzz_to_aff(q, &inf);  // never infinity: 2^(12j) is not a multiple of the group order
-        fb_store_coord(dst + j * stride, a.x);
-        fb_store_coord(dst + j * stride + 8, a.y);
+        for (int s = 0; s < FB_C; s++) q = x29_dbl(q);  // never infinity: 2^(12j) is not a multiple of r
+        acc = q_mul(acc, q.ZZZ);
+        uint32_t* sl = slot(j);
+        fb_store29(sl, q.X);
+        fb_store29(sl + 8, q.Y);
+        fb_store29(sl + 16, q.ZZ);
+        fb_store29(sl + 24, q.ZZZ);
+        fb_store29(sl + 32, acc);  // prod_{1..j} ZZZ
+    }
+    fq29 inv = q_inv(acc);  // 1 / prod_{1..FB_W-1} ZZZ
+    for (int j = FB_W - 1; j >= 1; j--) {
+        const uint32_t* sl = slot(j);
+        const fq29 X = fb_load_coord(sl), Y = fb_load_coord(sl + 8), ZZ = fb_load_coord(sl + 16),
+                   ZZZ = fb_load_coord(sl + 24);
+        const fq29 before = j > 1 ? fb_load_coord(slot(j - 1) + 32) : q_const(Q29::ONE);
+        const fq29 izzz = q_mul(inv, before);  // 1 / ZZZ_j
+        inv = q_mul(inv, ZZZ);                 // 1 / prod_{1..j-1} ZZZ
+        const fq29 izz = q_sqr(q_mul(ZZ, izzz));
+        fb_store29(dst + j * stride, q_canon(q_mul(X, izz)));
+        fb_store29(dst + j * stride + 8, q_canon(q_mul(Y, izzz)));
     }
 }
 

@@ -345,11 +345,18 @@ int build_work_lists(zgr_ctx* c) {
 int build_fb(zgr_ctx* c, int coset) {
     if (!c->fb) return ZGR_OK;
     if (!c->fbtab[coset]) DALLOC(c->fbtab[coset], (size_t)c->nlines * c->n * FB_W * 64);
+    // lane 0's bucket-sum buffer is the chain scratch: its stream runs the build and
+    // nothing else uses it meanwhile (the context lock is held)
     hipStream_t st = c->lanes[0].st;
-    const uint32_t per = (uint32_t)c->n;  // one launch per line
-    for (int L = 0; L < c->nlines; L++)
-        LAUNCH(c, st, "fb_build", k_fb_build, blocks(per, 64), 64, 0, c->bases[coset], c->fbtab[coset], c->log_n,
-               c->h, (uint32_t)L * per, per);
+    const size_t bkt_bytes = (size_t)c->nlines * c->nb * FB_REGION * X29_WORDS * 4;
+    const size_t per_base = (size_t)(FB_W - 1) * 40 * 4;
+    const size_t total = (size_t)c->nlines * c->n;
+    const size_t step = std::min(total, bkt_bytes / per_base);
+    for (size_t first = 0; first < total; first += step) {
+        const uint32_t count = (uint32_t)std::min(step, total - first);
+        LAUNCH(c, st, "fb_build", k_fb_build, blocks(count, 64), 64, 0, c->bases[coset], c->fbtab[coset],
+               c->lanes[0].bkt, c->log_n, c->h, (uint32_t)first, count);
+    }
     return ZGR_OK;
 }
 
@@ -819,6 +826,7 @@ int zgr_params_load(zgr_ctx* c, int coset, const void* bases_mont, size_t nbytes
     size_t want = (size_t)c->nlines * c->n * 64;
     if (nbytes != want) return fail(ZGR_ERR_INVALID_ARG, "params size %zu, expected %zu", nbytes, want);
     HIPCHK(hipSetDevice(c->device));
+    CHECK(sync_all(c));  // blobs still in flight read the bases and tables about to be replaced
     CHECK(alloc_bases(c, coset));
     c->loaded[coset] = false;
     HIPCHK(hipMemcpyAsync(c->bases[coset], bases_mont, want, hipMemcpyHostToDevice, c->lanes[0].st));
@@ -850,6 +858,7 @@ int zgr_params_load_bamt(zgr_ctx* c, int coset, const uint8_t* bamt, size_t nbyt
     if (nbytes != want)
         return fail(ZGR_ERR_INVALID_ARG, "bamt length %zu, expected %zu for degree %d", nbytes, want, degree);
     HIPCHK(hipSetDevice(c->device));
+    CHECK(sync_all(c));  // blobs still in flight read the bases and tables about to be replaced
     CHECK(alloc_bases(c, coset));
     c->loaded[coset] = false;
     hipStream_t st = c->lanes[0].st;
@@ -871,6 +880,7 @@ int zgr_params_generate(zgr_ctx* c, const uint8_t tau[32]) {
     if (!c || !tau) return fail(ZGR_ERR_INVALID_ARG, "null argument");
     std::lock_guard<std::mutex> lk(c->mu);
     HIPCHK(hipSetDevice(c->device));
+    CHECK(sync_all(c));
     hipStream_t st = c->lanes[0].st;
     uint32_t *table = nullptr, *tau_d = nullptr, *pc = nullptr;
     auto body = [&]() -> int {
