@@ -1,6 +1,6 @@
 // zg_encode_file -- encode raw blob files through the C++ host mirror (zg_encoder.hpp).
 //
-//   zg_encode_file LOG_COL LOG_ROW COSET_N PARAMS OUTDIR RAW... [--rows I,J,...]
+//   zg_encode_file LOG_COL LOG_ROW COSET_N PARAMS OUTDIR RAW... [--rows I,J,...] [--pb]
 //     PARAMS  tau:<64 hex digits, 32 B little-endian>   insecure setup from a known tau
 //             dir:<path>                                bamt prove-param files
 //                                                       (amt-prove-coset{c}-mont-5DWgDV-{h}-{n}.bin)
@@ -9,7 +9,10 @@
 //   reply's encoded_slice bytes of every row, concatenated in row order, and, with --rows,
 //   OUTDIR/k.rows = for each index: row || per level (commitment, proof) || high_commitment
 //   || commitment || merkle roots || merkle proof (leaf level first) || row root
-//   (EncodedBlob::get_row).
+//   (EncodedBlob::get_row), and OUTDIR/k.light = the LightEncodedSlice bytes of those rows.
+//   OUTDIR/k.pb is the reply as protobuf (the EncodeBlob response message).  With --pb
+//   each RAW file is an EncodeBlobRequest message: its data is encoded and its
+//   require_data decides whether k.pb carries encoded_data (k.reply always does).
 //   Exit codes: 0 ok, 2 usage, 3 TooLargeBlob, 4 other error.
 #include <cstdio>
 #include <cstdlib>
@@ -51,7 +54,7 @@ bool parse_tau(const std::string& hex, uint8_t tau[32]) {
 }
 
 int usage() {
-    fprintf(stderr, "usage: zg_encode_file LOG_COL LOG_ROW COSET_N tau:<hex>|dir:<path> OUTDIR RAW... [--rows I,J,...]\n");
+    fprintf(stderr, "usage: zg_encode_file LOG_COL LOG_ROW COSET_N tau:<hex>|dir:<path> OUTDIR RAW... [--rows I,J,...] [--pb]\n");
     return 2;
 }
 
@@ -63,12 +66,15 @@ int main(int argc, char** argv) {
     const std::string params = argv[4], outdir = argv[5];
     std::vector<std::string> raws;
     std::vector<size_t> rows;
+    bool pb = false;
     for (int i = 6; i < argc; i++) {
         std::string a = argv[i];
         if (a == "--rows" && i + 1 < argc) {
             std::stringstream ss(argv[++i]);
             std::string tok;
             while (std::getline(ss, tok, ',')) rows.push_back(std::stoul(tok));
+        } else if (a == "--pb") {
+            pb = true;
         } else {
             raws.push_back(a);
         }
@@ -87,9 +93,24 @@ int main(int argc, char** argv) {
         p.warmup();
         zg::EncoderService svc(std::move(p));
         for (size_t k = 0; k < raws.size(); k++) {
-            const std::vector<uint8_t> raw = read_file(raws[k]);
+            const std::vector<uint8_t> file = read_file(raws[k]);
+            zg::EncodeBlobRequest req;
+            if (pb) {
+                req = zg::EncodeBlobRequest::decode(file.data(), file.size());
+            } else {
+                req.data = file;
+                req.require_data = true;
+            }
+            const std::vector<uint8_t>& raw = req.data;
             const std::string stem = outdir + "/" + std::to_string(k);
             zg::EncodeBlobReply r = svc.process_data(raw.data(), raw.size(), true);
+            if (!req.require_data) {
+                zg::EncodeBlobReply lean = r;
+                lean.encoded_data.clear();
+                write_file(stem + ".pb", lean.encode());
+            } else {
+                write_file(stem + ".pb", r.encode());
+            }
             std::vector<uint8_t> reply;
             put(reply, r.erasure_commitment);
             put(reply, r.storage_root);
@@ -100,9 +121,10 @@ int main(int argc, char** argv) {
             write_file(stem + ".slices", slices);
             if (rows.empty()) continue;
             zg::EncodedBlob b = zg::EncodedBlob::build(zg::RawData::try_from(g, raw.data(), raw.size()), svc.params());
-            std::vector<uint8_t> out;
+            std::vector<uint8_t> out, light;
             for (size_t idx : rows) {
                 zg::EncodedSlice s = b.get_row(idx);
+                put(light, zg::LightEncodedSlice::from_slice(s).serialize());
                 for (const auto& e : s.row) put(out, e);
                 for (const auto& cp : s.amt_proof) {
                     put(out, cp.first);
@@ -115,6 +137,7 @@ int main(int argc, char** argv) {
                 put(out, s.row_root);
             }
             write_file(stem + ".rows", out);
+            write_file(stem + ".light", light);
         }
     } catch (const zg::TooLargeBlob& e) {
         fprintf(stderr, "%s\n", e.what());

@@ -200,6 +200,71 @@ struct EncodedSlice {
     }
 };
 
+// LightEncodedSlice (encoder/light_slice.rs:11-79): an EncodedSlice without its row.
+struct LightEncodedSlice {
+    size_t index = 0;
+    std::array<uint8_t, 64> amt_commitment{}, amt_high_commitment{};
+    std::vector<std::pair<std::array<uint8_t, 64>, std::array<uint8_t, 64>>> amt_proof;
+    std::vector<Bytes32> merkle_root, merkle_proof;
+    Bytes32 merkle_leaf{};
+
+    // from_slice (light_slice.rs:59-79); the slice's AMT and Merkle indices are its
+    // own index by construction (EncodedBlob::get_row)
+    static LightEncodedSlice from_slice(const EncodedSlice& s) {
+        LightEncodedSlice l;
+        l.index = s.index;
+        l.amt_commitment = s.commitment;
+        l.amt_proof = s.amt_proof;
+        l.amt_high_commitment = s.high_commitment;
+        l.merkle_root = s.merkle_root;
+        l.merkle_proof = s.merkle_proof;
+        l.merkle_leaf = s.row_root;
+        return l;
+    }
+
+    // into_slice (light_slice.rs:22-40): BlobRow.index = index % row_n
+    EncodedSlice into_slice(std::vector<Bytes32> row, size_t row_n) const {
+        EncodedSlice s;
+        s.index = index;
+        s.row_n = row_n;
+        s.row = std::move(row);
+        s.amt_proof = amt_proof;
+        s.high_commitment = amt_high_commitment;
+        s.commitment = amt_commitment;
+        s.merkle_root = merkle_root;
+        s.merkle_proof = merkle_proof;
+        s.row_root = merkle_leaf;
+        return s;
+    }
+
+    // serialize_uncompressed: index, amt_commitment, amt_proof, amt_high_commitment,
+    // merkle_root, merkle_proof, merkle_leaf
+    std::vector<uint8_t> serialize() const {
+        std::vector<uint8_t> o;
+        auto u64 = [&](uint64_t v) {
+            for (int i = 0; i < 8; i++) o.push_back((uint8_t)(v >> (8 * i)));
+        };
+        auto put = [&](const uint8_t* p, size_t n) { o.insert(o.end(), p, p + n); };
+        auto g1 = [&](const std::array<uint8_t, 64>& p) {
+            auto a = ark_g1_uncompressed(p);
+            put(a.data(), 64);
+        };
+        u64(index);
+        g1(amt_commitment);
+        u64(amt_proof.size());
+        for (const auto& cp : amt_proof) {
+            g1(cp.first);
+            g1(cp.second);
+        }
+        g1(amt_high_commitment);
+        for (const auto& m : merkle_root) put(m.data(), 32);
+        u64(merkle_proof.size());
+        for (const auto& m : merkle_proof) put(m.data(), 32);
+        put(merkle_leaf.data(), 32);
+        return o;
+    }
+};
+
 class EncodedBlob {
   public:
     // EncodedBlob::build over RawData (RawBlob packing happens on the device)
@@ -295,6 +360,88 @@ struct EncodeBlobReply {
     std::vector<uint8_t> storage_root;        // file root (service.rs:84)
     std::vector<uint8_t> encoded_data;        // 32 B per encoded element when require_data (service.rs:85-91)
     std::vector<std::vector<uint8_t>> encoded_slice;  // serialized get_row(i) for every row (service.rs:93-95)
+
+    // proto3 encoding of the message: fields with default values are not written,
+    // every element of the repeated field is
+    std::vector<uint8_t> encode() const {
+        std::vector<uint8_t> o;
+        size_t total = 16 + erasure_commitment.size() + storage_root.size() + encoded_data.size();
+        for (const auto& s : encoded_slice) total += 16 + s.size();
+        o.reserve(total);
+        if (version) {
+            proto_varint(o, 1 << 3);
+            proto_varint(o, version);
+        }
+        proto_bytes(o, 2, erasure_commitment, false);
+        proto_bytes(o, 3, storage_root, false);
+        proto_bytes(o, 4, encoded_data, false);
+        for (const auto& s : encoded_slice) proto_bytes(o, 5, s, true);
+        return o;
+    }
+
+    static void proto_varint(std::vector<uint8_t>& o, uint64_t v) {
+        while (v >= 0x80) {
+            o.push_back((uint8_t)(v | 0x80));
+            v >>= 7;
+        }
+        o.push_back((uint8_t)v);
+    }
+
+    static void proto_bytes(std::vector<uint8_t>& o, int field, const std::vector<uint8_t>& b, bool always) {
+        if (b.empty() && !always) return;
+        proto_varint(o, ((uint64_t)field << 3) | 2);
+        proto_varint(o, b.size());
+        o.insert(o.end(), b.begin(), b.end());
+    }
+};
+
+// EncodeBlobRequest (grpc/proto/encoder.proto:12-15): bytes data = 1; bool require_data = 2.
+struct EncodeBlobRequest {
+    std::vector<uint8_t> data;
+    bool require_data = false;
+
+    // proto3 decoding: unknown fields are skipped, a repeated scalar keeps its last
+    // value; malformed input throws std::invalid_argument
+    static EncodeBlobRequest decode(const uint8_t* p, size_t n) {
+        EncodeBlobRequest r;
+        size_t i = 0;
+        auto varint = [&]() -> uint64_t {
+            uint64_t v = 0;
+            for (int shift = 0; shift < 70; shift += 7) {
+                if (i >= n) throw std::invalid_argument("truncated varint");
+                uint8_t b = p[i++];
+                v |= (uint64_t)(b & 0x7f) << shift;
+                if (!(b & 0x80)) return v;
+            }
+            throw std::invalid_argument("varint too long");
+        };
+        auto skip = [&](size_t k) {
+            if (k > n - i) throw std::invalid_argument("truncated field");
+            i += k;
+        };
+        while (i < n) {
+            const uint64_t key = varint();
+            const uint64_t field = key >> 3, wt = key & 7;
+            if (field == 0) throw std::invalid_argument("field number 0");
+            if (wt == 0) {
+                const uint64_t v = varint();
+                if (field == 2) r.require_data = v != 0;
+                else if (field == 1) throw std::invalid_argument("field 1 is not a varint");
+            } else if (wt == 2) {
+                const uint64_t len = varint();
+                if (len > n - i) throw std::invalid_argument("truncated length-delimited field");
+                if (field == 1) r.data.assign(p + i, p + i + len);
+                else if (field == 2) throw std::invalid_argument("field 2 is not a varint");
+                i += len;
+            } else if (wt == 1 || wt == 5) {
+                if (field == 1 || field == 2) throw std::invalid_argument("fixed-width known field");
+                skip(wt == 1 ? 8 : 4);
+            } else {
+                throw std::invalid_argument("unsupported wire type");
+            }
+        }
+        return r;
+    }
 };
 
 class EncoderService {

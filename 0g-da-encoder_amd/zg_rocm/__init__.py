@@ -11,6 +11,8 @@ import os
 
 import numpy as np
 
+from . import wire
+
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "libzgr.so")
 
@@ -483,29 +485,54 @@ class EncodedBlob:
                 "commitment": self.get_affine_commitment(), "merkle_root": self.get_roots(),
                 "merkle_proof": mproof, "row_root": bytes(self.merkle_layer(g.log_row)[index])}
 
-    def encoded_slice(self, index):
-        """serialize_uncompressed(get_row(index)) (grpc/src/service.rs:93-95, 162-166):
-        EncodedSlice { index, amt: EncodedSliceAMT { index, commitment, row: BlobRow {
-        index, row, proof, high_commitment } }, merkle: EncodedSliceMerkle { root, proof,
-        leaf_index, leaf } } (encoder/slice.rs:15-20, amt/slice.rs:13-18,
-        amt/src/blob/encode.rs:203-209, merkle/slice.rs:12-20), fields in declaration
-        order; usize as u64 LE, Vec with a u64 LE length, arrays without one, Fr as 32 B
-        canonical LE, G1 affine as ark_g1_uncompressed."""
+    def slice_fields(self, index):
+        """get_row(index) as the EncodedSlice fields (encoder/blob.rs:44-49 with
+        EncodedSliceAMT / EncodedSliceMerkle::new), G1 values in wire form."""
         g = self.geo
         r = self.get_row(index)
-        u64 = lambda v: int(v).to_bytes(8, "little")  # noqa: E731
-        parts = [u64(index),                                   # EncodedSlice.index
-                 u64(index), ark_g1_uncompressed(r["commitment"]),  # EncodedSliceAMT.index, commitment
-                 u64(index % g.row_n),                         # BlobRow.index (row within the coset)
-                 u64(g.col_n), r["row"],                       # BlobRow.row
-                 u64(len(r["amt_proof"]))]                     # BlobRow.proof
-        for com, prf in r["amt_proof"]:
-            parts += [ark_g1_uncompressed(com), ark_g1_uncompressed(prf)]
-        parts.append(ark_g1_uncompressed(r["high_commitment"]))
-        parts += list(r["merkle_root"])                        # [Bytes32; COSET_N]
-        parts += [u64(len(r["merkle_proof"]))] + list(r["merkle_proof"])
-        parts += [u64(index), r["row_root"]]                   # leaf_index, leaf
-        return b"".join(parts)
+        g1 = ark_g1_uncompressed
+        return {"index": index, "amt_index": index, "commitment": g1(r["commitment"]),
+                "row_index": index % g.row_n, "row": r["row"],
+                "amt_proof": [(g1(c), g1(p)) for c, p in r["amt_proof"]],
+                "high_commitment": g1(r["high_commitment"]), "merkle_root": r["merkle_root"],
+                "merkle_proof": r["merkle_proof"], "leaf_index": index, "leaf": r["row_root"]}
+
+    def encoded_slice(self, index):
+        """serialize_uncompressed(get_row(index)) (grpc/src/service.rs:93-95, 162-166)."""
+        return wire.serialize_encoded_slice(self.slice_fields(index))
+
+    def encoded_slices(self):
+        """encoded_slice(i) for every row at once, as a (row_encoded, slice bytes) uint8
+        array: the same layout gathered with whole-array indexing."""
+        g = self.geo
+        R, h = g.row_encoded, g.h
+        idx = np.arange(R, dtype=np.uint64)
+        coset, local = idx // g.row_n, idx % g.row_n
+        rev = np.zeros(R, dtype=np.uint64)
+        for b in range(h):
+            rev |= ((local >> np.uint64(b)) & np.uint64(1)) << np.uint64(h - 1 - b)
+        u64 = lambda v: np.ascontiguousarray(v.astype("<u8")).view(np.uint8).reshape(R, 8)  # noqa: E731
+        ctree = ark_g1_uncompressed_array(self.out.comm_tree.reshape(-1, 64))
+        ptree = ark_g1_uncompressed_array(self.out.proof_tree.reshape(-1, 64))
+        cols = [u64(idx), u64(idx), np.broadcast_to(ark_g1_uncompressed_array(self.out.commitment.reshape(1, 64)), (R, 64)),
+                u64(local), u64(np.full(R, g.col_n, dtype=np.uint64)),
+                self.out.data.reshape(R, g.col_n * 32), u64(np.full(R, h, dtype=np.uint64))]
+        for d in range(h):
+            at = (coset * np.uint64(g.tree_pts) + np.uint64((2 << d) - 2) + ((rev >> np.uint64(h - 1 - d)) ^ np.uint64(1)))
+            at = at.astype(np.int64)
+            cols += [ctree[at], ptree[at]]
+        high = ark_g1_uncompressed_array(self.out.high.reshape(-1, 64))
+        cols.append(high[coset.astype(np.int64)])
+        cols.append(np.broadcast_to(self.merkle_layer(0).reshape(1, -1), (R, 32 * g.coset_n)))
+        cols.append(u64(np.full(R, g.log_row, dtype=np.uint64)))
+        for d in range(g.log_row, 0, -1):
+            cols.append(self.merkle_layer(d)[((idx >> np.uint64(g.log_row - d)) ^ np.uint64(1)).astype(np.int64)])
+        cols += [u64(idx), self.merkle_layer(g.log_row)]
+        return np.concatenate([np.asarray(c, dtype=np.uint8) for c in cols], axis=1)
+
+    def light_slice(self, index):
+        """serialize_uncompressed(get_row(index).into_light_slice()) (encoder/slice.rs:117-119)."""
+        return wire.serialize_light_slice(self.slice_fields(index))
 
 
 def ark_g1_uncompressed(p64):
@@ -520,20 +547,68 @@ def ark_g1_uncompressed(p64):
     return bytes(b)
 
 
-class EncoderService:
-    """EncoderService::process_data (grpc/src/service.rs:67-106)."""
+def ark_g1_uncompressed_array(pts):
+    """ark_g1_uncompressed over an (n, 64) uint8 array of canonical points."""
+    out = np.array(pts, dtype=np.uint8, copy=True).reshape(-1, 64)
+    y = out[:, 32:].copy().view("<u8")  # (n, 4) little-endian limbs
+    half = [((P_MOD - 1) // 2 >> (64 * k)) & ((1 << 64) - 1) for k in range(4)]
+    gt = np.zeros(len(out), dtype=bool)
+    eq = np.ones(len(out), dtype=bool)
+    for k in range(3, -1, -1):
+        hk = np.uint64(half[k])
+        gt |= eq & (y[:, k] > hk)
+        eq &= y[:, k] == hk
+    zero = ~out.any(axis=1)
+    out[gt, 63] |= 0x80
+    out[zero, 63] = 0x40
+    return out
 
-    def __init__(self, ctx):
+
+class EncoderService:
+    """EncoderService::process_data (grpc/src/service.rs:67-106).
+
+    With a Server, requests from several threads (the gRPC workers) share the
+    device through its admission queue and overlap; without one, each call encodes
+    on the context directly."""
+
+    VERSION = 0  # service.rs:98
+
+    def __init__(self, ctx, server=None):
         self.ctx = ctx
+        self.server = server
+
+    def _build(self, data, require_data):
+        g = self.ctx.geo
+        if len(data) > g.max_raw_data_size:  # RawData::try_from (raw_data.rs:55-73)
+            raise TooLargeBlob(len(data), g.max_raw_data_size)
+        if self.server is None:
+            return EncodedBlob(g, self.ctx.encode(data))
+        # the slices need the encoded rows even when the reply omits them
+        t = self.server.submit(data, True)
+        return EncodedBlob(g, self.server.poll(t))
 
     def process_data(self, data, require_data):
-        blob = EncodedBlob.build(self.ctx, data)
-        c = blob.get_affine_commitment()
+        """-> dict with the EncodeBlobReply fields (encoded_slice as a list of bytes)
+        and the EncodedBlob under "blob"."""
+        blob = self._build(data, require_data)
+        rows = blob.encoded_slices()
         return {
-            "version": 0,
-            "erasure_commitment": c,  # x || y, 32 B LE each (service.rs:77-83)
+            "version": self.VERSION,
+            "erasure_commitment": blob.get_affine_commitment(),  # x || y, 32 B LE each (service.rs:77-83)
             "storage_root": blob.get_file_root(),
             "encoded_data": bytes(blob.get_data()) if require_data else b"",
-            "encoded_slice": [blob.encoded_slice(i) for i in range(self.ctx.geo.row_encoded)],
+            "encoded_slice": [r.tobytes() for r in rows],
             "blob": blob,
         }
+
+    def encode_blob(self, data, require_data):
+        """The EncodeBlobReply protobuf bytes for one request (the gRPC handler's
+        body), serialized straight from the gathered slice array."""
+        blob = self._build(data, require_data)
+        return wire.encode_reply({
+            "version": self.VERSION,
+            "erasure_commitment": blob.get_affine_commitment(),
+            "storage_root": blob.get_file_root(),
+            "encoded_data": blob.get_data() if require_data else b"",
+            "encoded_slice": blob.encoded_slices(),
+        })

@@ -16,6 +16,7 @@ import oracle as O
 import zg_rocm as Z
 from helpers import DATA_SEED, OracleOutputs, bamt_image, oracle_params, seeded_tau
 from pyref import keccak256_ref
+from zg_rocm import wire
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 BIN = os.path.join(HERE, "..", "0g-da-encoder_amd", "lib", "zg_encode_file")
@@ -99,10 +100,18 @@ def raws(tmp_path_factory):
 ROWS = [0, GEO.row_n - 1, GEO.row_n + 5, GEO.row_encoded - 1]
 
 
-def _check_outputs(outdir, data):
+def _check_outputs(outdir, data, require=None):
     params = list(oracle_params(GEO.log_col, GEO.log_row, GEO.coset_n))
     for k, r in enumerate(data):
         exp = O.encode(params, GEO.log_col, GEO.log_row, r)
+        # the protobuf reply against the Python codec over the oracle's outputs
+        blob = Z.EncodedBlob(GEO, OracleOutputs(exp))
+        req = True if require is None else require[k]
+        want_pb = wire.encode_reply({"version": 0, "erasure_commitment": exp.commitment.tobytes(),
+                                     "storage_root": exp.root.tobytes(),
+                                     "encoded_data": exp.data.tobytes() if req else b"",
+                                     "encoded_slice": blob.encoded_slices()})
+        assert (outdir / ("%d.pb" % k)).read_bytes() == want_pb
         reply = (outdir / ("%d.reply" % k)).read_bytes()
         assert reply[:64] == exp.commitment.tobytes()
         assert reply[64:96] == exp.root.tobytes()
@@ -117,6 +126,8 @@ def _check_outputs(outdir, data):
             got = rows[j * per:(j + 1) * per]
             assert got == expected_row(exp, idx), "row %d of blob %d" % (idx, k)
             verify_merkle_slice(got, idx, exp.root.tobytes())
+        light = (outdir / ("%d.light" % k)).read_bytes()
+        assert light == b"".join(blob.light_slice(idx) for idx in ROWS)
 
 
 @pytest.mark.gpu
@@ -142,6 +153,24 @@ def test_host_mirror_from_bamt_dir(tmp_path, raws):
     res = _run(_geo_args() + ["dir:%s" % pdir, out] + paths + ["--rows", ",".join(map(str, ROWS))])
     assert res.returncode == 0, res.stderr.decode()
     _check_outputs(out, data)
+
+
+@pytest.mark.gpu
+def test_host_mirror_protobuf_requests(tmp_path, raws):
+    """EncodeBlobRequest files in, EncodeBlobReply protobuf out (require_data both ways)."""
+    _, data = raws
+    reqs = []
+    require = [k % 2 == 0 for k in range(len(data))]
+    for k, r in enumerate(data):
+        p = tmp_path / ("%d.req" % k)
+        p.write_bytes(wire.encode_request(r, require[k]))
+        reqs.append(str(p))
+    out = tmp_path / "out"
+    out.mkdir()
+    tau_hex = (seeded_tau() % O.R_MOD).to_bytes(32, "little").hex()
+    res = _run(_geo_args() + ["tau:" + tau_hex, out] + reqs + ["--rows", ",".join(map(str, ROWS)), "--pb"])
+    assert res.returncode == 0, res.stderr.decode()
+    _check_outputs(out, data, require)
 
 
 @pytest.mark.gpu

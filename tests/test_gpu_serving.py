@@ -73,3 +73,39 @@ def test_server_admission_errors(ctx):
     # the context is usable again once the server is gone
     out = ctx.encode(b"abc")
     assert out.commitment.any()
+
+
+def test_grpc_front_end_on_device(ctx):
+    """encoder.Encoder/EncodeBlob over loopback, served from the device through
+    zgr_server by concurrent gRPC workers; every reply equals the oracle's."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    import grpc
+
+    from helpers import OracleOutputs
+    from zg_rocm import grpc_front, wire
+
+    params = list(oracle_params(GEO.log_col, GEO.log_row, GEO.coset_n))
+    lengths = [0, 1, 1234, 40000, GEO.max_raw_data_size, 777]
+    raws = [Z.seeded_bytes(DATA_SEED + 90 + i, L) for i, L in enumerate(lengths)]
+    srv = Z.Server(ctx, hbm_budget=3 * GEO.max_blob_size, max_inflight=4)
+    server, port = grpc_front.serve(Z.EncoderService(ctx, srv), "127.0.0.1:0", max_workers=4)
+    try:
+        with grpc_front.EncoderClient("127.0.0.1:%d" % port) as client:
+            with ThreadPoolExecutor(4) as pool:
+                replies = list(pool.map(lambda kr: client.encode_blob_raw(kr[1], kr[0] % 2 == 0, timeout=60),
+                                        enumerate(raws)))
+            with pytest.raises(grpc.RpcError) as ei:
+                client.encode_blob(bytes(GEO.max_raw_data_size + 1), timeout=60)
+            assert ei.value.code() == grpc.StatusCode.UNKNOWN
+            assert ei.value.details().startswith("TooLargeBlob { actual: %d" % (GEO.max_raw_data_size + 1))
+    finally:
+        server.stop(0)
+        srv.close()
+    for k, (r, got) in enumerate(zip(raws, replies)):
+        exp = O.encode(params, GEO.log_col, GEO.log_row, r)
+        want = wire.encode_reply({"version": 0, "erasure_commitment": exp.commitment.tobytes(),
+                                  "storage_root": exp.root.tobytes(),
+                                  "encoded_data": exp.data.tobytes() if k % 2 == 0 else b"",
+                                  "encoded_slice": Z.EncodedBlob(GEO, OracleOutputs(exp)).encoded_slices()})
+        assert got == want, "request %d (length %d)" % (k, len(r))
