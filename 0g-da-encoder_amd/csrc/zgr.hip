@@ -26,6 +26,7 @@
 #include <algorithm>
 #include <map>
 #include <mutex>
+#include <random>
 #include <string>
 #include <thread>
 #include <unordered_map>
@@ -33,6 +34,7 @@
 
 #include "../../include/zgr.h"
 #include "kernels.h"
+#include "recover.h"
 
 using namespace zgr;
 
@@ -1245,6 +1247,128 @@ int zgr_merkle(zgr_ctx* c, const uint8_t* data, size_t n_elems, int levels, uint
     int rc = body();
     (void)hipStreamSynchronize(st);
     for (auto* p : layers) (void)hipFree(p);
+    return rc;
+}
+
+int zgr_recover(zgr_ctx* c, const uint32_t* row_ids, const uint8_t* rows, size_t n_rows, uint8_t* raw_out,
+                size_t raw_cap, size_t* raw_len) {
+    if (!c || (n_rows && (!row_ids || !rows)) || !raw_out || !raw_len) return fail(ZGR_ERR_INVALID_ARG, "null argument");
+    const int lc = c->log_col, h = c->log_row, log_n = c->log_n;
+    const size_t ROW = (size_t)1 << h, COL = (size_t)1 << lc, n = c->n;
+    const size_t max_raw = 31 * n - 4;
+    if (lc < 1 || lc > 11) return fail(ZGR_ERR_INVALID_ARG, "recovery needs 1 <= log_col <= 11, context has %d", lc);
+    if (raw_cap < max_raw) return fail(ZGR_ERR_INVALID_ARG, "raw_out holds %zu bytes, needs %zu", raw_cap, max_raw);
+    for (size_t k = 1; k < n_rows; k++)
+        if (row_ids[k] <= row_ids[k - 1]) return fail(ZGR_ERR_INVALID_ARG, "row ids must be strictly increasing");
+    std::lock_guard<std::mutex> lk(c->mu);
+    HIPCHK(hipSetDevice(c->device));
+    hipStream_t st = c->lanes[0].st;
+    std::vector<void*> bufs;
+    auto body = [&]() -> int {
+        auto alloc = [&](auto*& p, size_t bytes) -> int {
+            CHECK(dmalloc((void**)&p, bytes ? bytes : 32));
+            bufs.push_back((void*)p);
+            return ZGR_OK;
+        };
+        int* flag = nullptr;
+        uint8_t* d_rows = nullptr;
+        CHECK(alloc(flag, sizeof(int)));
+        CHECK(alloc(d_rows, n_rows * COL * 32));
+        LAUNCH(c, st, "recover", k_rec_clear, 1, 64, 0, flag);
+        if (n_rows) HIPCHK(hipMemcpyAsync(d_rows, rows, n_rows * COL * 32, hipMemcpyHostToDevice, st));
+        int hflag = 0;
+        // raw_slice_to_line runs over every row before the row count is looked at (lib.rs:21-36)
+        if (n_rows)
+            LAUNCH(c, st, "recover", k_rec_canon, blocks(n_rows * COL, 256), 256, 0, d_rows, (uint64_t)(n_rows * COL),
+                   flag);
+        HIPCHK(hipMemcpyAsync(&hflag, flag, sizeof(int), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        if (hflag & 1) return fail(ZGR_ERR_INVALID_ARG, "row element is not a canonical field element (the reference panics)");
+        if (n_rows < ROW) return fail(ZGR_ERR_INVALID_ARG, "Not enough valid lines");
+        if (row_ids[n_rows - 1] >= ROW * (size_t)c->coset_n)
+            return fail(ZGR_ERR_INVALID_ARG, "Cannot recover data: RowIdOverflow");
+        // interpolation set S = the first ROW rows (every received coset-0 row is among
+        // them), extra rows X = the rest, targets = the erased coset-0 rows, then X
+        const uint32_t m = (uint32_t)ROW, nx = (uint32_t)(n_rows - ROW);
+        std::vector<uint32_t> erased;
+        size_t n0 = 0;
+        while (n0 < n_rows && row_ids[n0] < ROW) n0++;
+        for (uint32_t l = 0, k = 0; l < ROW; l++) {
+            if (k < n0 && row_ids[k] == l) k++;
+            else erased.push_back(l);
+        }
+        const uint32_t e0 = (uint32_t)erased.size(), nt = e0 + nx;
+        uint64_t g[4];  // gamma < 2^252 < r, fresh for every call
+        {
+            std::random_device rd;
+            for (auto& w : g) w = ((uint64_t)rd() << 32) | rd();
+            g[3] &= 0x0fffffffffffffffull;
+        }
+        uint32_t *d_ids, *d_gamma, *d_cpow, *d_gpow, *d_u, *d_V, *d_yv, *d_b, *d_t, *d_A, *d_L, *d_out, *d_er,
+            *d_blob;
+        uint8_t* d_raw;
+        CHECK(alloc(d_ids, n_rows * 4));
+        CHECK(alloc(d_gamma, 32));
+        CHECK(alloc(d_cpow, (size_t)c->coset_n * COL * 32));
+        CHECK(alloc(d_gpow, COL * 32));
+        CHECK(alloc(d_u, n_rows * 32));
+        CHECK(alloc(d_V, n_rows * COL * 32));
+        CHECK(alloc(d_yv, n_rows * 32));
+        CHECK(alloc(d_b, (size_t)m * 32));
+        CHECK(alloc(d_t, (size_t)nt * 32));
+        CHECK(alloc(d_A, (size_t)nt * 32));
+        CHECK(alloc(d_L, (size_t)nt * m * 32));
+        CHECK(alloc(d_out, (size_t)e0 * COL * 32));
+        CHECK(alloc(d_er, (size_t)e0 * 4));
+        CHECK(alloc(d_blob, n * 32));
+        CHECK(alloc(d_raw, 31 * n));
+        HIPCHK(hipMemcpyAsync(d_ids, row_ids, n_rows * 4, hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(d_gamma, g, 32, hipMemcpyHostToDevice, st));
+        if (e0) HIPCHK(hipMemcpyAsync(d_er, erased.data(), (size_t)e0 * 4, hipMemcpyHostToDevice, st));
+        const uint32_t nr = (uint32_t)n_rows;
+        LAUNCH(c, st, "recover", k_rec_consts, blocks((size_t)(c->coset_n + 1) * COL, 64), 64, 0, c->consts, d_gamma,
+               c->coset_n, lc, d_cpow, d_gpow);
+        LAUNCH(c, st, "recover", k_rec_points, blocks(nr, 64), 64, 0, d_ids, nr, c->consts, c->tw, lc, h, d_u);
+        LAUNCH(c, st, "recover_rows", k_rec_rows, nr, 256, (COL + 256) * 32, d_rows, d_ids, d_cpow, d_gpow, c->tw_inv,
+               lc, h, d_V, d_yv);
+        LAUNCH(c, st, "recover", k_rec_bary, blocks(m, 64), 64, 0, d_u, m, d_b);
+        if (e0) LAUNCH(c, st, "recover", k_rec_targets, blocks(e0, 64), 64, 0, d_er, e0, c->tw, lc, h, d_t);
+        if (nx) HIPCHK(hipMemcpyAsync(d_t + 8 * (size_t)e0, d_u + 8 * (size_t)m, (size_t)nx * 32, hipMemcpyDeviceToDevice, st));
+        if (nt) {
+            const uint32_t chunks = (m + REC_CHUNK - 1) / REC_CHUNK;
+            LAUNCH(c, st, "recover", k_rec_node, blocks(nt, 64), 64, 0, d_t, nt, d_u, m, d_A);
+            LAUNCH(c, st, "recover_lagrange", k_rec_lagrange, blocks((size_t)nt * chunks, 64), 64, 0, d_t, nt, d_u, d_b,
+                   m, d_L);
+        }
+        if (e0) {
+            dim3 grid(blocks(COL, REC_TI), blocks(e0, REC_TL));
+            LAUNCH(c, st, "recover_gemm", k_rec_gemm, grid, 256, 0, d_L, d_V, e0, m, (uint32_t)COL, d_out);
+        }
+        if (nx) LAUNCH(c, st, "recover", k_rec_check, blocks(nx, 64), 64, 0, d_L + 8 * (size_t)e0 * m, d_A + 8 * (size_t)e0,
+                       d_yv, m, nx, flag);
+        if (n0) LAUNCH(c, st, "recover", k_rec_pass, blocks(n0 * COL, 256), 256, 0, d_rows, d_ids, (uint32_t)n0, lc, d_blob);
+        if (e0) LAUNCH(c, st, "recover_rows", k_rec_out, e0, 256, COL * 32, d_out, d_A, d_er, c->tw, lc, h, d_blob);
+        LAUNCH(c, st, "recover", k_rec_pack, blocks(31 * n / 4, 256), 256, 0, d_blob, (uint32_t)n, (uint32_t*)d_raw,
+               flag);
+        LAUNCH(c, st, "recover", k_rec_padding, blocks(max_raw, 256), 256, 0, d_raw, (uint64_t)max_raw, flag);
+        uint8_t lenb[4];
+        HIPCHK(hipMemcpyAsync(&hflag, flag, sizeof(int), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipMemcpyAsync(lenb, d_raw + max_raw, 4, hipMemcpyDeviceToHost, st));
+        CHECK(sync_all(c));
+        // the reference's order: the degree assert in data_poly, then RawBlob -> RawData
+        if (hflag & 2)
+            return fail(ZGR_ERR_INCONSISTENT, "rows do not lie on one polynomial of degree < N (the reference panics)");
+        if (hflag & 4) return fail(ZGR_ERR_INVALID_ARG, "Incorrect scalar");
+        if (hflag & 8) return fail(ZGR_ERR_INVALID_ARG, "Incorrect length field");
+        if (hflag & 16) return fail(ZGR_ERR_INVALID_ARG, "Non zero in padding range");
+        const size_t len = (size_t)lenb[0] | ((size_t)lenb[1] << 8) | ((size_t)lenb[2] << 16) | ((size_t)lenb[3] << 24);
+        if (len) HIPCHK(hipMemcpy(raw_out, d_raw, len, hipMemcpyDeviceToHost));
+        *raw_len = len;
+        return ZGR_OK;
+    };
+    int rc = body();
+    (void)hipStreamSynchronize(st);
+    for (void* p : bufs) (void)hipFree(p);
     return rc;
 }
 

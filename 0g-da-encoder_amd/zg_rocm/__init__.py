@@ -55,6 +55,7 @@ SIGNATURES = {
     "zgr_server_poll": (I, [P, ctypes.c_uint64, I]),
     "zgr_server_stats": (None, [P, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64),
                                 ctypes.POINTER(ctypes.c_uint64)]),
+    "zgr_recover": (I, [P, P, P, S, P, S, ctypes.POINTER(S)]),
     "zgr_kernel_time_ms": (ctypes.c_double, [P, ctypes.c_char_p, ctypes.POINTER(I)]),
     "zgr_kernel_time_reset": (None, [P]),
     "zgr_set_profiling": (None, [P, I]),
@@ -67,6 +68,7 @@ class ZgrError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__("zgr error %d: %s" % (code, msg))
         self.code = code
+        self.msg = msg
 
 
 class TooLargeBlob(ValueError):
@@ -337,6 +339,19 @@ class Context:
         _check(lib().zgr_merkle(self.h, _p(d), d.size // 32, levels, _p(layers), _p(root)))
         return layers.reshape(-1, 32), bytes(root)
 
+    def recover(self, row_ids, rows):
+        """zgr_recover: row_ids (strictly increasing) and their rows, one bytes-like of
+        32 * 2^log_col bytes each (or one concatenated buffer) -> the raw data."""
+        ids = np.ascontiguousarray(row_ids, dtype=np.uint32).reshape(-1)
+        buf = rows if isinstance(rows, (bytes, bytearray, memoryview)) else b"".join(bytes(r) for r in rows)
+        arr = np.frombuffer(bytes(buf), dtype=np.uint8) if len(buf) else np.zeros(1, dtype=np.uint8)
+        cap = self.geo.max_raw_data_size
+        out = np.zeros(cap, dtype=np.uint8)
+        n = ctypes.c_size_t(0)
+        _check(lib().zgr_recover(self.h, _p(ids if ids.size else np.zeros(1, dtype=np.uint32)), _p(arr), ids.size,
+                                 _p(out), cap, ctypes.byref(n)))
+        return out[:n.value].tobytes()
+
     # ---- profiling ----
     def set_profiling(self, on):
         lib().zgr_set_profiling(self.h, int(on))
@@ -545,6 +560,35 @@ def ark_g1_uncompressed(p64):
     elif int.from_bytes(bytes(b[32:]), "little") > (P_MOD - 1) // 2:
         b[63] |= 0x80
     return bytes(b)
+
+
+class RecoveryError(ValueError):
+    """An Err(String) of recover_from_da_slice; str() is the reference's text."""
+
+
+class RecoveryPanic(RuntimeError):
+    """An input on which the reference panics (an element >= r, or rows off one
+    polynomial of degree < N)."""
+
+
+def recover_from_da_slice(ctx, slices):
+    """recover_from_da_slice (crates/recovery/src/lib.rs:15-43) on the device.
+
+    slices: {row index: row bytes} (EncodedSlice::merkle_row, 32 B canonical per
+    element).  Rows of the wrong length are dropped (lib.rs:21-32); the rest go to
+    zgr_recover in key order.  Returns RawData::as_bytes; raises RecoveryError with
+    the reference's message, or RecoveryPanic where the reference panics."""
+    width = 32 * ctx.geo.col_n
+    ok = sorted((int(k), bytes(v)) for k, v in slices.items() if len(v) == width)
+    try:
+        return ctx.recover([k for k, _ in ok], b"".join(v for _, v in ok))
+    except ZgrError as e:
+        msg = e.msg
+        if e.code == ERR_INCONSISTENT or "panics" in msg:
+            raise RecoveryPanic(msg) from None
+        if e.code == ERR_INVALID_ARG:
+            raise RecoveryError(msg) from None
+        raise
 
 
 def ark_g1_uncompressed_array(pts):

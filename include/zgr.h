@@ -41,7 +41,8 @@ enum {
     ZGR_ERR_OOM = -3,            /* device allocation / admission failure */
     ZGR_ERR_DEVICE = -4,         /* HIP runtime error */
     ZGR_ERR_NOT_LOADED = -5,     /* params for a coset are not resident */
-    ZGR_ERR_INCONSISTENT = -6    /* coset commitments differ (crates/encoder/src/amt/blob.rs:42-47) */
+    ZGR_ERR_INCONSISTENT = -6    /* coset commitments differ (crates/encoder/src/amt/blob.rs:42-47);
+                                    recovery rows off one polynomial (crates/recovery/src/data_poly.rs:112) */
 };
 
 typedef struct zgr_ctx zgr_ctx;
@@ -158,6 +159,24 @@ int zgr_ecfft(zgr_ctx *ctx, const uint8_t *points, int log_n, int inverse, uint8
  * tree[0..=levels] and the file root (merkle/blob.rs:15-34, encoder/blob.rs:64-72). */
 int zgr_merkle(zgr_ctx *ctx, const uint8_t *data, size_t n_elems, int levels,
                uint8_t *out_layers, uint8_t *file_root);
+
+/* ---- RS recovery ---- */
+/* recover_from_da_slice (crates/recovery/src/lib.rs:15-43): the raw data of a blob
+ * from any 2^log_row of its encoded rows.  row_ids: n_rows strictly increasing row
+ * indices (the keys of the reference's BTreeMap); rows: n_rows * 32 * 2^log_col
+ * bytes, each row the canonical 32 B elements of EncodedSlice::merkle_row
+ * (raw_slice_to_line, crates/recovery/src/utils.rs:26-34).  Rows of another length
+ * are the caller's to drop, as the reference does (lib.rs:21-32).  raw_out (raw_cap
+ * >= MAX_RAW_DATA_SIZE = 31 * 2^log_n - 4) receives RawData::as_bytes, *raw_len its
+ * length.  Errors, in the reference's order (zgr_last_error gives its text):
+ *   ZGR_ERR_INVALID_ARG   an element >= r (the reference panics), "Not enough valid
+ *                         lines", "Cannot recover data: RowIdOverflow", "Incorrect
+ *                         scalar", "Incorrect length field", "Non zero in padding range";
+ *   ZGR_ERR_INCONSISTENT  the rows do not lie on one polynomial of degree < N (the
+ *                         reference's degree assert, data_poly.rs:112).
+ * Needs no params; runs synchronously on the context's stream. */
+int zgr_recover(zgr_ctx *ctx, const uint32_t *row_ids, const uint8_t *rows, size_t n_rows,
+                uint8_t *raw_out, size_t raw_cap, size_t *raw_len);
 
 /* ---- timing hooks for the benchmark (HIP events on the library's own stream) ---- */
 /* Average device time in ms of the named kernel over the calls since the last reset

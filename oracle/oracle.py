@@ -60,6 +60,7 @@ def lib():
             "zgo_merkle": (I, [P, S, I, I, P, P]),
             "zgo_raw_to_blob": (I, [P, S, I, P]),
             "zgo_encode": (I, [P, I, I, I, P, S, P, P, P, P, P, P, P]),
+            "zgo_recover": (I, [P, P, S, I, I, I, P, P]),
             "zgo_set_threads": (I, [I]),
         }
         for name, (res, args) in sig.items():
@@ -320,3 +321,34 @@ def encode(bases_list, log_col, log_row, raw):
     if rc == 2:
         raise AssertionError("coset commitments differ")
     return e
+
+
+# ---- RS recovery --------------------------------------------------------------
+RECOVER_STATUS = {
+    0: "Ok",
+    1: "Not enough valid lines",
+    2: "Cannot recover data: RowIdOverflow",
+    3: "Incorrect scalar",
+    4: "Incorrect length field",
+    5: "Non zero in padding range",
+    6: "panic: element out of range",
+    7: "panic: recovered degree above N",
+    8: "Cannot recover data: ExtaustiveK",
+}
+
+
+def recover(slices, log_col, log_row, coset_n=3):
+    """recover_from_da_slice (crates/recovery/src/lib.rs:15-43) over {row id: row bytes}
+    (each row 32 * 2^log_col canonical bytes, EncodedSlice::merkle_row).  Rows of the
+    wrong length are dropped first, as the reference does.  Returns (status, bytes):
+    the raw data for status 0, else b"" (RECOVER_STATUS names the outcome)."""
+    col = 1 << log_col
+    ok = sorted((int(k), bytes(v)) for k, v in slices.items() if len(v) == 32 * col)
+    ids = np.array([k for k, _ in ok], dtype=np.uint32) if ok else np.zeros(1, dtype=np.uint32)
+    rows = np.frombuffer(b"".join(v for _, v in ok), dtype=np.uint8).copy() if ok else np.zeros(1, dtype=np.uint8)
+    n = 1 << (log_col + log_row)
+    out = np.zeros(31 * n, dtype=np.uint8)
+    out_len = ctypes.c_size_t(0)
+    rc = lib().zgo_recover(_p(ids), _p(rows), len(ok), log_col, log_row, coset_n, _p(out),
+                           ctypes.byref(out_len))
+    return rc, (out[:out_len.value].tobytes() if rc == 0 else b"")

@@ -1177,3 +1177,245 @@ int zgo_encode(const uint64_t *const *bases, int log_col, int log_row, int coset
     free(ri);
     return rc;
 }
+
+/* ------------------------------------------------------------------ */
+/* RS recovery: recover_from_da_slice (crates/recovery/src/lib.rs:15-43) */
+/* ------------------------------------------------------------------ */
+
+/* A dense polynomial: coefficients c[0..len), len >= 1. */
+typedef struct { fe *c; size_t len; } rpoly;
+
+static int ilog2_sz(size_t n) {
+    int l = 0;
+    while (((size_t)1 << l) < n) l++;
+    return l;
+}
+
+/* Poly::dense_from_vec / to_vec (crates/recovery/src/poly.rs:236-246, 196-216):
+ * drop trailing zeros; the zero polynomial keeps one zero coefficient. */
+static void rp_trim(rpoly *p) {
+    while (p->len > 1 && fis_zero(&p->c[p->len - 1])) p->len--;
+}
+
+/* fx_to_fkx (crates/recovery/src/utils.rs:67-79): c_i -> c_i k^i */
+static void rp_scale_powers(fe *c, size_t len, const fe *k) {
+    fe base;
+    fset_one(&base, &FR);
+    for (size_t i = 0; i < len; i++) {
+        fmul(&c[i], &c[i], &base, &FR);
+        fmul(&base, &base, k, &FR);
+    }
+}
+
+/* coeffs_to_evals_coset (utils.rs:81-105): evaluations of f on coset idx of the
+ * N-domain (f(cw * w_N^s), s < N), then change_matrix_direction(ROW_LOG, COL_LOG)
+ * to row-major.  The FFT runs over max(next_pow2(len), N) points and keeps every
+ * (that / N)-th value. */
+static void rp_evals_coset(const rpoly *f, int log_col, int log_row, int idx, fe *out) {
+    int log_n = log_col + log_row;
+    size_t n = (size_t)1 << log_n;
+    int log_f = ilog2_sz(f->len);
+    if (log_f < log_n) log_f = log_n;
+    size_t F = (size_t)1 << log_f;
+    fe *a = (fe *)calloc(F, sizeof(fe));
+    memcpy(a, f->c, f->len * sizeof(fe));
+    if (idx) {
+        fe cw;
+        coset_factor_m(&cw, log_n, idx);
+        rp_scale_powers(a, f->len, &cw);
+    }
+    zgo_fft((uint64_t *)a, log_f);
+    size_t step = F / n;
+    for (size_t s = 0; s < n; s++) out[swap_bits(s, log_row, log_col)] = a[s * step];
+    free(a);
+}
+
+/* coeffs_to_evals_larger (utils.rs:107-111): the CM cosets one after the other */
+static void rp_evals_larger(const rpoly *f, int log_col, int log_row, int cm, fe *out) {
+    size_t n = (size_t)1 << (log_col + log_row);
+    for (int c = 0; c < cm; c++) rp_evals_coset(f, log_col, log_row, c, out + (size_t)c * n);
+}
+
+/* evals_to_poly (utils.rs:117-143): per chunk change_matrix_direction(COL_LOG, ROW_LOG),
+ * chunks in bit-reversed coset order, change_matrix_direction(log N, log CM), then
+ * the inverse DFT over the CM * N domain. */
+static rpoly rp_from_evals(const fe *evals, int log_col, int log_row, int cm) {
+    int log_n = log_col + log_row, log_cm = ilog2_sz((size_t)cm);
+    size_t n = (size_t)1 << log_n, big = n * (size_t)cm;
+    fe *t = (fe *)malloc(big * sizeof(fe));
+    for (int x = 0; x < cm; x++) {
+        const fe *chunk = evals + bitrev((size_t)x, log_cm) * n;
+        for (size_t i = 0; i < n; i++) {
+            size_t s = swap_bits(i, log_col, log_row); /* row-major -> natural order */
+            t[swap_bits((size_t)x * n + s, log_n, log_cm)] = chunk[i];
+        }
+    }
+    zgo_ifft((uint64_t *)t, log_n + log_cm);
+    rpoly p = {t, big};
+    rp_trim(&p);
+    return p;
+}
+
+/* Status codes of zgo_recover (the reference's outcome each stands for):
+ *   0 Ok(raw data)
+ *   1 Err("Not enough valid lines")                      lib.rs:34-36
+ *   2 Err("Cannot recover data: RowIdOverflow")          data_poly.rs:39-42, lib.rs:38-39
+ *   3 Err("Incorrect scalar")                            raw_blob.rs:47-49
+ *   4 Err("Incorrect length field")                      raw_data.rs:27-29
+ *   5 Err("Non zero in padding range")                   raw_data.rs:31-33
+ *   6 panic: an element >= r (MontConfig::from_bigint(..).unwrap(), utils.rs:26-34)
+ *   7 panic: recovered degree above N (assert, data_poly.rs:112)
+ *   8 Err("Cannot recover data: ExtaustiveK")            data_poly.rs:117
+ * rows: n_rows rows of 2^log_col canonical 32 B elements (merkle_row bytes), ids
+ * strictly increasing (a BTreeMap's keys).  out_raw: 31 * N bytes (the padded
+ * RawData); *out_len = its actual length. */
+int zgo_recover(const uint32_t *ids, const uint8_t *rows, size_t n_rows, int log_col, int log_row,
+                int coset_n, uint8_t *out_raw, size_t *out_len) {
+    const int log_n = log_col + log_row;
+    const size_t ROW = (size_t)1 << log_row, COL = (size_t)1 << log_col, n = (size_t)1 << log_n;
+    int cm = 1;
+    while (cm < coset_n) cm <<= 1; /* COSET_MORE = COSET_N.next_power_of_two() (zpoly.rs:13) */
+    const size_t big = n * (size_t)cm, cm_rows = ROW * (size_t)cm;
+    /* raw_slice_to_line (utils.rs:26-34) converts every row before the count is
+     * looked at (lib.rs:21-36) */
+    for (size_t e = 0; e < n_rows * COL; e++) {
+        uint64_t v[4];
+        memcpy(v, rows + e * 32, 32);
+        if (geq4(v, FR.m)) return 6;
+    }
+    if (n_rows < ROW) return 1;
+    if (ids[n_rows - 1] >= ROW * (size_t)coset_n) return 2; /* check_input (data_poly.rs:29-51) */
+    /* convert_input (data_poly.rs:53-70) */
+    fe *data = (fe *)calloc(big, sizeof(fe));
+    unsigned char *present = (unsigned char *)calloc(cm_rows, 1);
+    for (size_t k = 0; k < n_rows; k++) {
+        present[ids[k]] = 1;
+        for (size_t j = 0; j < COL; j++) {
+            uint64_t v[4];
+            memcpy(v, rows + (k * COL + j) * 32, 32);
+            fto_mont(&data[(size_t)ids[k] * COL + j], v, &FR);
+        }
+    }
+    /* zpoly (zpoly.rs:63-82): prod over erased rows of (x^COL + f0(row)), f0 from ZBlob
+     * (zpoly.rs:21-61): f0(c, l) = m0 * w_power^l, w_power = w_N^COL, m0 = -1 for
+     * coset 0 and -coset_factor(N, c)^COL otherwise.  A product of polynomials in
+     * x^COL: multiplied here in y = x^COL and spread out. */
+    fe wp, one, mone;
+    domain_root_m(&wp, log_n);
+    fpow_u64(&wp, &wp, (uint64_t)COL, &FR);
+    fset_one(&one, &FR);
+    fe zero_fe;
+    memset(&zero_fe, 0, sizeof zero_fe);
+    fsub(&mone, &zero_fe, &one, &FR);
+    size_t ne = 0;
+    for (size_t r = 0; r < cm_rows; r++) ne += !present[r];
+    fe *P = (fe *)calloc(ne + 1, sizeof(fe)); /* P(y) = prod (y + f0) */
+    P[0] = one;
+    size_t deg = 0;
+    for (size_t r = 0; r < cm_rows; r++) {
+        if (present[r]) continue;
+        size_t c = r / ROW, l = r % ROW;
+        fe m0, f0;
+        if (c == 0) {
+            m0 = mone;
+        } else {
+            fe cf;
+            coset_factor_m(&cf, log_n, (int)c);
+            fpow_u64(&cf, &cf, (uint64_t)COL, &FR);
+            fsub(&m0, &zero_fe, &cf, &FR);
+        }
+        fpow_u64(&f0, &wp, (uint64_t)l, &FR);
+        fmul(&f0, &f0, &m0, &FR);
+        /* P <- P * (y + f0) */
+        P[deg + 1] = P[deg];
+        for (size_t i = deg; i > 0; i--) {
+            fe t;
+            fmul(&t, &P[i], &f0, &FR);
+            fadd(&P[i], &P[i - 1], &t, &FR);
+        }
+        fmul(&P[0], &P[0], &f0, &FR);
+        deg++;
+    }
+    rpoly Z = {(fe *)calloc(deg * COL + 1, sizeof(fe)), deg * COL + 1};
+    for (size_t i = 0; i <= deg; i++) Z.c[i * COL] = P[i];
+    free(P);
+    free(present);
+    /* data_times_zpoly (data_times_zpoly.rs:14-32) */
+    fe *ev = (fe *)malloc(big * sizeof(fe));
+    rp_evals_larger(&Z, log_col, log_row, cm, ev);
+    for (size_t i = 0; i < big; i++) fmul(&ev[i], &ev[i], &data[i], &FR);
+    rpoly DZ = rp_from_evals(ev, log_col, log_row, cm);
+    free(data);
+    /* the retry loop over k (data_poly.rs:86-116); k = 5, 6, ... here in place of
+     * random draws: any k that is not a CM*N-th root of unity gives Z(kx) != 0 */
+    int rc = 8;
+    fe *zk = (fe *)malloc(big * sizeof(fe));
+    for (uint64_t kk = 5; kk < 105 && rc == 8; kk++) {
+        uint64_t kc[4] = {kk, 0, 0, 0};
+        fe k, kinv;
+        fto_mont(&k, kc, &FR);
+        finv(&kinv, &k, &FR);
+        rpoly Zk = {(fe *)malloc(Z.len * sizeof(fe)), Z.len};
+        memcpy(Zk.c, Z.c, Z.len * sizeof(fe));
+        rp_scale_powers(Zk.c, Zk.len, &k);
+        rp_evals_larger(&Zk, log_col, log_row, cm, zk);
+        free(Zk.c);
+        int has_zero = 0;
+        for (size_t i = 0; i < big && !has_zero; i++) has_zero = fis_zero(&zk[i]);
+        if (has_zero) continue;
+        { /* batch_inversion (Montgomery's trick) */
+            fe *pre = (fe *)malloc(big * sizeof(fe)), acc, inv;
+            fset_one(&acc, &FR);
+            for (size_t i = 0; i < big; i++) {
+                pre[i] = acc;
+                fmul(&acc, &acc, &zk[i], &FR);
+            }
+            finv(&inv, &acc, &FR);
+            for (size_t i = big; i-- > 0;) {
+                fe t;
+                fmul(&t, &inv, &pre[i], &FR);
+                fmul(&inv, &inv, &zk[i], &FR);
+                zk[i] = t;
+            }
+            free(pre);
+        }
+        rpoly DZk = {(fe *)malloc(DZ.len * sizeof(fe)), DZ.len};
+        memcpy(DZk.c, DZ.c, DZ.len * sizeof(fe));
+        rp_scale_powers(DZk.c, DZk.len, &k);
+        rp_evals_larger(&DZk, log_col, log_row, cm, ev);
+        free(DZk.c);
+        for (size_t i = 0; i < big; i++) fmul(&ev[i], &ev[i], &zk[i], &FR);
+        rpoly D = rp_from_evals(ev, log_col, log_row, cm);
+        rp_scale_powers(D.c, D.len, &kinv);
+        if (D.len > n + 1) { free(D.c); rc = 7; break; }
+        /* RawBlob::new(coeffs_to_evals(data_coeffs)) then RawData::try_from(RawBlob)
+         * (raw_blob.rs:43-57, raw_data.rs:14-38) */
+        fe *blob = (fe *)malloc(n * sizeof(fe));
+        rp_evals_coset(&D, log_col, log_row, 0, blob);
+        free(D.c);
+        rc = 0;
+        for (size_t i = 0; i < n && !rc; i++) {
+            uint64_t v[4];
+            ffrom_mont(v, &blob[i], &FR);
+            uint8_t b32[32];
+            memcpy(b32, v, 32);
+            if (b32[31]) rc = 3;
+            memcpy(out_raw + 31 * i, b32, 31);
+        }
+        free(blob);
+        if (!rc) {
+            size_t max_raw = 31 * n - 4;
+            uint32_t l32;
+            memcpy(&l32, out_raw + max_raw, 4);
+            if (l32 > max_raw) rc = 4;
+            for (size_t i = l32; i < max_raw && !rc; i++)
+                if (out_raw[i]) rc = 5;
+            *out_len = l32;
+        }
+    }
+    free(zk);
+    free(ev);
+    free(DZ.c);
+    free(Z.c);
+    return rc;
+}

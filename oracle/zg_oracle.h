@@ -113,6 +113,15 @@ int zgo_encode(const uint64_t *const *bases, int log_col, int log_row, int coset
                uint8_t *out_proof_tree, uint8_t *out_high, uint8_t *out_merkle,
                uint8_t *out_root);
 
+/* ---- RS recovery: recover_from_da_slice (crates/recovery/src/lib.rs:15-43) ---- */
+/* ids: n_rows strictly increasing row ids; rows: n_rows * 2^log_col canonical 32 B
+ * elements.  out_raw: 31 * 2^(log_col+log_row) bytes (padded RawData), *out_len its
+ * length.  Returns 0 ok, 1 not enough lines, 2 RowIdOverflow, 3 "Incorrect scalar",
+ * 4 "Incorrect length field", 5 "Non zero in padding range", 6 element >= r (the
+ * reference panics), 7 recovered degree above N (the reference asserts), 8 ExtaustiveK. */
+int zgo_recover(const uint32_t *ids, const uint8_t *rows, size_t n_rows, int log_col, int log_row,
+                int coset_n, uint8_t *out_raw, size_t *out_len);
+
 /* Thread count used by the parallel loops (OpenMP); returns the previous value. */
 int zgo_set_threads(int nthreads);
 

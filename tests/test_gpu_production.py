@@ -172,3 +172,28 @@ def test_windowed_path_equals_fixed_base(ctx, blob):
             assert np.array_equal(getattr(got, f), getattr(out0, f)), f
     finally:
         w.close()
+
+
+def test_recover_production(ctx, blob):
+    """recover_from_da_slice at 1024 x 1024: the raw data back from a third of the
+    rows (no coset-0 row among them), from a random 1100, and from all 3072; a
+    corrupted extra row is caught."""
+    raw, out = blob
+    w = 32 * GEO.col_n
+    rows = {i: out.data[i * w:(i + 1) * w].tobytes() for i in range(GEO.row_encoded)}
+    rng = np.random.default_rng(41)
+    sets = [list(range(GEO.row_encoded - GEO.row_n, GEO.row_encoded)),
+            sorted(int(i) for i in rng.choice(GEO.row_encoded, 1100, replace=False)),
+            list(range(GEO.row_encoded))]
+    for ids in sets:
+        t = time.time()
+        got = Z.recover_from_da_slice(ctx, {i: rows[i] for i in ids})
+        log("production recovery from %d rows: %.3f s" % (len(ids), time.time() - t))
+        assert got == raw
+    s = {i: rows[i] for i in sets[1]}
+    k = sets[1][-1]
+    b = bytearray(s[k])
+    b[77] ^= 4
+    s[k] = bytes(b)
+    with pytest.raises(Z.RecoveryPanic):
+        Z.recover_from_da_slice(ctx, s)
