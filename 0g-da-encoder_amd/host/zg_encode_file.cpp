@@ -1,6 +1,7 @@
 // zg_encode_file -- encode raw blob files through the C++ host mirror (zg_encoder.hpp).
 //
 //   zg_encode_file LOG_COL LOG_ROW COSET_N PARAMS OUTDIR RAW... [--rows I,J,...] [--pb]
+//                  [--recover I,J,...]
 //     PARAMS  tau:<64 hex digits, 32 B little-endian>   insecure setup from a known tau
 //             dir:<path>                                bamt prove-param files
 //                                                       (amt-prove-coset{c}-mont-5DWgDV-{h}-{n}.bin)
@@ -13,7 +14,9 @@
 //   OUTDIR/k.pb is the reply as protobuf (the EncodeBlob response message).  With --pb
 //   each RAW file is an EncodeBlobRequest message: its data is encoded and its
 //   require_data decides whether k.pb carries encoded_data (k.reply always does).
-//   Exit codes: 0 ok, 2 usage, 3 TooLargeBlob, 4 other error.
+//   With --recover, OUTDIR/k.recovered = recover_from_da_slice over the listed rows of
+//   the blob's own encoding (the reference's Err / panic text goes to stderr, exit 5).
+//   Exit codes: 0 ok, 2 usage, 3 TooLargeBlob, 4 other error, 5 recovery error.
 #include <cstdio>
 #include <cstdlib>
 #include <fstream>
@@ -54,7 +57,7 @@ bool parse_tau(const std::string& hex, uint8_t tau[32]) {
 }
 
 int usage() {
-    fprintf(stderr, "usage: zg_encode_file LOG_COL LOG_ROW COSET_N tau:<hex>|dir:<path> OUTDIR RAW... [--rows I,J,...] [--pb]\n");
+    fprintf(stderr, "usage: zg_encode_file LOG_COL LOG_ROW COSET_N tau:<hex>|dir:<path> OUTDIR RAW... [--rows I,J,...] [--pb] [--recover I,J,...]\n");
     return 2;
 }
 
@@ -65,7 +68,7 @@ int main(int argc, char** argv) {
     zg::Geometry g{atoi(argv[1]), atoi(argv[2]), atoi(argv[3])};
     const std::string params = argv[4], outdir = argv[5];
     std::vector<std::string> raws;
-    std::vector<size_t> rows;
+    std::vector<size_t> rows, recover_rows;
     bool pb = false;
     for (int i = 6; i < argc; i++) {
         std::string a = argv[i];
@@ -73,6 +76,10 @@ int main(int argc, char** argv) {
             std::stringstream ss(argv[++i]);
             std::string tok;
             while (std::getline(ss, tok, ',')) rows.push_back(std::stoul(tok));
+        } else if (a == "--recover" && i + 1 < argc) {
+            std::stringstream ss(argv[++i]);
+            std::string tok;
+            while (std::getline(ss, tok, ',')) recover_rows.push_back(std::stoul(tok));
         } else if (a == "--pb") {
             pb = true;
         } else {
@@ -119,6 +126,23 @@ int main(int argc, char** argv) {
             std::vector<uint8_t> slices;  // every encoded_slice, in row order
             for (const auto& sl : r.encoded_slice) put(slices, sl);
             write_file(stem + ".slices", slices);
+            if (!recover_rows.empty()) {
+                const size_t w = 32 * g.col_n();
+                std::map<size_t, std::vector<uint8_t>> lines;
+                for (size_t idx : recover_rows) {
+                    const size_t src = idx < g.row_encoded() ? idx : 0;  // ids past the blob: row 0's bytes
+                    lines[idx].assign(r.encoded_data.begin() + src * w, r.encoded_data.begin() + (src + 1) * w);
+                }
+                try {
+                    write_file(stem + ".recovered", zg::recover_from_da_slice(svc.params(), lines));
+                } catch (const zg::RecoveryError& e) {
+                    fprintf(stderr, "recovery error: %s\n", e.what());
+                    return 5;
+                } catch (const zg::RecoveryPanic& e) {
+                    fprintf(stderr, "recovery panic: %s\n", e.what());
+                    return 5;
+                }
+            }
             if (rows.empty()) continue;
             zg::EncodedBlob b = zg::EncodedBlob::build(zg::RawData::try_from(g, raw.data(), raw.size()), svc.params());
             std::vector<uint8_t> out, light;

@@ -15,6 +15,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -467,5 +468,41 @@ class EncoderService {
   private:
     EncoderParams params_;
 };
+
+// recover_from_da_slice (crates/recovery/src/lib.rs:15-43) on the device: the raw data
+// of a blob from any 2^log_row of its rows, keyed by row index (EncodedSlice::merkle_row
+// bytes).  Rows of the wrong length are dropped, as the reference does.  Throws
+// RecoveryError with the reference's Err text, RecoveryPanic where it panics.
+struct RecoveryError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+struct RecoveryPanic : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+inline std::vector<uint8_t> recover_from_da_slice(const EncoderParams& params,
+                                                  const std::map<size_t, std::vector<uint8_t>>& slices) {
+    const Geometry& g = params.geometry();
+    const size_t width = 32 * g.col_n();
+    std::vector<uint32_t> ids;
+    std::vector<uint8_t> rows;
+    for (const auto& kv : slices) {
+        if (kv.second.size() != width) continue;
+        if (kv.first > UINT32_MAX) throw RecoveryError("Cannot recover data: RowIdOverflow");
+        ids.push_back((uint32_t)kv.first);
+        rows.insert(rows.end(), kv.second.begin(), kv.second.end());
+    }
+    std::vector<uint8_t> out(31 * g.n() - 4);
+    size_t len = 0;
+    const int rc = zgr_recover(params.ctx(), ids.data(), rows.data(), ids.size(), out.data(), out.size(), &len);
+    if (rc != ZGR_OK) {
+        const std::string msg = zgr_last_error();
+        if (rc == ZGR_ERR_INCONSISTENT || msg.find("panics") != std::string::npos) throw RecoveryPanic(msg);
+        if (rc == ZGR_ERR_INVALID_ARG) throw RecoveryError(msg);
+        check(rc, "zgr_recover");
+    }
+    out.resize(len);
+    return out;
+}
 
 }  // namespace zg

@@ -6,7 +6,8 @@ the CPU oracle timed on a bounded sample of the same workload.
 
 A step is one encode of B seeded full-size blobs (raw bytes already resident in
 HBM) through the C ABI, outputs (encoded data, commitment, AMT proofs, Merkle
-tree) copied back to host memory.  Prints one JSON line.
+tree) copied back to host memory.  Also times RS recovery of the same blob from its
+rows (device and oracle).  Prints one JSON line.
 """
 import argparse
 import json
@@ -171,6 +172,32 @@ def main():
         if cnt:
             kernels[name] = {"avg_ms": round(ms, 4), "launches": cnt}
 
+    # RS recovery (recover_from_da_slice) of the plain blob from its encoded rows, host
+    # rows in and raw bytes out: the 1024 rows of coset 2 (every coset-0 row erased,
+    # the most work) and all 3072 rows (the consistency check over 2048 extra rows)
+    w = 32 * geo.col_n
+    rec_sets = {"coset2_1024_rows": list(range(2 * geo.row_n, 3 * geo.row_n)),
+                "all_3072_rows": list(range(geo.row_encoded))}
+    recovery = {}
+    for label, ids in rec_sets.items():
+        rows_b = b"".join(out1.data[i * w:(i + 1) * w].tobytes() for i in ids)
+        assert ctx.recover(ids, rows_b) == raws[0]
+        ctx.set_profiling(True)
+        ctx.kernel_time_reset()
+        tt = []
+        for _ in range(5):
+            s0 = time.perf_counter()
+            ctx.recover(ids, rows_b)
+            tt.append(time.perf_counter() - s0)
+        ctx.set_profiling(False)
+        ks = {}
+        for name in ("recover", "recover_rows", "recover_lagrange", "recover_gemm"):
+            ms, cnt = ctx.kernel_time_ms(name)
+            if cnt:
+                ks[name] = round(ms * cnt / 5, 3)  # per call, summed over its launches
+        recovery[label] = {"ms": round(float(np.median(tt)) * 1e3, 2), "kernel_ms_per_call": ks}
+    recovery["note"] = "zgr_recover, host rows in (PCIe included), raw bytes out; median of 5"
+
     # batched path: args.batch full-size blobs per call (two blobs in flight)
     nb_ = max(2, args.batch)
     braws = [Z.seeded_bytes(DATA_SEED + 1 + i, geo.max_raw_data_size) for i in range(nb_)]
@@ -295,9 +322,21 @@ def main():
             "valu_issue_frac": round(VALU_LANE_OPS / (acc_ms * 1e-3) / PEAK_MAC32, 4) if acc_ms else None,
         },
         "kernels": kernels,
+        "recovery": recovery,
     }
     if not args.no_cpu:
         res["cpu_baseline"] = cpu_baseline(ctx, geo, raws[0], args.cpu_threads)
+        # the oracle's recovery (the reference's 4N-domain algorithm) on the coset-2 rows
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+
+        O.set_threads(args.cpu_threads)
+        ids = rec_sets["coset2_1024_rows"]
+        t0 = time.time()
+        rc, got = O.recover({i: out1.data[i * w:(i + 1) * w].tobytes() for i in ids}, geo.log_col, geo.log_row)
+        recovery["coset2_1024_rows"]["cpu_oracle_ms"] = round((time.time() - t0) * 1e3, 1)
+        recovery["coset2_1024_rows"]["cpu_oracle_threads"] = args.cpu_threads
+        assert rc == 0 and got == raws[0]
     ctx.close()
     print(json.dumps(res))
 

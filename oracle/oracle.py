@@ -41,6 +41,7 @@ def lib():
             "zgo_to_mont": (None, [I, P, P]),
             "zgo_from_mont": (None, [I, P, P]),
             "zgo_finv": (None, [I, P, P]),
+            "zgo_from_mont_n": (None, [I, P, S, P]),
             "zgo_g1_mul_gen": (None, [P, P]),
             "zgo_g1_mul": (None, [P, P, P]),
             "zgo_g1_add": (None, [P, P, P]),
@@ -141,6 +142,14 @@ def fr_to_mont_array(vals):
     out = np.zeros((len(vals), 4), dtype=np.uint64)
     for k, v in enumerate(vals):
         out[k] = int_to_limbs(to_mont(0, v % R_MOD))
+    return out
+
+
+def fr_canonical_array(arr):
+    """uint64[n,4] mont -> uint64[n,4] canonical (bulk)."""
+    a = np.ascontiguousarray(arr, dtype=np.uint64).reshape(-1, 4)
+    out = np.zeros_like(a)
+    lib().zgo_from_mont_n(0, _p(a), a.shape[0], _p(out))
     return out
 
 

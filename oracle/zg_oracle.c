@@ -188,6 +188,15 @@ void zgo_from_mont(int which, const uint64_t a[4], uint64_t out[4]) {
     memcpy(x.v, a, 32);
     ffrom_mont(out, &x, pick(which));
 }
+void zgo_from_mont_n(int which, const uint64_t *a, size_t n, uint64_t *out) {
+#pragma omp parallel for num_threads(g_threads) schedule(static) if (n >= 4096)
+    for (size_t i = 0; i < n; i++) {
+        fe x;
+        memcpy(x.v, a + 4 * i, 32);
+        ffrom_mont(out + 4 * i, &x, pick(which));
+    }
+}
+
 void zgo_finv(int which, const uint64_t a[4], uint64_t out[4]) {
     fe x, z;
     memcpy(x.v, a, 32);

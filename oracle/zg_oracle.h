@@ -32,6 +32,8 @@ void zgo_fmul(int which, const uint64_t a[4], const uint64_t b[4], uint64_t out[
 void zgo_to_mont(int which, const uint64_t a[4], uint64_t out[4]);
 void zgo_from_mont(int which, const uint64_t a[4], uint64_t out[4]);
 void zgo_finv(int which, const uint64_t a[4], uint64_t out[4]);
+/* n elements out of Montgomery form */
+void zgo_from_mont_n(int which, const uint64_t *a, size_t n, uint64_t *out);
 
 /* ---- G1 primitives ---- */
 /* scalar (canonical) * generator -> canonical affine (64B, zeros = infinity) */

@@ -199,3 +199,23 @@ def test_host_mirror_usage_and_bad_params(tmp_path):
     assert _run([]).returncode == 2
     res = _run(_geo_args() + ["tau:zz", tmp_path, tmp_path / "x"])
     assert res.returncode == 4
+
+
+@pytest.mark.gpu
+def test_host_mirror_recovery(tmp_path, raws):
+    """zg::recover_from_da_slice (C++ mirror of crates/recovery/src/lib.rs:15-43) over
+    rows of the blob's own encoding: the raw bytes back; too few rows fail with the
+    reference's text."""
+    paths, data = raws
+    tau_hex = (seeded_tau() % O.R_MOD).to_bytes(32, "little").hex()
+    ids = list(range(GEO.row_n + 7, 2 * GEO.row_n + 7)) + [GEO.row_encoded - 1]
+    out = tmp_path / "ok"
+    out.mkdir()
+    res = _run(_geo_args() + ["tau:" + tau_hex, out] + paths + ["--recover", ",".join(map(str, ids))])
+    assert res.returncode == 0, res.stderr.decode()
+    for k, r in enumerate(data):
+        assert (out / ("%d.recovered" % k)).read_bytes() == r
+    few = tmp_path / "few"
+    few.mkdir()
+    res = _run(_geo_args() + ["tau:" + tau_hex, few, paths[1], "--recover", ",".join(map(str, ids[:GEO.row_n - 1]))])
+    assert res.returncode == 5 and b"Not enough valid lines" in res.stderr
