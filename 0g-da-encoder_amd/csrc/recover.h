@@ -34,23 +34,49 @@ __device__ __forceinline__ bool rec_canonical(const fr& a) {
     return false;
 }
 
-// Per-call constants:
-//   cpow[c * COL + i] = cf_c^-i / COL      (row iDFT scaling, c < coset_n)
-//   gpow[i]           = gamma^i             (random combination of a row's D_i)
-// gamma arrives canonical.  One thread per (c, i) plus COL threads for gpow.
-__global__ void k_rec_consts(const uint32_t* consts, const uint32_t* gamma_canon, int coset_n, int log_col,
-                             uint32_t* cpow, uint32_t* gpow) {
+// Per-call constants, in two steps:
+//   k_rec_inverses (one thread each): inv[c] = cf_c^-1 (c < coset_n), inv[coset_n] = 1 / COL,
+//                                     inv[coset_n + 1] = gamma (gamma arrives canonical)
+//   k_rec_consts: cpow[c * COL + i] = cf_c^-i / COL (row iDFT scaling), gpow[i] = gamma^i
+//                 (random combination of a row's D_i); one thread per entry.
+// No Fermat inversion: cf_c is a power of the 2^28-th root, so cf_c^-1 = cf_c^(2^28 - 1),
+// and 1 / 2^k = r - (r - 1) / 2^k because 2^28 divides r - 1.
+__global__ void k_rec_inverses(const uint32_t* consts, const uint32_t* gamma_canon, int coset_n, int log_col,
+                               uint32_t* inv) {
+    const int t = threadIdx.x;
+    if (blockIdx.x) return;
+    if (t < coset_n) {
+        f_store(inv + 8 * t, f_pow_u64(f_load<FrP>(consts + 8 * (K_CF + t)), (1ull << 28) - 1));
+    } else if (t == coset_n) {
+        uint32_t q[8], d[8];
+        for (int j = 0; j < 8; j++) {  // (r - 1) >> log_col
+            const uint32_t lo = j == 0 ? FrP::M[0] - 1 : FrP::M[j];
+            const uint32_t hi = j + 1 < 8 ? FrP::M[j + 1] : 0;
+            q[j] = log_col ? (lo >> log_col) | (hi << (32 - log_col)) : lo;
+        }
+        uint32_t br = 0;
+        for (int j = 0; j < 8; j++) {
+            const uint64_t x = (uint64_t)FrP::M[j] - q[j] - br;
+            d[j] = (uint32_t)x;
+            br = (uint32_t)(x >> 63);
+        }
+        fr c;
+        for (int j = 0; j < 8; j++) c.v[j] = d[j];
+        f_store(inv + 8 * coset_n, f_to_mont(c));
+    } else if (t == coset_n + 1) {
+        f_store(inv + 8 * (coset_n + 1), f_to_mont(f_load<FrP>(gamma_canon)));
+    }
+}
+
+__global__ void k_rec_consts(const uint32_t* inv, int coset_n, int log_col, uint32_t* cpow, uint32_t* gpow) {
     const uint32_t col = 1u << log_col;
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t < (uint32_t)coset_n * col) {
         const uint32_t c = t >> log_col, i = t & (col - 1);
-        fr cfi = f_inv(f_load<FrP>(consts + 8 * (K_CF + c)));
-        fr colinv = f_inv(fr_from_u64(col));
-        f_store(cpow + 8 * (size_t)t, f_mul(f_pow_u64(cfi, i), colinv));
+        f_store(cpow + 8 * (size_t)t, f_mul(f_pow_u64(f_load<FrP>(inv + 8 * c), i), f_load<FrP>(inv + 8 * coset_n)));
     } else if (t < (uint32_t)(coset_n + 1) * col) {
         const uint32_t i = t - (uint32_t)coset_n * col;
-        fr g = f_to_mont(f_load<FrP>(gamma_canon));
-        f_store(gpow + 8 * (size_t)i, f_pow_u64(g, i));
+        f_store(gpow + 8 * (size_t)i, f_pow_u64(f_load<FrP>(inv + 8 * (coset_n + 1)), i));
     }
 }
 
@@ -137,35 +163,35 @@ __global__ __launch_bounds__(256) void k_rec_rows(const uint8_t* rows, const uin
     if (threadIdx.x == 0) f_store(yv + 8 * (size_t)k, red[0]);
 }
 
-// Barycentric weights of the interpolation set (the first m = ROW points of u):
-// b_r = 1 / prod_{r' != r} (u_r - u_r'), one thread per r
-__global__ void k_rec_bary(const uint32_t* u, uint32_t m, uint32_t* b) {
-    const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= m) return;
-    const fr ur = f_load<FrP>(u + 8 * (size_t)r);
+// One workgroup (256 threads) per point a = a_pts[blockIdx.x]:
+//   out = prod_{q < m, q != skip} (a - u_q), inverted when invert != 0,
+// with skip = blockIdx.x when self_skip (B_r = prod_{q != r} (u_r - u_q) over the
+// interpolation set, the inverse barycentric weights), else no skip (A(x) = prod_q (x - u_q)).
+// Strided partial products, then a tree in LDS.
+__global__ __launch_bounds__(256) void k_rec_prod(const uint32_t* a_pts, const uint32_t* u, uint32_t m, int self_skip,
+                                                  int invert, uint32_t* out) {
+    __shared__ fr part[256];
+    const uint32_t b = blockIdx.x;
+    const fr a = f_load<FrP>(a_pts + 8 * (size_t)b);
     fr p = f_one<FrP>();
-    for (uint32_t q = 0; q < m; q++)
-        if (q != r) p = f_mul(p, f_sub(ur, f_load<FrP>(u + 8 * (size_t)q)));
-    f_store(b + 8 * (size_t)r, f_inv(p));
-}
-
-// A(t) = prod_r (t - u_r) over the interpolation set, one thread per target
-__global__ void k_rec_node(const uint32_t* t_pts, uint32_t nt, const uint32_t* u, uint32_t m, uint32_t* A) {
-    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= nt) return;
-    const fr x = f_load<FrP>(t_pts + 8 * (size_t)t);
-    fr p = f_one<FrP>();
-    for (uint32_t q = 0; q < m; q++) p = f_mul(p, f_sub(x, f_load<FrP>(u + 8 * (size_t)q)));
-    f_store(A + 8 * (size_t)t, p);
+    for (uint32_t q = threadIdx.x; q < m; q += blockDim.x)
+        if (!(self_skip && q == b)) p = f_mul(p, f_sub(a, f_load<FrP>(u + 8 * (size_t)q)));
+    part[threadIdx.x] = p;
+    __syncthreads();
+    for (uint32_t o = blockDim.x >> 1; o > 0; o >>= 1) {
+        if (threadIdx.x < o) part[threadIdx.x] = f_mul(part[threadIdx.x], part[threadIdx.x + o]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) f_store(out + 8 * (size_t)b, invert ? f_inv(part[0]) : part[0]);
 }
 
 constexpr uint32_t REC_CHUNK = 64;  // entries of one batched inversion in k_rec_lagrange
 
-// L[t][r] = b_r / (x_t - u_r) (the Lagrange matrix without its row factor A(x_t)):
-// one thread per (target t, chunk of REC_CHUNK sources), Montgomery's trick over the
-// chunk with the prefix products parked in L itself.  m is a multiple of REC_CHUNK or
-// smaller than it.
-__global__ void k_rec_lagrange(const uint32_t* t_pts, uint32_t nt, const uint32_t* u, const uint32_t* b, uint32_t m,
+// L[t][r] = 1 / ((x_t - u_r) B_r), B_r = prod_{q != r} (u_r - u_q): the Lagrange matrix
+// without its row factor A(x_t), b_r = 1 / B_r folded into the same inversion.  One
+// thread per (target t, chunk of REC_CHUNK sources), Montgomery's trick over the chunk
+// with the prefix products parked in L itself.
+__global__ void k_rec_lagrange(const uint32_t* t_pts, uint32_t nt, const uint32_t* u, const uint32_t* B, uint32_t m,
                                uint32_t* L) {
     const uint32_t chunks = (m + REC_CHUNK - 1) / REC_CHUNK;
     const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
@@ -177,72 +203,167 @@ __global__ void k_rec_lagrange(const uint32_t* t_pts, uint32_t nt, const uint32_
     fr acc = f_one<FrP>();
     for (uint32_t r = r0; r < r1; r++) {
         f_store(row + 8 * (size_t)r, acc);  // prefix prod_{r0 <= q < r} d_q
-        acc = f_mul(acc, f_sub(x, f_load<FrP>(u + 8 * (size_t)r)));
+        acc = f_mul(acc, f_mul(f_sub(x, f_load<FrP>(u + 8 * (size_t)r)), f_load<FrP>(B + 8 * (size_t)r)));
     }
     fr inv = f_inv(acc);
     for (uint32_t r = r1; r-- > r0;) {
-        const fr d = f_sub(x, f_load<FrP>(u + 8 * (size_t)r));
-        const fr di = f_mul(inv, f_load<FrP>(row + 8 * (size_t)r));
+        const fr d = f_mul(f_sub(x, f_load<FrP>(u + 8 * (size_t)r)), f_load<FrP>(B + 8 * (size_t)r));
+        f_store(row + 8 * (size_t)r, f_mul(inv, f_load<FrP>(row + 8 * (size_t)r)));
         inv = f_mul(inv, d);
-        f_store(row + 8 * (size_t)r, f_mul(di, f_load<FrP>(b + 8 * (size_t)r)));
     }
 }
 
-constexpr int REC_TL = 16;  // GEMM tile: targets
-constexpr int REC_TI = 64;  // GEMM tile: D_i columns
-constexpr int REC_TK = 16;  // GEMM tile: sources per LDS stage
+// Fr in 29-bit limbs for the GEMM: products of two limbs stay below 2^58, so a 64-bit
+// column takes 63 of them (7 terms of 9 products each) between carry passes, and the
+// Montgomery reduction runs once per output instead of once per product.
+struct R29 {
+    static constexpr uint32_t P[9] = {0x10000001u, 0x1f0fac9fu, 0x0e5c2450u, 0x07d090f3u, 0x1585d283u,
+                                      0x02db40c0u, 0x00a6e141u, 0x0e5c2634u, 0x0030644eu};
+    static constexpr uint32_t INV = 0x0fffffffu;  // -r^-1 mod 2^29
+    // r * 2^k for k = 5..0 (the final reduction of a value below 2^259)
+    static constexpr uint32_t RK[6][9] = {
+        {0x00000020u, 0x01f593f0u, 0x0b848a1fu, 0x1a121e6eu, 0x10ba5067u, 0x1b681815u, 0x14dc2822u, 0x0b84c680u, 0x060c89ceu},
+        {0x00000010u, 0x10fac9f8u, 0x05c2450fu, 0x1d090f37u, 0x185d2833u, 0x0db40c0au, 0x0a6e1411u, 0x05c26340u, 0x030644e7u},
+        {0x00000008u, 0x187d64fcu, 0x12e12287u, 0x1e84879bu, 0x0c2e9419u, 0x16da0605u, 0x05370a08u, 0x12e131a0u, 0x01832273u},
+        {0x00000004u, 0x1c3eb27eu, 0x19709143u, 0x1f4243cdu, 0x16174a0cu, 0x0b6d0302u, 0x029b8504u, 0x197098d0u, 0x00c19139u},
+        {0x00000002u, 0x1e1f593fu, 0x1cb848a1u, 0x0fa121e6u, 0x0b0ba506u, 0x05b68181u, 0x014dc282u, 0x1cb84c68u, 0x0060c89cu},
+        {0x10000001u, 0x1f0fac9fu, 0x0e5c2450u, 0x07d090f3u, 0x1585d283u, 0x02db40c0u, 0x00a6e141u, 0x0e5c2634u, 0x0030644eu}};
+};
 
-// Out[t][i] = sum_{r < m} L[t][r] V[r][i] for t < nt, i < col: 256 threads, a
-// 16 x 64 output tile per workgroup, each thread 4 targets of one column.
+constexpr uint32_t R29_MASK = 0x1fffffffu;
+
+// 256-bit value -> 9 limbs of 29 bits
+__device__ __forceinline__ void rec_to29(const fr& a, uint32_t* o) {
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+        const int bit = 29 * k, w = bit >> 5, sh = bit & 31;
+        uint32_t v = a.v[w] >> sh;
+        if (sh > 3 && w + 1 < 8) v |= a.v[w + 1] << (32 - sh);
+        o[k] = v & R29_MASK;
+    }
+}
+
+constexpr int REC_TL = 4;   // GEMM tile: targets (one per wave)
+constexpr int REC_TI = 64;  // GEMM tile: D_i columns (one per lane)
+constexpr int REC_TK = 14;  // GEMM tile: sources per LDS stage (two carry passes)
+
+// Out[t][i] = sum_{r < m} L[t][r] V[r][i] for t < nt, i < col (Montgomery form in and
+// out, values < r): 256 threads, a 4 x 64 output tile per workgroup, one output per
+// thread.  The products accumulate unreduced in 18 64-bit columns; with L, V < r and
+// m <= 2^10 the sum is below 2^518, the single Montgomery reduction by 2^261 leaves it
+// below 2^257, and six conditional subtractions of r 2^k bring it below r.  That
+// reduction divides by 2^261 where the Montgomery product divides by 2^256, so the
+// result is doubled five times.
 __global__ __launch_bounds__(256) void k_rec_gemm(const uint32_t* L, const uint32_t* V, uint32_t nt, uint32_t m,
                                                   uint32_t col, uint32_t* Out) {
-    __shared__ fr Ls[REC_TL][REC_TK];
-    __shared__ fr Vs[REC_TK][REC_TI];
-    const uint32_t tx = threadIdx.x % REC_TI, ty = threadIdx.x / REC_TI;  // ty < 4
+    __shared__ uint32_t Ls[REC_TL][REC_TK][9];
+    __shared__ uint32_t Vs[REC_TK][REC_TI][9];
+    const uint32_t tx = threadIdx.x % REC_TI, ty = threadIdx.x / REC_TI;
     const uint32_t t0 = blockIdx.y * REC_TL, i0 = blockIdx.x * REC_TI;
-    fr acc[4];
+    uint64_t T[18];
 #pragma unroll
-    for (int q = 0; q < 4; q++) acc[q] = f_zero<FrP>();
+    for (int k = 0; k < 18; k++) T[k] = 0;
     for (uint32_t k0 = 0; k0 < m; k0 += REC_TK) {
-        {  // L tile: 16 x 16 = 256 entries, one per thread
-            const uint32_t a = threadIdx.x / REC_TK, bb = threadIdx.x % REC_TK;
-            const uint32_t t = t0 + a, r = k0 + bb;
-            Ls[a][bb] = (t < nt && r < m) ? f_load<FrP>(L + 8 * ((size_t)t * m + r)) : f_zero<FrP>();
+        for (uint32_t e = threadIdx.x; e < REC_TL * REC_TK; e += blockDim.x) {
+            const uint32_t a = e / REC_TK, b = e % REC_TK;
+            const uint32_t t = t0 + a, r = k0 + b;
+            const fr x = (t < nt && r < m) ? f_load<FrP>(L + 8 * ((size_t)t * m + r)) : f_zero<FrP>();
+            rec_to29(x, Ls[a][b]);
         }
-        for (uint32_t e = threadIdx.x; e < REC_TK * REC_TI; e += blockDim.x) {  // V tile: 16 x 64
-            const uint32_t a = e / REC_TI, bb = e % REC_TI;
-            const uint32_t r = k0 + a, i = i0 + bb;
-            Vs[a][bb] = (r < m && i < col) ? f_load<FrP>(V + 8 * ((size_t)r * col + i)) : f_zero<FrP>();
+        for (uint32_t e = threadIdx.x; e < REC_TK * REC_TI; e += blockDim.x) {
+            const uint32_t a = e / REC_TI, b = e % REC_TI;
+            const uint32_t r = k0 + a, i = i0 + b;
+            const fr x = (r < m && i < col) ? f_load<FrP>(V + 8 * ((size_t)r * col + i)) : f_zero<FrP>();
+            rec_to29(x, Vs[a][b]);
         }
         __syncthreads();
-#pragma unroll 2
-        for (int kk = 0; kk < REC_TK; kk++) {
-            const fr v = Vs[kk][tx];
+#pragma unroll 1
+        for (int half = 0; half < 2; half++) {
+            for (int kk = 7 * half; kk < 7 * half + 7; kk++) {
+                uint32_t l[9], v[9];
 #pragma unroll
-            for (int q = 0; q < 4; q++) acc[q] = f_add(acc[q], f_mul(Ls[ty * 4 + q][kk], v));
+                for (int j = 0; j < 9; j++) {
+                    l[j] = Ls[ty][kk][j];
+                    v[j] = Vs[kk][tx][j];
+                }
+#pragma unroll
+                for (int i = 0; i < 9; i++)
+#pragma unroll
+                    for (int j = 0; j < 9; j++) T[i + j] += (uint64_t)l[i] * v[j];
+            }
+#pragma unroll
+            for (int k = 0; k < 17; k++) {  // carry pass: same value, columns back below 2^29
+                T[k + 1] += T[k] >> 29;
+                T[k] &= R29_MASK;
+            }
         }
         __syncthreads();
     }
-    const uint32_t i = i0 + tx;
-    if (i >= col) return;
+    // Montgomery reduction by 2^261
 #pragma unroll
-    for (int q = 0; q < 4; q++) {
-        const uint32_t t = t0 + ty * 4 + q;
-        if (t < nt) f_store(Out + 8 * ((size_t)t * col + i), acc[q]);
+    for (int i = 0; i < 9; i++) {
+        const uint32_t mm = ((uint32_t)T[i] * R29::INV) & R29_MASK;
+#pragma unroll
+        for (int j = 0; j < 9; j++) T[i + j] += (uint64_t)mm * R29::P[j];
+        T[i + 1] += T[i] >> 29;
     }
+    uint32_t v[9];
+    uint64_t cy = 0;
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+        const uint64_t x = T[9 + k] + cy;
+        v[k] = (uint32_t)x & R29_MASK;
+        cy = x >> 29;
+    }
+    v[8] |= (uint32_t)cy << 29;  // below 2^259: the top limb keeps its extra bits
+#pragma unroll
+    for (int q = 0; q < 6; q++) {  // v -= r 2^(5-q) where that stays non-negative
+        uint32_t d[9];
+        int32_t br = 0;
+#pragma unroll
+        for (int k = 0; k < 9; k++) {
+            const int64_t x = (int64_t)v[k] - (int64_t)R29::RK[q][k] - br;
+            br = x < 0;
+            d[k] = k < 8 ? (uint32_t)x & R29_MASK : (uint32_t)x;
+        }
+#pragma unroll
+        for (int k = 0; k < 9; k++) v[k] = br ? v[k] : d[k];
+    }
+    fr o;
+#pragma unroll
+    for (int w = 0; w < 8; w++) {  // 29-bit limbs -> 32-bit words
+        const int bit = 32 * w, k = bit / 29, sh = bit % 29;
+        uint64_t x = (uint64_t)v[k] >> sh;
+        if (k + 1 < 9) x |= (uint64_t)v[k + 1] << (29 - sh);
+        if (k + 2 < 9 && 29 - sh + 29 < 32) x |= (uint64_t)v[k + 2] << (58 - sh);
+        o.v[w] = (uint32_t)x;
+    }
+#pragma unroll
+    for (int q = 0; q < 5; q++) o = f_add(o, o);
+    const uint32_t i = i0 + tx, t = t0 + ty;
+    if (i < col && t < nt) f_store(Out + 8 * ((size_t)t * col + i), o);
 }
 
 // Consistency of the extra rows (indices m..nrows-1 of the received list): their
-// combination yv must equal the interpolant A(u_x) sum_r L[x][r] yv[r].  flag bit 1.
-__global__ void k_rec_check(const uint32_t* L, const uint32_t* A, const uint32_t* yv, uint32_t m, uint32_t nx,
-                            int* flag) {
-    const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
-    if (x >= nx) return;
+// combination yv must equal the interpolant A(u_x) sum_r L[x][r] yv[r].  One workgroup
+// (256 threads) per extra row x.  flag bit 1.
+__global__ __launch_bounds__(256) void k_rec_check(const uint32_t* L, const uint32_t* A, const uint32_t* yv,
+                                                   uint32_t m, int* flag) {
+    __shared__ fr part[256];
+    const uint32_t x = blockIdx.x;
     fr s = f_zero<FrP>();
-    for (uint32_t r = 0; r < m; r++)
+    for (uint32_t r = threadIdx.x; r < m; r += blockDim.x)
         s = f_add(s, f_mul(f_load<FrP>(L + 8 * ((size_t)x * m + r)), f_load<FrP>(yv + 8 * (size_t)r)));
-    s = f_mul(s, f_load<FrP>(A + 8 * (size_t)x));
-    if (!f_eq(s, f_load<FrP>(yv + 8 * (size_t)(m + x)))) atomicOr(flag, 2);
+    part[threadIdx.x] = s;
+    __syncthreads();
+    for (uint32_t o = blockDim.x >> 1; o > 0; o >>= 1) {
+        if (threadIdx.x < o) part[threadIdx.x] = f_add(part[threadIdx.x], part[threadIdx.x + o]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const fr v = f_mul(part[0], f_load<FrP>(A + 8 * (size_t)x));
+        if (!f_eq(v, f_load<FrP>(yv + 8 * (size_t)(m + x)))) atomicOr(flag, 2);
+    }
 }
 
 // One workgroup per erased coset-0 row l = erased[t]: w_i = A(x_l) Out[t][i] w_N^(l i),

@@ -29,6 +29,7 @@
 #include <random>
 #include <string>
 #include <thread>
+#include <type_traits>
 #include <unordered_map>
 #include <vector>
 
@@ -180,6 +181,10 @@ struct zgr_ctx {
     std::vector<size_t> mlens;
     int next_lane = 0;
     int* flag = nullptr;
+
+    // RS recovery workspace, grown on demand and kept until the context ends
+    std::vector<void*> rec_buf;
+    std::vector<size_t> rec_cap;
 
     // profiling
     bool profiling = false;
@@ -791,6 +796,8 @@ void zgr_ctx_destroy(zgr_ctx* c) {
     void* ptrs[] = {c->consts, c->tw, c->tw_inv, c->flag, c->work[0], c->work[1]};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
+    for (void* p : c->rec_buf)
+        if (p) (void)hipFree(p);
     for (int i = 0; i < 3; i++) {
         if (c->scale[i]) (void)hipFree(c->scale[i]);
         if (c->bases[i]) (void)hipFree(c->bases[i]);
@@ -1263,11 +1270,23 @@ int zgr_recover(zgr_ctx* c, const uint32_t* row_ids, const uint8_t* rows, size_t
     std::lock_guard<std::mutex> lk(c->mu);
     HIPCHK(hipSetDevice(c->device));
     hipStream_t st = c->lanes[0].st;
-    std::vector<void*> bufs;
+    size_t slot = 0;
     auto body = [&]() -> int {
+        // buffer i of the workspace, grown to at least `bytes`
         auto alloc = [&](auto*& p, size_t bytes) -> int {
-            CHECK(dmalloc((void**)&p, bytes ? bytes : 32));
-            bufs.push_back((void*)p);
+            bytes = bytes ? bytes : 32;
+            if (slot == c->rec_buf.size()) {
+                c->rec_buf.push_back(nullptr);
+                c->rec_cap.push_back(0);
+            }
+            if (c->rec_cap[slot] < bytes) {
+                if (c->rec_buf[slot]) HIPCHK(hipFree(c->rec_buf[slot]));
+                c->rec_buf[slot] = nullptr;
+                c->rec_cap[slot] = 0;
+                CHECK(dmalloc(&c->rec_buf[slot], bytes));
+                c->rec_cap[slot] = bytes;
+            }
+            p = (std::remove_reference_t<decltype(p)>)c->rec_buf[slot++];
             return ZGR_OK;
         };
         int* flag = nullptr;
@@ -1304,11 +1323,12 @@ int zgr_recover(zgr_ctx* c, const uint32_t* row_ids, const uint8_t* rows, size_t
             for (auto& w : g) w = ((uint64_t)rd() << 32) | rd();
             g[3] &= 0x0fffffffffffffffull;
         }
-        uint32_t *d_ids, *d_gamma, *d_cpow, *d_gpow, *d_u, *d_V, *d_yv, *d_b, *d_t, *d_A, *d_L, *d_out, *d_er,
-            *d_blob;
+        uint32_t *d_ids, *d_gamma, *d_inv, *d_cpow, *d_gpow, *d_u, *d_V, *d_yv, *d_b, *d_t, *d_A, *d_L, *d_out,
+            *d_er, *d_blob;
         uint8_t* d_raw;
         CHECK(alloc(d_ids, n_rows * 4));
         CHECK(alloc(d_gamma, 32));
+        CHECK(alloc(d_inv, (size_t)(c->coset_n + 2) * 32));
         CHECK(alloc(d_cpow, (size_t)c->coset_n * COL * 32));
         CHECK(alloc(d_gpow, COL * 32));
         CHECK(alloc(d_u, n_rows * 32));
@@ -1326,17 +1346,18 @@ int zgr_recover(zgr_ctx* c, const uint32_t* row_ids, const uint8_t* rows, size_t
         HIPCHK(hipMemcpyAsync(d_gamma, g, 32, hipMemcpyHostToDevice, st));
         if (e0) HIPCHK(hipMemcpyAsync(d_er, erased.data(), (size_t)e0 * 4, hipMemcpyHostToDevice, st));
         const uint32_t nr = (uint32_t)n_rows;
-        LAUNCH(c, st, "recover", k_rec_consts, blocks((size_t)(c->coset_n + 1) * COL, 64), 64, 0, c->consts, d_gamma,
-               c->coset_n, lc, d_cpow, d_gpow);
+        LAUNCH(c, st, "recover", k_rec_inverses, 1, 64, 0, c->consts, d_gamma, c->coset_n, lc, d_inv);
+        LAUNCH(c, st, "recover", k_rec_consts, blocks((size_t)(c->coset_n + 1) * COL, 64), 64, 0, d_inv, c->coset_n,
+               lc, d_cpow, d_gpow);
         LAUNCH(c, st, "recover", k_rec_points, blocks(nr, 64), 64, 0, d_ids, nr, c->consts, c->tw, lc, h, d_u);
         LAUNCH(c, st, "recover_rows", k_rec_rows, nr, 256, (COL + 256) * 32, d_rows, d_ids, d_cpow, d_gpow, c->tw_inv,
                lc, h, d_V, d_yv);
-        LAUNCH(c, st, "recover", k_rec_bary, blocks(m, 64), 64, 0, d_u, m, d_b);
+        LAUNCH(c, st, "recover", k_rec_prod, m, 256, 0, d_u, d_u, m, 1, 0, d_b);  // B_r, inverted in k_rec_lagrange
         if (e0) LAUNCH(c, st, "recover", k_rec_targets, blocks(e0, 64), 64, 0, d_er, e0, c->tw, lc, h, d_t);
         if (nx) HIPCHK(hipMemcpyAsync(d_t + 8 * (size_t)e0, d_u + 8 * (size_t)m, (size_t)nx * 32, hipMemcpyDeviceToDevice, st));
         if (nt) {
             const uint32_t chunks = (m + REC_CHUNK - 1) / REC_CHUNK;
-            LAUNCH(c, st, "recover", k_rec_node, blocks(nt, 64), 64, 0, d_t, nt, d_u, m, d_A);
+            LAUNCH(c, st, "recover", k_rec_prod, nt, 256, 0, d_t, d_u, m, 0, 0, d_A);
             LAUNCH(c, st, "recover_lagrange", k_rec_lagrange, blocks((size_t)nt * chunks, 64), 64, 0, d_t, nt, d_u, d_b,
                    m, d_L);
         }
@@ -1344,8 +1365,8 @@ int zgr_recover(zgr_ctx* c, const uint32_t* row_ids, const uint8_t* rows, size_t
             dim3 grid(blocks(COL, REC_TI), blocks(e0, REC_TL));
             LAUNCH(c, st, "recover_gemm", k_rec_gemm, grid, 256, 0, d_L, d_V, e0, m, (uint32_t)COL, d_out);
         }
-        if (nx) LAUNCH(c, st, "recover", k_rec_check, blocks(nx, 64), 64, 0, d_L + 8 * (size_t)e0 * m, d_A + 8 * (size_t)e0,
-                       d_yv, m, nx, flag);
+        if (nx) LAUNCH(c, st, "recover", k_rec_check, nx, 256, 0, d_L + 8 * (size_t)e0 * m, d_A + 8 * (size_t)e0, d_yv,
+                       m, flag);
         if (n0) LAUNCH(c, st, "recover", k_rec_pass, blocks(n0 * COL, 256), 256, 0, d_rows, d_ids, (uint32_t)n0, lc, d_blob);
         if (e0) LAUNCH(c, st, "recover_rows", k_rec_out, e0, 256, COL * 32, d_out, d_A, d_er, c->tw, lc, h, d_blob);
         LAUNCH(c, st, "recover", k_rec_pack, blocks(31 * n / 4, 256), 256, 0, d_blob, (uint32_t)n, (uint32_t*)d_raw,
@@ -1368,7 +1389,6 @@ int zgr_recover(zgr_ctx* c, const uint32_t* row_ids, const uint8_t* rows, size_t
     };
     int rc = body();
     (void)hipStreamSynchronize(st);
-    for (void* p : bufs) (void)hipFree(p);
     return rc;
 }
 
