@@ -36,6 +36,7 @@
 #include "../../include/zgr.h"
 #include "kernels.h"
 #include "recover.h"
+#include "verify.h"
 
 using namespace zgr;
 
@@ -1135,6 +1136,27 @@ int zgr_coset_extend(zgr_ctx* c, const uint8_t* data_mont, int coset, uint8_t* o
     return sync_all(c);
 }
 
+// Windowed MSM sum_i s_i P_i over 2^log_np device-resident points (param layout) with
+// the scalars (Montgomery) already in ln.y; the canonical affine result goes to out.
+int msm_windowed(zgr_ctx* c, Lane& ln, const uint32_t* dbases, int log_np, uint32_t* out) {
+    const int log_bs = log_np < 10 ? log_np : 10;
+    const int h = log_np - log_bs;
+    const size_t np = (size_t)1 << log_np;
+    LAUNCH(c, ln.st, "msm_digits", k_msm_digits, blocks(np, 256), 256, 0, ln.y, log_np, log_bs, ln.digits, 0);
+    uint32_t bs = 1u << log_bs, nb = 1u << h;
+    LAUNCH(c, ln.st, "msm_sort", k_msm_sort, nb * MSM_W, bs < 256 ? bs : 256u, 0, ln.digits, log_bs, ln.entries,
+           ln.counts);
+    LAUNCH(c, ln.st, "msm_accum", k_msm_accum, nb, ACC_THREADS, accum_lds_bytes(bs), dbases, ln.entries, ln.counts,
+           ln.heap, log_np, h, 1);
+    for (int d = h - 1; d >= 0; d--) {
+        uint32_t total = (uint32_t)MSM_W << d;
+        LAUNCH(c, ln.st, "msm_tree", k_msm_tree_level, blocks(total, 64), 64, 0, ln.heap, h, d, total, MSM_W,
+               (uint64_t)0);
+    }
+    LAUNCH(c, ln.st, "msm_root", k_msm_root_out, 1, 64, 0, ln.heap, h, out);
+    return ZGR_OK;
+}
+
 int zgr_msm(zgr_ctx* c, const uint8_t* bases_mont, const uint8_t* scalars_mont, size_t n, uint8_t* out_point) {
     if (!c || !bases_mont || !scalars_mont || !out_point) return fail(ZGR_ERR_INVALID_ARG, "null argument");
     if (n == 0 || n > c->n) return fail(ZGR_ERR_INVALID_ARG, "msm size %zu outside [1, %zu]", n, c->n);
@@ -1154,19 +1176,8 @@ int zgr_msm(zgr_ctx* c, const uint8_t* bases_mont, const uint8_t* scalars_mont, 
         HIPCHK(hipMemsetAsync(dbases, 0, np * 64, ln.st));
         HIPCHK(hipMemcpyAsync(ln.y, scalars_mont, n * 32, hipMemcpyHostToDevice, ln.st));
         HIPCHK(hipMemcpyAsync(dbases, bases_mont, n * 64, hipMemcpyHostToDevice, ln.st));
-        LAUNCH(c, ln.st, "msm_digits", k_msm_digits, blocks(np, 256), 256, 0, ln.y, log_np, log_bs, ln.digits, 0);
-        uint32_t bs = 1u << log_bs, nb = 1u << h;
-        LAUNCH(c, ln.st, "msm_sort", k_msm_sort, nb * MSM_W, bs < 256 ? bs : 256u, 0, ln.digits, log_bs,
-               ln.entries, ln.counts);
-        LAUNCH(c, ln.st, "msm_accum", k_msm_accum, nb, ACC_THREADS, accum_lds_bytes(bs), dbases, ln.entries,
-               ln.counts, ln.heap, log_np, h, 1);
-        for (int d = h - 1; d >= 0; d--) {
-            uint32_t total = (uint32_t)MSM_W << d;
-            LAUNCH(c, ln.st, "msm_tree", k_msm_tree_level, blocks(total, 64), 64, 0, ln.heap, h, d, total, MSM_W,
-                   (uint64_t)0);
-        }
         uint32_t* opts = c->slots[0].outpts;
-        LAUNCH(c, ln.st, "msm_root", k_msm_root_out, 1, 64, 0, ln.heap, h, opts);
+        CHECK(msm_windowed(c, ln, dbases, log_np, opts));
         HIPCHK(hipMemcpyAsync(out_point, opts, 64, hipMemcpyDeviceToHost, ln.st));
         return sync_all(c);
     };
@@ -1254,6 +1265,70 @@ int zgr_merkle(zgr_ctx* c, const uint8_t* data, size_t n_elems, int levels, uint
     int rc = body();
     (void)hipStreamSynchronize(st);
     for (auto* p : layers) (void)hipFree(p);
+    return rc;
+}
+
+int zgr_verify_msm(zgr_ctx* c, const uint32_t* row_ids, const uint8_t* rows, const uint8_t* alphas, size_t n_rows,
+                   uint8_t* out_point) {
+    if (!c || !out_point || (n_rows && (!row_ids || !rows || !alphas))) return fail(ZGR_ERR_INVALID_ARG, "null argument");
+    const size_t ROW = (size_t)1 << c->log_row, COL = (size_t)1 << c->log_col;
+    for (size_t k = 0; k < n_rows; k++) {
+        if (row_ids[k] >= ROW * (size_t)c->coset_n)
+            return fail(ZGR_ERR_INVALID_ARG, "row index %u past the encoded blob (%zu rows)", row_ids[k], ROW * c->coset_n);
+        if (k && row_ids[k] <= row_ids[k - 1]) return fail(ZGR_ERR_INVALID_ARG, "row ids must be strictly increasing");
+    }
+    std::lock_guard<std::mutex> lk(c->mu);
+    HIPCHK(hipSetDevice(c->device));
+    std::vector<std::vector<uint32_t>> per(c->coset_n);
+    for (size_t k = 0; k < n_rows; k++) per[row_ids[k] / ROW].push_back((uint32_t)k);
+    for (int cs = 0; cs < c->coset_n; cs++)
+        if (!per[cs].empty()) CHECK(ensure_params(c, cs));
+    Lane& ln = c->lanes[0];
+    hipStream_t st = ln.st;
+    uint8_t *d_rows = nullptr, *d_alpha = nullptr;
+    uint32_t *d_ids = nullptr, *d_ks = nullptr, *d_pts = nullptr;
+    int* flag = nullptr;
+    auto body = [&]() -> int {
+        CHECK(dmalloc((void**)&d_rows, n_rows * COL * 32 + 32));
+        CHECK(dmalloc((void**)&d_alpha, n_rows * 32 + 32));
+        CHECK(dmalloc((void**)&d_ids, n_rows * 4 + 4));
+        CHECK(dmalloc((void**)&d_ks, n_rows * 4 + 4));
+        CHECK(dmalloc((void**)&d_pts, (size_t)(c->coset_n + 1) * 64));
+        CHECK(dmalloc((void**)&flag, sizeof(int)));
+        LAUNCH(c, st, "verify", k_rec_clear, 1, 64, 0, flag);
+        if (n_rows) {
+            HIPCHK(hipMemcpyAsync(d_rows, rows, n_rows * COL * 32, hipMemcpyHostToDevice, st));
+            HIPCHK(hipMemcpyAsync(d_alpha, alphas, n_rows * 32, hipMemcpyHostToDevice, st));
+            HIPCHK(hipMemcpyAsync(d_ids, row_ids, n_rows * 4, hipMemcpyHostToDevice, st));
+        }
+        int npts = 0;
+        size_t off = 0;
+        std::vector<uint32_t> ks_all;
+        for (int cs = 0; cs < c->coset_n; cs++) ks_all.insert(ks_all.end(), per[cs].begin(), per[cs].end());
+        if (n_rows) HIPCHK(hipMemcpyAsync(d_ks, ks_all.data(), n_rows * 4, hipMemcpyHostToDevice, st));
+        for (int cs = 0; cs < c->coset_n; cs++) {
+            const uint32_t nk = (uint32_t)per[cs].size();
+            if (!nk) continue;
+            HIPCHK(hipMemsetAsync(ln.y, 0, c->n * 32, st));
+            LAUNCH(c, st, "verify", k_ver_scalars, blocks((size_t)nk * COL, 256), 256, 0, d_rows, d_alpha, d_ks + off,
+                   nk, d_ids, c->log_col, c->log_row, ln.y, flag);
+            const uint32_t* basis = c->bases[cs] + (size_t)c->h * c->n * 16;  // line h: the basis
+            CHECK(msm_windowed(c, ln, basis, c->log_n, d_pts + 16 * npts));
+            npts++;
+            off += nk;
+        }
+        LAUNCH(c, st, "verify", k_ver_sum, 1, 64, 0, d_pts, npts, d_pts + 16 * c->coset_n);
+        int hflag = 0;
+        HIPCHK(hipMemcpyAsync(&hflag, flag, sizeof(int), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipMemcpyAsync(out_point, d_pts + 16 * c->coset_n, 64, hipMemcpyDeviceToHost, st));
+        CHECK(sync_all(c));
+        if (hflag) return fail(ZGR_ERR_INVALID_ARG, "row element or alpha is not a canonical field element");
+        return ZGR_OK;
+    };
+    int rc = body();
+    (void)hipStreamSynchronize(st);
+    for (void* p : {(void*)d_rows, (void*)d_alpha, (void*)d_ids, (void*)d_ks, (void*)d_pts, (void*)flag})
+        if (p) (void)hipFree(p);
     return rc;
 }
 

@@ -56,6 +56,7 @@ SIGNATURES = {
     "zgr_server_stats": (None, [P, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64),
                                 ctypes.POINTER(ctypes.c_uint64)]),
     "zgr_recover": (I, [P, P, P, S, P, S, ctypes.POINTER(S)]),
+    "zgr_verify_msm": (I, [P, P, P, P, S, P]),
     "zgr_kernel_time_ms": (ctypes.c_double, [P, ctypes.c_char_p, ctypes.POINTER(I)]),
     "zgr_kernel_time_reset": (None, [P]),
     "zgr_set_profiling": (None, [P, I]),
@@ -352,6 +353,19 @@ class Context:
                                  _p(out), cap, ctypes.byref(n)))
         return out[:n.value].tobytes()
 
+    def verify_msm(self, row_ids, rows, alphas):
+        """zgr_verify_msm: row_ids (strictly increasing), their rows (canonical bytes, one
+        buffer or a list) and one canonical alpha (int) per row -> 64 B canonical point."""
+        ids = np.ascontiguousarray(row_ids, dtype=np.uint32).reshape(-1)
+        buf = rows if isinstance(rows, (bytes, bytearray, memoryview)) else b"".join(bytes(r) for r in rows)
+        arr = np.frombuffer(bytes(buf), dtype=np.uint8) if len(buf) else np.zeros(1, dtype=np.uint8)
+        al = np.frombuffer(b"".join(int(a).to_bytes(32, "little") for a in alphas), dtype=np.uint8) \
+            if len(alphas) else np.zeros(1, dtype=np.uint8)
+        out = np.zeros(64, dtype=np.uint8)
+        _check(lib().zgr_verify_msm(self.h, _p(ids if ids.size else np.zeros(1, dtype=np.uint32)), _p(arr), _p(al),
+                                    ids.size, _p(out)))
+        return bytes(out)
+
     # ---- profiling ----
     def set_profiling(self, on):
         lib().zgr_set_profiling(self.h, int(on))
@@ -589,6 +603,64 @@ def recover_from_da_slice(ctx, slices):
         if e.code == ERR_INVALID_ARG:
             raise RecoveryError(msg) from None
         raise
+
+
+class DeferredVerifier:
+    """The MSM half of DeferredVerifier (crates/amt/src/deferred_verification.rs:34-222)
+    on the device, for the rows of one encoded blob.
+
+    record_slice() takes EncodedSlice bytes, draws a random alpha for the row and
+    keeps alpha * (commitment - sum_d proof[d].commitment), as record_msm does after
+    verify_amt_proof's pairing checks (crates/amt/src/verify_params.rs:176-186).
+    fast_check_msm() (fast_check_msm_gpu, deferred_verification.rs:203-222) compares the
+    device MSM of the recorded rows over the resident basis with the device MSM of the
+    recorded answers.  The pairing half (fast_check_pairing) stays with the caller."""
+
+    def __init__(self, ctx, rng=None):
+        import secrets
+
+        self.ctx = ctx
+        self._rand = rng if rng is not None else (lambda: secrets.randbelow(R_MOD))
+        self._rows = {}
+
+    def record_slice(self, slice_bytes):
+        f = wire.parse_encoded_slice(slice_bytes, self.ctx.geo.coset_n)
+        idx = f["index"]
+        if idx in self._rows:
+            raise ValueError("row %d recorded twice" % idx)
+        self._rows[idx] = (b"".join(f["row"]), self._rand(), f["commitment"], [c for c, _ in f["amt_proof"]])
+
+    def fast_check_msm(self):
+        if not self._rows:
+            return True
+        ids = sorted(self._rows)
+        lhs = self.ctx.verify_msm(ids, [self._rows[i][0] for i in ids], [self._rows[i][1] for i in ids])
+        bases, scal = [], []
+        for i in ids:
+            _, alpha, com, proof = self._rows[i]
+            for pt, k in [(com, alpha)] + [(p, R_MOD - alpha) for p in proof]:
+                xy = _g1_from_wire(pt)
+                if xy is not None:
+                    bases.append(xy)
+                    scal.append(k % R_MOD)
+        if not bases:
+            return lhs == bytes(64)
+        R256 = 1 << 256
+        bm = b"".join((x * R256 % P_MOD).to_bytes(32, "little") + (y * R256 % P_MOD).to_bytes(32, "little")
+                      for x, y in bases)
+        sm = b"".join((k * R256 % R_MOD).to_bytes(32, "little") for k in scal)
+        rhs = self.ctx.msm(np.frombuffer(bm, dtype=np.uint64), np.frombuffer(sm, dtype=np.uint64))
+        return lhs == bytes(rhs)
+
+
+def _g1_from_wire(b):
+    """(x, y) ints of an ark-uncompressed G1 point, None for infinity."""
+    raw = bytearray(b)
+    flags = raw[63] & 0xC0
+    raw[63] &= 0x3F
+    if flags & 0x40:
+        return None
+    return int.from_bytes(bytes(raw[:32]), "little"), int.from_bytes(bytes(raw[32:]), "little")
 
 
 def ark_g1_uncompressed_array(pts):

@@ -160,6 +160,20 @@ int zgr_ecfft(zgr_ctx *ctx, const uint8_t *points, int log_n, int inverse, uint8
 int zgr_merkle(zgr_ctx *ctx, const uint8_t *data, size_t n_elems, int levels,
                uint8_t *out_layers, uint8_t *file_root);
 
+/* ---- deferred AMT verification (the GPU half of DeferredVerifier) ---- */
+/* DeferredVerifier::fast_check_msm_gpu (crates/amt/src/deferred_verification.rs:203-222)
+ * over the rows record_msm collected (deferred_verification.rs:71-92), with the
+ * verify basis = the resident prove basis line: for each row k = coset c, local l,
+ *   out = sum_k alpha_k * sum_i basis_c[bitrev(l) * COL + i] * row_k[bitrev(i)]
+ * (BlobRow::verify, crates/amt/src/blob/encode.rs:224-239, and verify_amt_proof,
+ * crates/amt/src/verify_params.rs:176-186).  row_ids strictly increasing, < coset_n *
+ * 2^log_row; rows: n_rows * 32 * 2^log_col canonical bytes (BlobRow.row); alphas: n_rows
+ * canonical Fr (the random factors; fast_check_msm compares out with the sum of
+ * alpha_k (commitment - sum_d proof_k[d].commitment)).  out_point: canonical affine
+ * (64 zero bytes = infinity).  Needs the params of every coset named. */
+int zgr_verify_msm(zgr_ctx *ctx, const uint32_t *row_ids, const uint8_t *rows, const uint8_t *alphas,
+                   size_t n_rows, uint8_t *out_point);
+
 /* ---- RS recovery ---- */
 /* recover_from_da_slice (crates/recovery/src/lib.rs:15-43): the raw data of a blob
  * from any 2^log_row of its encoded rows.  row_ids: n_rows strictly increasing row

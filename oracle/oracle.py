@@ -62,6 +62,7 @@ def lib():
             "zgo_raw_to_blob": (I, [P, S, I, P]),
             "zgo_encode": (I, [P, I, I, I, P, S, P, P, P, P, P, P, P]),
             "zgo_recover": (I, [P, P, S, I, I, I, P, P]),
+            "zgo_verify_msm": (I, [P, I, I, I, P, P, P, S, P]),
             "zgo_set_threads": (I, [I]),
         }
         for name, (res, args) in sig.items():
@@ -361,3 +362,23 @@ def recover(slices, log_col, log_row, coset_n=3):
     rc = lib().zgo_recover(_p(ids), _p(rows), len(ok), log_col, log_row, coset_n, _p(out),
                            ctypes.byref(out_len))
     return rc, (out[:out_len.value].tobytes() if rc == 0 else b"")
+
+
+# ---- deferred verification MSM ---------------------------------------------------
+def verify_msm(bases_list, log_col, log_row, ids, rows, alphas):
+    """DeferredVerifier::record_msm + fast_check_msm's MSM over rows (see zg_oracle.h):
+    ids strictly increasing row indices, rows their canonical bytes (one buffer),
+    alphas canonical ints.  Returns the 64 B canonical point."""
+    coset_n = len(bases_list)
+    arr = (ctypes.c_void_p * coset_n)(*[b.ctypes.data for b in bases_list])
+    n = len(ids)
+    idv = np.array(ids, dtype=np.uint32) if n else np.zeros(1, dtype=np.uint32)
+    rw = np.frombuffer(bytes(rows), dtype=np.uint8).copy() if n else np.zeros(1, dtype=np.uint8)
+    al = np.frombuffer(b"".join(int(a).to_bytes(32, "little") for a in alphas), dtype=np.uint8).copy() \
+        if n else np.zeros(1, dtype=np.uint8)
+    out = np.zeros(64, dtype=np.uint8)
+    rc = lib().zgo_verify_msm(ctypes.cast(arr, ctypes.c_void_p), log_col, log_row, coset_n, _p(idv), _p(rw), _p(al),
+                              n, _p(out))
+    if rc:
+        raise ValueError("row index past the encoded blob")
+    return bytes(out)

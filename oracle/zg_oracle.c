@@ -1428,3 +1428,44 @@ int zgo_recover(const uint32_t *ids, const uint8_t *rows, size_t n_rows, int log
     free(Z.c);
     return rc;
 }
+
+/* ------------------------------------------------------------------ */
+/* deferred verification MSM: DeferredVerifier::record_msm + fast_check_msm */
+/* (crates/amt/src/deferred_verification.rs:71-92, 133-150) over rows verified by */
+/* BlobRow::verify (crates/amt/src/blob/encode.rs:224-239) -> verify_amt_proof   */
+/* (crates/amt/src/verify_params.rs:176-186)                                   */
+/* ------------------------------------------------------------------ */
+
+/* out = sum_k alpha_k * sum_i basis_c[bitrev(l) * COL + i] * index_reverse(row_k)[i],
+ * row k = (c, l) = divmod(ids[k], ROW); basis_c = line h of bases[c] (param layout).
+ * rows: canonical 32 B elements, COL per row; alphas canonical.  Returns 0, or -1 for
+ * an id past the encoded blob. */
+int zgo_verify_msm(const uint64_t *const *bases, int log_col, int log_row, int coset_n, const uint32_t *ids,
+                   const uint8_t *rows, const uint8_t *alphas, size_t n_rows, uint8_t out[64]) {
+    const size_t ROW = (size_t)1 << log_row, COL = (size_t)1 << log_col, n = ROW * COL;
+    for (size_t k = 0; k < n_rows; k++)
+        if (ids[k] >= ROW * (size_t)coset_n) return -1;
+    uint64_t *pts = (uint64_t *)malloc((n_rows * COL + 1) * 64);
+    uint64_t *sc = (uint64_t *)malloc((n_rows * COL + 1) * 32);
+    for (size_t k = 0; k < n_rows; k++) {
+        const size_t c = ids[k] / ROW, l = ids[k] % ROW, b = bitrev(l, log_row);
+        fe a;
+        uint64_t av[4];
+        memcpy(av, alphas + 32 * k, 32);
+        fto_mont(&a, av, &FR);
+        for (size_t i = 0; i < COL; i++) {
+            /* data = index_reverse(row): data[i] = row[bitrev(i)] */
+            uint64_t v[4];
+            memcpy(v, rows + 32 * (k * COL + bitrev(i, log_col)), 32);
+            fe m;
+            fto_mont(&m, v, &FR);
+            fmul(&m, &m, &a, &FR);
+            ffrom_mont(sc + 4 * (k * COL + i), &m, &FR);
+            memcpy(pts + 8 * (k * COL + i), bases[c] + 8 * ((size_t)log_row * n + b * COL + i), 64);
+        }
+    }
+    zgo_msm(pts, sc, n_rows * COL, out);
+    free(pts);
+    free(sc);
+    return 0;
+}

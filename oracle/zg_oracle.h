@@ -124,6 +124,13 @@ int zgo_encode(const uint64_t *const *bases, int log_col, int log_row, int coset
 int zgo_recover(const uint32_t *ids, const uint8_t *rows, size_t n_rows, int log_col, int log_row,
                 int coset_n, uint8_t *out_raw, size_t *out_len);
 
+/* ---- deferred verification MSM (crates/amt/src/deferred_verification.rs:71-92, 133-150) ---- */
+/* out = sum_k alpha_k * sum_i basis_c[bitrev(l) * COL + i] * row_k[bitrev(i)] for rows
+ * ids[k] = c * ROW + l (BlobRow::verify, crates/amt/src/blob/encode.rs:224-239); rows and
+ * alphas canonical; basis_c = line log_row of bases[c].  Returns 0, -1 for a bad id. */
+int zgo_verify_msm(const uint64_t *const *bases, int log_col, int log_row, int coset_n, const uint32_t *ids,
+                   const uint8_t *rows, const uint8_t *alphas, size_t n_rows, uint8_t out[64]);
+
 /* Thread count used by the parallel loops (OpenMP); returns the previous value. */
 int zgo_set_threads(int nthreads);
 

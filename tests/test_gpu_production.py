@@ -197,3 +197,28 @@ def test_recover_production(ctx, blob):
     s[k] = bytes(b)
     with pytest.raises(Z.RecoveryPanic):
         Z.recover_from_da_slice(ctx, s)
+
+
+def test_deferred_verify_production(ctx, blob):
+    """The deferred verifier's MSM over all 3072 rows of the production blob (one
+    2^20-point MSM per coset over the resident basis) accepts, and rejects one changed
+    element."""
+    from zg_rocm import wire
+
+    raw, out = blob
+    eb = Z.EncodedBlob(GEO, out)
+    rows = eb.encoded_slices()
+    v = Z.DeferredVerifier(ctx)
+    for i in range(GEO.row_encoded):
+        v.record_slice(rows[i].tobytes())
+    t = time.time()
+    assert v.fast_check_msm()
+    log("deferred MSM check over 3072 rows: %.3f s" % (time.time() - t))
+    f = wire.parse_encoded_slice(rows[1500].tobytes())
+    r = bytearray(b"".join(f["row"]))
+    r[32 * 17] ^= 1
+    f["row"] = bytes(r)
+    v = Z.DeferredVerifier(ctx)
+    v.record_slice(wire.serialize_encoded_slice(f))
+    v.record_slice(rows[0].tobytes())
+    assert not v.fast_check_msm()
