@@ -146,11 +146,11 @@ def fr_to_mont_array(vals):
     return out
 
 
-def fr_canonical_array(arr):
-    """uint64[n,4] mont -> uint64[n,4] canonical (bulk)."""
+def fr_canonical_array(arr, which=0):
+    """uint64[n,4] mont -> uint64[n,4] canonical (bulk); which = 1 for Fq."""
     a = np.ascontiguousarray(arr, dtype=np.uint64).reshape(-1, 4)
     out = np.zeros_like(a)
-    lib().zgo_from_mont_n(0, _p(a), a.shape[0], _p(out))
+    lib().zgo_from_mont_n(which, _p(a), a.shape[0], _p(out))
     return out
 
 

@@ -86,3 +86,33 @@ def test_verify_msm_argument_errors(ctx, slices):
         ctx.verify_msm([0], row, [O.R_MOD])  # alpha not canonical
     with pytest.raises(Z.ZgrError):
         ctx.verify_msm([2, 1], row + row, [1, 1])
+
+
+def test_device_slices_pass_reference_verification(ctx, slices):
+    """Slices of the device-encoded blob pass the restated EncodedSlice::verify with
+    its pairing checks (crates/encoder/src/encoder/slice.rs:72-95), one by one and as a
+    deferred batch whose MSM half runs on the device."""
+    import random
+
+    import verify as V
+
+    tau = seeded_tau()
+    params = list(oracle_params(GEO.log_col, GEO.log_row, GEO.coset_n))
+    vps = []
+    for c in range(GEO.coset_n):
+        canon = O.fr_canonical_array(params[c].reshape(-1, 4), which=1).reshape(GEO.h + 2, GEO.n, 2, 4)
+        vps.append(V.VerifyParams(tau, GEO.log_col, GEO.log_row, c,
+                                  [V.g1_from_canon(canon[GEO.h][i].tobytes()) for i in range(GEO.n)]))
+    first = wire.parse_encoded_slice(slices[0])
+    com = V.g1_from_wire(first["commitment"])
+    root = V.file_root([bytes(x) for x in first["merkle_root"]])
+    for i in (3, GEO.row_n + 40, GEO.row_encoded - 2):
+        assert V.verify_slice(wire.parse_encoded_slice(slices[i]), vps, com, root, GEO) is None, i
+    col = V.Collector(True)
+    dv = Z.DeferredVerifier(ctx)
+    for i in (0, 17, GEO.row_n + 1, 2 * GEO.row_n + 50):
+        assert V.verify_slice(wire.parse_encoded_slice(slices[i]), vps, com, root, GEO, col) is None
+        dv.record_slice(slices[i])
+    col.msm = []  # the MSM equations go to the device instead
+    assert col.fast_check(random.Random(9))
+    assert dv.fast_check_msm()
