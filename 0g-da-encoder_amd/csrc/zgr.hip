@@ -37,6 +37,7 @@
 #include "kernels.h"
 #include "recover.h"
 #include "verify.h"
+#include "params_pp.h"
 
 using namespace zgr;
 
@@ -884,6 +885,57 @@ int zgr_params_load_bamt(zgr_ctx* c, int coset, const uint8_t* bamt, size_t nbyt
     CHECK(sync_all(c));
     c->loaded[coset] = true;
     return ZGR_OK;
+}
+
+int zgr_params_from_pp(zgr_ctx* c, int coset, const uint8_t* g1pp, const uint8_t* high_g1pp, size_t n_points) {
+    if (!c || !g1pp || !high_g1pp) return fail(ZGR_ERR_INVALID_ARG, "null argument");
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (coset < 0 || coset >= c->coset_n) return fail(ZGR_ERR_INVALID_ARG, "coset %d out of range", coset);
+    if (n_points != c->n)
+        return fail(ZGR_ERR_INVALID_ARG, "%zu powers of tau, the context's domain has %zu", n_points, c->n);
+    HIPCHK(hipSetDevice(c->device));
+    CHECK(sync_all(c));  // blobs still in flight read the bases and tables about to be replaced
+    CHECK(alloc_bases(c, coset));
+    c->loaded[coset] = false;
+    hipStream_t st = c->lanes[0].st;
+    const size_t n = c->n;
+    const int log_n = c->log_n;
+    uint32_t *din = nullptr, *gp = nullptr, *hp = nullptr, *pts = nullptr, *w = nullptr;
+    auto body = [&]() -> int {
+        DALLOC(din, n * 64);
+        DALLOC(gp, n * 128);
+        DALLOC(hp, n * 128);
+        DALLOC(pts, n * 128);
+        DALLOC(w, 32);
+        LAUNCH(c, st, "params_pp", k_pp_coset_w, 1, 64, 0, c->consts, coset, w);
+        // coset-scaled powers (generate.rs:152-170); coset 0 is unscaled
+        HIPCHK(hipMemcpyAsync(din, g1pp, n * 64, hipMemcpyHostToDevice, st));
+        LAUNCH(c, st, "params_pp", k_pp_scale, blocks(n, 64), 64, 0, din, w, coset != 0, (uint32_t)n, gp);
+        HIPCHK(hipMemcpyAsync(din, high_g1pp, n * 64, hipMemcpyHostToDevice, st));
+        LAUNCH(c, st, "params_pp", k_pp_scale, blocks(n, 64), 64, 0, din, w, coset != 0, (uint32_t)n, hp);
+        const uint32_t half_n = (uint32_t)(n / 2);
+        auto transform = [&](const uint32_t* src, uint32_t m, bool inverse, int line) -> int {
+            LAUNCH(c, st, "params_pp", k_pp_load, blocks(n, 256), 256, 0, src, pts, log_n, m);
+            const uint32_t* tw = inverse ? c->tw_inv : c->tw;
+            for (int s = 1; s <= log_n; s++)
+                LAUNCH(c, st, "params_pp_fft", k_ec_dit_stage, blocks(half_n, 64), 64, 0, pts, tw, s, log_n - s, half_n);
+            // both directions end with size_inv: the iDFT's own, and gen_quotients' explicit one
+            LAUNCH(c, st, "params_pp", k_pp_store, blocks(n, 64), 64, 0, pts, c->consts, 1, log_n,
+                   c->bases[coset] + (size_t)line * n * 16);
+            return ZGR_OK;
+        };
+        for (int d = 1; d <= c->h; d++) CHECK(transform(gp, (uint32_t)(n >> d), false, d - 1));  // quotients
+        CHECK(transform(gp, 0, true, c->h));                                                    // basis
+        CHECK(transform(hp, 0, true, c->h + 1));                                                // high_basis
+        CHECK(build_fb(c, coset));
+        return sync_all(c);
+    };
+    int rc = body();
+    (void)hipStreamSynchronize(st);
+    for (void* p : {(void*)din, (void*)gp, (void*)hp, (void*)pts, (void*)w})
+        if (p) (void)hipFree(p);
+    if (rc == ZGR_OK) c->loaded[coset] = true;
+    return rc;
 }
 
 int zgr_params_generate(zgr_ctx* c, const uint8_t tau[32]) {

@@ -38,6 +38,7 @@ SIGNATURES = {
     "zgr_params_load": (I, [P, I, P, S]),
     "zgr_params_load_bamt": (I, [P, I, P, S]),
     "zgr_params_generate": (I, [P, P]),
+    "zgr_params_from_pp": (I, [P, I, P, P, S]),
     "zgr_params_download": (I, [P, I, P, S]),
     "zgr_amt_prove": (I, [P, I, P, S, P, P, P, P]),
     "zgr_multiple_multiexp": (I, [P, I, P, S, P]),
@@ -257,6 +258,12 @@ class Context:
         """One coset's bamt file image (bytes / uint8 array), as AMTParams::from_dir_mont reads it."""
         b = np.frombuffer(image, dtype=np.uint8) if isinstance(image, (bytes, bytearray)) else np.ascontiguousarray(image)
         _check(lib().zgr_params_load_bamt(self.h, coset, _p(b), b.nbytes))
+
+    def params_from_pp(self, coset, g1pp, high_g1pp):
+        """zgr_params_from_pp: canonical affine powers of tau (2^log_n points each)."""
+        a = np.ascontiguousarray(g1pp, dtype=np.uint8).reshape(-1)
+        b = np.ascontiguousarray(high_g1pp, dtype=np.uint8).reshape(-1)
+        _check(lib().zgr_params_from_pp(self.h, coset, _p(a), _p(b), a.size // 64))
 
     def params_download(self, coset):
         out = np.zeros(self.geo.params_bytes() // 8, dtype=np.uint64)

@@ -88,6 +88,15 @@ int zgr_params_load_bamt(zgr_ctx *ctx, int coset, const uint8_t *bamt, size_t nb
  * (crates/amt/src/power_tau.rs:54-80) followed by AMTParams::from_pp
  * (crates/amt/src/prove_params/generate.rs:139-195). */
 int zgr_params_generate(zgr_ctx *ctx, const uint8_t tau[32]);
+/* AMTParams::from_pp (crates/amt/src/prove_params/generate.rs:139-232) on the device:
+ * the prove bases of one coset from powers of tau, without tau -- g1pp[i] = [tau^i] G1
+ * and high_g1pp[i] = [tau^(H+i)] G1 for i < 2^log_n (PowerTau, crates/amt/src/
+ * power_tau.rs:20-80, as a ceremony provides them), canonical affine x || y, 64 B each.
+ * Coset scaling by cf_c^-i, then the EC-FFTs of gen_basis / gen_quotients (Pairing::
+ * fast_fft / fast_ifft) and enact, straight into the resident layout; the shifted-copy
+ * tables follow.  n_points must be 2^log_n. */
+int zgr_params_from_pp(zgr_ctx *ctx, int coset, const uint8_t *g1pp, const uint8_t *high_g1pp,
+                       size_t n_points);
 /* Copy one coset's resident bases back to host memory (layout above). */
 int zgr_params_download(zgr_ctx *ctx, int coset, void *out, size_t nbytes);
 

@@ -63,6 +63,7 @@ def lib():
             "zgo_encode": (I, [P, I, I, I, P, S, P, P, P, P, P, P, P]),
             "zgo_recover": (I, [P, P, S, I, I, I, P, P]),
             "zgo_verify_msm": (I, [P, I, I, I, P, P, P, S, P]),
+            "zgo_power_tau": (None, [P, ctypes.c_uint64, I, P]),
             "zgo_set_threads": (I, [I]),
         }
         for name, (res, args) in sig.items():
@@ -382,3 +383,11 @@ def verify_msm(bases_list, log_col, log_row, ids, rows, alphas):
     if rc:
         raise ValueError("row index past the encoded blob")
     return bytes(out)
+
+
+def power_tau(tau, start, log_n):
+    """[tau^(start + i)] G1 for i < 2^log_n (power_tau, crates/amt/src/power_tau.rs:28-42),
+    canonical affine, as one (2^log_n * 64) uint8 array."""
+    out = np.zeros((1 << log_n) * 64, dtype=np.uint8)
+    lib().zgo_power_tau(_p(int_to_limbs(tau % R_MOD)), start, log_n, _p(out))
+    return out

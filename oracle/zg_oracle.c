@@ -1469,3 +1469,26 @@ int zgo_verify_msm(const uint64_t *const *bases, int log_col, int log_row, int c
     free(sc);
     return 0;
 }
+
+/* ------------------------------------------------------------------ */
+/* power_tau (crates/amt/src/power_tau.rs:28-42): out[i] = [tau^(start + i)] G1,      */
+/* i < 2^log_n, canonical affine (64 B)                                             */
+/* ------------------------------------------------------------------ */
+void zgo_power_tau(const uint64_t tau[4], uint64_t start, int log_n, uint8_t *out) {
+    const size_t n = (size_t)1 << log_n;
+    fe t;
+    fto_mont(&t, tau, &FR);
+#pragma omp parallel num_threads(g_threads)
+    {
+        const int nt = omp_get_num_threads(), id = omp_get_thread_num();
+        const size_t lo = n * (size_t)id / (size_t)nt, hi = n * (size_t)(id + 1) / (size_t)nt;
+        fe k;
+        fpow_u64(&k, &t, start + lo, &FR);
+        for (size_t i = lo; i < hi; i++) {
+            uint64_t kc[4];
+            ffrom_mont(kc, &k, &FR);
+            zgo_g1_mul_gen(kc, out + 64 * i);
+            fmul(&k, &k, &t, &FR);
+        }
+    }
+}

@@ -131,6 +131,10 @@ int zgo_recover(const uint32_t *ids, const uint8_t *rows, size_t n_rows, int log
 int zgo_verify_msm(const uint64_t *const *bases, int log_col, int log_row, int coset_n, const uint32_t *ids,
                    const uint8_t *rows, const uint8_t *alphas, size_t n_rows, uint8_t out[64]);
 
+/* power_tau (crates/amt/src/power_tau.rs:28-42): out[i] = [tau^(start + i)] G1 for
+ * i < 2^log_n, canonical affine (64 B each); tau canonical. */
+void zgo_power_tau(const uint64_t tau[4], uint64_t start, int log_n, uint8_t *out);
+
 /* Thread count used by the parallel loops (OpenMP); returns the previous value. */
 int zgo_set_threads(int nthreads);
 

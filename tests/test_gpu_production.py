@@ -56,6 +56,22 @@ def test_params_sampled_points(ctx):
             assert np.array_equal(got[line, pos], exp), (c, line, pos)
 
 
+def test_params_from_pp_production(ctx):
+    """zgr_params_from_pp over 2^20 powers of tau gives, for coset 2, exactly the
+    buffer the closed-form generator left (and the shifted-copy tables are rebuilt)."""
+    O.set_threads(CPU_THREADS)
+    tau = seeded_tau()
+    t = time.time()
+    g1pp = O.power_tau(tau, 0, GEO.log_n)
+    high = O.power_tau(tau, 3 * GEO.n, GEO.log_n)
+    log("powers of tau (oracle, %d threads): %.1f s" % (CPU_THREADS, time.time() - t))
+    before = ctx.params_download(2)
+    t = time.time()
+    ctx.params_from_pp(2, g1pp, high)
+    log("params_from_pp, one coset at 2^20: %.2f s" % (time.time() - t))
+    assert np.array_equal(ctx.params_download(2), before)
+
+
 def test_encode_closed_form(blob):
     raw, out = blob
     O.set_threads(CPU_THREADS)
