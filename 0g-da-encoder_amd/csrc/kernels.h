@@ -851,59 +851,94 @@ __global__ __launch_bounds__(FB_LANES, 2) void k_fb_reduce(FbLevels lv, const ui
 // radix-2 DIT over XYZZ points, twiddle products by 254-bit scalar multiplication.
 // ---------------------------------------------------------------------------
 // canonical affine (zeros = infinity) -> XYZZ mont, at bit-reversed position
+// EC-FFT points live as reduced-radix XYZZ (X29_WORDS u32 each) between the stages.
+// canonical affine (64 zero bytes = infinity) -> x29
+__device__ __forceinline__ x29 ec29_from_canon(const uint32_t* s) {
+    fq x = f_load<FqP>(s), y = f_load<FqP>(s + 8);
+    const bool inf = f_is_zero(x) && f_is_zero(y);
+    a29 a;
+    a.x = q_from_fq(f_to_mont(x));
+    a.y = q_from_fq(f_to_mont(y));
+    return x29_select(inf, x29_inf(), x29_from_aff(a));
+}
+
 __global__ void k_ec_load_bitrev(const uint32_t* in, uint32_t* pts, int log_n) {
     uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= (1u << log_n)) return;
-    const uint32_t* s = in + 16 * (size_t)bitrev_bits(p, log_n);
-    fq x = f_load<FqP>(s), y = f_load<FqP>(s + 8);
-    bool inf = f_is_zero(x) && f_is_zero(y);
-    g1_aff a;
-    a.x = f_to_mont(x);
-    a.y = f_to_mont(y);
-    g1_xyzz r = xyzz_select(inf, xyzz_inf(), xyzz_from_aff(a));
-    xyzz_store(pts + 32 * (size_t)p, r);
+    x29_store(pts + X29_WORDS * (size_t)p, ec29_from_canon(in + 16 * (size_t)bitrev_bits(p, log_n)));
 }
 
-// k * P for a full-width canonical scalar (MSB-first, uniform per lane)
-__device__ __forceinline__ g1_xyzz xyzz_mul_scalar(const g1_xyzz& p, const fr& k) {
-    g1_xyzz r = xyzz_inf();
-    for (int i = 253; i >= 0; i--) {
-        r = xyzz_dbl(r);
-        g1_xyzz s = xyzz_add(r, p);
-        r = xyzz_select((k.v[i >> 5] >> (i & 31)) & 1, s, r);
+// k * P for a canonical scalar k < 2^254: 2-bit windows over the multiples P, 2P, 3P
+// (127 x (2 doublings + 1 addition), the addition selected away on zero digits).  The
+// multiples sit in LDS, thread-interleaved: a block of EC_THREADS threads needs
+// EC_LDS_WORDS u32.
+constexpr int EC_THREADS = 64;
+constexpr int EC_LDS_WORDS = 3 * X29_WORDS * EC_THREADS;
+
+__device__ __forceinline__ void ec_tab_store(uint32_t* tab, int k, const x29& v) {
+    uint32_t w[X29_WORDS];
+    x29_store(w, v);
+#pragma unroll
+    for (int i = 0; i < X29_WORDS; i++) tab[(k * X29_WORDS + i) * EC_THREADS + threadIdx.x] = w[i];
+}
+
+__device__ __forceinline__ x29 ec_tab_load(const uint32_t* tab, int k) {
+    uint32_t w[X29_WORDS];
+#pragma unroll
+    for (int i = 0; i < X29_WORDS; i++) w[i] = tab[(k * X29_WORDS + i) * EC_THREADS + threadIdx.x];
+    return x29_load(w);
+}
+
+__device__ __forceinline__ x29 x29_mul_scalar(const x29& p, const fr& k, uint32_t* tab) {
+    ec_tab_store(tab, 0, p);
+    const x29 p2 = x29_dbl(p);
+    ec_tab_store(tab, 1, p2);
+    ec_tab_store(tab, 2, x29_add(p2, p));
+    x29 r = x29_inf();
+#pragma unroll
+    for (int w = 7; w >= 0; w--) {  // scalar words by constant index: no private array
+        const uint32_t kw = k.v[w];
+        for (int j = w == 7 ? 14 : 15; j >= 0; j--) {  // bits 2j, 2j + 1 of word w (k < 2^254)
+            r = x29_dbl(x29_dbl(r));
+            const uint32_t d = (kw >> (2 * j)) & 3;
+            const x29 t = ec_tab_load(tab, d ? (int)d - 1 : 0);
+            r = x29_select(d != 0, x29_add(r, t), r);
+        }
     }
     return r;
 }
 
-__global__ __launch_bounds__(64) void k_ec_dit_stage(uint32_t* pts, const uint32_t* tw, int s, int tw_shift,
-                                                     uint32_t half_n) {
+// one radix-2 DIT stage: (u, v) -> (u + w v, u - w v), w = tw[j << tw_shift]
+__global__ __launch_bounds__(EC_THREADS) void k_ec_dit_stage(uint32_t* pts, const uint32_t* tw, int s, int tw_shift,
+                                                             uint32_t half_n) {
+    __shared__ uint32_t tab[EC_LDS_WORDS];
     uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= half_n) return;
     uint32_t half = 1u << (s - 1);
     uint32_t j = t & (half - 1);
     uint32_t i0 = ((t >> (s - 1)) << s) | j, i1 = i0 + half;
-    g1_xyzz u = xyzz_load(pts + 32 * (size_t)i0);
-    g1_xyzz v = xyzz_load(pts + 32 * (size_t)i1);
-    fr w = f_from_mont(f_load<FrP>(tw + 8 * (size_t)(j << tw_shift)));
-    v = xyzz_mul_scalar(v, w);
-    g1_xyzz nv = v;
-    nv.Y = f_neg(v.Y);
-    xyzz_store(pts + 32 * (size_t)i0, xyzz_add(u, v));
-    xyzz_store(pts + 32 * (size_t)i1, xyzz_add(u, nv));
+    x29 v = x29_load(pts + X29_WORDS * (size_t)i1);
+    v = x29_mul_scalar(v, f_from_mont(f_load<FrP>(tw + 8 * (size_t)(j << tw_shift))), tab);
+    const x29 u = x29_load(pts + X29_WORDS * (size_t)i0);
+    x29 nv = v;
+    nv.Y = q_sub(q_zero(), v.Y);
+    x29_store(pts + X29_WORDS * (size_t)i0, x29_add(u, v));
+    x29_store(pts + X29_WORDS * (size_t)i1, x29_add(u, nv));
 }
 
 // size_inv scaling of the inverse transform (N_ctx^-1 * 2^shift), then canonical affine out
-__global__ __launch_bounds__(64) void k_ec_store(const uint32_t* pts, const uint32_t* consts, int scale_shift,
-                                                 int scale, uint32_t* out, uint32_t n) {
+__global__ __launch_bounds__(EC_THREADS) void k_ec_store(const uint32_t* pts, const uint32_t* consts, int scale_shift,
+                                                         int scale, uint32_t* out, uint32_t n) {
+    __shared__ uint32_t tab[EC_LDS_WORDS];
     uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n) return;
-    g1_xyzz v = xyzz_load(pts + 32 * (size_t)p);
+    x29 v = x29_load(pts + X29_WORDS * (size_t)p);
     if (scale) {
         fr f = f_load<FrP>(consts + 8 * K_NINV);
         for (int i = 0; i < scale_shift; i++) f = f_dbl(f);
-        v = xyzz_mul_scalar(v, f_from_mont(f));
+        v = x29_mul_scalar(v, f_from_mont(f), tab);
     }
-    out_store_canon(out + 16 * (size_t)p, v);
+    out_store_canon(out + 16 * (size_t)p, x29_to_xyzz(v));
 }
 
 // ---------------------------------------------------------------------------

@@ -1,7 +1,7 @@
 // Device kernels of AMTParams::from_pp (crates/amt/src/prove_params/generate.rs:139-232):
 // prove bases from powers of tau [tau^i] G1 and [tau^(H+i)] G1 without knowing tau, as
 // a ceremony file provides them.  Included once, by zgr.hip, after kernels.h (the EC-FFT
-// stages k_ec_dit_stage and xyzz_mul_scalar).
+// stages k_ec_dit_stage and x29_mul_scalar).
 //
 //   coset scaling     g1pp'[i] = w^i g1pp[i], w = 1 / cf_c            (generate.rs:152-170)
 //   basis             iDFT(g1pp') = fast_ifft                          (gen_basis, :196-202)
@@ -9,27 +9,23 @@
 //                                                                      (gen_quotients, :204-232)
 //   high_basis        iDFT(high_g1pp')
 //   enact             affine, index_reverse                            (:122-126)
-// The transforms run as radix-2 DIT stages over XYZZ points: a bit-reversed load, log N
-// stages of butterflies with one 254-bit scalar multiplication each, then the store.
+// The transforms run as radix-2 DIT stages over reduced-radix XYZZ points: a
+// bit-reversed load, log N stages of butterflies with one 254-bit scalar
+// multiplication each (2-bit windows), then the store.
 #pragma once
 
 namespace zgr {
 
-// XYZZ point of a canonical affine input (64 zero bytes = infinity) times w^i, w^i from
+// x29 point of a canonical affine input (64 zero bytes = infinity) times w^i, w^i from
 // the Montgomery w by a 64-bit exponent (w = 1 for coset 0: no scaling).
-__global__ __launch_bounds__(64) void k_pp_scale(const uint32_t* in, const uint32_t* w_mont, int scale, uint32_t n,
-                                                 uint32_t* out) {
+__global__ __launch_bounds__(EC_THREADS) void k_pp_scale(const uint32_t* in, const uint32_t* w_mont, int scale,
+                                                         uint32_t n, uint32_t* out) {
+    __shared__ uint32_t tab[EC_LDS_WORDS];
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const uint32_t* s = in + 16 * (size_t)i;
-    fq x = f_load<FqP>(s), y = f_load<FqP>(s + 8);
-    const bool inf = f_is_zero(x) && f_is_zero(y);
-    g1_aff a;
-    a.x = f_to_mont(x);
-    a.y = f_to_mont(y);
-    g1_xyzz p = xyzz_select(inf, xyzz_inf(), xyzz_from_aff(a));
-    if (scale) p = xyzz_mul_scalar(p, f_from_mont(f_pow_u64(f_load<FrP>(w_mont), i)));
-    xyzz_store(out + 32 * (size_t)i, p);
+    x29 p = ec29_from_canon(in + 16 * (size_t)i);
+    if (scale) p = x29_mul_scalar(p, f_from_mont(f_pow_u64(f_load<FrP>(w_mont), i)), tab);
+    x29_store(out + X29_WORDS * (size_t)i, p);
 }
 
 // Bit-reversed load of the transform input: pts[p] = src[q], q = bitrev(p); with m > 0
@@ -38,27 +34,27 @@ __global__ void k_pp_load(const uint32_t* src, uint32_t* pts, int log_n, uint32_
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= (1u << log_n)) return;
     const uint32_t q = bitrev_bits(p, log_n);
-    uint4* d = (uint4*)(pts + 32 * (size_t)p);
+    uint32_t* d = pts + X29_WORDS * (size_t)p;
     if (m && (q == 0 || q > m)) {
-        const g1_xyzz z = xyzz_inf();
-        xyzz_store(pts + 32 * (size_t)p, z);
+        x29_store(d, x29_inf());
         return;
     }
-    const uint4* s = (const uint4*)(src + 32 * (size_t)(m ? m - q : q));
+    const uint32_t* s = src + X29_WORDS * (size_t)(m ? m - q : q);
 #pragma unroll
-    for (int k = 0; k < 8; k++) d[k] = s[k];
+    for (int k = 0; k < X29_WORDS; k++) d[k] = s[k];
 }
 
 // Transform output p -> param layout (x_mont || y_mont) at index bitrev(p) (enact's
 // index_reverse), scaled by N^-1 when scale != 0; infinity is stored as zeros.
-__global__ __launch_bounds__(64) void k_pp_store(const uint32_t* pts, const uint32_t* consts, int scale, int log_n,
-                                                 uint32_t* line) {
+__global__ __launch_bounds__(EC_THREADS) void k_pp_store(const uint32_t* pts, const uint32_t* consts, int scale,
+                                                         int log_n, uint32_t* line) {
+    __shared__ uint32_t tab[EC_LDS_WORDS];
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= (1u << log_n)) return;
-    g1_xyzz v = xyzz_load(pts + 32 * (size_t)p);
-    if (scale) v = xyzz_mul_scalar(v, f_from_mont(f_load<FrP>(consts + 8 * K_NINV)));
+    x29 v = x29_load(pts + X29_WORDS * (size_t)p);
+    if (scale) v = x29_mul_scalar(v, f_from_mont(f_load<FrP>(consts + 8 * K_NINV)), tab);
     bool inf;
-    const g1_aff a = xyzz_to_aff(v, &inf);
+    const g1_aff a = xyzz_to_aff(x29_to_xyzz(v), &inf);
     uint32_t* o = line + 16 * (size_t)bitrev_bits(p, log_n);
 #pragma unroll
     for (int k = 0; k < 8; k++) {

@@ -903,9 +903,9 @@ int zgr_params_from_pp(zgr_ctx* c, int coset, const uint8_t* g1pp, const uint8_t
     uint32_t *din = nullptr, *gp = nullptr, *hp = nullptr, *pts = nullptr, *w = nullptr;
     auto body = [&]() -> int {
         DALLOC(din, n * 64);
-        DALLOC(gp, n * 128);
-        DALLOC(hp, n * 128);
-        DALLOC(pts, n * 128);
+        DALLOC(gp, n * X29_WORDS * 4);
+        DALLOC(hp, n * X29_WORDS * 4);
+        DALLOC(pts, n * X29_WORDS * 4);
         DALLOC(w, 32);
         LAUNCH(c, st, "params_pp", k_pp_coset_w, 1, 64, 0, c->consts, coset, w);
         // coset-scaled powers (generate.rs:152-170); coset 0 is unscaled
@@ -1250,7 +1250,7 @@ int zgr_ecfft(zgr_ctx* c, const uint8_t* points, int log_n, int inverse, uint8_t
     uint32_t *din = nullptr, *dpts = nullptr;
     auto body = [&]() -> int {
         DALLOC(din, n * 64);
-        DALLOC(dpts, n * 128);
+        DALLOC(dpts, n * X29_WORDS * 4);
         HIPCHK(hipMemcpyAsync(din, points, n * 64, hipMemcpyHostToDevice, st));
         LAUNCH(c, st, "ecfft", k_ec_load_bitrev, blocks(n, 256), 256, 0, din, dpts, log_n);
         const uint32_t* tw = inverse ? c->tw_inv : c->tw;
