@@ -7,7 +7,8 @@ the CPU oracle timed on a bounded sample of the same workload.
 A step is one encode of B seeded full-size blobs (raw bytes already resident in
 HBM) through the C ABI, outputs (encoded data, commitment, AMT proofs, Merkle
 tree) copied back to host memory.  Also times RS recovery of the same blob from its
-rows (device and oracle).  Prints one JSON line.
+rows, the deferred-verification MSM over its rows and a 2^20-point EC-FFT (device,
+and the oracle where it finishes in seconds).  Prints one JSON line.
 """
 import argparse
 import json
@@ -198,6 +199,44 @@ def main():
         recovery[label] = {"ms": round(float(np.median(tt)) * 1e3, 2), "kernel_ms_per_call": ks}
     recovery["note"] = "zgr_recover, host rows in (PCIe included), raw bytes out; median of 5"
 
+    # deferred verification MSM (zgr_verify_msm): every row of the plain blob, one random
+    # alpha each -- one 2^20-point MSM per coset over the resident basis
+    vrng = np.random.default_rng(DATA_SEED + 3)
+    v_ids = list(range(geo.row_encoded))
+    v_rows = out1.data.tobytes()
+    v_alphas = [int.from_bytes(vrng.bytes(32), "little") % Z.R_MOD for _ in v_ids]
+    ctx.verify_msm(v_ids, v_rows, v_alphas)
+    vt = []
+    for _ in range(3):
+        s0 = time.perf_counter()
+        ctx.verify_msm(v_ids, v_rows, v_alphas)
+        vt.append(time.perf_counter() - s0)
+    verify = {"rows": len(v_ids), "ms": round(float(np.median(vt)) * 1e3, 2),
+              "note": "zgr_verify_msm over 3 x 1024 rows (host rows in), median of 3"}
+
+    # G1 EC-FFT (zgr_ecfft) over 2^20 points: 1024 distinct multiples kG, tiled
+    gx, gy, pm = 1, 2, Z.P_MOD
+    pts_small = []
+    x, y = gx, gy
+    for _ in range(1024):
+        pts_small.append(x.to_bytes(32, "little") + y.to_bytes(32, "little"))
+        lam = (y - gy) * pow(x - gx, pm - 2, pm) % pm if x != gx else 3 * x * x * pow(2 * y, pm - 2, pm) % pm
+        nx = (lam * lam - x - gx) % pm
+        x, y = nx, (lam * (x - nx) - y) % pm
+    ec_in = np.frombuffer(b"".join(pts_small) * (geo.n // 1024), dtype=np.uint8)
+    ctx.set_profiling(True)
+    ctx.kernel_time_reset()
+    s0 = time.perf_counter()
+    ctx.ecfft(ec_in, geo.log_n, inverse=False)
+    ec_wall = time.perf_counter() - s0
+    ec_ms, ec_cnt = ctx.kernel_time_ms("ecfft")
+    ctx.set_profiling(False)
+    butterflies = geo.log_n * geo.n // 2
+    ecfft = {"log_n": geo.log_n, "wall_ms": round(ec_wall * 1e3, 1), "kernel_ms": round(ec_ms * ec_cnt, 1),
+             "butterflies_per_s": round(butterflies / (ec_ms * ec_cnt * 1e-3)) if ec_cnt else None,
+             "note": "one forward DFT of 2^20 G1 points (Pairing::fast_fft); 20 x 2^19 butterflies, "
+                     "each a 254-bit scalar multiplication"}
+
     # batched path: args.batch full-size blobs per call (two blobs in flight)
     nb_ = max(2, args.batch)
     braws = [Z.seeded_bytes(DATA_SEED + 1 + i, geo.max_raw_data_size) for i in range(nb_)]
@@ -323,6 +362,8 @@ def main():
         },
         "kernels": kernels,
         "recovery": recovery,
+        "verify_msm": verify,
+        "ecfft": ecfft,
     }
     if not args.no_cpu:
         res["cpu_baseline"] = cpu_baseline(ctx, geo, raws[0], args.cpu_threads)
@@ -337,6 +378,14 @@ def main():
         recovery["coset2_1024_rows"]["cpu_oracle_ms"] = round((time.time() - t0) * 1e3, 1)
         recovery["coset2_1024_rows"]["cpu_oracle_threads"] = args.cpu_threads
         assert rc == 0 and got == raws[0]
+        # the oracle's deferred-verification MSM over coset 0's 1024 rows (a third)
+        bases = [ctx.params_download(c) for c in range(geo.coset_n)]
+        w1 = 32 * geo.col_n
+        t0 = time.time()
+        O.verify_msm(bases, geo.log_col, geo.log_row, v_ids[:geo.row_n], v_rows[:geo.row_n * w1],
+                     v_alphas[:geo.row_n])
+        verify["cpu_oracle_ms_1024_rows"] = round((time.time() - t0) * 1e3, 1)
+        verify["cpu_oracle_threads"] = args.cpu_threads
     ctx.close()
     print(json.dumps(res))
 

@@ -1464,7 +1464,9 @@ int zgo_verify_msm(const uint64_t *const *bases, int log_col, int log_row, int c
             memcpy(pts + 8 * (k * COL + i), bases[c] + 8 * ((size_t)log_row * n + b * COL + i), 64);
         }
     }
-    zgo_msm(pts, sc, n_rows * COL, out);
+    g1j r;
+    par_msm(&r, pts, sc, n_rows * COL);
+    j_store_canon(out, &r);
     free(pts);
     free(sc);
     return 0;
