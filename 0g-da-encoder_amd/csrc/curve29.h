@@ -103,6 +103,28 @@ __device__ __forceinline__ x29 x29_madd(const x29& p, const a29& q, bool live = 
     return x29_select(pinf, x29_from_aff(q), r);
 }
 
+// p + q for affine q by the generic formula alone (madd-2008-s without its exceptional
+// cases), for the bucket loop.  *rare is set on every lane where x(p) may equal x(q) or
+// p may be infinity: H = x(q) ZZ - X lies in [0, 4p) and is 0 mod p only when it is 0,
+// p, 2p or 3p, which its low limb screens; the all-zero infinity (x29_inf) gives
+// H = 2p exactly.  Those lanes (and, at random, about one in 2^27 others) must redo the
+// addition with x29_madd; the value returned there is meaningless.
+__device__ __forceinline__ x29 x29_madd_unchecked(const x29& p, const a29& q, bool* rare) {
+    fq29 P = q_sub_lazy(q_mul(q.x, p.ZZ), p.X);
+    fq29 R = q_sub_lazy(q_mul(q.y, p.ZZZ), p.Y);
+    const uint32_t h0 = P.l[0];
+    *rare = h0 == 0u || h0 == Q29::P[0] || h0 == Q29::P2[0] || h0 == Q29::P3_L0;
+    x29 r;
+    fq29 PP = q_sqr(P);
+    r.ZZ = q_mul(p.ZZ, PP);
+    fq29 PPP = q_mul(P, PP);
+    r.ZZZ = q_mul(p.ZZZ, PPP);
+    fq29 Q = q_mul(p.X, PP);
+    r.X = q_sub(q_sub(q_sqr(R), PPP), q_dbl(Q));
+    r.Y = q_mul2(R, q_sub_lazy(Q, r.X), q_sub_lazy(q_zero(), p.Y), PPP);  // R (Q - X3) - Y1 PPP
+    return r;
+}
+
 // memory layout of a stored XYZZ point: X, Y, ZZ, ZZZ, 9 limbs each (36 u32)
 constexpr int X29_WORDS = 36;
 

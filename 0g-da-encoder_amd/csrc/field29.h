@@ -23,6 +23,7 @@ struct Q29 {
                                       0x02db40c0u, 0x00a6e141u, 0x0e5c2634u, 0x0030644eu};
     static constexpr uint32_t P2[9] = {0x10f9fa8eu, 0x0208c16du, 0x18e5469eu, 0x05aa45a1u, 0x0b0bb2f0u,
                                        0x05b68181u, 0x014dc282u, 0x1cb84c68u, 0x0060c89cu};
+    static constexpr uint32_t P3_L0 = 0x0976f7d5u;  // low limb of 3p
     static constexpr uint32_t ONE[9] = {0x157ccc21u, 0x141c2758u, 0x185230d3u, 0x014c0419u, 0x0aa36fb9u,
                                         0x1d4240ceu, 0x11d54c07u, 0x052ac7a8u, 0x000dc836u};
     // 2^5 * R' mod p: mont29(v, C_TO) = v * 2^5, i.e. 2^256-Montgomery -> 2^261-Montgomery
@@ -185,6 +186,19 @@ __device__ __forceinline__ fq29 q_sub(const fq29& a, const fq29& b) {
 }
 
 __device__ __forceinline__ fq29 q_neg(const fq29& a) { return q_sub(q_zero(), a); }
+
+// neg ? p - a : a for a in [0, p]; p - a needs no reduction, so the result stays in [0, p]
+__device__ __forceinline__ fq29 q_cneg_canon(const fq29& a, bool neg) {
+    fq29 r;
+    int32_t c = 0;
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+        int32_t x = (int32_t)Q29::P[k] - (int32_t)a.l[k] + c;
+        r.l[k] = neg ? ((uint32_t)x & M29) : a.l[k];
+        c = x >> 29;  // arithmetic: -1 or 0
+    }
+    return r;
+}
 
 // a - b + 2p in (0, 4p) without the final reduction: only for values that feed a
 // multiplication (which accepts inputs below 4p) or q_is_zero4

@@ -771,9 +771,14 @@ __global__ __launch_bounds__(FB_LANES, 2) void k_fb_sums(const uint32_t* table, 
         p1 = pn[1];
         p2 = pn[2];
         p3 = pn[3];
-        if ((ev >> FB_SIGN_SHIFT) & 1u) P.y = q_neg(P.y);
+        P.y = q_cneg_canon(P.y, (ev >> FB_SIGN_SHIFT) & 1u);  // table points are canonical
         const bool same = m == cur;
-        x29 M = x29_madd(R, P, active);
+        bool rare;
+        x29 M = x29_madd_unchecked(R, P, &rare);
+        // a point added to itself or its negative, or to a bucket that cancelled to
+        // infinity: the exact formula, on a wave-uniform branch that is almost never taken
+        if (__builtin_expect(__any(active && same && rare), 0))
+            if (active && same && rare) M = x29_madd(R, P);
         if (active && !same && cur != 0) {  // bucket cur is complete in this share
             if (head && cont)
                 x29_store(first_p + s * X29_WORDS, R);
