@@ -104,24 +104,38 @@ __device__ __forceinline__ x29 x29_madd(const x29& p, const a29& q, bool live = 
 }
 
 // p + q for affine q by the generic formula alone (madd-2008-s without its exceptional
-// cases), for the bucket loop.  *rare is set on every lane where x(p) may equal x(q) or
-// p may be infinity: H = x(q) ZZ - X lies in [0, 4p) and is 0 mod p only when it is 0,
-// p, 2p or 3p, which its low limb screens; the all-zero infinity (x29_inf) gives
-// H = 2p exactly.  Those lanes (and, at random, about one in 2^27 others) must redo the
-// addition with x29_madd; the value returned there is meaningless.
+// cases), for the bucket loop, on lazy-X points: X in [0, 8p), Y, ZZ, ZZZ in [0, 2p), as
+// this function returns them (X only ever feeds multiplications and q_sub_lazy8, which
+// accept it; x29_norm brings a point back to the usual form).  *rare is set on every
+// lane where x(p) may equal x(q) or p may be infinity: H = x(q) ZZ - X + 8p lies in
+// (0, 10p) and is 0 mod p only when it is one of 0, p, .., 9p, which its low limb
+// screens; the all-zero infinity (x29_inf) gives H = 8p exactly.  Those lanes (and, at
+// random, about one in 2^26 others) must redo the addition exactly (x29_madd on
+// x29_norm(p)); the value returned there is meaningless.
 __device__ __forceinline__ x29 x29_madd_unchecked(const x29& p, const a29& q, bool* rare) {
-    fq29 P = q_sub_lazy(q_mul(q.x, p.ZZ), p.X);
+    fq29 H = q_sub_lazy8(q_mul(q.x, p.ZZ), p.X);
     fq29 R = q_sub_lazy(q_mul(q.y, p.ZZZ), p.Y);
-    const uint32_t h0 = P.l[0];
-    *rare = h0 == 0u || h0 == Q29::P[0] || h0 == Q29::P2[0] || h0 == Q29::P3_L0;
+    const uint32_t h0 = H.l[0];
+    bool hit = false;
+#pragma unroll
+    for (int k = 0; k < 10; k++) hit |= h0 == Q29::KP_L0[k];
+    *rare = hit;
     x29 r;
-    fq29 PP = q_sqr(P);
+    fq29 PP = q_sqr(H);  // H^2 < 100 p^2: below 2p (q_mul's bound)
     r.ZZ = q_mul(p.ZZ, PP);
-    fq29 PPP = q_mul(P, PP);
+    fq29 PPP = q_mul(H, PP);
     r.ZZZ = q_mul(p.ZZZ, PPP);
     fq29 Q = q_mul(p.X, PP);
-    r.X = q_sub(q_sub(q_sqr(R), PPP), q_dbl(Q));
-    r.Y = q_mul2(R, q_sub_lazy(Q, r.X), q_sub_lazy(q_zero(), p.Y), PPP);  // R (Q - X3) - Y1 PPP
+    r.X = q_x3_lazy(q_sqr(R), PPP, Q);
+    // R (Q - X3) - Y1 PPP: products below 4p * 10p + 2p * 2p = 44 p^2
+    r.Y = q_mul2(R, q_sub_lazy8(Q, r.X), q_sub_lazy(q_zero(), p.Y), PPP);
+    return r;
+}
+
+// a lazy-X point (x29_madd_unchecked) in the usual form: every coordinate in [0, 2p)
+__device__ __forceinline__ x29 x29_norm(const x29& p) {
+    x29 r = p;
+    r.X = q_reduce_8p(p.X);
     return r;
 }
 

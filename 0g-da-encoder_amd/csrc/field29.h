@@ -23,7 +23,15 @@ struct Q29 {
                                       0x02db40c0u, 0x00a6e141u, 0x0e5c2634u, 0x0030644eu};
     static constexpr uint32_t P2[9] = {0x10f9fa8eu, 0x0208c16du, 0x18e5469eu, 0x05aa45a1u, 0x0b0bb2f0u,
                                        0x05b68181u, 0x014dc282u, 0x1cb84c68u, 0x0060c89cu};
-    static constexpr uint32_t P3_L0 = 0x0976f7d5u;  // low limb of 3p
+    static constexpr uint32_t P4[9] = {0x01f3f51cu, 0x041182dbu, 0x11ca8d3cu, 0x0b548b43u, 0x161765e0u,
+                                       0x0b6d0302u, 0x029b8504u, 0x197098d0u, 0x00c19139u};
+    static constexpr uint32_t P6[9] = {0x12edefaau, 0x061a4448u, 0x0aafd3dau, 0x10fed0e5u, 0x012318d0u,
+                                       0x11238484u, 0x03e94786u, 0x1628e538u, 0x012259d6u};
+    static constexpr uint32_t P8[9] = {0x03e7ea38u, 0x082305b6u, 0x03951a78u, 0x16a91687u, 0x0c2ecbc0u,
+                                       0x16da0605u, 0x05370a08u, 0x12e131a0u, 0x01832273u};
+    // low limbs of k p, k = 0..9
+    static constexpr uint32_t KP_L0[10] = {0x00000000u, 0x187cfd47u, 0x10f9fa8eu, 0x0976f7d5u, 0x01f3f51cu,
+                                           0x1a70f263u, 0x12edefaau, 0x0b6aecf1u, 0x03e7ea38u, 0x1c64e77fu};
     static constexpr uint32_t ONE[9] = {0x157ccc21u, 0x141c2758u, 0x185230d3u, 0x014c0419u, 0x0aa36fb9u,
                                         0x1d4240ceu, 0x11d54c07u, 0x052ac7a8u, 0x000dc836u};
     // 2^5 * R' mod p: mont29(v, C_TO) = v * 2^5, i.e. 2^256-Montgomery -> 2^261-Montgomery
@@ -55,7 +63,9 @@ __device__ __forceinline__ fq29 q_zero() {
 
 __device__ __forceinline__ fq29 q_one() { return q_const(Q29::ONE); }
 
-// Montgomery product a * b / 2^261 for a, b in [0, 4p); result in [0, 2p)
+// Montgomery product a * b / 2^261; result in [0, 2p) whenever a * b < p 2^261 (about
+// 169 p^2: both inputs below 8p, or one below 10p and the other below 16p, qualify),
+// since the result is below a b / 2^261 + p.  Inputs below 4p are the common case.
 __device__ __forceinline__ fq29 q_mul(const fq29& a, const fq29& b) {
     uint64_t T[18];
 #pragma unroll
@@ -212,6 +222,50 @@ __device__ __forceinline__ fq29 q_sub_lazy(const fq29& a, const fq29& b) {
         c = x >> 29;
     }
     return s;
+}
+
+// a - b + 8p for a in [0, 2p), b in [0, 8p): (0, 10p) with normalised limbs, for
+// values that only feed multiplications (see q_mul)
+__device__ __forceinline__ fq29 q_sub_lazy8(const fq29& a, const fq29& b) {
+    fq29 s;
+    int32_t c = 0;
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+        int32_t x = (int32_t)a.l[k] + (int32_t)Q29::P8[k] - (int32_t)b.l[k] + c;
+        s.l[k] = (uint32_t)x & M29;
+        c = x >> 29;
+    }
+    return s;
+}
+
+// r2 - t - 2 u + 6p for r2, t, u in [0, 2p): (0, 8p) in one carry pass (the X3 of an
+// XYZZ addition, R^2 - PPP - 2Q, left unreduced)
+__device__ __forceinline__ fq29 q_x3_lazy(const fq29& r2, const fq29& t, const fq29& u) {
+    fq29 s;
+    int32_t c = 0;
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+        int32_t x = (int32_t)r2.l[k] + (int32_t)Q29::P6[k] - (int32_t)t.l[k] - 2 * (int32_t)u.l[k] + c;
+        s.l[k] = (uint32_t)x & M29;
+        c = x >> 29;  // arithmetic: -3 .. 1
+    }
+    return s;
+}
+
+// [0, 8p) -> [0, 2p)
+__device__ __forceinline__ fq29 q_reduce_8p(const fq29& s) {
+    fq29 d;
+    int32_t br = 0;
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+        int32_t x = (int32_t)s.l[k] - (int32_t)Q29::P4[k] - br;
+        br = (x >> 31) & 1;
+        d.l[k] = (uint32_t)x & M29;
+    }
+    fq29 r;
+#pragma unroll
+    for (int k = 0; k < 9; k++) r.l[k] = br ? s.l[k] : d.l[k];
+    return q_reduce_2p(r);
 }
 
 // [0, 2p) -> [0, p)

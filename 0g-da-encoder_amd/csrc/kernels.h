@@ -708,6 +708,8 @@ __global__ __launch_bounds__(1024) void k_fb_scatter(const uint32_t* y, int log_
 //     summed through LDS into the group's heap node of line L.
 // Splitting the work this way keeps every step of a phase the same addition on every
 // lane: phase 1 never adds two projective points, phase 2 never touches a base.
+// Phase 1 keeps X unreduced in [0, 8p) (x29_madd_unchecked); phase 2 normalises each
+// bucket sum as it loads it.
 constexpr int FB_PER_LANE = FB_BUCKETS / FB_LANES;  // buckets per lane in phase 2 (32)
 constexpr uint32_t FB_NO_SHARE = 0xffffffffu;        // lane with no entries
 // XYZZ slots per (line, chunk) in `bkt`: the buckets, then each lane's first- and
@@ -778,7 +780,7 @@ __global__ __launch_bounds__(FB_LANES, 2) void k_fb_sums(const uint32_t* table, 
         // a point added to itself or its negative, or to a bucket that cancelled to
         // infinity: the exact formula, on a wave-uniform branch that is almost never taken
         if (__builtin_expect(__any(active && same && rare), 0))
-            if (active && same && rare) M = x29_madd(R, P);
+            if (active && same && rare) M = x29_madd(x29_norm(R), P);
         if (active && !same && cur != 0) {  // bucket cur is complete in this share
             if (head && cont)
                 x29_store(first_p + s * X29_WORDS, R);
@@ -830,7 +832,7 @@ __global__ __launch_bounds__(FB_LANES, 2) void k_fb_reduce(FbLevels lv, const ui
         const uint32_t m = (uint32_t)(s * FB_PER_LANE + q);
         const bool filled = bst[m + 1] > bst[m];
         x29 B = x29_inf();
-        if (filled) B = x29_load(bk + (size_t)(m - 1) * X29_WORDS);
+        if (filled) B = x29_norm(x29_load(bk + (size_t)(m - 1) * X29_WORDS));  // lazy X from k_fb_sums
         Rr = x29_add(Rr, B);
         T = x29_add(T, Rr);
     }
