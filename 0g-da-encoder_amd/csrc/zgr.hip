@@ -321,18 +321,44 @@ int amt_outputs(zgr_ctx* c, Lane& ln, uint32_t* out, int lines, bool grouped) {
 // Work lists of the grouped bucket kernels (one u32 per wave: L | gl << 8 | S << 10),
 // deepest levels first so that the longest waves start first; within a level the lines
 // of one chunk group are adjacent (they read the same sorted entries).
-// Every line sits at the largest level its outputs allow (fb_group_log): on MI355X this
-// measured faster than plans that keep some lines at smaller groups to shorten the
-// longest waves.  ZGR_GROUP_PLAN=none keeps every line at level 0 (one bucket set per
-// chunk, as multiple_multiexp needs).
+// Every line sits at the largest level its outputs allow (fb_group_log), with one
+// exception: when the waves of the deepest level outnumber the wave slots k_fb_sums has
+// on the device (4 SIMDs per CU, 2 waves each at its register count), the ones left over
+// would run last and alone, so the high-basis line (read only at the root) drops one
+// level.  At 2^20 x 12 lines that turns 2304 four-chunk waves into 2048 and measured 4 %
+// faster; other plans that keep lines at smaller groups measured slower.
+// ZGR_GROUP_PLAN=none keeps every line at level 0 (one bucket set per chunk, as
+// multiple_multiexp needs); a comma-separated list sets each line's level explicitly.
+constexpr int FB_SUMS_WAVES_PER_SIMD = 2;
+
 int build_work_lists(zgr_ctx* c) {
+    int cus = 0;
+    HIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device));
+    const size_t slots = (size_t)cus * 4 * FB_SUMS_WAVES_PER_SIMD;
     const char* mode = getenv("ZGR_GROUP_PLAN");
     const bool none = mode && !strcmp(mode, "none");
+    // an explicit plan: one level per line, comma-separated, each capped at fb_group_log
+    std::vector<int> given;
+    if (mode && !none)
+        for (const char* q = mode; *q;) {
+            given.push_back((int)strtol(q, (char**)&q, 10));
+            if (*q == ',') q++;
+            else if (*q) return ZGR_ERR_INVALID_ARG;
+        }
     for (int v = 0; v < 2; v++) {
         const int lines = c->nlines - v;
         std::vector<int> gl(lines, 0);
         if (!none)
-            for (int L = 0; L < lines; L++) gl[L] = fb_group_log(L, c->h);
+            for (int L = 0; L < lines; L++) {
+                gl[L] = fb_group_log(L, c->h);
+                if (L < (int)given.size()) gl[L] = std::max(0, std::min(gl[L], given[L]));
+            }
+        if (!none && given.empty() && lines == c->h + 2 && gl[c->h + 1] > 0) {
+            const int top = *std::max_element(gl.begin(), gl.end());
+            size_t deepest = 0;
+            for (int L = 0; L < lines; L++) deepest += gl[L] == top ? c->nb >> top : 0;
+            if (gl[c->h + 1] == top && deepest > slots && deepest - (c->nb >> top) <= slots) gl[c->h + 1]--;
+        }
         std::vector<uint32_t> w;
         c->plan[v] = 0;
         for (int L = 0; L < lines; L++) {

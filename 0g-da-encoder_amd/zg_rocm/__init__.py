@@ -390,9 +390,12 @@ def make_context(geo=PRODUCTION, fixed_base=True, group_plan=None, device=0):
     """Context with its MSM path chosen explicitly.  zgr_ctx_create reads
     ZGR_FIXED_BASE: the default keeps shifted copies of the resident bases (one bucket
     set per chunk, FB_W = 22 copies: 22 x the params' HBM); "0" selects the windowed
-    kernels.  ZGR_GROUP_PLAN="none" gives every AMT line one bucket set per chunk
-    (default: each line shares bucket sets over the largest chunk groups its outputs
-    allow, up to 4 chunks).  All give identical results."""
+    kernels.  ZGR_GROUP_PLAN="none" gives every AMT line one bucket set per chunk, and a
+    comma-separated list ("2,2,1,0,...") sets each line's chunk-group level (log2 of the
+    chunks sharing one bucket set), capped at what the line's outputs allow (default:
+    the largest such groups, up to 4 chunks, with the high-basis line one level lower
+    when that fits the longest waves into the device's wave slots).  All give identical
+    results."""
     want = {"ZGR_FIXED_BASE": "1" if fixed_base else "0", "ZGR_GROUP_PLAN": group_plan}
     old = {k: os.environ.get(k) for k in want}
     try:

@@ -12,14 +12,19 @@ pytestmark = pytest.mark.gpu
 GEO = Z.TEST_SETTING
 
 
-@pytest.fixture(scope="module", params=["fixed_base", "fixed_base_per_chunk", "windowed"])
+PLANS = {"fixed_base": None, "fixed_base_per_chunk": "none", "fixed_base_mixed_plan": "1,0,2,1,0,2,1,0,2,1,0,2",
+         "windowed": None}
+
+
+@pytest.fixture(scope="module", params=list(PLANS))
 def ctx(request):
     """Every MSM path over the resident bases: the shifted-copy tables (default: the
     AMT lines share bucket sets over chunk groups), the same with one bucket set per
-    chunk on every line (ZGR_GROUP_PLAN=none), and the windowed kernels
-    (ZGR_FIXED_BASE=0); both variables are read at context creation."""
+    chunk on every line (ZGR_GROUP_PLAN=none) and with an explicit per-line plan that
+    mixes the group levels (each capped at what the line's outputs allow), and the
+    windowed kernels (ZGR_FIXED_BASE=0); both variables are read at context creation."""
     fixed = request.param != "windowed"
-    c = Z.make_context(GEO, fixed_base=fixed, group_plan="none" if request.param == "fixed_base_per_chunk" else None)
+    c = Z.make_context(GEO, fixed_base=fixed, group_plan=PLANS[request.param])
     c.set_profiling(True)
     c.params_generate(seeded_tau())
     # the table build runs only on the fixed-base path
@@ -39,7 +44,7 @@ def test_msm_path_in_use(ctx):
     merges = ctx.kernel_time_ms("msm_group_scatter")[1]
     ctx.set_profiling(False)
     ctx.kernel_time_reset()
-    assert (merges > 0) == (ctx.mode == "fixed_base")
+    assert (merges > 0) == (ctx.mode in ("fixed_base", "fixed_base_mixed_plan"))
 
 
 @pytest.fixture(scope="module")
