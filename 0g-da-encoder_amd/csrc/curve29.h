@@ -62,7 +62,11 @@ __device__ __forceinline__ x29 x29_dbl(const x29& p) {
     return r;
 }
 
-__device__ __forceinline__ x29 x29_add(const x29& p, const x29& q) {
+// p + q.  Inputs may carry X unreduced in [0, 8p) (X only feeds multiplications here and
+// in x29_dbl).  LAZY_X: the result's X is left in [0, 8p) as well (one carry pass for
+// X3 = R^2 - PPP - 2Q instead of three reduced subtractions); x29_norm restores it.
+template <bool LAZY_X = false>
+__device__ __forceinline__ x29 x29_add_t(const x29& p, const x29& q) {
     const bool pinf = x29_is_inf(p), qinf = x29_is_inf(q);
     fq29 U1 = q_mul(p.X, q.ZZ);
     fq29 P = q_sub_lazy(q_mul(q.X, p.ZZ), U1);  // < 4p, feeds multiplications only
@@ -77,12 +81,20 @@ __device__ __forceinline__ x29 x29_add(const x29& p, const x29& q) {
     fq29 PPP = q_mul(P, PP);
     r.ZZZ = q_mul(q_mul(p.ZZZ, q.ZZZ), PPP);
     fq29 Q = q_mul(U1, PP);
-    r.X = q_sub(q_sub(q_sqr(R), PPP), q_dbl(Q));
-    r.Y = q_mul2(R, q_sub_lazy(Q, r.X), q_sub_lazy(q_zero(), S1), PPP);  // R (Q - X3) - S1 PPP
+    if (LAZY_X) {
+        r.X = q_x3_lazy(q_sqr(R), PPP, Q);
+        r.Y = q_mul2(R, q_sub_lazy8(Q, r.X), q_sub_lazy(q_zero(), S1), PPP);  // R (Q - X3) - S1 PPP
+    } else {
+        r.X = q_sub(q_sub(q_sqr(R), PPP), q_dbl(Q));
+        r.Y = q_mul2(R, q_sub_lazy(Q, r.X), q_sub_lazy(q_zero(), S1), PPP);
+    }
     r = x29_select(pinf, q, r);
     r = x29_select(qinf, p, r);
     return r;
 }
+
+__device__ __forceinline__ x29 x29_add(const x29& p, const x29& q) { return x29_add_t<false>(p, q); }
+__device__ __forceinline__ x29 x29_add_lazy(const x29& p, const x29& q) { return x29_add_t<true>(p, q); }
 
 // p + q for affine q (madd-2008-s, 8M + 2S); q is never infinity.  Lanes whose
 // result is discarded pass live = false so that they never enter the doubling branch.

@@ -708,8 +708,8 @@ __global__ __launch_bounds__(1024) void k_fb_scatter(const uint32_t* y, int log_
 //     summed through LDS into the group's heap node of line L.
 // Splitting the work this way keeps every step of a phase the same addition on every
 // lane: phase 1 never adds two projective points, phase 2 never touches a base.
-// Phase 1 keeps X unreduced in [0, 8p) (x29_madd_unchecked); phase 2 normalises each
-// bucket sum as it loads it.
+// Both phases keep X unreduced in [0, 8p) (x29_madd_unchecked, x29_add_lazy); phase 2
+// normalises only the group's result.
 constexpr int FB_PER_LANE = FB_BUCKETS / FB_LANES;  // buckets per lane in phase 2 (32)
 constexpr uint32_t FB_NO_SHARE = 0xffffffffu;        // lane with no entries
 // XYZZ slots per (line, chunk) in `bkt`: the buckets, then each lane's first- and
@@ -832,25 +832,25 @@ __global__ __launch_bounds__(FB_LANES, 2) void k_fb_reduce(FbLevels lv, const ui
         const uint32_t m = (uint32_t)(s * FB_PER_LANE + q);
         const bool filled = bst[m + 1] > bst[m];
         x29 B = x29_inf();
-        if (filled) B = x29_norm(x29_load(bk + (size_t)(m - 1) * X29_WORDS));  // lazy X from k_fb_sums
-        Rr = x29_add(Rr, B);
-        T = x29_add(T, Rr);
+        if (filled) B = x29_load(bk + (size_t)(m - 1) * X29_WORDS);
+        Rr = x29_add_lazy(Rr, B);
+        T = x29_add_lazy(T, Rr);
     }
     // T + 32 s Rr = (s Rr) doubled 5 times, plus T
     x29 off = x29_mul_small_uniform(Rr, (uint32_t)s, 6);
     for (int k = 0; k < 5; k++) off = x29_dbl(off);
-    x29 W = x29_add(T, off);
+    x29 W = x29_add_lazy(T, off);
 
     x29_store(red + s * X29_WORDS, W);
     __syncthreads();
     for (int off = 1; off < FB_LANES; off <<= 1) {
         if ((s & (2 * off - 1)) == 0) {
-            W = x29_add(W, x29_load(red + (s + off) * X29_WORDS));
+            W = x29_add_lazy(W, x29_load(red + (s + off) * X29_WORDS));
             x29_store(red + s * X29_WORDS, W);
         }
         __syncthreads();
     }
-    if (s == 0) x29_store(leaf, W);
+    if (s == 0) x29_store(leaf, x29_norm(W));
 }
 
 // ---------------------------------------------------------------------------
