@@ -598,6 +598,13 @@ __device__ __forceinline__ void fb_digits(const uint32_t* y, int log_n, int from
 //                 gl <= gl_used: group S = b >> gl lists bucket m of its chunks one after
 //                 the other (chunk g's indices offset by g * FB_W * bs, its table region),
 //                 so the group's bucket start is the sum of its chunks' bucket starts.
+//                 The chunk's list is assembled in LDS, FB_STAGE slots at a time (a
+//                 window of the sorted order; the slots stay in registers meanwhile),
+//                 and each window is copied out: at level 0 as one contiguous block, at
+//                 a group level one bucket run at a time, so the global writes are runs
+//                 of consecutive words rather than single words at random slots.  The
+//                 window keeps the block's LDS near 47 KB, so that the scatter of one
+//                 coset still finds room on CUs that run another coset's bucket kernels.
 __global__ __launch_bounds__(1024) void k_fb_count(const uint32_t* y, int log_n, int log_bs, int from_y,
                                                    uint32_t* counts, uint32_t* bstart) {
     __shared__ uint32_t hist[FB_BUCKETS + 1];  // by |digit|; [0] unused
@@ -642,6 +649,9 @@ __global__ __launch_bounds__(1024) void k_fb_count(const uint32_t* y, int log_n,
     }
 }
 
+constexpr int FB_STAGE = (FB_W << FB_MAX_LOG_BS) / 4;  // k_fb_scatter's LDS window (5632 slots)
+constexpr uint32_t FB_NO_SLOT = 0xffffffffu;
+
 // level-gl outputs of k_fb_scatter (level 0: ent only; cnt / bst come from k_fb_count)
 struct FbOut {
     uint32_t* ent[FB_GL_MAX + 1];
@@ -653,6 +663,7 @@ __global__ __launch_bounds__(1024) void k_fb_scatter(const uint32_t* y, int log_
                                                      const uint32_t* bstart, int gl_used, FbOut out) {
     __shared__ uint32_t run[FB_BUCKETS + 1];                // next level-0 slot of bucket m
     __shared__ uint32_t base[FB_GL_MAX][FB_BUCKETS + 1];    // level-gl slot - level-0 slot
+    __shared__ uint32_t staged[FB_STAGE];                   // one window of the sorted list
     const uint32_t b = blockIdx.x, k = threadIdx.x, T = blockDim.x;
     const size_t cap = (size_t)FB_W << log_bs;  // entries (and table points) of one chunk
     const uint32_t* b0 = bstart + (size_t)b * FB_BSTART;
@@ -681,18 +692,37 @@ __global__ __launch_bounds__(1024) void k_fb_scatter(const uint32_t* y, int log_
     __syncthreads();
     int dig[FB_W];
     fb_digits(y, log_n, from_y, (b << log_bs) + k, dig);
+    uint32_t pos[FB_W];  // level-0 slot of each digit's entry, or FB_NO_SLOT
 #pragma unroll
     for (int w = 0; w < FB_W; w++) {
         const int d = dig[w];
-        if (!d) continue;
-        const uint32_t mag = d < 0 ? -d : d;
-        const uint32_t pos = atomicAdd(&run[mag], 1u);
-        const uint32_t e = (((uint32_t)w << log_bs) | k) | ((uint32_t)(d < 0) << FB_SIGN_SHIFT) | (mag << FB_MAG_SHIFT);
-        out.ent[0][(size_t)b * cap + pos] = e;
+        pos[w] = d ? atomicAdd(&run[d < 0 ? -d : d], 1u) : FB_NO_SLOT;  // < count <= cap
+    }
+    const uint32_t count = b0[FB_BUCKETS + 1];
+    for (uint32_t lo = 0; lo < count; lo += FB_STAGE) {  // block-uniform
+        __syncthreads();  // the previous window is copied out (first pass: every slot taken)
+#pragma unroll
+        for (int w = 0; w < FB_W; w++) {
+            const uint32_t q = pos[w] - lo;  // wraps below lo; FB_NO_SLOT - lo >= 2^31
+            if (q < FB_STAGE) {
+                const int d = dig[w];
+                const uint32_t mag = d < 0 ? -d : d;
+                staged[q] = (((uint32_t)w << log_bs) | k) | ((uint32_t)(d < 0) << FB_SIGN_SHIFT) |
+                            (mag << FB_MAG_SHIFT);
+            }
+        }
+        __syncthreads();
+        const uint32_t hi = min(count, lo + (uint32_t)FB_STAGE);
+        for (uint32_t i = lo + k; i < hi; i += T) out.ent[0][(size_t)b * cap + i] = staged[i - lo];
         for (int gl = 1; gl <= gl_used; gl++) {
             const uint32_t S = b >> gl, g = b & ((1u << gl) - 1);
-            // the index field absorbs g * cap: j * bs + k + g * cap < 2^gl * cap < 2^18
-            out.ent[gl][((size_t)S << gl) * cap + base[gl - 1][mag] + pos] = e + g * (uint32_t)cap;
+            uint32_t* dst = out.ent[gl] + ((size_t)S << gl) * cap;
+            const uint32_t* bs_gl = base[gl - 1];
+            for (uint32_t i = lo + k; i < hi; i += T) {
+                const uint32_t e = staged[i - lo];
+                // the index field absorbs g * cap: j * bs + k + g * cap < 2^gl * cap < 2^18
+                dst[bs_gl[e >> FB_MAG_SHIFT] + i] = e + g * (uint32_t)cap;
+            }
         }
     }
 }
