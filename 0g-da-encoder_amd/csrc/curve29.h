@@ -186,13 +186,30 @@ __device__ __forceinline__ x29 x29_from_xyzz(const g1_xyzz& p) {
     return r;
 }
 
-__device__ __forceinline__ g1_xyzz x29_to_xyzz(const x29& p) {
-    g1_xyzz r;
-    r.X = q_to_fq(p.X);
-    r.Y = q_to_fq(p.Y);
-    r.ZZ = q_to_fq(p.ZZ);
-    r.ZZZ = q_to_fq(p.ZZZ);
-    return r;
+// Affine coordinates of p in the 29-bit form, from one inversion of ZZ ZZZ = z^5
+// (x = X ZZZ / z^5, y = Y ZZ / z^5); *inf reports infinity (the coordinates are then 0).
+__device__ __forceinline__ void x29_to_aff(const x29& p, fq29& x, fq29& y, bool* inf) {
+    *inf = x29_is_inf(p);
+    const fq29 I = q_inv(q_mul(p.ZZ, p.ZZZ));
+    x = q_mul(p.X, q_mul(I, p.ZZZ));
+    y = q_mul(p.Y, q_mul(I, p.ZZ));
+}
+
+// canonical output point: x || y as 8 little-endian u32 each, all zeros for infinity
+__device__ __forceinline__ void x29_store_canon(uint32_t* out, const x29& p) {
+    bool inf;
+    fq29 x, y;
+    x29_to_aff(p, x, y, &inf);
+    fq29 one = q_zero();
+    one.l[0] = 1;
+    uint32_t wx[8], wy[8];
+    q_pack(q_canon(q_mul(x, one)), wx);  // x 2^261 / 2^261, reduced to [0, p)
+    q_pack(q_canon(q_mul(y, one)), wy);
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+        out[i] = inf ? 0u : wx[i];
+        out[8 + i] = inf ? 0u : wy[i];
+    }
 }
 
 // k * p for k < 2^bits, uniform per lane

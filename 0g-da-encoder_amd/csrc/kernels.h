@@ -434,7 +434,7 @@ __global__ __launch_bounds__(64) void k_msm_horner(const uint32_t* heap, int h, 
         node = 1;
         line = h + 1;
     }
-    out_store_canon(out + (size_t)slot * 16, x29_to_xyzz(horner_windows(heap, nb, nw, line, node)));
+    x29_store_canon(out + (size_t)slot * 16, horner_windows(heap, nb, nw, line, node));
 }
 
 // multiple_multiexp output: out[L * nb + b] = sum_w 2^(8w) heap leaf (L, w, nb + b)
@@ -444,13 +444,13 @@ __global__ __launch_bounds__(64) void k_msm_leaves_out(const uint32_t* heap, int
     uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nlines * nb) return;
     uint32_t L = t >> h, b = t & (nb - 1);
-    out_store_canon(out + (size_t)t * 16, x29_to_xyzz(horner_windows(heap, nb, nw, L, nb + b)));
+    x29_store_canon(out + (size_t)t * 16, horner_windows(heap, nb, nw, L, nb + b));
 }
 
 // single MSM result: Horner over the windows of tree node 1 of line 0
 __global__ __launch_bounds__(64) void k_msm_root_out(const uint32_t* heap, int h, uint32_t* out) {
     if (threadIdx.x || blockIdx.x) return;
-    out_store_canon(out, x29_to_xyzz(horner_windows(heap, 1u << h, MSM_W, 0, 1)));
+    x29_store_canon(out, horner_windows(heap, 1u << h, MSM_W, 0, 1));
 }
 
 // ---------------------------------------------------------------------------
@@ -975,7 +975,7 @@ __global__ __launch_bounds__(EC_THREADS) void k_ec_store(const uint32_t* pts, co
         for (int i = 0; i < scale_shift; i++) f = f_dbl(f);
         v = x29_mul_scalar(v, f_from_mont(f), tab);
     }
-    out_store_canon(out + 16 * (size_t)p, x29_to_xyzz(v));
+    x29_store_canon(out + 16 * (size_t)p, v);
 }
 
 // ---------------------------------------------------------------------------

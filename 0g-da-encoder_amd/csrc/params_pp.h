@@ -54,7 +54,11 @@ __global__ __launch_bounds__(EC_THREADS) void k_pp_store(const uint32_t* pts, co
     x29 v = x29_load(pts + X29_WORDS * (size_t)p);
     if (scale) v = x29_mul_scalar(v, f_from_mont(f_load<FrP>(consts + 8 * K_NINV)), tab);
     bool inf;
-    const g1_aff a = xyzz_to_aff(x29_to_xyzz(v), &inf);
+    fq29 x, y;
+    x29_to_aff(v, x, y, &inf);
+    g1_aff a;
+    a.x = q_to_fq(x);
+    a.y = q_to_fq(y);
     uint32_t* o = line + 16 * (size_t)bitrev_bits(p, log_n);
 #pragma unroll
     for (int k = 0; k < 8; k++) {
