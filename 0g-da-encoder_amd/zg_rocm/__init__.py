@@ -117,10 +117,37 @@ def hip():
         H.hipFree.argtypes = [ctypes.c_void_p]
         H.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
         H.hipDeviceSynchronize.argtypes = []
-        for f in (H.hipMalloc, H.hipFree, H.hipMemcpy, H.hipDeviceSynchronize):
+        H.hipHostMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t, ctypes.c_uint]
+        H.hipHostFree.argtypes = [ctypes.c_void_p]
+        for f in (H.hipMalloc, H.hipFree, H.hipMemcpy, H.hipDeviceSynchronize, H.hipHostMalloc, H.hipHostFree):
             f.restype = ctypes.c_int
         _hip = H
     return _hip
+
+
+class PinnedHostArray:
+    """A page-locked host byte array (hipHostMalloc) as a numpy uint8 array (`.array`),
+    so that rows in and raw data out cross PCIe at the link's rate."""
+
+    def __init__(self, size):
+        self.size = int(size)
+        self.ptr = ctypes.c_void_p()
+        rc = hip().hipHostMalloc(ctypes.byref(self.ptr), max(self.size, 1), 0)
+        if rc:
+            raise ZgrError(ERR_OOM, "hipHostMalloc(%d) failed: %d" % (self.size, rc))
+        self.array = np.ctypeslib.as_array((ctypes.c_uint8 * max(self.size, 1)).from_address(self.ptr.value))[:self.size]
+
+    def free(self):
+        if self.ptr.value:
+            self.array = None
+            hip().hipHostFree(self.ptr)
+            self.ptr = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
 
 
 class DeviceBuffer:
@@ -350,22 +377,42 @@ class Context:
     def recover(self, row_ids, rows):
         """zgr_recover: row_ids (strictly increasing) and their rows, one bytes-like of
         32 * 2^log_col bytes each (or one concatenated buffer) -> the raw data."""
-        ids = np.ascontiguousarray(row_ids, dtype=np.uint32).reshape(-1)
         buf = rows if isinstance(rows, (bytes, bytearray, memoryview)) else b"".join(bytes(r) for r in rows)
-        arr = np.frombuffer(bytes(buf), dtype=np.uint8) if len(buf) else np.zeros(1, dtype=np.uint8)
-        cap = self.geo.max_raw_data_size
-        out = np.zeros(cap, dtype=np.uint8)
+        arr = np.frombuffer(buf, dtype=np.uint8) if len(buf) else np.zeros(1, dtype=np.uint8)
+        out = np.empty(self.geo.max_raw_data_size, dtype=np.uint8)
+        n = self.recover_into(row_ids, arr, out)
+        return out[:n].tobytes()
+
+    def recover_into(self, row_ids, rows, out):
+        """zgr_recover over caller-owned host arrays: rows a contiguous uint8 array of the
+        rows in id order, out a writable uint8 array of at least MAX_RAW_DATA_SIZE bytes;
+        returns the raw data's length (its bytes are out[:length])."""
+        ids = np.ascontiguousarray(row_ids, dtype=np.uint32).reshape(-1)
+        if rows.dtype != np.uint8 or not rows.flags.c_contiguous:
+            raise ValueError("rows must be a contiguous uint8 array")
+        if out.dtype != np.uint8 or not out.flags.c_contiguous or not out.flags.writeable:
+            raise ValueError("out must be a writable contiguous uint8 array")
+        if rows.size < ids.size * 32 * self.geo.col_n:
+            raise ValueError("rows holds %d bytes, %d rows need %d" % (rows.size, ids.size, ids.size * 32 * self.geo.col_n))
         n = ctypes.c_size_t(0)
-        _check(lib().zgr_recover(self.h, _p(ids if ids.size else np.zeros(1, dtype=np.uint32)), _p(arr), ids.size,
-                                 _p(out), cap, ctypes.byref(n)))
-        return out[:n.value].tobytes()
+        _check(lib().zgr_recover(self.h, _p(ids if ids.size else np.zeros(1, dtype=np.uint32)),
+                                 _p(rows if rows.size else np.zeros(1, dtype=np.uint8)), ids.size, _p(out), out.size,
+                                 ctypes.byref(n)))
+        return n.value
 
     def verify_msm(self, row_ids, rows, alphas):
         """zgr_verify_msm: row_ids (strictly increasing), their rows (canonical bytes, one
         buffer or a list) and one canonical alpha (int) per row -> 64 B canonical point."""
         ids = np.ascontiguousarray(row_ids, dtype=np.uint32).reshape(-1)
-        buf = rows if isinstance(rows, (bytes, bytearray, memoryview)) else b"".join(bytes(r) for r in rows)
-        arr = np.frombuffer(bytes(buf), dtype=np.uint8) if len(buf) else np.zeros(1, dtype=np.uint8)
+        if isinstance(rows, np.ndarray):
+            arr = np.ascontiguousarray(rows, dtype=np.uint8).reshape(-1)
+        else:
+            buf = rows if isinstance(rows, (bytes, bytearray, memoryview)) else b"".join(bytes(r) for r in rows)
+            arr = np.frombuffer(buf, dtype=np.uint8)
+        if arr.size < ids.size * 32 * self.geo.col_n:
+            raise ValueError("rows holds %d bytes, %d rows need %d" % (arr.size, ids.size, ids.size * 32 * self.geo.col_n))
+        if not arr.size:
+            arr = np.zeros(1, dtype=np.uint8)
         al = np.frombuffer(b"".join(int(a).to_bytes(32, "little") for a in alphas), dtype=np.uint8) \
             if len(alphas) else np.zeros(1, dtype=np.uint8)
         out = np.zeros(64, dtype=np.uint8)

@@ -136,3 +136,25 @@ def test_malformed_raw_data(ctx):
 def test_abi_argument_errors(ctx):
     with pytest.raises(Z.ZgrError):
         ctx.recover([3, 2], bytes(2 * 32 * GEO.col_n))  # not increasing
+
+
+def test_recover_into_host_arrays(ctx, blobs):
+    """recover_into over caller-owned arrays, pageable and page-locked (hipHostMalloc):
+    the same bytes as the oracle; undersized arrays are refused before the call."""
+    raw, rows = blobs[3]
+    ids = list(range(R + 2, 2 * R + 2))
+    buf = b"".join(rows[i] for i in ids)
+    pin_rows, pin_out = Z.PinnedHostArray(len(buf)), Z.PinnedHostArray(GEO.max_raw_data_size)
+    try:
+        pin_rows.array[:] = np.frombuffer(buf, dtype=np.uint8)
+        for ra, oa in ((np.frombuffer(buf, dtype=np.uint8), np.empty(GEO.max_raw_data_size, dtype=np.uint8)),
+                       (pin_rows.array, pin_out.array)):
+            n = ctx.recover_into(ids, ra, oa)
+            assert oa[:n].tobytes() == raw
+    finally:
+        pin_rows.free()
+        pin_out.free()
+    with pytest.raises(ValueError):
+        ctx.recover_into(ids, np.frombuffer(buf[:-1], dtype=np.uint8), np.empty(GEO.max_raw_data_size, np.uint8))
+    with pytest.raises(Z.ZgrError):  # raw_out below MAX_RAW_DATA_SIZE: the C ABI refuses it
+        ctx.recover_into(ids, np.frombuffer(buf, dtype=np.uint8), np.empty(GEO.max_raw_data_size - 1, np.uint8))
