@@ -447,10 +447,10 @@ __global__ __launch_bounds__(64) void k_msm_leaves_out(const uint32_t* heap, int
     x29_store_canon(out + (size_t)t * 16, horner_windows(heap, nb, nw, L, nb + b));
 }
 
-// single MSM result: Horner over the windows of tree node 1 of line 0
-__global__ __launch_bounds__(64) void k_msm_root_out(const uint32_t* heap, int h, uint32_t* out) {
+// single MSM result: Horner over the nw windows of tree node 1 of line 0
+__global__ __launch_bounds__(64) void k_msm_root_out(const uint32_t* heap, int h, int nw, uint32_t* out) {
     if (threadIdx.x || blockIdx.x) return;
-    x29_store_canon(out, horner_windows(heap, 1u << h, MSM_W, 0, 1));
+    x29_store_canon(out, horner_windows(heap, 1u << h, nw, 0, 1));
 }
 
 // ---------------------------------------------------------------------------

@@ -1231,7 +1231,36 @@ int msm_windowed(zgr_ctx* c, Lane& ln, const uint32_t* dbases, int log_np, uint3
         LAUNCH(c, ln.st, "msm_tree", k_msm_tree_level, blocks(total, 64), 64, 0, ln.heap, h, d, total, MSM_W,
                (uint64_t)0);
     }
-    LAUNCH(c, ln.st, "msm_root", k_msm_root_out, 1, 64, 0, ln.heap, h, out);
+    LAUNCH(c, ln.st, "msm_root", k_msm_root_out, 1, 64, 0, ln.heap, h, MSM_W, out);
+    return ZGR_OK;
+}
+
+// sum_i s_i B_i over line L of a coset's resident bases, through that line of the
+// shifted-copy tables: scalars in ln.y (Montgomery, in order), one bucket set per
+// chunk (one wave each), the chunk sums summed by the tree kernel, canonical affine
+// result to out.  Uses the lane's sort, bucket and heap buffers.
+int msm_fb_line(zgr_ctx* c, Lane& ln, int coset, int L, uint32_t* out) {
+    const int log_bs = c->log_n - c->h;
+    FbLevels lv{};
+    FbOut fo{};
+    lv.ent[0] = fo.ent[0] = ln.entries;
+    lv.cnt[0] = ln.counts;
+    lv.bst[0] = ln.bstart;
+    LAUNCH(c, ln.st, "msm_sort", k_fb_count, (uint32_t)c->nb, (uint32_t)c->bs, 0, ln.y, c->log_n, log_bs, 0, ln.counts,
+           ln.bstart);
+    LAUNCH(c, ln.st, "msm_scatter", k_fb_scatter, (uint32_t)c->nb, (uint32_t)c->bs, 0, ln.y, c->log_n, log_bs, 0,
+           ln.bstart, 0, fo);
+    const size_t cap = (size_t)FB_W << log_bs;
+    const uint32_t* table = c->fbtab[coset] + ((size_t)L << c->h) * cap * 16;  // line L's chunks
+    uint32_t* heap = ln.heap + (size_t)L * 2 * c->nb * X29_WORDS;               // line L's tree
+    LAUNCH(c, ln.st, "msm_accum", k_fb_sums, (uint32_t)c->nb, FB_LANES, 0, table, lv, (const uint32_t*)nullptr,
+           ln.bkt, c->log_n, c->h, 1);
+    LAUNCH(c, ln.st, "msm_reduce", k_fb_reduce, (uint32_t)c->nb, FB_LANES, 0, lv, (const uint32_t*)nullptr, ln.bkt,
+           heap, c->h, 1);
+    for (int d = c->h - 1; d >= 0; d--)
+        LAUNCH(c, ln.st, "msm_tree", k_msm_tree_level, blocks((size_t)1 << d, 64), 64, 0, heap, c->h, d,
+               (uint32_t)1 << d, 1, (uint64_t)0);
+    LAUNCH(c, ln.st, "msm_root", k_msm_root_out, 1, 64, 0, heap, c->h, 1, out);
     return ZGR_OK;
 }
 
@@ -1366,7 +1395,10 @@ int zgr_verify_msm(zgr_ctx* c, const uint32_t* row_ids, const uint8_t* rows, con
     uint8_t *d_rows = nullptr, *d_alpha = nullptr;
     uint32_t *d_ids = nullptr, *d_ks = nullptr, *d_pts = nullptr;
     int* flag = nullptr;
+    hipEvent_t ev_up = nullptr, ev_done = nullptr;
     auto body = [&]() -> int {
+        HIPCHK(hipEventCreateWithFlags(&ev_up, hipEventDisableTiming));
+        HIPCHK(hipEventCreateWithFlags(&ev_done, hipEventDisableTiming));
         CHECK(dmalloc((void**)&d_rows, n_rows * COL * 32 + 32));
         CHECK(dmalloc((void**)&d_alpha, n_rows * 32 + 32));
         CHECK(dmalloc((void**)&d_ids, n_rows * 4 + 4));
@@ -1384,16 +1416,34 @@ int zgr_verify_msm(zgr_ctx* c, const uint32_t* row_ids, const uint8_t* rows, con
         std::vector<uint32_t> ks_all;
         for (int cs = 0; cs < c->coset_n; cs++) ks_all.insert(ks_all.end(), per[cs].begin(), per[cs].end());
         if (n_rows) HIPCHK(hipMemcpyAsync(d_ks, ks_all.data(), n_rows * 4, hipMemcpyHostToDevice, st));
+        // with the shifted-copy tables the cosets alternate between the two lanes (their
+        // streams wait for the uploads on lane 0's, and lane 0's for lane 1's results)
+        const bool fb = c->fb;
+        HIPCHK(hipEventRecord(ev_up, st));
+        bool used1 = false;
         for (int cs = 0; cs < c->coset_n; cs++) {
             const uint32_t nk = (uint32_t)per[cs].size();
             if (!nk) continue;
-            HIPCHK(hipMemsetAsync(ln.y, 0, c->n * 32, st));
-            LAUNCH(c, st, "verify", k_ver_scalars, blocks((size_t)nk * COL, 256), 256, 0, d_rows, d_alpha, d_ks + off,
-                   nk, d_ids, c->log_col, c->log_row, ln.y, flag);
-            const uint32_t* basis = c->bases[cs] + (size_t)c->h * c->n * 16;  // line h: the basis
-            CHECK(msm_windowed(c, ln, basis, c->log_n, d_pts + 16 * npts));
+            Lane& lc = fb ? c->lanes[npts & 1] : ln;
+            if (&lc != &ln && !used1) {
+                HIPCHK(hipStreamWaitEvent(lc.st, ev_up, 0));
+                used1 = true;
+            }
+            HIPCHK(hipMemsetAsync(lc.y, 0, c->n * 32, lc.st));
+            LAUNCH(c, lc.st, "verify", k_ver_scalars, blocks((size_t)nk * COL, 256), 256, 0, d_rows, d_alpha,
+                   d_ks + off, nk, d_ids, c->log_col, c->log_row, lc.y, flag);
+            if (fb) {
+                CHECK(msm_fb_line(c, lc, cs, c->h, d_pts + 16 * npts));  // line h: the basis
+            } else {
+                const uint32_t* basis = c->bases[cs] + (size_t)c->h * c->n * 16;
+                CHECK(msm_windowed(c, lc, basis, c->log_n, d_pts + 16 * npts));
+            }
             npts++;
             off += nk;
+        }
+        if (used1) {
+            HIPCHK(hipEventRecord(ev_done, c->lanes[1].st));
+            HIPCHK(hipStreamWaitEvent(st, ev_done, 0));
         }
         LAUNCH(c, st, "verify", k_ver_sum, 1, 64, 0, d_pts, npts, d_pts + 16 * c->coset_n);
         int hflag = 0;
@@ -1404,9 +1454,11 @@ int zgr_verify_msm(zgr_ctx* c, const uint32_t* row_ids, const uint8_t* rows, con
         return ZGR_OK;
     };
     int rc = body();
-    (void)hipStreamSynchronize(st);
+    (void)sync_all(c);
     for (void* p : {(void*)d_rows, (void*)d_alpha, (void*)d_ids, (void*)d_ks, (void*)d_pts, (void*)flag})
         if (p) (void)hipFree(p);
+    for (hipEvent_t e : {ev_up, ev_done})
+        if (e) (void)hipEventDestroy(e);
     return rc;
 }
 
