@@ -905,12 +905,103 @@ __global__ void k_ec_load_bitrev(const uint32_t* in, uint32_t* pts, int log_n) {
     x29_store(pts + X29_WORDS * (size_t)p, ec29_from_canon(in + 16 * (size_t)bitrev_bits(p, log_n)));
 }
 
-// k * P for a canonical scalar k < 2^254: 2-bit windows over the multiples P, 2P, 3P
-// (127 x (2 doublings + 1 addition), the addition selected away on zero digits).  The
-// multiples sit in LDS, thread-interleaved: a block of EC_THREADS threads needs
-// EC_LDS_WORDS u32.
+// k * P for a canonical scalar k < r by the GLV split of the BN254 endomorphism
+// phi(x, y) = (beta x, y) = lambda (x, y): k = k1 + lambda k2 (mod r) with |k1|, |k2| <
+// 2^127, and k P = k1 P + k2 phi(P).  Both halves share 2-bit windows over the multiples
+// P, 2P, 3P (phi of a multiple is one multiplication of its X): 64 x (2 doublings + 2
+// additions), the additions selected away on zero digits, where the plain 254-bit scalar
+// needs 127 x (2 doublings + 1 addition).  The multiples sit in LDS, thread-interleaved:
+// a block of EC_THREADS threads needs EC_LDS_WORDS u32.
 constexpr int EC_THREADS = 64;
 constexpr int EC_LDS_WORDS = 3 * X29_WORDS * EC_THREADS;
+
+struct Glv {
+    // lambda = 0xb3c4d79d41a917585bfc41088d8daaa78b17ea66b99c90dd (a cube root of unity mod r),
+    // beta = 0x59e26bcea0d48bacd4f263f1acdb5c4f5763473177fffffe (mod p), in the 29-bit form
+    static constexpr uint32_t BETA29[9] = {0x0a337995u, 0x158d1d23u, 0x189c9b98u, 0x12fa4e45u, 0x185faadcu,
+                                           0x0176f16du, 0x0eed93bau, 0x14291140u, 0x000c0afeu};
+    // a reduced basis of {(x, y) : x + lambda y = 0 mod r}: (a1, b1), (a2, b2) with b1 < 0,
+    // determinant r; g1 = round(2^256 b2 / r), g2 = round(2^256 (-b1) / r)
+    static constexpr uint32_t G1[3] = {0xc7e0b3d7u, 0xd91d232eu, 0x00000002u};
+    static constexpr uint32_t G2[5] = {0x391eb18eu, 0x7a7bd9d4u, 0xa773d2cfu, 0x4ccef014u, 0x00000002u};
+    static constexpr uint32_t A1[2] = {0x94d213e3u, 0x89d32568u};
+    static constexpr uint32_t A2[4] = {0x1221250bu, 0x0be4e154u, 0xeeb859fdu, 0x6f4d8248u};
+    static constexpr uint32_t NB1[4] = {0x7d4f1128u, 0x8211bbebu, 0xeeb859fcu, 0x6f4d8248u};  // -b1
+    static constexpr uint32_t B2[2] = {0x94d213e3u, 0x89d32568u};
+};
+
+// out[0 .. NA + NB) = a * b (schoolbook with a carry per row)
+template <int NA, int NB>
+__device__ __forceinline__ void u32_mul(const uint32_t* a, const uint32_t* b, uint32_t* out) {
+#pragma unroll
+    for (int i = 0; i < NA + NB; i++) out[i] = 0;
+#pragma unroll
+    for (int i = 0; i < NA; i++) {
+        uint64_t c = 0;
+#pragma unroll
+        for (int j = 0; j < NB; j++) {
+            const uint64_t t = (uint64_t)a[i] * b[j] + out[i + j] + c;
+            out[i + j] = (uint32_t)t;
+            c = t >> 32;
+        }
+        out[i + NB] = (uint32_t)c;
+    }
+}
+
+// c = (k g + 2^255) >> 256 for the 8-limb k and an NG-limb g; the low NC limbs of c
+template <int NG, int NC>
+__device__ __forceinline__ void glv_round(const uint32_t* k, const uint32_t* g, uint32_t* c) {
+    uint32_t t[8 + NG];
+    u32_mul<8, NG>(k, g, t);
+    uint64_t x = (uint64_t)t[7] + 0x80000000u;  // + 2^255
+    uint32_t carry = (uint32_t)(x >> 32);
+#pragma unroll
+    for (int i = 0; i < NC; i++) {
+        const uint64_t y = (uint64_t)t[8 + i] + carry;
+        c[i] = (uint32_t)y;
+        carry = (uint32_t)(y >> 32);
+    }
+}
+
+// (x mod 2^256, a two's-complement value of magnitude < 2^127) -> magnitude, sign
+__device__ __forceinline__ bool glv_abs(const uint32_t* x, uint32_t* m) {
+    const bool neg = x[7] >> 31;
+    uint32_t c = neg ? 1u : 0u;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        const uint64_t y = (uint64_t)(neg ? ~x[i] : x[i]) + c;
+        m[i] = (uint32_t)y;
+        c = (uint32_t)(y >> 32);
+    }
+    return neg;
+}
+
+// k (canonical, < r) -> |k1|, |k2| (< 2^127, 4 limbs each) and their signs, with
+// k1 = k - c1 a1 - c2 a2 and k2 = c1 (-b1) - c2 b2 for c1 = round(k b2 / r),
+// c2 = round(k (-b1) / r) (c1 < 2^64, c2 < 2^127)
+__device__ __forceinline__ void glv_split(const fr& k, uint32_t* m1, bool* n1, uint32_t* m2, bool* n2) {
+    uint32_t c1[2], c2[4];
+    glv_round<3, 2>(k.v, Glv::G1, c1);
+    glv_round<5, 4>(k.v, Glv::G2, c2);
+    uint32_t p1[4], p2[8], q1[6], q2[6];
+    u32_mul<2, 2>(c1, Glv::A1, p1);
+    u32_mul<4, 4>(c2, Glv::A2, p2);
+    u32_mul<2, 4>(c1, Glv::NB1, q1);
+    u32_mul<4, 2>(c2, Glv::B2, q2);
+    uint32_t x[8], y[8];
+    int64_t bx = 0, by = 0;
+#pragma unroll
+    for (int i = 0; i < 8; i++) {  // x = k - p1 - p2, y = q1 - q2 (mod 2^256)
+        const int64_t tx = (int64_t)k.v[i] - (i < 4 ? p1[i] : 0u) - p2[i] + bx;
+        x[i] = (uint32_t)tx;
+        bx = tx >> 32;
+        const int64_t ty = (int64_t)(i < 6 ? q1[i] : 0u) - (i < 6 ? q2[i] : 0u) + by;
+        y[i] = (uint32_t)ty;
+        by = ty >> 32;
+    }
+    *n1 = glv_abs(x, m1);
+    *n2 = glv_abs(y, m2);
+}
 
 __device__ __forceinline__ void ec_tab_store(uint32_t* tab, int k, const x29& v) {
     uint32_t w[X29_WORDS];
@@ -927,21 +1018,33 @@ __device__ __forceinline__ x29 ec_tab_load(const uint32_t* tab, int k) {
 }
 
 __device__ __forceinline__ x29 x29_mul_scalar(const x29& p, const fr& k, uint32_t* tab) {
+    uint32_t m1[4], m2[4];
+    bool n1, n2;
+    glv_split(k, m1, &n1, m2, &n2);
     ec_tab_store(tab, 0, p);
     const x29 p2 = x29_dbl(p);
     ec_tab_store(tab, 1, p2);
     ec_tab_store(tab, 2, x29_add(p2, p));
+    // r = |k1| P + (+-)|k2| phi(P), the second half negated when the signs differ; the
+    // sign of k1 is applied at the end
+    const bool flip = n1 != n2;
+    const fq29 beta = q_const(Glv::BETA29);
     x29 r = x29_inf();
 #pragma unroll
-    for (int w = 7; w >= 0; w--) {  // scalar words by constant index: no private array
-        const uint32_t kw = k.v[w];
-        for (int j = w == 7 ? 14 : 15; j >= 0; j--) {  // bits 2j, 2j + 1 of word w (k < 2^254)
+    for (int w = 3; w >= 0; w--) {  // magnitude words by constant index: no private array
+        const uint32_t a = m1[w], b = m2[w];
+        for (int j = 15; j >= 0; j--) {  // bits 2j, 2j + 1 of word w
             r = x29_dbl(x29_dbl(r));
-            const uint32_t d = (kw >> (2 * j)) & 3;
-            const x29 t = ec_tab_load(tab, d ? (int)d - 1 : 0);
-            r = x29_select(d != 0, x29_add(r, t), r);
+            const uint32_t d1 = (a >> (2 * j)) & 3, d2 = (b >> (2 * j)) & 3;
+            const x29 t1 = ec_tab_load(tab, d1 ? (int)d1 - 1 : 0);
+            r = x29_select(d1 != 0, x29_add(r, t1), r);
+            x29 t2 = ec_tab_load(tab, d2 ? (int)d2 - 1 : 0);
+            t2.X = q_mul(t2.X, beta);
+            if (flip) t2.Y = q_sub(q_zero(), t2.Y);
+            r = x29_select(d2 != 0, x29_add(r, t2), r);
         }
     }
+    if (n1) r.Y = q_sub(q_zero(), r.Y);
     return r;
 }
 
