@@ -249,7 +249,7 @@ def main():
     ecfft = {"log_n": geo.log_n, "wall_ms": round(ec_wall * 1e3, 1), "kernel_ms": round(ec_ms * ec_cnt, 1),
              "butterflies_per_s": round(butterflies / (ec_ms * ec_cnt * 1e-3)) if ec_cnt else None,
              "note": "one forward DFT of 2^20 G1 points (Pairing::fast_fft); 20 x 2^19 butterflies, "
-                     "each a 254-bit scalar multiplication"}
+                     "each a scalar multiplication by a GLV-split twiddle"}
 
     # batched path: args.batch full-size blobs per call (two blobs in flight)
     nb_ = max(2, args.batch)
