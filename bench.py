@@ -34,13 +34,13 @@ MAC32_PER_PAIR_WINDOW = 10 * 136
 # microbenchmark of v_mad_u64_u32 alone reaches 25.0 T/s on the GPU.)
 PEAK_MAC32 = 256 * 4 * 16 * 2.4e9
 # HBM bytes per launch pair (k_fb_sums + k_fb_reduce) from two rocprofv3 counter passes
-# over `bench.py --roofline-only`: FETCH_SIZE (21,955,494 + 1,218,484) KiB x 2.0 (a 1 GiB
-# device copy reads as 524,321 KiB) + WRITE_SIZE (1,739,829 + 770) KiB x 1.0 (the copy
+# over `bench.py --roofline-only`: FETCH_SIZE (22,002,553 + 1,230,987) KiB x 2.0 (a 1 GiB
+# device copy reads as 524,322 KiB) + WRITE_SIZE (1,739,867 + 769) KiB x 1.0 (the copy
 # writes as 1,048,576 KiB).
-MSM_TRAFFIC_BYTES = ((21955494 + 1218484) * 2.0 + 1739829 + 770) * 1024
+MSM_TRAFFIC_BYTES = ((22002553 + 1230987) * 2.0 + 1739867 + 769) * 1024
 # VALU lane-instructions per launch pair: rocprofv3 SQ_INSTS_VALU (wave instructions)
 # of k_fb_sums + k_fb_reduce over the same command, x 64 lanes.
-VALU_LANE_OPS = (10150435430 + 1497634564) * 64.0
+VALU_LANE_OPS = (10150435430 + 1430224644) * 64.0
 
 
 def log(*a):
