@@ -344,6 +344,283 @@ __global__ __launch_bounds__(256) void k_rec_gemm(const uint32_t* L, const uint3
     if (i < col && t < nt) f_store(Out + 8 * ((size_t)t * col + i), o);
 }
 
+// ---------------------------------------------------------------------------
+// The same GEMM on the matrix cores (v_mfma_i32_32x32x32_i8).  Every element, a
+// Montgomery residue below r < 2^254, is split into 37 limbs of 7 bits (0..127, valid
+// signed int8), so Out[t][i] * 2^256 = sum_s D_s[t][i] 2^(7 s) with, for the 73 limb-
+// weight diagonals s = a + b,
+//     D_s[t][i] = sum_{a + b = s} sum_r Lp_a[t][r] Vp_b[r][i]    (< 37 * 2^10 * 127^2 < 2^30)
+// a sum of int8 matrix products.  k_rec_gemm_mfma computes the D_s of a 32 x 32 output
+// tile, k_rec_fold folds them: sum_s D_s (2^(7 s + 5) mod r) in 29-bit columns, one
+// Montgomery reduction by 2^261 (= X 2^-256 mod r, the Montgomery form of the product
+// sum), one conditional subtraction of r.
+// MFMA operand layout (32x32x32, int8): lane l holds A[l % 32][16 (l / 32) .. + 16) and
+// B[16 (l / 32) .. + 16)[l % 32] as 16 packed bytes; accumulator register q of lane l is
+// C[8 (q / 4) + 4 (l / 32) + q % 4][l % 32].
+constexpr int RM_LIMBS = 37;
+constexpr int RM_DIAG = 2 * RM_LIMBS - 1;
+constexpr int RM_WAVES = 16;                              // k_rec_gemm_mfma: 1024 threads
+constexpr int RM_ACC = (RM_DIAG + RM_WAVES - 1) / RM_WAVES;  // diagonals per wave (s = w + 16 q)
+constexpr uint32_t RM_MAX_M = 2048;                       // keeps D_s below 2^31
+
+typedef int rm_v16i __attribute__((ext_vector_type(16)));
+typedef int rm_v4i __attribute__((ext_vector_type(4)));
+
+// 2^(7 s + 5) mod r, 29-bit limbs
+__constant__ uint32_t RM_KS[RM_DIAG][9] = {
+    {0x00000020u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u},
+    {0x00001000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u},
+    {0x00080000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u},
+    {0x04000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u},
+    {0x00000000u, 0x00000010u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u},
+    {0x00000000u, 0x00000800u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u},
+    {0x00000000u, 0x00040000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u},
+    {0x00000000u, 0x02000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u},
+    {0x00000000u, 0x00000000u, 0x00000008u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u},
+    {0x00000000u, 0x00000000u, 0x00000400u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u},
+    {0x00000000u, 0x00000000u, 0x00020000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u},
+    {0x00000000u, 0x00000000u, 0x01000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u},
+    {0x00000000u, 0x00000000u, 0x00000000u, 0x00000004u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u},
+    {0x00000000u, 0x00000000u, 0x00000000u, 0x00000200u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u},
+    {0x00000000u, 0x00000000u, 0x00000000u, 0x00010000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u},
+    {0x00000000u, 0x00000000u, 0x00000000u, 0x00800000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u},
+    {0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000002u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u},
+    {0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000100u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u},
+    {0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00008000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u},
+    {0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00400000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u},
+    {0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000001u, 0x00000000u, 0x00000000u, 0x00000000u},
+    {0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000080u, 0x00000000u, 0x00000000u, 0x00000000u},
+    {0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00004000u, 0x00000000u, 0x00000000u, 0x00000000u},
+    {0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00200000u, 0x00000000u, 0x00000000u, 0x00000000u},
+    {0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x10000000u, 0x00000000u, 0x00000000u, 0x00000000u},
+    {0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000040u, 0x00000000u, 0x00000000u},
+    {0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00002000u, 0x00000000u, 0x00000000u},
+    {0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00100000u, 0x00000000u, 0x00000000u},
+    {0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x08000000u, 0x00000000u, 0x00000000u},
+    {0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000020u, 0x00000000u},
+    {0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00001000u, 0x00000000u},
+    {0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00080000u, 0x00000000u},
+    {0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x04000000u, 0x00000000u},
+    {0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000010u},
+    {0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000800u},
+    {0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00040000u},
+    {0x1ffffff6u, 0x096341c4u, 0x106694d6u, 0x11da567du, 0x08c5c6dfu, 0x036f7879u, 0x197b3375u, 0x106681f7u, 0x001c14efu},
+    {0x1ffffab6u, 0x1718fc64u, 0x0ca7ebbdu, 0x0ae15862u, 0x0a3495d7u, 0x045b84f1u, 0x0d5c9dbdu, 0x0c9df0dcu, 0x000d7914u},
+    {0x0ffd5addu, 0x0d5998b1u, 0x1d5ce7cau, 0x1f285fe9u, 0x08ff23b9u, 0x09c89e51u, 0x177e12abu, 0x185f3518u, 0x001ed378u},
+    {0x0ead6e2fu, 0x18d6ba48u, 0x034c6796u, 0x1b3217eeu, 0x103a4175u, 0x1cefabadu, 0x0a3c100eu, 0x047275e8u, 0x0019ff8fu},
+    {0x16b7173cu, 0x0b3349dcu, 0x15ba25e1u, 0x05a57662u, 0x0594d010u, 0x1598a393u, 0x11b43229u, 0x08c0ce57u, 0x002522bbu},
+    {0x1b8b9d9eu, 0x15a4d94bu, 0x1dcd09adu, 0x14e3b424u, 0x0d2d71d8u, 0x1462ffd4u, 0x1a36d9ebu, 0x01208bdcu, 0x000af79bu},
+    {0x15cecee3u, 0x0da617dcu, 0x0614b9aau, 0x0f39a6e3u, 0x06901375u, 0x1ea99461u, 0x08857171u, 0x0fd59a84u, 0x000070a1u},
+    {0x1767717fu, 0x13fc41b7u, 0x1c00b0e5u, 0x1502e0a4u, 0x1283e839u, 0x11eeefd9u, 0x0211d7b9u, 0x1c711beeu, 0x0007ec70u},
+    {0x13b8bf6cu, 0x10e75f67u, 0x01259c7cu, 0x0524ff6bu, 0x137faa93u, 0x1e56ddbcu, 0x1be243b1u, 0x195afaf7u, 0x002e6250u},
+    {0x1c5fb586u, 0x06376fcbu, 0x1ae4efadu, 0x191aa17fu, 0x1e0ef708u, 0x0ef2027bu, 0x019a7ff4u, 0x15934725u, 0x00215b02u},
+    {0x0fdac2a8u, 0x0e548f1du, 0x02cb5ac3u, 0x1d9eec3cu, 0x017b2746u, 0x1da2fbbdu, 0x13e28bdbu, 0x19f670a4u, 0x000b065eu},
+    {0x1d6153e3u, 0x058100adu, 0x053d448du, 0x0cd5b277u, 0x0d6aca98u, 0x1ea788b2u, 0x1e5e6996u, 0x1ac7fe6au, 0x0007d284u},
+    {0x10a9f16cu, 0x1346da7fu, 0x1f6f7042u, 0x0e8de88fu, 0x06f0d9f2u, 0x1aa34a28u, 0x022b3264u, 0x04cc3969u, 0x00216c4au},
+    {0x14f8b5a8u, 0x1609e8eeu, 0x080ba577u, 0x1742744eu, 0x126c9c1cu, 0x1646d1e0u, 0x1c3bc40au, 0x166f92a6u, 0x0013aa1bu},
+    {0x1c5ad3ccu, 0x15c566edu, 0x1b1b5b65u, 0x0adcb5acu, 0x171f4bb4u, 0x0edfc926u, 0x1bfc4420u, 0x0d1190dfu, 0x0000adeau},
+    {0x1d69e5ffu, 0x03a3ca51u, 0x1f518e86u, 0x068a4578u, 0x1a2007a8u, 0x0d09529bu, 0x1d7b2efau, 0x1a6c49bbu, 0x002690e5u},
+    {0x14f2ff1au, 0x11a66168u, 0x1010cacbu, 0x0808fb7du, 0x1cb1f3cfu, 0x014d8123u, 0x1b19bd45u, 0x1d6da53bu, 0x00007ba7u},
+    {0x097f8cffu, 0x142107b4u, 0x1a094175u, 0x1cad2dccu, 0x0374151cu, 0x03e55132u, 0x0c37c144u, 0x087677b8u, 0x000d6fa7u},
+    {0x0fc67f5du, 0x115f4057u, 0x0e07c3beu, 0x0513151fu, 0x08bec681u, 0x0eaebeb6u, 0x050fd629u, 0x04a2a314u, 0x001a1ce8u},
+    {0x133fae3bu, 0x1066a4c1u, 0x050c1572u, 0x0e537e1au, 0x12518334u, 0x1246e734u, 0x1af05e2fu, 0x127b3e0eu, 0x00036aedu},
+    {0x0fd71d77u, 0x1bc54f31u, 0x04cd7268u, 0x0369f485u, 0x070d339cu, 0x09be5383u, 0x12512c7fu, 0x1c61af97u, 0x0001f007u},
+    {0x1b8ebb7bu, 0x075939a1u, 0x1eec7edau, 0x0de76dd1u, 0x1afcb17du, 0x10e17dd8u, 0x0553d961u, 0x090b0cc5u, 0x00060e69u},
+    {0x075dbd70u, 0x1ba206f6u, 0x107d280du, 0x16add9c4u, 0x05fb9683u, 0x030ae061u, 0x1f7e9cb2u, 0x1fc3ff54u, 0x0000efbcu},
+    {0x0edeb7feu, 0x12e421deu, 0x01dbbe4cu, 0x074bc05bu, 0x12bf9cd4u, 0x1fb9af16u, 0x1e009689u, 0x05475e15u, 0x001715e2u},
+    {0x1f5bfec3u, 0x0b54cd39u, 0x01ea7f00u, 0x092da385u, 0x1eeb40d7u, 0x0e991d61u, 0x1887987cu, 0x17b9f092u, 0x00030a63u},
+    {0x0dff6178u, 0x11e93801u, 0x025e5da5u, 0x184d3aecu, 0x0971d78au, 0x15b4aaf6u, 0x1e953431u, 0x0a1717c1u, 0x00020f6bu},
+    {0x0fb0bbfbu, 0x194da19au, 0x07621d32u, 0x1f8aa148u, 0x0d4ea8d0u, 0x0c0d3762u, 0x0757b291u, 0x03bf21f6u, 0x0015c020u},
+    {0x085dfd47u, 0x1c535dbbu, 0x1e8a835du, 0x07e05de8u, 0x1c888945u, 0x03ca464eu, 0x06b12132u, 0x0d0c7988u, 0x0019ba97u},
+    {0x0efea33cu, 0x09850343u, 0x14c8096fu, 0x1cc873cfu, 0x0cb8b7c2u, 0x02e5f444u, 0x0c3cc3c5u, 0x15c29e49u, 0x0002a6ddu},
+    {0x0f519df9u, 0x0913e95fu, 0x1f7fb96fu, 0x0d85f12au, 0x05b31fdcu, 0x1efb5ceeu, 0x19d1b9c3u, 0x1cca1944u, 0x0000b0b1u},
+    {0x18cefc7fu, 0x0ae5031du, 0x11809353u, 0x1b28048au, 0x040a1bb2u, 0x1ad33656u, 0x083600bau, 0x16b07c33u, 0x0027f4a4u},
+    {0x177e3f17u, 0x1513c177u, 0x1c7ec475u, 0x1f76d16bu, 0x112a8197u, 0x1dad9c09u, 0x168df9b8u, 0x14726e4au, 0x00212e2du},
+    {0x0f1f8b29u, 0x1b8d11a9u, 0x1e11e34fu, 0x13877335u, 0x04c64163u, 0x1e4b034au, 0x0e464f57u, 0x17e629acu, 0x00250028u},
+    {0x1fc5941fu, 0x01986c4cu, 0x1807e540u, 0x0db2aeb9u, 0x1b6bee13u, 0x106e1c11u, 0x03ec524fu, 0x022a5c83u, 0x002a12a6u},
+    {0x12ca0f11u, 0x146a4d56u, 0x09fee0eau, 0x15e88351u, 0x00f0c2ceu, 0x19fcf563u, 0x0dcd7c88u, 0x1b39b101u, 0x000dd504u},
+    {0x0507885cu, 0x16f264ddu, 0x1a7b59eeu, 0x1aed466bu, 0x118fcce2u, 0x17a5966bu, 0x0f469740u, 0x17e32166u, 0x001c6764u},
+    {0x13c42db5u, 0x1f9adbd9u, 0x08ae53a2u, 0x0c88be97u, 0x19b1c4f8u, 0x1c8f3d53u, 0x1267a24cu, 0x1c9181ffu, 0x00064f63u}};
+
+// 7-bit limb a of the 256-bit value v (little-endian words)
+__device__ __forceinline__ uint32_t rm_limb(const fr& v, int a) {
+    const int bit = 7 * a, w = bit >> 5, sh = bit & 31;
+    uint32_t x = v.v[w] >> sh;
+    if (sh > 25 && w + 1 < 8) x |= v.v[w + 1] << (32 - sh);
+    return x & 0x7fu;
+}
+
+// Limb planes in slabs: plane a of an x-by-mp operand (x = tp or cp, mp: multiples of
+// 32) stores, for row tile x / 32 and K step k / 32, the 32 x 32 bytes [row % 32][k % 32]
+// as one contiguous 1 KB slab, so that staging a slab is fully coalesced.
+__device__ __forceinline__ size_t rm_slab_index(int a, uint32_t x_tiles, uint32_t k_steps, uint32_t row,
+                                                uint32_t k) {
+    return ((((size_t)a * x_tiles + (row >> 5)) * k_steps + (k >> 5)) << 10) + ((row & 31) << 5) + (k & 31);
+}
+
+// Lp = limb planes of L[t][r] (t < nt, r < m), zero in the padding up to tp x mp
+__global__ void k_rec_planes_l(const uint32_t* L, uint32_t nt, uint32_t m, uint32_t tp, uint32_t mp, int8_t* Lp) {
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (size_t)tp * mp) return;
+    const uint32_t t = (uint32_t)(e / mp), r = (uint32_t)(e % mp);
+    fr v{};
+    if (t < nt && r < m) v = f_load<FrP>(L + 8 * ((size_t)t * m + r));
+#pragma unroll
+    for (int a = 0; a < RM_LIMBS; a++) Lp[rm_slab_index(a, tp >> 5, mp >> 5, t, r)] = (int8_t)rm_limb(v, a);
+}
+
+// Vp = limb planes of V transposed, Vp[b][i][r] = limb b of V[r][i] (r < m, i < col),
+// so that a lane's 8 bytes of B are contiguous; zero in the padding up to cp x mp
+__global__ void k_rec_planes_v(const uint32_t* V, uint32_t m, uint32_t col, uint32_t mp, uint32_t cp, int8_t* Vp) {
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (size_t)cp * mp) return;
+    const uint32_t i = (uint32_t)(e % cp), r = (uint32_t)(e / cp);  // consecutive threads: consecutive i
+    fr v{};
+    if (i < col && r < m) v = f_load<FrP>(V + 8 * ((size_t)r * col + i));
+#pragma unroll
+    for (int b = 0; b < RM_LIMBS; b++) Vp[rm_slab_index(b, cp >> 5, mp >> 5, i, r)] = (int8_t)rm_limb(v, b);
+}
+
+// One K step of wave W: for every limb a, its A fragment once, then an MFMA for each of
+// the wave's diagonals s = W + 16 q that pairs it with a limb b = s - a.  W, a and q are
+// compile-time constants after unrolling, so the wave runs exactly its MFMAs as
+// straight-line code (no branch between them for the hazard checks to guard).
+// a lane's 16-byte fragment of slab x
+__device__ __forceinline__ rm_v4i rm_frag(const uint64_t (*S)[32][4], int x, int row, int kh) {
+    const uint64_t lo = S[x][row][2 * kh], hi = S[x][row][2 * kh + 1];
+    rm_v4i f = {(int)(uint32_t)lo, (int)(uint32_t)(lo >> 32), (int)(uint32_t)hi, (int)(uint32_t)(hi >> 32)};
+    return f;
+}
+
+template <int W>
+__device__ __forceinline__ void rm_step(const uint64_t (*As)[32][4], const uint64_t (*Bs)[32][4], rm_v16i* acc,
+                                        int row, int kh) {
+#pragma unroll
+    for (int a = 0; a < RM_LIMBS; a++) {
+        const rm_v4i af = rm_frag(As, a, row, kh);
+#pragma unroll
+        for (int q = 0; q < RM_ACC; q++) {
+            const int s = W + RM_WAVES * q, b = s - a;
+            if (s < RM_DIAG && b >= 0 && b < RM_LIMBS)
+                acc[q] = __builtin_amdgcn_mfma_i32_32x32x32_i8(af, rm_frag(Bs, b, row, kh), acc[q], 0, 0, 0);
+        }
+        // a compiler barrier per limb: the fragments a wave reuses are read again from LDS
+        // rather than kept in registers, where they do not fit beside the accumulators
+        asm volatile("" ::: "memory");
+    }
+}
+
+// rm_step<w> for the runtime (wave-uniform) wave index w
+template <int W>
+__device__ __forceinline__ void rm_dispatch(int w, const uint64_t (*As)[32][4], const uint64_t (*Bs)[32][4],
+                                            rm_v16i* acc, int row, int kh) {
+    if (w == W) {
+        rm_step<W>(As, Bs, acc, row, kh);
+    } else {
+        if constexpr (W + 1 < RM_WAVES) rm_dispatch<W + 1>(w, As, Bs, acc, row, kh);
+    }
+}
+
+// D[s][t][i] for one 32 x 32 tile (t0 = 32 blockIdx.y, i0 = 32 blockIdx.x); tp, cp and
+// mp multiples of 32.  Wave w accumulates the diagonals s = w + 16 q; every K step stages
+// the 37 limb slabs of both operands (2 x 37 KB) in LDS, and a wave keeps its A fragment
+// of limb a while it runs the b = s - a its diagonals need.
+__global__ __launch_bounds__(RM_WAVES * 64) void k_rec_gemm_mfma(const int8_t* Lp, const int8_t* Vp, uint32_t tp,
+                                                                 uint32_t mp, uint32_t cp, int* D) {
+    __shared__ uint64_t As[RM_LIMBS][32][4], Bs[RM_LIMBS][32][4];
+    const uint32_t t0 = blockIdx.y * 32, i0 = blockIdx.x * 32;
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    const int row = l & 31, kh = l >> 5;
+    rm_v16i acc[RM_ACC];
+#pragma unroll
+    for (int q = 0; q < RM_ACC; q++)
+#pragma unroll
+        for (int k = 0; k < 16; k++) acc[q][k] = 0;
+    const uint32_t k_steps = mp >> 5;
+    for (uint32_t k0 = 0; k0 < mp; k0 += 32) {
+        for (uint32_t c = threadIdx.x; c < 2u * RM_LIMBS * 128; c += RM_WAVES * 64) {
+            // one 8-byte word of a slab: word j = 4 row + quarter of the 1 KB slab
+            const uint32_t which = c / (RM_LIMBS * 128), rem = c % (RM_LIMBS * 128);
+            const uint32_t a = rem >> 7, j = rem & 127;
+            if (which == 0)
+                As[a][j >> 2][j & 3] = ((const uint64_t*)Lp)[rm_slab_index(a, tp >> 5, k_steps, t0, k0) / 8 + j];
+            else
+                Bs[a][j >> 2][j & 3] = ((const uint64_t*)Vp)[rm_slab_index(a, cp >> 5, k_steps, i0, k0) / 8 + j];
+        }
+        __syncthreads();
+        rm_dispatch<0>(w, As, Bs, acc, row, kh);  // wave-uniform
+        __syncthreads();
+    }
+#pragma unroll
+    for (int q = 0; q < RM_ACC; q++) {
+        const int s = w + RM_WAVES * q;
+        if (s >= RM_DIAG) continue;
+#pragma unroll
+        for (int k = 0; k < 16; k++) {
+            const uint32_t t = t0 + 8 * (k >> 2) + 4 * kh + (k & 3);
+            D[((size_t)s * tp + t) * cp + i0 + row] = acc[q][k];
+        }
+    }
+}
+
+// Out[t][i] (Montgomery, < r) from the diagonals: sum_s D_s K_s in 29-bit columns (a
+// carry pass every 24 terms keeps them below 2^63), a Montgomery reduction by 2^261 (the
+// sum is below 2^290 < r 2^261, so the result is below 2r), one conditional subtraction.
+__global__ void k_rec_fold(const int* D, uint32_t tp, uint32_t cp, uint32_t nt, uint32_t col, uint32_t* Out) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x, t = blockIdx.y;
+    if (i >= col || t >= nt) return;
+    uint64_t c[19];
+#pragma unroll
+    for (int k = 0; k < 19; k++) c[k] = 0;
+    for (int s0 = 0; s0 < RM_DIAG; s0 += 24) {
+        for (int s = s0; s < s0 + 24 && s < RM_DIAG; s++) {
+            const uint64_t d = (uint32_t)D[((size_t)s * tp + t) * cp + i];
+#pragma unroll
+            for (int k = 0; k < 9; k++) c[k] += d * RM_KS[s][k];
+        }
+#pragma unroll
+        for (int k = 0; k < 10; k++) {  // carry pass: columns 0..9 back below 2^29 (+ carry)
+            c[k + 1] += c[k] >> 29;
+            c[k] &= R29_MASK;
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 9; k++) {  // REDC by 2^261
+        const uint32_t mk = ((uint32_t)c[k] * R29::INV) & R29_MASK;
+#pragma unroll
+        for (int j = 0; j < 9; j++) c[k + j] += (uint64_t)mk * R29::P[j];
+        c[k + 1] += c[k] >> 29;
+    }
+    uint32_t v[9];
+    uint64_t carry = 0;
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+        const uint64_t x = c[9 + k] + carry;
+        v[k] = (uint32_t)x & R29_MASK;
+        carry = x >> 29;
+    }
+    uint32_t d[9];  // v - r, kept when it does not borrow
+    int32_t br = 0;
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+        const int32_t x = (int32_t)v[k] - (int32_t)R29::P[k] - br;
+        br = (x >> 31) & 1;
+        d[k] = (uint32_t)x & R29_MASK;
+    }
+#pragma unroll
+    for (int k = 0; k < 9; k++) v[k] = br ? v[k] : d[k];
+    fr o;
+#pragma unroll
+    for (int ww = 0; ww < 8; ww++) {  // 29-bit limbs -> 32-bit words
+        const int bit = 32 * ww, k = bit / 29, sh = bit % 29;
+        uint64_t x = (uint64_t)v[k] >> sh;
+        if (k + 1 < 9) x |= (uint64_t)v[k + 1] << (29 - sh);
+        if (k + 2 < 9 && 29 - sh + 29 < 32) x |= (uint64_t)v[k + 2] << (58 - sh);
+        o.v[ww] = (uint32_t)x;
+    }
+    f_store(Out + 8 * ((size_t)t * col + i), o);
+}
+
 // Consistency of the extra rows (indices m..nrows-1 of the received list): their
 // combination yv must equal the interpolant A(u_x) sum_r L[x][r] yv[r].  One workgroup
 // (256 threads) per extra row x.  flag bit 1.

@@ -1567,8 +1567,27 @@ int zgr_recover(zgr_ctx* c, const uint32_t* row_ids, const uint8_t* rows, size_t
                    m, d_L);
         }
         if (e0) {
-            dim3 grid(blocks(COL, REC_TI), blocks(e0, REC_TL));
-            LAUNCH(c, st, "recover_gemm", k_rec_gemm, grid, 256, 0, d_L, d_V, e0, m, (uint32_t)COL, d_out);
+            const char* gm = getenv("ZGR_REC_GEMM");
+            if (gm && !strcmp(gm, "valu")) {  // 29-bit limb products on the VALU
+                dim3 grid(blocks(COL, REC_TI), blocks(e0, REC_TL));
+                LAUNCH(c, st, "recover_gemm", k_rec_gemm, grid, 256, 0, d_L, d_V, e0, m, (uint32_t)COL, d_out);
+            } else {  // int8 limb planes on the matrix cores
+                if (m > RM_MAX_M) return fail(ZGR_ERR_INVALID_ARG, "recovery GEMM over %u sources (max %u)", m, RM_MAX_M);
+                const uint32_t tp = (e0 + 31) & ~31u, mp = (m + 31) & ~31u, cp = ((uint32_t)COL + 31) & ~31u;
+                int8_t *d_lp, *d_vp;
+                int* d_diag;
+                CHECK(alloc(d_lp, (size_t)RM_LIMBS * tp * mp));
+                CHECK(alloc(d_vp, (size_t)RM_LIMBS * cp * mp));
+                CHECK(alloc(d_diag, (size_t)RM_DIAG * tp * cp * 4));
+                LAUNCH(c, st, "recover_planes", k_rec_planes_l, blocks((size_t)tp * mp, 256), 256, 0, d_L, e0, m, tp, mp,
+                       d_lp);
+                LAUNCH(c, st, "recover_planes", k_rec_planes_v, blocks((size_t)cp * mp, 256), 256, 0, d_V, m,
+                       (uint32_t)COL, mp, cp, d_vp);
+                LAUNCH(c, st, "recover_gemm", k_rec_gemm_mfma, dim3(cp / 32, tp / 32), RM_WAVES * 64, 0, d_lp, d_vp,
+                       tp, mp, cp, d_diag);
+                LAUNCH(c, st, "recover_fold", k_rec_fold, dim3(blocks(COL, 64), e0), 64, 0, d_diag, tp, cp, e0,
+                       (uint32_t)COL, d_out);
+            }
         }
         if (nx) LAUNCH(c, st, "recover", k_rec_check, nx, 256, 0, d_L + 8 * (size_t)e0 * m, d_A + 8 * (size_t)e0, d_yv,
                        m, flag);

@@ -202,7 +202,8 @@ def main():
             timing[kind] = round(float(np.median(tt)) * 1e3, 2)
             if kind == "pageable":
                 ks = {}
-                for name in ("recover", "recover_rows", "recover_lagrange", "recover_gemm"):
+                for name in ("recover", "recover_rows", "recover_lagrange", "recover_planes", "recover_gemm",
+                             "recover_fold"):
                     ms, cnt = ctx.kernel_time_ms(name)
                     if cnt:
                         ks[name] = round(ms * cnt / 5, 3)  # per call, summed over its launches

@@ -158,3 +158,23 @@ def test_recover_into_host_arrays(ctx, blobs):
         ctx.recover_into(ids, np.frombuffer(buf[:-1], dtype=np.uint8), np.empty(GEO.max_raw_data_size, np.uint8))
     with pytest.raises(Z.ZgrError):  # raw_out below MAX_RAW_DATA_SIZE: the C ABI refuses it
         ctx.recover_into(ids, np.frombuffer(buf, dtype=np.uint8), np.empty(GEO.max_raw_data_size - 1, np.uint8))
+
+
+def test_gemm_paths(ctx, blobs):
+    """The interpolation GEMM on the matrix cores (default) and on the VALU
+    (ZGR_REC_GEMM=valu, read at every call) recover the same bytes, over row sets that
+    erase every coset-0 row, some of them, or none."""
+    import os
+    raw, rows = blobs[3]
+    sets = [list(range(2 * R, 3 * R)), list(range(5, R)) + list(range(R, R + 5)), list(range(R))]
+    old = os.environ.get("ZGR_REC_GEMM")
+    try:
+        for path in ("valu", "mfma"):
+            os.environ["ZGR_REC_GEMM"] = path
+            for ids in sets:
+                assert Z.recover_from_da_slice(ctx, {i: rows[i] for i in ids}) == raw, (path, ids[0])
+    finally:
+        if old is None:
+            os.environ.pop("ZGR_REC_GEMM", None)
+        else:
+            os.environ["ZGR_REC_GEMM"] = old
