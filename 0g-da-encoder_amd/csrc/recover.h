@@ -346,18 +346,19 @@ __global__ __launch_bounds__(256) void k_rec_gemm(const uint32_t* L, const uint3
 
 // ---------------------------------------------------------------------------
 // The same GEMM on the matrix cores (v_mfma_i32_32x32x32_i8).  Every element, a
-// Montgomery residue below r < 2^254, is split into 37 limbs of 7 bits (0..127, valid
-// signed int8), so Out[t][i] * 2^256 = sum_s D_s[t][i] 2^(7 s) with, for the 73 limb-
-// weight diagonals s = a + b,
-//     D_s[t][i] = sum_{a + b = s} sum_r Lp_a[t][r] Vp_b[r][i]    (< 37 * 2^10 * 127^2 < 2^30)
+// Montgomery residue below r < 2^254, is written with 32 balanced base-256 digits
+// (-128..127, int8; the top digit stays below 64, so no carry leaves it), so
+// Out[t][i] * 2^256 = sum_s D_s[t][i] 2^(8 s) with, for the 63 digit-weight diagonals
+// s = a + b,
+//     D_s[t][i] = sum_{a + b = s} sum_r Lp_a[t][r] Vp_b[r][i]    (|D_s| <= 32 * 2^10 * 2^14 = 2^29)
 // a sum of int8 matrix products.  k_rec_gemm_mfma computes the D_s of a 32 x 32 output
-// tile, k_rec_fold folds them: sum_s D_s (2^(7 s + 5) mod r) in 29-bit columns, one
+// tile, k_rec_fold folds them: sum_s |D_s| (+-2^(8 s + 5) mod r) in 29-bit columns, one
 // Montgomery reduction by 2^261 (= X 2^-256 mod r, the Montgomery form of the product
 // sum), one conditional subtraction of r.
 // MFMA operand layout (32x32x32, int8): lane l holds A[l % 32][16 (l / 32) .. + 16) and
 // B[16 (l / 32) .. + 16)[l % 32] as 16 packed bytes; accumulator register q of lane l is
 // C[8 (q / 4) + 4 (l / 32) + q % 4][l % 32].
-constexpr int RM_LIMBS = 37;
+constexpr int RM_LIMBS = 32;
 constexpr int RM_DIAG = 2 * RM_LIMBS - 1;
 constexpr int RM_WAVES = 16;                              // k_rec_gemm_mfma: 1024 threads
 constexpr int RM_ACC = (RM_DIAG + RM_WAVES - 1) / RM_WAVES;  // diagonals per wave (s = w + 16 q)
@@ -366,88 +367,81 @@ constexpr uint32_t RM_MAX_M = 2048;                       // keeps D_s below 2^3
 typedef int rm_v16i __attribute__((ext_vector_type(16)));
 typedef int rm_v4i __attribute__((ext_vector_type(4)));
 
-// 2^(7 s + 5) mod r, 29-bit limbs
-__constant__ uint32_t RM_KS[RM_DIAG][9] = {
-    {0x00000020u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u},
-    {0x00001000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u},
-    {0x00080000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u},
-    {0x04000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u},
-    {0x00000000u, 0x00000010u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u},
-    {0x00000000u, 0x00000800u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u},
-    {0x00000000u, 0x00040000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u},
-    {0x00000000u, 0x02000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u},
-    {0x00000000u, 0x00000000u, 0x00000008u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u},
-    {0x00000000u, 0x00000000u, 0x00000400u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u},
-    {0x00000000u, 0x00000000u, 0x00020000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u},
-    {0x00000000u, 0x00000000u, 0x01000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u},
-    {0x00000000u, 0x00000000u, 0x00000000u, 0x00000004u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u},
-    {0x00000000u, 0x00000000u, 0x00000000u, 0x00000200u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u},
-    {0x00000000u, 0x00000000u, 0x00000000u, 0x00010000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u},
-    {0x00000000u, 0x00000000u, 0x00000000u, 0x00800000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u},
-    {0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000002u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u},
-    {0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000100u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u},
-    {0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00008000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u},
-    {0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00400000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u},
-    {0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000001u, 0x00000000u, 0x00000000u, 0x00000000u},
-    {0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000080u, 0x00000000u, 0x00000000u, 0x00000000u},
-    {0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00004000u, 0x00000000u, 0x00000000u, 0x00000000u},
-    {0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00200000u, 0x00000000u, 0x00000000u, 0x00000000u},
-    {0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x10000000u, 0x00000000u, 0x00000000u, 0x00000000u},
-    {0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000040u, 0x00000000u, 0x00000000u},
-    {0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00002000u, 0x00000000u, 0x00000000u},
-    {0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00100000u, 0x00000000u, 0x00000000u},
-    {0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x08000000u, 0x00000000u, 0x00000000u},
-    {0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000020u, 0x00000000u},
-    {0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00001000u, 0x00000000u},
-    {0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00080000u, 0x00000000u},
-    {0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x04000000u, 0x00000000u},
-    {0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000010u},
-    {0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000800u},
-    {0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00040000u},
-    {0x1ffffff6u, 0x096341c4u, 0x106694d6u, 0x11da567du, 0x08c5c6dfu, 0x036f7879u, 0x197b3375u, 0x106681f7u, 0x001c14efu},
-    {0x1ffffab6u, 0x1718fc64u, 0x0ca7ebbdu, 0x0ae15862u, 0x0a3495d7u, 0x045b84f1u, 0x0d5c9dbdu, 0x0c9df0dcu, 0x000d7914u},
-    {0x0ffd5addu, 0x0d5998b1u, 0x1d5ce7cau, 0x1f285fe9u, 0x08ff23b9u, 0x09c89e51u, 0x177e12abu, 0x185f3518u, 0x001ed378u},
-    {0x0ead6e2fu, 0x18d6ba48u, 0x034c6796u, 0x1b3217eeu, 0x103a4175u, 0x1cefabadu, 0x0a3c100eu, 0x047275e8u, 0x0019ff8fu},
-    {0x16b7173cu, 0x0b3349dcu, 0x15ba25e1u, 0x05a57662u, 0x0594d010u, 0x1598a393u, 0x11b43229u, 0x08c0ce57u, 0x002522bbu},
-    {0x1b8b9d9eu, 0x15a4d94bu, 0x1dcd09adu, 0x14e3b424u, 0x0d2d71d8u, 0x1462ffd4u, 0x1a36d9ebu, 0x01208bdcu, 0x000af79bu},
-    {0x15cecee3u, 0x0da617dcu, 0x0614b9aau, 0x0f39a6e3u, 0x06901375u, 0x1ea99461u, 0x08857171u, 0x0fd59a84u, 0x000070a1u},
-    {0x1767717fu, 0x13fc41b7u, 0x1c00b0e5u, 0x1502e0a4u, 0x1283e839u, 0x11eeefd9u, 0x0211d7b9u, 0x1c711beeu, 0x0007ec70u},
-    {0x13b8bf6cu, 0x10e75f67u, 0x01259c7cu, 0x0524ff6bu, 0x137faa93u, 0x1e56ddbcu, 0x1be243b1u, 0x195afaf7u, 0x002e6250u},
-    {0x1c5fb586u, 0x06376fcbu, 0x1ae4efadu, 0x191aa17fu, 0x1e0ef708u, 0x0ef2027bu, 0x019a7ff4u, 0x15934725u, 0x00215b02u},
-    {0x0fdac2a8u, 0x0e548f1du, 0x02cb5ac3u, 0x1d9eec3cu, 0x017b2746u, 0x1da2fbbdu, 0x13e28bdbu, 0x19f670a4u, 0x000b065eu},
-    {0x1d6153e3u, 0x058100adu, 0x053d448du, 0x0cd5b277u, 0x0d6aca98u, 0x1ea788b2u, 0x1e5e6996u, 0x1ac7fe6au, 0x0007d284u},
-    {0x10a9f16cu, 0x1346da7fu, 0x1f6f7042u, 0x0e8de88fu, 0x06f0d9f2u, 0x1aa34a28u, 0x022b3264u, 0x04cc3969u, 0x00216c4au},
-    {0x14f8b5a8u, 0x1609e8eeu, 0x080ba577u, 0x1742744eu, 0x126c9c1cu, 0x1646d1e0u, 0x1c3bc40au, 0x166f92a6u, 0x0013aa1bu},
-    {0x1c5ad3ccu, 0x15c566edu, 0x1b1b5b65u, 0x0adcb5acu, 0x171f4bb4u, 0x0edfc926u, 0x1bfc4420u, 0x0d1190dfu, 0x0000adeau},
-    {0x1d69e5ffu, 0x03a3ca51u, 0x1f518e86u, 0x068a4578u, 0x1a2007a8u, 0x0d09529bu, 0x1d7b2efau, 0x1a6c49bbu, 0x002690e5u},
-    {0x14f2ff1au, 0x11a66168u, 0x1010cacbu, 0x0808fb7du, 0x1cb1f3cfu, 0x014d8123u, 0x1b19bd45u, 0x1d6da53bu, 0x00007ba7u},
-    {0x097f8cffu, 0x142107b4u, 0x1a094175u, 0x1cad2dccu, 0x0374151cu, 0x03e55132u, 0x0c37c144u, 0x087677b8u, 0x000d6fa7u},
-    {0x0fc67f5du, 0x115f4057u, 0x0e07c3beu, 0x0513151fu, 0x08bec681u, 0x0eaebeb6u, 0x050fd629u, 0x04a2a314u, 0x001a1ce8u},
-    {0x133fae3bu, 0x1066a4c1u, 0x050c1572u, 0x0e537e1au, 0x12518334u, 0x1246e734u, 0x1af05e2fu, 0x127b3e0eu, 0x00036aedu},
-    {0x0fd71d77u, 0x1bc54f31u, 0x04cd7268u, 0x0369f485u, 0x070d339cu, 0x09be5383u, 0x12512c7fu, 0x1c61af97u, 0x0001f007u},
-    {0x1b8ebb7bu, 0x075939a1u, 0x1eec7edau, 0x0de76dd1u, 0x1afcb17du, 0x10e17dd8u, 0x0553d961u, 0x090b0cc5u, 0x00060e69u},
-    {0x075dbd70u, 0x1ba206f6u, 0x107d280du, 0x16add9c4u, 0x05fb9683u, 0x030ae061u, 0x1f7e9cb2u, 0x1fc3ff54u, 0x0000efbcu},
-    {0x0edeb7feu, 0x12e421deu, 0x01dbbe4cu, 0x074bc05bu, 0x12bf9cd4u, 0x1fb9af16u, 0x1e009689u, 0x05475e15u, 0x001715e2u},
-    {0x1f5bfec3u, 0x0b54cd39u, 0x01ea7f00u, 0x092da385u, 0x1eeb40d7u, 0x0e991d61u, 0x1887987cu, 0x17b9f092u, 0x00030a63u},
-    {0x0dff6178u, 0x11e93801u, 0x025e5da5u, 0x184d3aecu, 0x0971d78au, 0x15b4aaf6u, 0x1e953431u, 0x0a1717c1u, 0x00020f6bu},
-    {0x0fb0bbfbu, 0x194da19au, 0x07621d32u, 0x1f8aa148u, 0x0d4ea8d0u, 0x0c0d3762u, 0x0757b291u, 0x03bf21f6u, 0x0015c020u},
-    {0x085dfd47u, 0x1c535dbbu, 0x1e8a835du, 0x07e05de8u, 0x1c888945u, 0x03ca464eu, 0x06b12132u, 0x0d0c7988u, 0x0019ba97u},
-    {0x0efea33cu, 0x09850343u, 0x14c8096fu, 0x1cc873cfu, 0x0cb8b7c2u, 0x02e5f444u, 0x0c3cc3c5u, 0x15c29e49u, 0x0002a6ddu},
-    {0x0f519df9u, 0x0913e95fu, 0x1f7fb96fu, 0x0d85f12au, 0x05b31fdcu, 0x1efb5ceeu, 0x19d1b9c3u, 0x1cca1944u, 0x0000b0b1u},
-    {0x18cefc7fu, 0x0ae5031du, 0x11809353u, 0x1b28048au, 0x040a1bb2u, 0x1ad33656u, 0x083600bau, 0x16b07c33u, 0x0027f4a4u},
-    {0x177e3f17u, 0x1513c177u, 0x1c7ec475u, 0x1f76d16bu, 0x112a8197u, 0x1dad9c09u, 0x168df9b8u, 0x14726e4au, 0x00212e2du},
-    {0x0f1f8b29u, 0x1b8d11a9u, 0x1e11e34fu, 0x13877335u, 0x04c64163u, 0x1e4b034au, 0x0e464f57u, 0x17e629acu, 0x00250028u},
-    {0x1fc5941fu, 0x01986c4cu, 0x1807e540u, 0x0db2aeb9u, 0x1b6bee13u, 0x106e1c11u, 0x03ec524fu, 0x022a5c83u, 0x002a12a6u},
-    {0x12ca0f11u, 0x146a4d56u, 0x09fee0eau, 0x15e88351u, 0x00f0c2ceu, 0x19fcf563u, 0x0dcd7c88u, 0x1b39b101u, 0x000dd504u},
-    {0x0507885cu, 0x16f264ddu, 0x1a7b59eeu, 0x1aed466bu, 0x118fcce2u, 0x17a5966bu, 0x0f469740u, 0x17e32166u, 0x001c6764u},
-    {0x13c42db5u, 0x1f9adbd9u, 0x08ae53a2u, 0x0c88be97u, 0x19b1c4f8u, 0x1c8f3d53u, 0x1267a24cu, 0x1c9181ffu, 0x00064f63u}};
+// {2^(8 s + 5) mod r, r - that}, 29-bit limbs
+__constant__ uint32_t RM_KS[RM_DIAG][2][9] = {
+    {{0x00000020u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u}, {0x0fffffe1u, 0x1f0fac9fu, 0x0e5c2450u, 0x07d090f3u, 0x1585d283u, 0x02db40c0u, 0x00a6e141u, 0x0e5c2634u, 0x0030644eu}},
+    {{0x00002000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u}, {0x0fffe001u, 0x1f0fac9fu, 0x0e5c2450u, 0x07d090f3u, 0x1585d283u, 0x02db40c0u, 0x00a6e141u, 0x0e5c2634u, 0x0030644eu}},
+    {{0x00200000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u}, {0x0fe00001u, 0x1f0fac9fu, 0x0e5c2450u, 0x07d090f3u, 0x1585d283u, 0x02db40c0u, 0x00a6e141u, 0x0e5c2634u, 0x0030644eu}},
+    {{0x00000000u, 0x00000001u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u}, {0x10000001u, 0x1f0fac9eu, 0x0e5c2450u, 0x07d090f3u, 0x1585d283u, 0x02db40c0u, 0x00a6e141u, 0x0e5c2634u, 0x0030644eu}},
+    {{0x00000000u, 0x00000100u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u}, {0x10000001u, 0x1f0fab9fu, 0x0e5c2450u, 0x07d090f3u, 0x1585d283u, 0x02db40c0u, 0x00a6e141u, 0x0e5c2634u, 0x0030644eu}},
+    {{0x00000000u, 0x00010000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u}, {0x10000001u, 0x1f0eac9fu, 0x0e5c2450u, 0x07d090f3u, 0x1585d283u, 0x02db40c0u, 0x00a6e141u, 0x0e5c2634u, 0x0030644eu}},
+    {{0x00000000u, 0x01000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u}, {0x10000001u, 0x1e0fac9fu, 0x0e5c2450u, 0x07d090f3u, 0x1585d283u, 0x02db40c0u, 0x00a6e141u, 0x0e5c2634u, 0x0030644eu}},
+    {{0x00000000u, 0x00000000u, 0x00000008u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u}, {0x10000001u, 0x1f0fac9fu, 0x0e5c2448u, 0x07d090f3u, 0x1585d283u, 0x02db40c0u, 0x00a6e141u, 0x0e5c2634u, 0x0030644eu}},
+    {{0x00000000u, 0x00000000u, 0x00000800u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u}, {0x10000001u, 0x1f0fac9fu, 0x0e5c1c50u, 0x07d090f3u, 0x1585d283u, 0x02db40c0u, 0x00a6e141u, 0x0e5c2634u, 0x0030644eu}},
+    {{0x00000000u, 0x00000000u, 0x00080000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u}, {0x10000001u, 0x1f0fac9fu, 0x0e542450u, 0x07d090f3u, 0x1585d283u, 0x02db40c0u, 0x00a6e141u, 0x0e5c2634u, 0x0030644eu}},
+    {{0x00000000u, 0x00000000u, 0x08000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u}, {0x10000001u, 0x1f0fac9fu, 0x065c2450u, 0x07d090f3u, 0x1585d283u, 0x02db40c0u, 0x00a6e141u, 0x0e5c2634u, 0x0030644eu}},
+    {{0x00000000u, 0x00000000u, 0x00000000u, 0x00000040u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u}, {0x10000001u, 0x1f0fac9fu, 0x0e5c2450u, 0x07d090b3u, 0x1585d283u, 0x02db40c0u, 0x00a6e141u, 0x0e5c2634u, 0x0030644eu}},
+    {{0x00000000u, 0x00000000u, 0x00000000u, 0x00004000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u}, {0x10000001u, 0x1f0fac9fu, 0x0e5c2450u, 0x07d050f3u, 0x1585d283u, 0x02db40c0u, 0x00a6e141u, 0x0e5c2634u, 0x0030644eu}},
+    {{0x00000000u, 0x00000000u, 0x00000000u, 0x00400000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u}, {0x10000001u, 0x1f0fac9fu, 0x0e5c2450u, 0x079090f3u, 0x1585d283u, 0x02db40c0u, 0x00a6e141u, 0x0e5c2634u, 0x0030644eu}},
+    {{0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000002u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u}, {0x10000001u, 0x1f0fac9fu, 0x0e5c2450u, 0x07d090f3u, 0x1585d281u, 0x02db40c0u, 0x00a6e141u, 0x0e5c2634u, 0x0030644eu}},
+    {{0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000200u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u}, {0x10000001u, 0x1f0fac9fu, 0x0e5c2450u, 0x07d090f3u, 0x1585d083u, 0x02db40c0u, 0x00a6e141u, 0x0e5c2634u, 0x0030644eu}},
+    {{0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00020000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u}, {0x10000001u, 0x1f0fac9fu, 0x0e5c2450u, 0x07d090f3u, 0x1583d283u, 0x02db40c0u, 0x00a6e141u, 0x0e5c2634u, 0x0030644eu}},
+    {{0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x02000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u}, {0x10000001u, 0x1f0fac9fu, 0x0e5c2450u, 0x07d090f3u, 0x1385d283u, 0x02db40c0u, 0x00a6e141u, 0x0e5c2634u, 0x0030644eu}},
+    {{0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000010u, 0x00000000u, 0x00000000u, 0x00000000u}, {0x10000001u, 0x1f0fac9fu, 0x0e5c2450u, 0x07d090f3u, 0x1585d283u, 0x02db40b0u, 0x00a6e141u, 0x0e5c2634u, 0x0030644eu}},
+    {{0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00001000u, 0x00000000u, 0x00000000u, 0x00000000u}, {0x10000001u, 0x1f0fac9fu, 0x0e5c2450u, 0x07d090f3u, 0x1585d283u, 0x02db30c0u, 0x00a6e141u, 0x0e5c2634u, 0x0030644eu}},
+    {{0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00100000u, 0x00000000u, 0x00000000u, 0x00000000u}, {0x10000001u, 0x1f0fac9fu, 0x0e5c2450u, 0x07d090f3u, 0x1585d283u, 0x02cb40c0u, 0x00a6e141u, 0x0e5c2634u, 0x0030644eu}},
+    {{0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x10000000u, 0x00000000u, 0x00000000u, 0x00000000u}, {0x10000001u, 0x1f0fac9fu, 0x0e5c2450u, 0x07d090f3u, 0x1585d283u, 0x12db40c0u, 0x00a6e140u, 0x0e5c2634u, 0x0030644eu}},
+    {{0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000080u, 0x00000000u, 0x00000000u}, {0x10000001u, 0x1f0fac9fu, 0x0e5c2450u, 0x07d090f3u, 0x1585d283u, 0x02db40c0u, 0x00a6e0c1u, 0x0e5c2634u, 0x0030644eu}},
+    {{0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00008000u, 0x00000000u, 0x00000000u}, {0x10000001u, 0x1f0fac9fu, 0x0e5c2450u, 0x07d090f3u, 0x1585d283u, 0x02db40c0u, 0x00a66141u, 0x0e5c2634u, 0x0030644eu}},
+    {{0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00800000u, 0x00000000u, 0x00000000u}, {0x10000001u, 0x1f0fac9fu, 0x0e5c2450u, 0x07d090f3u, 0x1585d283u, 0x02db40c0u, 0x0026e141u, 0x0e5c2634u, 0x0030644eu}},
+    {{0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000004u, 0x00000000u}, {0x10000001u, 0x1f0fac9fu, 0x0e5c2450u, 0x07d090f3u, 0x1585d283u, 0x02db40c0u, 0x00a6e141u, 0x0e5c2630u, 0x0030644eu}},
+    {{0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000400u, 0x00000000u}, {0x10000001u, 0x1f0fac9fu, 0x0e5c2450u, 0x07d090f3u, 0x1585d283u, 0x02db40c0u, 0x00a6e141u, 0x0e5c2234u, 0x0030644eu}},
+    {{0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00040000u, 0x00000000u}, {0x10000001u, 0x1f0fac9fu, 0x0e5c2450u, 0x07d090f3u, 0x1585d283u, 0x02db40c0u, 0x00a6e141u, 0x0e582634u, 0x0030644eu}},
+    {{0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x04000000u, 0x00000000u}, {0x10000001u, 0x1f0fac9fu, 0x0e5c2450u, 0x07d090f3u, 0x1585d283u, 0x02db40c0u, 0x00a6e141u, 0x0a5c2634u, 0x0030644eu}},
+    {{0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000020u}, {0x10000001u, 0x1f0fac9fu, 0x0e5c2450u, 0x07d090f3u, 0x1585d283u, 0x02db40c0u, 0x00a6e141u, 0x0e5c2634u, 0x0030642eu}},
+    {{0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00002000u}, {0x10000001u, 0x1f0fac9fu, 0x0e5c2450u, 0x07d090f3u, 0x1585d283u, 0x02db40c0u, 0x00a6e141u, 0x0e5c2634u, 0x0030444eu}},
+    {{0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00200000u}, {0x10000001u, 0x1f0fac9fu, 0x0e5c2450u, 0x07d090f3u, 0x1585d283u, 0x02db40c0u, 0x00a6e141u, 0x0e5c2634u, 0x0010644eu}},
+    {{0x0fffff57u, 0x1ea70ab4u, 0x052c068bu, 0x17504f49u, 0x0aa8075bu, 0x1d4240ceu, 0x11d54c07u, 0x052ac7a8u, 0x000dc836u}, {0x000000aau, 0x0068a1ebu, 0x09301dc5u, 0x108041aau, 0x0addcb27u, 0x0598fff2u, 0x0ed19539u, 0x09315e8bu, 0x00229c18u}},
+    {{0x1fff56b8u, 0x0aa227a3u, 0x021c552fu, 0x1da684b1u, 0x1a6426d0u, 0x14969824u, 0x065cad9bu, 0x00dce9edu, 0x002c0019u}, {0x1000a949u, 0x146d84fbu, 0x0c3fcf21u, 0x0a2a0c42u, 0x1b21abb2u, 0x0e44a89bu, 0x1a4a33a5u, 0x0d7f3c46u, 0x00046435u}},
+    {{0x1f56b718u, 0x1bf33373u, 0x18d445f3u, 0x11815470u, 0x02e009fcu, 0x1fe57637u, 0x057178a7u, 0x19674e0eu, 0x002531eeu}, {0x10a948e9u, 0x031c792bu, 0x1587de5du, 0x164f3c82u, 0x12a5c886u, 0x02f5ca89u, 0x1b356899u, 0x14f4d825u, 0x000b325fu}},
+    {{0x16b7173cu, 0x0b3349dcu, 0x15ba25e1u, 0x05a57662u, 0x0594d010u, 0x1598a393u, 0x11b43229u, 0x08c0ce57u, 0x002522bbu}, {0x1948e8c5u, 0x13dc62c2u, 0x18a1fe6fu, 0x022b1a90u, 0x0ff10273u, 0x0d429d2du, 0x0ef2af17u, 0x059b57dcu, 0x000b4193u}},
+    {{0x17173b3cu, 0x0b49b297u, 0x1b9a135bu, 0x09c76849u, 0x1a5ae3b1u, 0x08c5ffa8u, 0x146db3d7u, 0x024117b9u, 0x0015ef36u}, {0x18e8c4c5u, 0x13c5fa07u, 0x12c210f5u, 0x1e0928a9u, 0x1b2aeed1u, 0x1a154117u, 0x0c392d69u, 0x0c1b0e7au, 0x001a7518u}},
+    {{0x173b3b8cu, 0x16985f72u, 0x1852e6a9u, 0x1ce69b8cu, 0x1a404dd5u, 0x1aa65184u, 0x0215c5c7u, 0x1f566a11u, 0x0001c285u}, {0x18c4c475u, 0x08774d2cu, 0x16093da7u, 0x0ae9f566u, 0x1b4584adu, 0x0834ef3bu, 0x1e911b79u, 0x0f05bc22u, 0x002ea1c8u}},
+    {{0x0b3b8bf7u, 0x00d2611eu, 0x11a962dcu, 0x00467433u, 0x1e996f4au, 0x0c9c3e0bu, 0x0fe7dc8bu, 0x152cb93cu, 0x000eff38u}, {0x04c4740au, 0x1e3d4b81u, 0x1cb2c174u, 0x078a1cbfu, 0x16ec6339u, 0x163f02b4u, 0x10bf04b5u, 0x192f6cf7u, 0x00216515u}},
+    {{0x0b8bf6b1u, 0x1c8ad921u, 0x1af3a709u, 0x1d17786cu, 0x15235381u, 0x1a95107fu, 0x145d084eu, 0x1e497271u, 0x00104473u}, {0x04740950u, 0x0284d37eu, 0x13687d47u, 0x0ab91886u, 0x00627f01u, 0x08463041u, 0x0c49d8f2u, 0x1012b3c2u, 0x00201fdau}},
+    {{0x0bf6b0aau, 0x1b9523c7u, 0x00b2d6b0u, 0x1767bb0fu, 0x085ec9d1u, 0x1f68beefu, 0x04f8a2f6u, 0x167d9c29u, 0x0002c197u}, {0x04094f57u, 0x037a88d8u, 0x0da94da0u, 0x1068d5e4u, 0x0d2708b1u, 0x037281d1u, 0x1bae3e4au, 0x17de8a0au, 0x002da2b6u}},
+    {{0x16b0a9f2u, 0x024856a6u, 0x09ccb46fu, 0x1a5321b5u, 0x11784e8du, 0x00c164b9u, 0x0f82a56cu, 0x1492124fu, 0x001c1b69u}, {0x194f560fu, 0x1cc755f8u, 0x048f6fe1u, 0x0d7d6f3eu, 0x040d83f5u, 0x0219dc07u, 0x11243bd5u, 0x19ca13e4u, 0x001448e4u}},
+    {{0x10a9f16cu, 0x1346da7fu, 0x1f6f7042u, 0x0e8de88fu, 0x06f0d9f2u, 0x1aa34a28u, 0x022b3264u, 0x04cc3969u, 0x00216c4au}, {0x1f560e95u, 0x0bc8d21fu, 0x0eecb40eu, 0x1942a863u, 0x0e94f890u, 0x0837f698u, 0x1e7baedcu, 0x098feccau, 0x000ef804u}},
+    {{0x09f16b50u, 0x0c13d1ddu, 0x10174aefu, 0x0e84e89cu, 0x04d93839u, 0x0c8da3c1u, 0x18778815u, 0x0cdf254du, 0x00275437u}, {0x060e94b1u, 0x12fbdac2u, 0x1e44d961u, 0x194ba856u, 0x10ac9a49u, 0x164d9cffu, 0x082f592bu, 0x017d00e6u, 0x00091017u}},
+    {{0x116b4f30u, 0x17159bb7u, 0x0c6d6d96u, 0x0b72d6b3u, 0x1c7d2ed1u, 0x1b7f249au, 0x0ff11081u, 0x1446437fu, 0x0002b7a9u}, {0x1e94b0d1u, 0x07fa10e7u, 0x01eeb6bau, 0x1c5dba40u, 0x1908a3b1u, 0x075c1c25u, 0x10b5d0bfu, 0x1a15e2b4u, 0x002daca4u}},
+    {{0x0b4f2ff2u, 0x02c046d2u, 0x04639a4bu, 0x056ec613u, 0x0fdd4e2eu, 0x1727105au, 0x07f0304cu, 0x1d3968a7u, 0x00122d57u}, {0x04b0d00fu, 0x1c4f65cdu, 0x09f88a05u, 0x0261cae0u, 0x05a88455u, 0x0bb43066u, 0x18b6b0f4u, 0x1122bd8cu, 0x001e36f6u}},
+    {{0x0f2ff1a0u, 0x1a66168au, 0x010cacb8u, 0x008fb7d8u, 0x0b1f3cf4u, 0x14d8123eu, 0x119bd450u, 0x16da53bdu, 0x0007ba7eu}, {0x00d00e61u, 0x04a99615u, 0x0d4f7798u, 0x0740d91bu, 0x0a66958fu, 0x0e032e82u, 0x0f0b0cf0u, 0x1781d276u, 0x0028a9cfu}},
+    {{0x0ff19fd8u, 0x0ba3918du, 0x0e470c2cu, 0x172131feu, 0x02540f82u, 0x05d0203eu, 0x01c11e7bu, 0x1bedc56cu, 0x002ad274u}, {0x000e6029u, 0x136c1b12u, 0x00151824u, 0x10af5ef5u, 0x1331c300u, 0x1d0b2082u, 0x1ee5c2c5u, 0x126e60c7u, 0x000591d9u}},
+    {{0x119fd71eu, 0x17bb28b0u, 0x19b41ce1u, 0x1b120786u, 0x13ebaadbu, 0x0a9113fau, 0x0dcb9fb8u, 0x006bb221u, 0x0019e79eu}, {0x1e6028e3u, 0x075483eeu, 0x14a8076fu, 0x0cbe896cu, 0x019a27a7u, 0x184a2cc6u, 0x12db4188u, 0x0df07412u, 0x00167cb0u}},
+    {{0x0fd71d77u, 0x1bc54f31u, 0x04cd7268u, 0x0369f485u, 0x070d339cu, 0x09be5383u, 0x12512c7fu, 0x1c61af97u, 0x0001f007u}, {0x0028e28au, 0x034a5d6eu, 0x098eb1e8u, 0x04669c6eu, 0x0e789ee7u, 0x191ced3du, 0x0e55b4c1u, 0x11fa769cu, 0x002e7446u}},
+    {{0x171d76f6u, 0x0eb27343u, 0x1dd8fdb4u, 0x1bcedba3u, 0x15f962fau, 0x01c2fbb1u, 0x0aa7b2c3u, 0x1216198au, 0x000c1cd2u}, {0x18e2890bu, 0x105d395bu, 0x1083269cu, 0x0c01b54fu, 0x1f8c6f88u, 0x0118450eu, 0x15ff2e7eu, 0x1c460ca9u, 0x0024477bu}},
+    {{0x1d76f5c0u, 0x0e881bd8u, 0x01f4a037u, 0x1ab76712u, 0x17ee5a0eu, 0x0c2b8184u, 0x1dfa72c8u, 0x1f0ffd53u, 0x0003bef3u}, {0x12890a41u, 0x108790c6u, 0x0c678419u, 0x0d1929e1u, 0x1d977874u, 0x16afbf3bu, 0x02ac6e78u, 0x0f4c28e0u, 0x002ca55au}},
+    {{0x06f5bfedu, 0x19f20915u, 0x03c98571u, 0x02ec4ffeu, 0x156b6f18u, 0x153bb672u, 0x0e10108cu, 0x1f267e13u, 0x00278225u}, {0x090a4014u, 0x051da38au, 0x0a929edfu, 0x04e440f5u, 0x001a636bu, 0x0d9f8a4eu, 0x1296d0b4u, 0x0f35a820u, 0x0008e228u}},
+    {{0x05bfec2fu, 0x163d2700u, 0x104bcbb4u, 0x0b09a75du, 0x192e3af1u, 0x06b6955eu, 0x07d2a686u, 0x0d42e2f8u, 0x000041edu}, {0x0a4013d2u, 0x08d2859fu, 0x1e10589cu, 0x1cc6e995u, 0x1c579791u, 0x1c24ab61u, 0x18d43abau, 0x0119433bu, 0x00302261u}},
+    {{0x0fec2effu, 0x1e17538eu, 0x1d6f9060u, 0x01d6cc8eu, 0x18b51ed5u, 0x13ba1e08u, 0x11ffa4f4u, 0x1486d20au, 0x0011891bu}, {0x0013d102u, 0x00f85911u, 0x10ec93f0u, 0x05f9c464u, 0x1cd0b3aeu, 0x0f2122b7u, 0x0ea73c4cu, 0x19d55429u, 0x001edb32u}},
+    {{0x0c2efea4u, 0x0db1852du, 0x067353d7u, 0x07d8776eu, 0x19072de4u, 0x1352c387u, 0x03ac0139u, 0x1db44fdeu, 0x00250f72u}, {0x03d1015du, 0x115e2772u, 0x07e8d079u, 0x1ff81985u, 0x1c7ea49eu, 0x0f887d38u, 0x1cfae007u, 0x10a7d655u, 0x000b54dbu}},
+    {{0x0efea33cu, 0x09850343u, 0x14c8096fu, 0x1cc873cfu, 0x0cb8b7c2u, 0x02e5f444u, 0x0c3cc3c5u, 0x15c29e49u, 0x0002a6ddu}, {0x01015cc5u, 0x158aa95cu, 0x19941ae1u, 0x0b081d23u, 0x08cd1ac0u, 0x1ff54c7cu, 0x146a1d7bu, 0x189987eau, 0x002dbd70u}},
+    {{0x1ea33bf2u, 0x1227d2beu, 0x1eff72deu, 0x1b0be255u, 0x0b663fb8u, 0x1df6b9dcu, 0x13a37387u, 0x19943289u, 0x00016163u}, {0x115cc40fu, 0x0ce7d9e0u, 0x0f5cb172u, 0x0cc4ae9du, 0x0a1f92cau, 0x04e486e4u, 0x0d036db9u, 0x14c7f3aau, 0x002f02eau}},
+    {{0x133bf1f9u, 0x0e650698u, 0x1aede05au, 0x152e5f4fu, 0x0f96f741u, 0x02bb1716u, 0x1ee35f28u, 0x0fad7e30u, 0x000ea5a7u}, {0x1cc40e08u, 0x10aaa606u, 0x136e43f6u, 0x12a231a3u, 0x05eedb41u, 0x002029aau, 0x01c38219u, 0x1eaea803u, 0x0021bea6u}},
+    {{0x0bf1f8b3u, 0x0d4faca0u, 0x1c296e18u, 0x14a3b69du, 0x1db6f02fu, 0x1f249c88u, 0x112d6781u, 0x1bc6b351u, 0x00177be4u}, {0x040e074eu, 0x11bfffffu, 0x1232b638u, 0x132cda55u, 0x17cee253u, 0x03b6a437u, 0x0f7979bfu, 0x129572e2u, 0x0018e869u}},
+    {{0x11f8b284u, 0x0415031du, 0x14cc8132u, 0x1ab067f5u, 0x0a1e3812u, 0x02692b9au, 0x1c926672u, 0x1210d056u, 0x000b4edeu}, {0x1e074d7du, 0x1afaa981u, 0x198fa31eu, 0x0d2028fdu, 0x0b679a70u, 0x00721526u, 0x04147acfu, 0x1c4b55ddu, 0x0025156fu}},
+    {{0x08b283c5u, 0x0c6654cdu, 0x1d44d377u, 0x03568d8au, 0x08608e96u, 0x00a3ade9u, 0x0bf08813u, 0x019388e7u, 0x0027c07cu}, {0x074d7c3cu, 0x12a957d2u, 0x111750d9u, 0x047a0368u, 0x0d2543edu, 0x023792d7u, 0x14b6592eu, 0x0cc89d4cu, 0x0008a3d2u}},
+    {{0x1283c42eu, 0x0b79326eu, 0x1d3dacf7u, 0x0d76a335u, 0x18c7e671u, 0x0bd2cb35u, 0x07a34ba0u, 0x0bf190b3u, 0x000e33b2u}, {0x1d7c3bd3u, 0x13967a30u, 0x111e7759u, 0x1a59edbdu, 0x1cbdec11u, 0x1708758au, 0x190395a0u, 0x026a9580u, 0x0022309cu}}};
 
-// 7-bit limb a of the 256-bit value v (little-endian words)
-__device__ __forceinline__ uint32_t rm_limb(const fr& v, int a) {
-    const int bit = 7 * a, w = bit >> 5, sh = bit & 31;
-    uint32_t x = v.v[w] >> sh;
-    if (sh > 25 && w + 1 < 8) x |= v.v[w + 1] << (32 - sh);
-    return x & 0x7fu;
+// the 32 balanced base-256 digits of v < 2^254 (little-endian words)
+__device__ __forceinline__ void rm_digits(const fr& v, int8_t* d) {
+    uint32_t c = 0;
+#pragma unroll
+    for (int a = 0; a < RM_LIMBS; a++) {
+        const uint32_t x = ((v.v[a >> 2] >> (8 * (a & 3))) & 0xffu) + c;
+        c = x >= 128u;
+        d[a] = (int8_t)(int32_t)(x - 256u * c);
+    }
 }
 
 // Limb planes in slabs: plane a of an x-by-mp operand (x = tp or cp, mp: multiples of
@@ -458,18 +452,20 @@ __device__ __forceinline__ size_t rm_slab_index(int a, uint32_t x_tiles, uint32_
     return ((((size_t)a * x_tiles + (row >> 5)) * k_steps + (k >> 5)) << 10) + ((row & 31) << 5) + (k & 31);
 }
 
-// Lp = limb planes of L[t][r] (t < nt, r < m), zero in the padding up to tp x mp
+// Lp = digit planes of L[t][r] (t < nt, r < m), zero in the padding up to tp x mp
 __global__ void k_rec_planes_l(const uint32_t* L, uint32_t nt, uint32_t m, uint32_t tp, uint32_t mp, int8_t* Lp) {
     const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= (size_t)tp * mp) return;
     const uint32_t t = (uint32_t)(e / mp), r = (uint32_t)(e % mp);
     fr v{};
     if (t < nt && r < m) v = f_load<FrP>(L + 8 * ((size_t)t * m + r));
+    int8_t d[RM_LIMBS];
+    rm_digits(v, d);
 #pragma unroll
-    for (int a = 0; a < RM_LIMBS; a++) Lp[rm_slab_index(a, tp >> 5, mp >> 5, t, r)] = (int8_t)rm_limb(v, a);
+    for (int a = 0; a < RM_LIMBS; a++) Lp[rm_slab_index(a, tp >> 5, mp >> 5, t, r)] = d[a];
 }
 
-// Vp = limb planes of V transposed, Vp[b][i][r] = limb b of V[r][i] (r < m, i < col),
+// Vp = digit planes of V transposed, Vp[b][i][r] = digit b of V[r][i] (r < m, i < col),
 // so that a lane's 8 bytes of B are contiguous; zero in the padding up to cp x mp
 __global__ void k_rec_planes_v(const uint32_t* V, uint32_t m, uint32_t col, uint32_t mp, uint32_t cp, int8_t* Vp) {
     const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -477,8 +473,10 @@ __global__ void k_rec_planes_v(const uint32_t* V, uint32_t m, uint32_t col, uint
     const uint32_t i = (uint32_t)(e % cp), r = (uint32_t)(e / cp);  // consecutive threads: consecutive i
     fr v{};
     if (i < col && r < m) v = f_load<FrP>(V + 8 * ((size_t)r * col + i));
+    int8_t d[RM_LIMBS];
+    rm_digits(v, d);
 #pragma unroll
-    for (int b = 0; b < RM_LIMBS; b++) Vp[rm_slab_index(b, cp >> 5, mp >> 5, i, r)] = (int8_t)rm_limb(v, b);
+    for (int b = 0; b < RM_LIMBS; b++) Vp[rm_slab_index(b, cp >> 5, mp >> 5, i, r)] = d[b];
 }
 
 // One K step of wave W: for every limb a, its A fragment once, then an MFMA for each of
@@ -563,9 +561,10 @@ __global__ __launch_bounds__(RM_WAVES * 64) void k_rec_gemm_mfma(const int8_t* L
     }
 }
 
-// Out[t][i] (Montgomery, < r) from the diagonals: sum_s D_s K_s in 29-bit columns (a
-// carry pass every 24 terms keeps them below 2^63), a Montgomery reduction by 2^261 (the
-// sum is below 2^290 < r 2^261, so the result is below 2r), one conditional subtraction.
+// Out[t][i] (Montgomery, < r) from the diagonals: sum_s |D_s| K_s, with K_s = 2^(8 s + 5)
+// mod r for D_s >= 0 and r - that otherwise, in 29-bit columns (a carry pass every 24
+// terms keeps them below 2^63), a Montgomery reduction by 2^261 (the sum is below
+// 2^289 < r 2^261, so the result is below 2r), one conditional subtraction.
 __global__ void k_rec_fold(const int* D, uint32_t tp, uint32_t cp, uint32_t nt, uint32_t col, uint32_t* Out) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x, t = blockIdx.y;
     if (i >= col || t >= nt) return;
@@ -574,9 +573,11 @@ __global__ void k_rec_fold(const int* D, uint32_t tp, uint32_t cp, uint32_t nt, 
     for (int k = 0; k < 19; k++) c[k] = 0;
     for (int s0 = 0; s0 < RM_DIAG; s0 += 24) {
         for (int s = s0; s < s0 + 24 && s < RM_DIAG; s++) {
-            const uint64_t d = (uint32_t)D[((size_t)s * tp + t) * cp + i];
+            const int32_t ds = D[((size_t)s * tp + t) * cp + i];
+            const uint64_t d = (uint32_t)(ds < 0 ? -ds : ds);
+            const bool neg = ds < 0;
 #pragma unroll
-            for (int k = 0; k < 9; k++) c[k] += d * RM_KS[s][k];
+            for (int k = 0; k < 9; k++) c[k] += d * (neg ? RM_KS[s][1][k] : RM_KS[s][0][k]);  // uniform loads
         }
 #pragma unroll
         for (int k = 0; k < 10; k++) {  // carry pass: columns 0..9 back below 2^29 (+ carry)
