@@ -33,6 +33,8 @@ MAC32_PER_PAIR_WINDOW = 10 * 136
 # i.e. one v_mad_u64_u32 (32x32->64 MAC) per lane per clock.  (A dependent-chain
 # microbenchmark of v_mad_u64_u32 alone reaches 25.0 T/s on the GPU.)
 PEAK_MAC32 = 256 * 4 * 16 * 2.4e9
+# dense int8 MFMA peak of MI355X: 5.0 POP/s = 2.5 x 10^15 multiply-accumulates per second
+PEAK_INT8_MAC = 2.5e15
 # HBM bytes per launch pair (k_fb_sums + k_fb_reduce) from two rocprofv3 counter passes
 # over `bench.py --roofline-only`: FETCH_SIZE (22,002,553 + 1,230,987) KiB x 2.0 (a 1 GiB
 # device copy reads as 524,322 KiB) + WRITE_SIZE (1,739,867 + 769) KiB x 1.0 (the copy
@@ -211,6 +213,16 @@ def main():
         pin_out.free()
         recovery[label] = {"ms": timing["pageable"], "ms_pinned_host_buffers": timing["pinned"],
                            "kernel_ms_per_call": ks}
+        erased = sum(1 for i in range(geo.row_n) if i not in set(ids))
+        if erased and ks.get("recover_gemm"):
+            # the interpolation GEMM on the matrix cores: 32 x 32 int8 digit-plane products
+            # of (erased rows x ROW) by (ROW x COL), padded to multiples of 32
+            pad = lambda v: (v + 31) // 32 * 32  # noqa: E731
+            macs = 32 * 32 * pad(erased) * pad(geo.row_n) * pad(geo.col_n)
+            recovery[label]["mfma_gemm"] = {
+                "int8_macs": macs, "ms": ks["recover_gemm"],
+                "achieved_TMAC_s": round(macs / (ks["recover_gemm"] * 1e-3) / 1e12, 1),
+                "frac_of_dense_int8_peak": round(macs / (ks["recover_gemm"] * 1e-3) / PEAK_INT8_MAC, 3)}
     recovery["note"] = ("zgr_recover over host arrays (rows in and raw bytes out cross PCIe inside the timed "
                         "call): pageable numpy arrays, and page-locked (hipHostMalloc) ones; median of 5")
 
