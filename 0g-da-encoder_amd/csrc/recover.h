@@ -519,13 +519,33 @@ __device__ __forceinline__ void rm_dispatch(int w, const uint64_t (*As)[32][4], 
     }
 }
 
+// Copy the K-step k0 slabs of the tile into one LDS buffer pair: 64 slabs of 1 KB, four
+// per wave, each one global_load_lds_dwordx4 (lane l's 16 bytes land at base + 16 l).  The
+// copies complete asynchronously (vmcnt), so a buffer fills while the other is read.
+__device__ __forceinline__ void rm_stage(const int8_t* Lp, const int8_t* Vp, uint32_t tp, uint32_t cp,
+                                         uint32_t k_steps, uint32_t t0, uint32_t i0, uint32_t k0, int w, int l,
+                                         uint64_t (*As)[32][4], uint64_t (*Bs)[32][4]) {
+#pragma unroll
+    for (int jj = 0; jj < 4; jj++) {
+        const int j = w + RM_WAVES * jj;  // slab: A digit j (j < 32) or B digit j - 32
+        if (j < RM_LIMBS)
+            __builtin_amdgcn_global_load_lds((const void*)(Lp + rm_slab_index(j, tp >> 5, k_steps, t0, k0) + 16 * l),
+                                             (void __attribute__((address_space(3)))*)&As[j][0][0], 16, 0, 0);
+        else
+            __builtin_amdgcn_global_load_lds(
+                (const void*)(Vp + rm_slab_index(j - RM_LIMBS, cp >> 5, k_steps, i0, k0) + 16 * l),
+                (void __attribute__((address_space(3)))*)&Bs[j - RM_LIMBS][0][0], 16, 0, 0);
+    }
+}
+
 // D[s][t][i] for one 32 x 32 tile (t0 = 32 blockIdx.y, i0 = 32 blockIdx.x); tp, cp and
-// mp multiples of 32.  Wave w accumulates the diagonals s = w + 16 q; every K step stages
-// the 37 limb slabs of both operands (2 x 37 KB) in LDS, and a wave keeps its A fragment
-// of limb a while it runs the b = s - a its diagonals need.
+// mp multiples of 32.  Wave w accumulates the diagonals s = w + 16 q; a wave keeps its A
+// fragment of digit a while it runs the b = s - a its diagonals need.  The 32 digit slabs
+// of both operands of a K step (64 KB) sit in one of two LDS buffer pairs (128 KB): the
+// next step's slabs are copied into the other pair while this step's MFMAs run.
 __global__ __launch_bounds__(RM_WAVES * 64) void k_rec_gemm_mfma(const int8_t* Lp, const int8_t* Vp, uint32_t tp,
                                                                  uint32_t mp, uint32_t cp, int* D) {
-    __shared__ uint64_t As[RM_LIMBS][32][4], Bs[RM_LIMBS][32][4];
+    __shared__ uint64_t As0[RM_LIMBS][32][4], Bs0[RM_LIMBS][32][4], As1[RM_LIMBS][32][4], Bs1[RM_LIMBS][32][4];
     const uint32_t t0 = blockIdx.y * 32, i0 = blockIdx.x * 32;
     const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
     const int row = l & 31, kh = l >> 5;
@@ -535,19 +555,17 @@ __global__ __launch_bounds__(RM_WAVES * 64) void k_rec_gemm_mfma(const int8_t* L
 #pragma unroll
         for (int k = 0; k < 16; k++) acc[q][k] = 0;
     const uint32_t k_steps = mp >> 5;
-    for (uint32_t k0 = 0; k0 < mp; k0 += 32) {
-        for (uint32_t c = threadIdx.x; c < 2u * RM_LIMBS * 128; c += RM_WAVES * 64) {
-            // one 8-byte word of a slab: word j = 4 row + quarter of the 1 KB slab
-            const uint32_t which = c / (RM_LIMBS * 128), rem = c % (RM_LIMBS * 128);
-            const uint32_t a = rem >> 7, j = rem & 127;
-            if (which == 0)
-                As[a][j >> 2][j & 3] = ((const uint64_t*)Lp)[rm_slab_index(a, tp >> 5, k_steps, t0, k0) / 8 + j];
-            else
-                Bs[a][j >> 2][j & 3] = ((const uint64_t*)Vp)[rm_slab_index(a, cp >> 5, k_steps, i0, k0) / 8 + j];
-        }
+    rm_stage(Lp, Vp, tp, cp, k_steps, t0, i0, 0, w, l, As0, Bs0);
+    for (uint32_t ks = 0; ks < k_steps; ks += 2) {  // block-uniform; k_steps >= 1
+        __builtin_amdgcn_s_waitcnt(0);  // this wave's copies into pair 0
+        __syncthreads();                // everyone's, and pair 1 is no longer read
+        if (ks + 1 < k_steps) rm_stage(Lp, Vp, tp, cp, k_steps, t0, i0, (ks + 1) * 32, w, l, As1, Bs1);
+        rm_dispatch<0>(w, As0, Bs0, acc, row, kh);  // wave-uniform
+        if (ks + 1 >= k_steps) break;
+        __builtin_amdgcn_s_waitcnt(0);
         __syncthreads();
-        rm_dispatch<0>(w, As, Bs, acc, row, kh);  // wave-uniform
-        __syncthreads();
+        if (ks + 2 < k_steps) rm_stage(Lp, Vp, tp, cp, k_steps, t0, i0, (ks + 2) * 32, w, l, As0, Bs0);
+        rm_dispatch<0>(w, As1, Bs1, acc, row, kh);
     }
 #pragma unroll
     for (int q = 0; q < RM_ACC; q++) {
