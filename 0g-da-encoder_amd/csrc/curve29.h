@@ -96,14 +96,15 @@ __device__ __forceinline__ x29 x29_add_t(const x29& p, const x29& q) {
 __device__ __forceinline__ x29 x29_add(const x29& p, const x29& q) { return x29_add_t<false>(p, q); }
 __device__ __forceinline__ x29 x29_add_lazy(const x29& p, const x29& q) { return x29_add_t<true>(p, q); }
 
-// p + q for affine q (madd-2008-s, 8M + 2S); q is never infinity.  Lanes whose
-// result is discarded pass live = false so that they never enter the doubling branch.
-__device__ __forceinline__ x29 x29_madd(const x29& p, const a29& q, bool live = true) {
+// p + q for affine q (madd-2008-s, 8M + 2S) with every exceptional case handled; q is
+// never infinity.  The bucket loop runs x29_madd_unchecked and comes here only for the
+// lanes that one flags.
+__device__ __forceinline__ x29 x29_madd(const x29& p, const a29& q) {
     const bool pinf = x29_is_inf(p);
     fq29 P = q_sub_lazy(q_mul(q.x, p.ZZ), p.X);
     fq29 R = q_sub_lazy(q_mul(q.y, p.ZZZ), p.Y);
-    // equal x: p == q (double) or p == -q (infinity); one test on the common path
-    if (live && !pinf && q_is_zero4(P)) return q_is_zero4(R) ? x29_dbl(p) : x29_inf();
+    // equal x: p == q (double) or p == -q (infinity)
+    if (!pinf && q_is_zero4(P)) return q_is_zero4(R) ? x29_dbl(p) : x29_inf();
     x29 r;
     fq29 PP = q_sqr(P);
     r.ZZ = q_mul(p.ZZ, PP);
