@@ -363,6 +363,7 @@ constexpr int RM_DIAG = 2 * RM_LIMBS - 1;
 constexpr int RM_WAVES = 16;                              // k_rec_gemm_mfma: 1024 threads
 constexpr int RM_ACC = (RM_DIAG + RM_WAVES - 1) / RM_WAVES;  // diagonals per wave (s = w + 16 q)
 constexpr uint32_t RM_MAX_M = 2048;                       // keeps D_s below 2^31
+static_assert(2 * RM_LIMBS == 4 * RM_WAVES, "rm_stage: four slabs per wave cover both operands' digits");
 
 typedef int rm_v16i __attribute__((ext_vector_type(16)));
 typedef int rm_v4i __attribute__((ext_vector_type(4)));
@@ -479,10 +480,6 @@ __global__ void k_rec_planes_v(const uint32_t* V, uint32_t m, uint32_t col, uint
     for (int b = 0; b < RM_LIMBS; b++) Vp[rm_slab_index(b, cp >> 5, mp >> 5, i, r)] = d[b];
 }
 
-// One K step of wave W: for every limb a, its A fragment once, then an MFMA for each of
-// the wave's diagonals s = W + 16 q that pairs it with a limb b = s - a.  W, a and q are
-// compile-time constants after unrolling, so the wave runs exactly its MFMAs as
-// straight-line code (no branch between them for the hazard checks to guard).
 // a lane's 16-byte fragment of slab x
 __device__ __forceinline__ rm_v4i rm_frag(const uint64_t (*S)[32][4], int x, int row, int kh) {
     const uint64_t lo = S[x][row][2 * kh], hi = S[x][row][2 * kh + 1];
@@ -490,6 +487,10 @@ __device__ __forceinline__ rm_v4i rm_frag(const uint64_t (*S)[32][4], int x, int
     return f;
 }
 
+// One K step of wave W: for every digit a, its A fragment once, then an MFMA for each of
+// the wave's diagonals s = W + 16 q that pairs it with a digit b = s - a.  W, a and q are
+// compile-time constants after unrolling, so the wave runs exactly its MFMAs as
+// straight-line code (no branch between them for the hazard checks to guard).
 template <int W>
 __device__ __forceinline__ void rm_step(const uint64_t (*As)[32][4], const uint64_t (*Bs)[32][4], rm_v16i* acc,
                                         int row, int kh) {
@@ -502,7 +503,7 @@ __device__ __forceinline__ void rm_step(const uint64_t (*As)[32][4], const uint6
             if (s < RM_DIAG && b >= 0 && b < RM_LIMBS)
                 acc[q] = __builtin_amdgcn_mfma_i32_32x32x32_i8(af, rm_frag(Bs, b, row, kh), acc[q], 0, 0, 0);
         }
-        // a compiler barrier per limb: the fragments a wave reuses are read again from LDS
+        // a compiler barrier per digit: the fragments a wave reuses are read again from LDS
         // rather than kept in registers, where they do not fit beside the accumulators
         asm volatile("" ::: "memory");
     }
