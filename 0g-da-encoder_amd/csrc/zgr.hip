@@ -343,7 +343,7 @@ int build_work_lists(zgr_ctx* c) {
         for (const char* q = mode; *q;) {
             given.push_back((int)strtol(q, (char**)&q, 10));
             if (*q == ',') q++;
-            else if (*q) return ZGR_ERR_INVALID_ARG;
+            else if (*q) return fail(ZGR_ERR_INVALID_ARG, "ZGR_GROUP_PLAN=%s: expected none or levels like 2,2,1,0", mode);
         }
     for (int v = 0; v < 2; v++) {
         const int lines = c->nlines - v;

@@ -47,6 +47,12 @@ def test_msm_path_in_use(ctx):
     assert (merges > 0) == (ctx.mode in ("fixed_base", "fixed_base_mixed_plan"))
 
 
+def test_group_plan_rejected():
+    """A malformed ZGR_GROUP_PLAN fails context creation with the variable in the message."""
+    with pytest.raises(Z.ZgrError, match="ZGR_GROUP_PLAN"):
+        Z.make_context(GEO, group_plan="2,x,1")
+
+
 @pytest.fixture(scope="module")
 def params():
     return oracle_params(GEO.log_col, GEO.log_row, GEO.coset_n)
