@@ -1563,8 +1563,15 @@ int zgr_recover(zgr_ctx* c, const uint32_t* row_ids, const uint8_t* rows, size_t
         if (nt) {
             const uint32_t chunks = (m + REC_CHUNK - 1) / REC_CHUNK;
             LAUNCH(c, st, "recover", k_rec_prod, nt, 256, 0, d_t, d_u, m, 0, 0, d_A);
-            LAUNCH(c, st, "recover_lagrange", k_rec_lagrange, blocks((size_t)nt * chunks, 64), 64, 0, d_t, nt, d_u, d_b,
-                   m, d_L);
+            uint32_t *d_t29, *d_u29, *d_b29;  // targets, points and B_r in the 29-bit form
+            CHECK(alloc(d_t29, (size_t)nt * 36));
+            CHECK(alloc(d_u29, (size_t)m * 36));
+            CHECK(alloc(d_b29, (size_t)m * 36));
+            LAUNCH(c, st, "recover_lagrange", k_rec_to29, blocks(nt, 256), 256, 0, d_t, nt, d_t29);
+            LAUNCH(c, st, "recover_lagrange", k_rec_to29, blocks(m, 256), 256, 0, d_u, m, d_u29);
+            LAUNCH(c, st, "recover_lagrange", k_rec_to29, blocks(m, 256), 256, 0, d_b, m, d_b29);
+            LAUNCH(c, st, "recover_lagrange", k_rec_lagrange, blocks((size_t)nt * chunks, 64), 64, 0, d_t29, nt, d_u29,
+                   d_b29, m, d_L);
         }
         if (e0) {
             const char* gm = getenv("ZGR_REC_GEMM");
