@@ -47,16 +47,9 @@ __global__ void k_ver_scalars(const uint8_t* rows, const uint8_t* alphas, const 
 // out = sum of count canonical affine points (64 zero bytes = infinity), canonical
 __global__ void k_ver_sum(const uint32_t* pts, int count, uint32_t* out) {
     if (threadIdx.x || blockIdx.x) return;
-    g1_xyzz acc = xyzz_inf();
-    for (int q = 0; q < count; q++) {
-        fq x = f_load<FqP>(pts + 16 * q), y = f_load<FqP>(pts + 16 * q + 8);
-        if (f_is_zero(x) && f_is_zero(y)) continue;
-        g1_aff a;
-        a.x = f_to_mont(x);
-        a.y = f_to_mont(y);
-        acc = xyzz_madd(acc, a);
-    }
-    out_store_canon(out, acc);
+    x29 acc = x29_inf();  // in the 29-bit form: the one inversion of the output is its cost
+    for (int q = 0; q < count; q++) acc = x29_add(acc, ec29_from_canon(pts + 16 * q));
+    x29_store_canon(out, acc);
 }
 
 }  // namespace zgr
