@@ -481,6 +481,7 @@ int finish_blob(zgr_ctx* c, Slot& s) {
     if (!s.busy) return ZGR_OK;
     s.busy = false;
     const zgr_encoded* out = s.out;
+    s.out = nullptr;
     if (out->data)
         for (int cs = 0; cs < c->coset_n; cs++) {  // coset by coset, as the staging fills
             HIPCHK(hipEventSynchronize(s.ev_data[cs]));
@@ -504,23 +505,43 @@ int finish_blob(zgr_ctx* c, Slot& s) {
     return ZGR_OK;
 }
 
+// Abandon whatever the slots hold: wait for the device, then mark both slots free so
+// that no later call copies their staging into output buffers of a call that has
+// returned (the slot's `out` points into the caller's array of that call).
+void drop_slots(zgr_ctx* c) {
+    (void)sync_all(c);
+    for (auto& s : c->slots) {
+        s.busy = false;
+        s.out = nullptr;
+    }
+}
+
 // encode nblobs blobs already in device memory, two in flight
 int encode_pipeline(zgr_ctx* c, int nblobs, const uint8_t* const* raws, const size_t* lens,
                     const zgr_encoded* outs) {
     for (int cs = 0; cs < c->coset_n; cs++) CHECK(ensure_params(c, cs));
+    if (c->slots[0].busy || c->slots[1].busy) drop_slots(c);  // never inherit another call's slot
     int first_err = ZGR_OK;
     for (int i = 0; i < nblobs; i++) {
         Slot& s = c->slots[i & 1];
         int rc = finish_blob(c, s);  // the blob issued two steps ago
         if (rc && !first_err) first_err = rc;
-        CHECK(issue_blob(c, s, raws[i], lens[i], &outs[i]));
+        rc = issue_blob(c, s, raws[i], lens[i], &outs[i]);
+        if (rc) {  // the other slot may still be in flight into this call's outputs
+            drop_slots(c);
+            return rc;
+        }
     }
     for (int k = 0; k < 2; k++) {
         Slot& s = c->slots[(nblobs + k) & 1];
         int rc = finish_blob(c, s);
         if (rc && !first_err) first_err = rc;
     }
-    CHECK(sync_all(c));
+    int rc = sync_all(c);
+    if (rc) {
+        drop_slots(c);
+        return rc;
+    }
     return first_err;
 }
 
@@ -586,7 +607,9 @@ struct zgr_server {
     std::deque<Request*> queue;   // submitted, waiting for admission
     std::deque<Request*> staged;  // admitted and uploading / uploaded, waiting for a slot
     Request* in_slot[2] = {nullptr, nullptr};
-    std::unordered_map<uint64_t, int> done;  // ticket -> final status, until polled
+    // ticket -> final status and, for a failure, the worker thread's error message
+    // (zgr_last_error is per thread), until polled
+    std::unordered_map<uint64_t, std::pair<int, std::string>> done;
     std::unordered_map<uint64_t, bool> pending;  // submitted tickets not yet completed
     uint64_t next_id = 1, n_completed = 0;
     int n_admitted = 0;
@@ -641,7 +664,7 @@ void server_loop(zgr_server* sv) {
                 sv->in_slot[k] = nullptr;
                 sv->n_admitted--;
                 sv->n_completed++;
-                sv->done[r->id] = st;
+                sv->done[r->id] = {st, st ? g_err : std::string()};
                 sv->pending.erase(r->id);
             }
             sv->cv_done.notify_all();
@@ -672,8 +695,8 @@ void server_loop(zgr_server* sv) {
                     sv->arena.release(r->off, r->len);
                     sv->n_admitted--;
                     sv->n_completed++;
-                    sv->done[r->id] = st;
-                sv->pending.erase(r->id);
+                    sv->done[r->id] = {st, g_err};
+                    sv->pending.erase(r->id);
                 }
                 sv->cv_done.notify_all();
                 delete r;
@@ -1160,9 +1183,10 @@ int zgr_server_poll(zgr_server* sv, uint64_t ticket, int timeout_ms) {
         sv->cv_done.wait(lk, ready);
     else if (!sv->cv_done.wait_for(lk, std::chrono::milliseconds(timeout_ms), ready))
         return 1;
-    int st = sv->done[ticket];
+    const auto res = sv->done[ticket];
     sv->done.erase(ticket);
-    return st;
+    if (res.first != ZGR_OK) g_err = res.second;  // the worker's message, on the polling thread
+    return res.first;
 }
 
 void zgr_server_stats(zgr_server* sv, uint64_t* queued, uint64_t* admitted, uint64_t* completed) {

@@ -404,6 +404,8 @@ class Context:
         """zgr_verify_msm: row_ids (strictly increasing), their rows (canonical bytes, one
         buffer or a list) and one canonical alpha (int) per row -> 64 B canonical point."""
         ids = np.ascontiguousarray(row_ids, dtype=np.uint32).reshape(-1)
+        if len(alphas) != ids.size:
+            raise ValueError("%d alphas for %d rows" % (len(alphas), ids.size))
         if isinstance(rows, np.ndarray):
             arr = np.ascontiguousarray(rows, dtype=np.uint8).reshape(-1)
         else:
@@ -662,26 +664,72 @@ def recover_from_da_slice(ctx, slices):
         raise
 
 
+class VerifierError(ValueError):
+    """A VerifierError of EncodedSlice::verify (crates/encoder/src/encoder/error.rs);
+    str() is the reference's Debug text."""
+
+
 class DeferredVerifier:
     """The MSM half of DeferredVerifier (crates/amt/src/deferred_verification.rs:34-222)
     on the device, for the rows of one encoded blob.
 
-    record_slice() takes EncodedSlice bytes, draws a random alpha for the row and
-    keeps alpha * (commitment - sum_d proof[d].commitment), as record_msm does after
+    record_slice() takes EncodedSlice bytes and runs the checks EncodedSlice::verify
+    makes before its AMT proof check (crates/encoder/src/encoder/slice.rs:72-95,
+    crates/encoder/src/amt/slice.rs:51-88, and verify_amt_proof's length and position
+    checks, crates/amt/src/verify_params.rs:149-159), raising VerifierError with the
+    reference's text; then it draws a random alpha for the row and keeps
+    alpha * (commitment - sum_d proof[d].commitment), as record_msm does after
     verify_amt_proof's pairing checks (crates/amt/src/verify_params.rs:176-186).
     fast_check_msm() (fast_check_msm_gpu, deferred_verification.rs:203-222) compares the
     device MSM of the recorded rows over the resident basis with the device MSM of the
-    recorded answers.  The pairing half (fast_check_pairing) stays with the caller."""
+    recorded answers.  The pairing half (fast_check_pairing) and the Merkle proof stay
+    with the caller."""
 
-    def __init__(self, ctx, rng=None):
+    def __init__(self, ctx, commitment, rng=None):
+        """commitment: the authoritative commitment, canonical affine x || y (64 B, as
+        EncodedBlob.get_affine_commitment returns it)."""
         import secrets
 
         self.ctx = ctx
+        self.commitment = bytes(commitment)
+        if len(self.commitment) != 64:
+            raise ValueError("commitment must be 64 bytes (canonical x || y)")
         self._rand = rng if rng is not None else (lambda: secrets.randbelow(R_MOD))
         self._rows = {}
 
+    def check_slice(self, f):
+        """The reference's checks on one parsed slice, in its order; raises VerifierError."""
+        g = self.ctx.geo
+        idx, amt_index = f["index"], f["amt_index"]
+        if idx != amt_index:
+            raise VerifierError("UnmatchedAMTIndex { row_index: %d, amt_index: %d }" % (idx, amt_index))
+        if idx != f["leaf_index"]:
+            raise VerifierError("UnmatchedMerkleIndex { row_index: %d, merkle_index: %d }" % (idx, f["leaf_index"]))
+        if _g1_from_wire(f["commitment"]) != _g1_from_canon(self.commitment):
+            raise VerifierError("AMT(IncorrectCommitment)")
+        if len(f["row"]) != g.col_n:
+            raise VerifierError("AMT(IncorrectRowSize { actual: %d, expected: %d })" % (len(f["row"]), g.col_n))
+        if amt_index >= g.row_encoded:
+            raise VerifierError("AMT(RowIndexOverflow { actual: %d, expected_max: %d })" % (amt_index, g.row_encoded))
+        coset, local = divmod(amt_index, g.row_n)
+        if local != f["row_index"]:
+            raise VerifierError("AMT(UnmatchedCosetIndex { coset_index: %d, local_index: %d, amt_index: %d })"
+                                % (coset, local, f["row_index"]))
+        depth = len(f["amt_proof"])
+        # verify_amt_proof: batch = basis.len() >> depth must be the row length, and the
+        # row's bit-reversed index must name one of the 2^depth batches
+        err = None
+        if depth > g.log_n or (g.n >> depth) != g.col_n:
+            err = "UnexpectedDataLength"
+        elif f["row_index"] >= (1 << depth):
+            err = "IncorrectPosition"
+        if err:
+            raise VerifierError("AMT(IncorrectProof { coset_index: %d, amt_index: %d, error: %s })"
+                                % (coset, f["row_index"], err))
+
     def record_slice(self, slice_bytes):
         f = wire.parse_encoded_slice(slice_bytes, self.ctx.geo.coset_n)
+        self.check_slice(f)
         idx = f["index"]
         if idx in self._rows:
             raise ValueError("row %d recorded twice" % idx)
@@ -708,6 +756,13 @@ class DeferredVerifier:
         sm = b"".join((k * R256 % R_MOD).to_bytes(32, "little") for k in scal)
         rhs = self.ctx.msm(np.frombuffer(bm, dtype=np.uint64), np.frombuffer(sm, dtype=np.uint64))
         return lhs == bytes(rhs)
+
+
+def _g1_from_canon(b):
+    """(x, y) ints of a canonical affine x || y, None for infinity (64 zero bytes)."""
+    if not any(b):
+        return None
+    return int.from_bytes(bytes(b[:32]), "little"), int.from_bytes(bytes(b[32:64]), "little")
 
 
 def _g1_from_wire(b):

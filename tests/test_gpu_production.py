@@ -224,7 +224,7 @@ def test_deferred_verify_production(ctx, blob):
     raw, out = blob
     eb = Z.EncodedBlob(GEO, out)
     rows = eb.encoded_slices()
-    v = Z.DeferredVerifier(ctx)
+    v = Z.DeferredVerifier(ctx, bytes(out.commitment))
     for i in range(GEO.row_encoded):
         v.record_slice(rows[i].tobytes())
     t = time.time()
@@ -234,7 +234,7 @@ def test_deferred_verify_production(ctx, blob):
     r = bytearray(b"".join(f["row"]))
     r[32 * 17] ^= 1
     f["row"] = bytes(r)
-    v = Z.DeferredVerifier(ctx)
+    v = Z.DeferredVerifier(ctx, bytes(out.commitment))
     v.record_slice(wire.serialize_encoded_slice(f))
     v.record_slice(rows[0].tobytes())
     assert not v.fast_check_msm()

@@ -31,6 +31,13 @@ def slices(ctx):
     return [blob.encoded_slice(i) for i in range(GEO.row_encoded)]
 
 
+def _auth(slices):
+    """The authoritative commitment (canonical x || y) from a slice's wire form."""
+    b = bytearray(wire.parse_encoded_slice(slices[0])["commitment"])
+    b[63] &= 0x3F
+    return bytes(b)
+
+
 def test_verify_msm_matches_oracle(ctx, slices):
     params = list(oracle_params(GEO.log_col, GEO.log_row, GEO.coset_n))
     rng = np.random.default_rng(8)
@@ -45,11 +52,11 @@ def test_verify_msm_matches_oracle(ctx, slices):
 
 
 def test_deferred_verifier_accepts_honest_rows(ctx, slices):
-    v = Z.DeferredVerifier(ctx)
+    v = Z.DeferredVerifier(ctx, _auth(slices))
     for s in slices:
         v.record_slice(s)
     assert v.fast_check_msm()
-    v = Z.DeferredVerifier(ctx)
+    v = Z.DeferredVerifier(ctx, _auth(slices))
     for i in range(5, GEO.row_encoded, 11):
         v.record_slice(slices[i])
     assert v.fast_check_msm()
@@ -64,7 +71,7 @@ def _tamper_row(s, pos):
 
 
 def test_deferred_verifier_rejects_tampering(ctx, slices):
-    v = Z.DeferredVerifier(ctx)
+    v = Z.DeferredVerifier(ctx, _auth(slices))
     for i, s in enumerate(slices):
         v.record_slice(_tamper_row(s, 70) if i == 77 else s)
     assert not v.fast_check_msm()
@@ -72,7 +79,7 @@ def test_deferred_verifier_rejects_tampering(ctx, slices):
     f = wire.parse_encoded_slice(slices[20])
     g = wire.parse_encoded_slice(slices[21])
     f["amt_proof"] = [g["amt_proof"][0]] + list(f["amt_proof"][1:])
-    v = Z.DeferredVerifier(ctx)
+    v = Z.DeferredVerifier(ctx, _auth(slices))
     v.record_slice(wire.serialize_encoded_slice(f))
     v.record_slice(slices[30])
     assert not v.fast_check_msm()
@@ -109,7 +116,7 @@ def test_device_slices_pass_reference_verification(ctx, slices):
     for i in (3, GEO.row_n + 40, GEO.row_encoded - 2):
         assert V.verify_slice(wire.parse_encoded_slice(slices[i]), vps, com, root, GEO) is None, i
     col = V.Collector(True)
-    dv = Z.DeferredVerifier(ctx)
+    dv = Z.DeferredVerifier(ctx, _auth(slices))
     for i in (0, 17, GEO.row_n + 1, 2 * GEO.row_n + 50):
         assert V.verify_slice(wire.parse_encoded_slice(slices[i]), vps, com, root, GEO, col) is None
         dv.record_slice(slices[i])
