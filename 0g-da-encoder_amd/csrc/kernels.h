@@ -105,7 +105,8 @@ __global__ void k_powers(uint32_t* tw, const uint32_t* consts, int idx, uint32_t
     f_store(tw + 8 * (size_t)k, r);
 }
 
-// scale[p] = N^-1 * cf^bitrev(p): coefficient bitrev(p) sits at DIF position p
+// scale[p] = N^-1 * cf^bitrev(p) (2^256 form; the context converts it to the 2^261 form
+// the NTT passes multiply by): coefficient bitrev(p) sits at DIF position p
 __global__ void k_coset_scale_table(uint32_t* scale, const uint32_t* consts, int coset, int log_n) {
     uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= (1u << log_n)) return;
@@ -148,70 +149,6 @@ __global__ void k_pack(const uint8_t* raw, size_t len, int log_col, int log_row,
     }
     uint32_t r = i >> log_col, c = i & ((1u << log_col) - 1);
     f_store(pts + 8 * (size_t)((c << log_row) | r), f_to_mont(a));
-}
-
-// ---------------------------------------------------------------------------
-// radix-2 NTT stages (global memory).  tw holds omega^k of the context domain;
-// a stage of half-width 2^(s-1) uses tw[j << tw_shift].
-// ---------------------------------------------------------------------------
-// decimation in frequency: natural in, bit-reversed out
-__global__ void k_ntt_dif_stage(uint32_t* a, const uint32_t* tw, int s, int tw_shift, uint32_t half_n) {
-    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= half_n) return;
-    uint32_t half = 1u << (s - 1);
-    uint32_t j = t & (half - 1);
-    uint32_t i0 = ((t >> (s - 1)) << s) | j, i1 = i0 + half;
-    fr u = f_load<FrP>(a + 8 * (size_t)i0), v = f_load<FrP>(a + 8 * (size_t)i1);
-    fr w = f_load<FrP>(tw + 8 * (size_t)(j << tw_shift));
-    f_store(a + 8 * (size_t)i0, f_add(u, v));
-    f_store(a + 8 * (size_t)i1, f_mul(f_sub(u, v), w));
-}
-
-// decimation in time: bit-reversed in, natural out
-__global__ void k_ntt_dit_stage(uint32_t* a, const uint32_t* tw, int s, int tw_shift, uint32_t half_n) {
-    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= half_n) return;
-    uint32_t half = 1u << (s - 1);
-    uint32_t j = t & (half - 1);
-    uint32_t i0 = ((t >> (s - 1)) << s) | j, i1 = i0 + half;
-    fr u = f_load<FrP>(a + 8 * (size_t)i0);
-    fr v = f_mul(f_load<FrP>(a + 8 * (size_t)i1), f_load<FrP>(tw + 8 * (size_t)(j << tw_shift)));
-    f_store(a + 8 * (size_t)i0, f_add(u, v));
-    f_store(a + 8 * (size_t)i1, f_sub(u, v));
-}
-
-// out[p] = in[p] * scale[p]
-__global__ void k_mul_table(uint32_t* out, const uint32_t* in, const uint32_t* scale, uint32_t n) {
-    uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n) return;
-    f_store(out + 8 * (size_t)p, f_mul(f_load<FrP>(in + 8 * (size_t)p), f_load<FrP>(scale + 8 * (size_t)p)));
-}
-
-// out[p] = in[p] * c (c = consts[idx])
-__global__ void k_mul_const(uint32_t* out, const uint32_t* in, const uint32_t* consts, int idx, uint32_t n) {
-    uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n) return;
-    f_store(out + 8 * (size_t)p, f_mul(f_load<FrP>(in + 8 * (size_t)p), f_load<FrP>(consts + 8 * idx)));
-}
-
-// a[p] *= 1 / 2^(log_n_ctx - shift) = N_ctx^-1 * 2^shift (size_inv of a sub-domain)
-__global__ void k_mul_ninv_sub(uint32_t* a, const uint32_t* consts, int shift, uint32_t n) {
-    uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n) return;
-    fr f = f_load<FrP>(consts + 8 * K_NINV);
-    for (int i = 0; i < shift; i++) f = f_dbl(f);
-    f_store(a + 8 * (size_t)p, f_mul(f_load<FrP>(a + 8 * (size_t)p), f));
-}
-
-// out[p] = in[bitrev(p)]
-__global__ void k_bitrev_copy(uint32_t* out, const uint32_t* in, int log_n) {
-    uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= (1u << log_n)) return;
-    uint32_t q = bitrev_bits(p, log_n);
-    const uint4* s = (const uint4*)(in + 8 * (size_t)q);
-    uint4* d = (uint4*)(out + 8 * (size_t)p);
-    d[0] = s[0];
-    d[1] = s[1];
 }
 
 // ---------------------------------------------------------------------------

@@ -38,6 +38,7 @@
 #include "recover.h"
 #include "verify.h"
 #include "params_pp.h"
+#include "ntt.h"
 
 using namespace zgr;
 
@@ -164,7 +165,16 @@ struct zgr_ctx {
     uint32_t* consts = nullptr;   // K_COUNT Fr
     uint32_t* tw = nullptr;       // n/2 Fr: omega^k
     uint32_t* tw_inv = nullptr;   // n/2 Fr: omega^-k
-    uint32_t* scale[3] = {nullptr, nullptr, nullptr};  // coset scale tables (DIF order)
+    // NTT tables in the 2^261 form (k_ntt_tables): omega^+-k (n/2 each), the stage
+    // twiddles of the in-LDS sub-transforms, and 2^k / n (k <= log_n)
+    uint32_t* tw29 = nullptr;
+    uint32_t* tw29_inv = nullptr;
+    uint32_t* stw = nullptr;
+    uint32_t* stw_inv = nullptr;
+    uint32_t* ninv29 = nullptr;
+    // coset scale tables, 2^261 form: scale[c][p] = cf_c^bitrev(p) / n, the factor of the
+    // coefficient at DIF position p
+    uint32_t* scale[3] = {nullptr, nullptr, nullptr};
 
     // resident params
     uint32_t* bases[3] = {nullptr, nullptr, nullptr};
@@ -241,18 +251,109 @@ int sync_all(zgr_ctx* c) {
     return ZGR_OK;
 }
 
-// radix-2 transform of one 2^k row in place; dif: natural -> bitrev, else bitrev -> natural
-int ntt_stages(zgr_ctx* c, hipStream_t st, uint32_t* a, int k, bool dif, bool inverse) {
-    const uint32_t* tw = inverse ? c->tw_inv : c->tw;
-    uint32_t half_n = 1u << (k - 1);
-    if (dif) {
-        for (int s = k; s >= 1; s--)
-            LAUNCH(c, st, "ntt", k_ntt_dif_stage, blocks(half_n, 256), 256, 0, a, tw, s, c->log_n - s, half_n);
-    } else {
-        for (int s = 1; s <= k; s++)
-            LAUNCH(c, st, "ntt", k_ntt_dit_stage, blocks(half_n, 256), 256, 0, a, tw, s, c->log_n - s, half_n);
-    }
+// ---------------------------------------------------------------------------
+// Fr NTT (ntt.h): one pass up to 2^11 points, two passes (four-step) up to 2^22
+// ---------------------------------------------------------------------------
+int ntt_launch(zgr_ctx* c, hipStream_t st, const NttPass& p, uint32_t groups, uint32_t rows) {
+    const uint32_t threads = std::max(64u, std::min(1024u, (1u << p.log_m) / 2));
+    LAUNCH(c, st, "ntt", k_ntt_pass, dim3(groups, rows), threads, ntt_lds_bytes(p.log_m), p);
     return ZGR_OK;
+}
+
+NttPass ntt_pass_base(const zgr_ctx* c, bool inverse, int n) {
+    NttPass p{};
+    p.tw = inverse ? c->tw29_inv : c->tw29;
+    p.stw = inverse ? c->stw_inv : c->stw;
+    p.log_ctx = c->log_n;
+    p.row_len = 1ull << n;
+    return p;
+}
+
+// natural -> bit-reversed order (DIF) on `rows` consecutive rows of 2^n points, src ->
+// dst (they may be the same buffer); no 1/N scaling
+int ntt_dif(zgr_ctx* c, hipStream_t st, const uint32_t* src, uint32_t* dst, int n, uint32_t rows, bool inverse) {
+    NttPass p = ntt_pass_base(c, inverse, n);
+    p.src = src;
+    p.dst = dst;
+    p.in_rev = p.out_rev = 1;
+    if (n <= NTT_MAX_LOG) {
+        p.estride = 1;
+        p.log_m = n;
+        return ntt_launch(c, st, p, 1, rows);
+    }
+    const int n1 = (n + 1) / 2, n2 = n - n1;
+    p.gstride = 1;  // pass A: columns
+    p.estride = 1u << n2;
+    p.log_m = n1;
+    p.log_g = n2;
+    p.post = 1;
+    p.log_post = n;
+    CHECK(ntt_launch(c, st, p, 1u << n2, rows));
+    p.src = dst;  // pass B: blocks, in place
+    p.gstride = 1u << n2;
+    p.estride = 1;
+    p.log_m = n2;
+    p.log_g = n1;
+    p.post = 0;
+    return ntt_launch(c, st, p, 1u << n1, rows);
+}
+
+// bit-reversed -> natural order (DIT), src -> dst; pre (2^261 form, one factor per
+// input point, or null) multiplies the input
+int ntt_dit(zgr_ctx* c, hipStream_t st, const uint32_t* src, uint32_t* dst, int n, uint32_t rows, bool inverse,
+            const uint32_t* pre) {
+    NttPass p = ntt_pass_base(c, inverse, n);
+    p.src = src;
+    p.dst = dst;
+    p.pre = pre;
+    if (n <= NTT_MAX_LOG) {
+        p.estride = 1;
+        p.log_m = n;
+        return ntt_launch(c, st, p, 1, rows);
+    }
+    const int n1 = (n + 1) / 2, n2 = n - n1;
+    p.gstride = 1u << n2;  // pass 1: blocks
+    p.estride = 1;
+    p.log_m = n2;
+    p.log_g = n1;
+    p.post = 2;
+    p.log_post = n;
+    CHECK(ntt_launch(c, st, p, 1u << n1, rows));
+    p.src = dst;  // pass 2: columns, in place
+    p.pre = nullptr;
+    p.gstride = 1;
+    p.estride = 1u << n2;
+    p.log_m = n1;
+    p.log_g = n2;
+    p.post = 0;
+    return ntt_launch(c, st, p, 1u << n2, rows);
+}
+
+// ark-poly fft / ifft (natural -> natural, ifft scaled by 1/2^n) of `rows` rows in place;
+// above 2^11 points the bit-reversed result goes through tmp (rows * 2^n points)
+int ntt_natural(zgr_ctx* c, hipStream_t st, uint32_t* data, uint32_t* tmp, int n, uint32_t rows, bool inverse) {
+    const uint32_t* scale = inverse ? c->ninv29 + 8 * (size_t)(c->log_n - n) : nullptr;
+    if (n <= NTT_MAX_LOG) {
+        NttPass p = ntt_pass_base(c, inverse, n);
+        p.src = data;
+        p.dst = data;
+        p.scale = scale;
+        p.estride = 1;
+        p.log_m = n;
+        p.in_rev = 1;
+        return ntt_launch(c, st, p, 1, rows);
+    }
+    CHECK(ntt_dif(c, st, data, data, n, rows, inverse));
+    LAUNCH(c, st, "ntt_unrev", k_ntt_unrev, dim3(blocks((size_t)1 << n, 256), rows), 256, 0, data, tmp, n, scale);
+    HIPCHK(hipMemcpyAsync(data, tmp, ((size_t)rows << n) * 32, hipMemcpyDeviceToDevice, st));
+    return ZGR_OK;
+}
+
+// rows of one ntt_natural launch: grid.y <= 65535, and above 2^11 points what the
+// context's staging buffer (n_ctx points) holds
+uint32_t ntt_chunk_rows(const zgr_ctx* c, int n, int batch) {
+    size_t r = n <= NTT_MAX_LOG ? 65535 : std::min<size_t>(65535, c->n >> n);
+    return (uint32_t)std::max<size_t>(1, std::min<size_t>(r, (size_t)batch));
 }
 
 // windows per heap tree set: 1 with the fixed-base tables, else MSM_W
@@ -418,10 +519,7 @@ int check_len(const zgr_ctx* c, size_t len) {
 int issue_blob(zgr_ctx* c, Slot& s, const uint8_t* raw_dev, size_t len, const zgr_encoded* out) {
     const uint32_t n = (uint32_t)c->n;
     LAUNCH(c, s.st, "pack", k_pack, blocks(n, 256), 256, 0, raw_dev, len, c->log_col, c->log_row, s.pts);
-    if (c->coset_n > 1) {
-        HIPCHK(hipMemcpyAsync(s.coeff, s.pts, c->n * 32, hipMemcpyDeviceToDevice, s.st));
-        CHECK(ntt_stages(c, s.st, s.coeff, c->log_n, true, true));  // coefficients, bit-reversed
-    }
+    if (c->coset_n > 1) CHECK(ntt_dif(c, s.st, s.pts, s.coeff, c->log_n, 1, true));  // coefficients, bit-reversed
     HIPCHK(hipEventRecord(s.ev_pre, s.st));
     const uint32_t slots = (uint32_t)out_slots(c);
     for (int cs = 0; cs < c->coset_n; cs++) {
@@ -429,9 +527,8 @@ int issue_blob(zgr_ctx* c, Slot& s, const uint8_t* raw_dev, size_t len, const zg
         c->next_lane ^= 1;
         HIPCHK(hipStreamWaitEvent(ln.st, s.ev_pre, 0));
         const uint32_t* y = s.pts;
-        if (cs > 0) {
-            LAUNCH(c, ln.st, "coset_scale", k_mul_table, blocks(n, 256), 256, 0, ln.y, s.coeff, c->scale[cs], n);
-            CHECK(ntt_stages(c, ln.st, ln.y, c->log_n, false, false));
+        if (cs > 0) {  // evaluations on coset cs: DIT of the coefficients times cf^i / N
+            CHECK(ntt_dit(c, ln.st, s.coeff, ln.y, c->log_n, 1, false, c->scale[cs]));
             y = ln.y;
         }
         // high_commitment = [tau^(3N) f(tau)] G1 is the same group element for every
@@ -730,7 +827,7 @@ const char* zgr_last_error(void) { return g_err.c_str(); }
 int zgr_ctx_create(int device, int log_col, int log_row, int coset_n, zgr_ctx** out) {
     if (!out) return fail(ZGR_ERR_INVALID_ARG, "out is NULL");
     *out = nullptr;
-    if (log_col < 3 || log_row < 1 || log_col + log_row > 26 || coset_n < 1 || coset_n > 3 ||
+    if (log_col < 3 || log_row < 1 || log_col + log_row > NTT_MAX_TWO_PASS || coset_n < 1 || coset_n > 3 ||
         (1 << (log_col + log_row)) * coset_n > (1 << 28))
         return fail(ZGR_ERR_INVALID_ARG, "unsupported geometry log_col=%d log_row=%d coset_n=%d", log_col,
                     log_row, coset_n);
@@ -761,9 +858,18 @@ int zgr_ctx_create(int device, int log_col, int log_row, int coset_n, zgr_ctx** 
         if (need + 1024 > (size_t)max_shm)
             return fail(ZGR_ERR_INVALID_ARG, "bucket kernel needs %zu B of LDS, device allows %d", need, max_shm);
         HIPCHK(hipFuncSetAttribute((const void*)k_msm_accum, hipFuncAttributeMaxDynamicSharedMemorySize, (int)need));
+        const size_t ntt_lds = ntt_lds_bytes(std::min(c->log_n, NTT_MAX_LOG));
+        if (ntt_lds > (size_t)max_shm)
+            return fail(ZGR_ERR_INVALID_ARG, "NTT pass needs %zu B of LDS, device allows %d", ntt_lds, max_shm);
+        HIPCHK(hipFuncSetAttribute((const void*)k_ntt_pass, hipFuncAttributeMaxDynamicSharedMemorySize, (int)ntt_lds));
         DALLOC(c->consts, K_COUNT * 32);
         DALLOC(c->tw, c->n / 2 * 32);
         DALLOC(c->tw_inv, c->n / 2 * 32);
+        DALLOC(c->tw29, c->n / 2 * 32);
+        DALLOC(c->tw29_inv, c->n / 2 * 32);
+        DALLOC(c->stw, 9 * NTT_STAGE_TW * 4);
+        DALLOC(c->stw_inv, 9 * NTT_STAGE_TW * 4);
+        DALLOC(c->ninv29, (c->log_n + 1) * 32);
         DALLOC(c->flag, sizeof(int));
         // Merkle layer lengths: leaves, then halving down to coset_n roots
         for (size_t len = (size_t)coset_n * c->n / 8;; len /= 2) {
@@ -820,10 +926,13 @@ int zgr_ctx_create(int device, int log_col, int log_row, int coset_n, zgr_ctx** 
                (uint32_t)(c->n / 2));
         LAUNCH(c, st, "setup", k_powers, blocks(c->n / 2, 256), 256, 0, c->tw_inv, c->consts, (int)K_OMEGA_INV,
                (uint32_t)(c->n / 2));
+        LAUNCH(c, st, "setup", k_ntt_tables, blocks(c->n / 2, 256), 256, 0, c->tw, c->tw_inv, c->consts, c->log_n,
+               c->tw29, c->tw29_inv, c->stw, c->stw_inv, c->ninv29);
         for (int cs = 1; cs < coset_n; cs++) {
             DALLOC(c->scale[cs], c->n * 32);
             LAUNCH(c, st, "setup", k_coset_scale_table, blocks(c->n, 256), 256, 0, c->scale[cs], c->consts, cs,
                    c->log_n);
+            LAUNCH(c, st, "setup", k_to261_inplace, blocks(c->n, 256), 256, 0, c->scale[cs], (uint32_t)c->n);
         }
         return sync_all(c);
     };
@@ -844,7 +953,8 @@ void zgr_ctx_destroy(zgr_ctx* c) {
         if (l.st) (void)hipStreamSynchronize(l.st);
     for (auto& s : c->slots)
         if (s.st) (void)hipStreamSynchronize(s.st);
-    void* ptrs[] = {c->consts, c->tw, c->tw_inv, c->flag, c->work[0], c->work[1]};
+    void* ptrs[] = {c->consts, c->tw, c->tw_inv, c->tw29, c->tw29_inv, c->stw, c->stw_inv, c->ninv29, c->flag,
+                    c->work[0], c->work[1]};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     for (void* p : c->rec_buf)
@@ -1198,22 +1308,37 @@ void zgr_server_stats(zgr_server* sv, uint64_t* queued, uint64_t* admitted, uint
 }
 
 int zgr_ntt(zgr_ctx* c, uint8_t* data_mont, int log_n, int batch, int inverse) {
-    if (!c || !data_mont || batch < 0) return fail(ZGR_ERR_INVALID_ARG, "bad argument");
+    if (!c || (!data_mont && batch) || batch < 0) return fail(ZGR_ERR_INVALID_ARG, "bad argument");
     if (log_n < 1 || log_n > c->log_n)
         return fail(ZGR_ERR_INVALID_ARG, "log_n %d outside [1, %d] of this context", log_n, c->log_n);
     std::lock_guard<std::mutex> lk(c->mu);
     HIPCHK(hipSetDevice(c->device));
-    Lane& ln = c->lanes[0];
-    uint32_t* tmp = c->slots[0].pts;
+    hipStream_t st = c->lanes[0].st;
+    uint32_t* buf = c->slots[0].pts;   // rows staged here (n_ctx points)
+    uint32_t* tmp = c->slots[0].coeff;
     const size_t row = (size_t)32 << log_n;
-    for (int r = 0; r < batch; r++) {
-        HIPCHK(hipMemcpyAsync(tmp, data_mont + r * row, row, hipMemcpyHostToDevice, ln.st));
-        LAUNCH(c, ln.st, "ntt", k_bitrev_copy, blocks((size_t)1 << log_n, 256), 256, 0, ln.y, tmp, log_n);
-        CHECK(ntt_stages(c, ln.st, ln.y, log_n, false, inverse != 0));
-        if (inverse)
-            LAUNCH(c, ln.st, "ntt", k_mul_ninv_sub, blocks((size_t)1 << log_n, 256), 256, 0, ln.y, c->consts,
-                   c->log_n - log_n, 1u << log_n);
-        HIPCHK(hipMemcpyAsync(data_mont + r * row, ln.y, row, hipMemcpyDeviceToHost, ln.st));
+    for (int r = 0; r < batch;) {
+        const uint32_t rows = std::min<uint32_t>(ntt_chunk_rows(c, log_n, batch - r), c->n >> log_n);
+        HIPCHK(hipMemcpyAsync(buf, data_mont + r * row, rows * row, hipMemcpyHostToDevice, st));
+        CHECK(ntt_natural(c, st, buf, tmp, log_n, rows, inverse != 0));
+        HIPCHK(hipMemcpyAsync(data_mont + r * row, buf, rows * row, hipMemcpyDeviceToHost, st));
+        r += rows;
+    }
+    return sync_all(c);
+}
+
+int zgr_ntt_device(zgr_ctx* c, void* data_dev, int log_n, int batch, int inverse) {
+    if (!c || (!data_dev && batch) || batch < 0) return fail(ZGR_ERR_INVALID_ARG, "bad argument");
+    if (log_n < 1 || log_n > c->log_n)
+        return fail(ZGR_ERR_INVALID_ARG, "log_n %d outside [1, %d] of this context", log_n, c->log_n);
+    std::lock_guard<std::mutex> lk(c->mu);
+    HIPCHK(hipSetDevice(c->device));
+    hipStream_t st = c->lanes[0].st;
+    uint32_t* data = (uint32_t*)data_dev;
+    for (int r = 0; r < batch;) {
+        const uint32_t rows = ntt_chunk_rows(c, log_n, batch - r);
+        CHECK(ntt_natural(c, st, data + ((size_t)r << log_n) * 8, c->slots[0].coeff, log_n, rows, inverse != 0));
+        r += rows;
     }
     return sync_all(c);
 }
@@ -1223,16 +1348,14 @@ int zgr_coset_extend(zgr_ctx* c, const uint8_t* data_mont, int coset, uint8_t* o
     if (coset < 0 || coset >= c->coset_n) return fail(ZGR_ERR_INVALID_ARG, "coset %d out of range", coset);
     std::lock_guard<std::mutex> lk(c->mu);
     HIPCHK(hipSetDevice(c->device));
-    const uint32_t n = (uint32_t)c->n;
     Lane& ln = c->lanes[0];
     uint32_t* coeff = c->slots[0].coeff;
     HIPCHK(hipMemcpyAsync(coeff, data_mont, c->n * 32, hipMemcpyHostToDevice, ln.st));
     if (coset == 0) {
         HIPCHK(hipMemcpyAsync(ln.y, coeff, c->n * 32, hipMemcpyDeviceToDevice, ln.st));
     } else {
-        CHECK(ntt_stages(c, ln.st, coeff, c->log_n, true, true));
-        LAUNCH(c, ln.st, "coset_scale", k_mul_table, blocks(n, 256), 256, 0, ln.y, coeff, c->scale[coset], n);
-        CHECK(ntt_stages(c, ln.st, ln.y, c->log_n, false, false));
+        CHECK(ntt_dif(c, ln.st, coeff, coeff, c->log_n, 1, true));
+        CHECK(ntt_dit(c, ln.st, coeff, ln.y, c->log_n, 1, false, c->scale[coset]));
     }
     HIPCHK(hipMemcpyAsync(out_mont, ln.y, c->n * 32, hipMemcpyDeviceToHost, ln.st));
     return sync_all(c);

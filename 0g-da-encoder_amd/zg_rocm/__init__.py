@@ -46,6 +46,7 @@ SIGNATURES = {
     "zgr_encode_device": (I, [P, P, S, P]),
     "zgr_encode_batch_device": (I, [P, I, P, P, P]),
     "zgr_ntt": (I, [P, P, I, I, I]),
+    "zgr_ntt_device": (I, [P, P, I, I, I]),
     "zgr_coset_extend": (I, [P, P, I, P]),
     "zgr_msm": (I, [P, P, P, S, P]),
     "zgr_ecfft": (I, [P, P, I, I, P]),
@@ -169,6 +170,15 @@ class DeviceBuffer:
     @property
     def address(self):
         return self.ptr.value or 0
+
+    def to_host(self):
+        """The buffer's current bytes."""
+        out = ctypes.create_string_buffer(max(self.size, 1))
+        if self.size:
+            rc = hip().hipMemcpy(out, self.ptr, self.size, 2)  # hipMemcpyDeviceToHost
+            if rc:
+                raise ZgrError(ERR_DEVICE, "hipMemcpy failed: %d" % rc)
+        return out.raw[:self.size]
 
     def free(self):
         if self.ptr.value:
@@ -346,6 +356,10 @@ class Context:
         batch = a.size // (4 << log_n)
         _check(lib().zgr_ntt(self.h, _p(a), log_n, batch, int(inverse)))
         return a
+
+    def ntt_device(self, data_dev_ptr, log_n, batch, inverse=False):
+        """zgr_ntt_device: batch rows of 2^log_n Fr (mont) in device memory, in place."""
+        _check(lib().zgr_ntt_device(self.h, ctypes.c_void_p(data_dev_ptr), log_n, batch, int(inverse)))
 
     def coset_extend(self, data_mont, coset):
         d = np.ascontiguousarray(data_mont, dtype=np.uint64)

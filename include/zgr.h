@@ -67,7 +67,8 @@ const char *zgr_last_error(void);
 
 /* Context on one device for one blob geometry (log_col, log_row, coset_n).
  * Replaces the per-thread workspace of ag_cuda_ec::init_local_workspace
- * (crates/amt/src/prove_params/interfaces.rs:41-44).  coset_n <= 3. */
+ * (crates/amt/src/prove_params/interfaces.rs:41-44).  coset_n <= 3, log_col >= 3,
+ * log_row >= 1, log_col + log_row <= 22 (the NTT's two-pass limit; production is 20). */
 int zgr_ctx_create(int device, int log_col, int log_row, int coset_n, zgr_ctx **out);
 void zgr_ctx_destroy(zgr_ctx *ctx);
 
@@ -151,10 +152,14 @@ int zgr_server_poll(zgr_server *srv, uint64_t ticket, int timeout_ms);
 void zgr_server_stats(zgr_server *srv, uint64_t *queued, uint64_t *admitted, uint64_t *completed);
 
 /* ---- primitives (reference call sites named per function) ---- */
-/* Radix-2 DFT over the 2^log_n domain of ark-poly Radix2EvaluationDomain
- * (to_coset_blob, crates/amt/src/blob/encode.rs:142-157).  inverse != 0 applies
- * the domain's size_inv.  batch rows of 2^log_n Fr (mont), in place, host memory. */
+/* Radix2EvaluationDomain::fft / ifft of ark-poly over the 2^log_n domain, natural
+ * order in and out (to_coset_blob, crates/amt/src/blob/encode.rs:142-157).  inverse != 0
+ * is ifft (inverse root, then size_inv).  batch rows of 2^log_n Fr (mont), in place,
+ * host memory.  log_n <= the context's log_col + log_row. */
 int zgr_ntt(zgr_ctx *ctx, uint8_t *data_mont, int log_n, int batch, int inverse);
+/* The same on rows already in device memory (in place): up to 2^11 points per row one
+ * launch for the whole batch, each row in LDS; larger rows two passes over HBM. */
+int zgr_ntt_device(zgr_ctx *ctx, void *data_dev, int log_n, int batch, int inverse);
 /* to_coset_blob (encode.rs:135-158) for coset 1..coset_n-1 of ctx's geometry:
  * data_mont (2^log_n, column-major) -> out_mont evaluations on that coset. */
 int zgr_coset_extend(zgr_ctx *ctx, const uint8_t *data_mont, int coset, uint8_t *out_mont);

@@ -133,6 +133,59 @@ def test_msm_2pow20(ctx):
     assert np.array_equal(got, exp)
 
 
+def _fr_rows(rng, rows, log_n):
+    """uniform values below 2^252 < r as Montgomery limbs (every such value is a valid
+    Montgomery representative)"""
+    a = rng.integers(0, 2**63, (rows, 1 << log_n, 4), dtype=np.uint64)
+    a[:, :, 3] &= np.uint64(0x0FFFFFFFFFFFFFFF)
+    return a
+
+
+@pytest.mark.parametrize("inverse", [False, True])
+def test_ntt_batch_1024_rows_2pow10(ctx, inverse):
+    """config 3: the batched Fr NTT of 1024 rows x 2^10 (one launch, each row in LDS),
+    every row bit-exact against the oracle's radix-2 fft / ifft, through both the
+    host-memory and the device-memory entry points."""
+    rng = np.random.default_rng(3030 + inverse)
+    rows = _fr_rows(rng, 1024, 10)
+    t = time.time()
+    got = ctx.ntt(rows, 10, inverse).reshape(1024, 1 << 10, 4)
+    log("ntt 1024 x 2^10 inverse=%d (host in/out): %.4f s" % (inverse, time.time() - t))
+    f = O.ifft if inverse else O.fft
+    for r in range(1024):
+        assert np.array_equal(got[r], f(rows[r], 10)), r
+    dev = Z.DeviceBuffer(rows.tobytes())
+    ctx.ntt_device(dev.address, 10, 1024, inverse)
+    assert np.array_equal(np.frombuffer(dev.to_host(), dtype=np.uint64).reshape(got.shape), got)
+
+
+@pytest.mark.parametrize("log_n,batch", [(12, 3), (13, 2), (17, 1), (20, 1), (20, 2)])
+def test_ntt_two_pass(ctx, log_n, batch):
+    """Transforms above 2^11 points run as two passes (four-step, n = ceil(n/2) +
+    floor(n/2)) and come back to natural order; both directions bit-exact against the
+    oracle, at even and odd splits and at the full 2^20 domain."""
+    rng = np.random.default_rng(4000 + log_n)
+    rows = _fr_rows(rng, batch, log_n)
+    for inverse in (False, True):
+        got = ctx.ntt(rows, log_n, inverse).reshape(rows.shape)
+        f = O.ifft if inverse else O.fft
+        for r in range(batch):
+            assert np.array_equal(got[r], f(rows[r], log_n)), (inverse, r)
+    dev = Z.DeviceBuffer(rows.tobytes())
+    ctx.ntt_device(dev.address, log_n, batch, False)
+    assert np.array_equal(np.frombuffer(dev.to_host(), dtype=np.uint64).reshape(rows.shape),
+                          ctx.ntt(rows, log_n, False).reshape(rows.shape))
+
+
+def test_coset_extend_production(ctx):
+    """to_coset_blob at 2^20 (four-step iNTT, coset scaling fused into the first DIT
+    pass): every coset bit-exact against the oracle."""
+    rng = np.random.default_rng(5050)
+    data = _fr_rows(rng, 1, GEO.log_n)[0]
+    for c in range(GEO.coset_n):
+        assert np.array_equal(ctx.coset_extend(data, c).reshape(-1, 4), O.to_coset_blob(data, GEO.log_n, c)), c
+
+
 def test_batch_equals_single(ctx, blob):
     # config 4 shape: uneven full-geometry blobs in one batched call, each bit-equal
     # to the same blob encoded alone
