@@ -42,6 +42,12 @@
 
 using namespace zgr;
 
+// Intermediate Merkle layers of zgr_merkle_noalloc: a fixed array per GPU, so that
+// entry point allocates nothing.  Host code never uses this name as an address (it is a
+// host-side stand-in there): it uses the device address kept per device by
+// noalloc_scratch(), taken with hipGetSymbolAddress when a context is created.
+static __device__ __align__(256) uint8_t scratch[ZGR_MERKLE_SCRATCH_BYTES];
+
 namespace {
 
 thread_local std::string g_err;
@@ -71,6 +77,26 @@ int fail(int code, const char* fmt, ...) {
     } while (0)
 
 inline uint32_t blocks(size_t n, uint32_t t) { return (uint32_t)((n + t - 1) / t); }
+
+// device address of `scratch` on each device (each GPU holds its own copy of the array)
+std::mutex g_scratch_mu;
+std::map<int, uint8_t*> g_scratch_dev;
+
+// Resolve `scratch` on `device` once; the device must be current.
+int noalloc_scratch(int device, uint8_t** out) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    auto it = g_scratch_dev.find(device);
+    if (it == g_scratch_dev.end()) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(ZGR_ERR_DEVICE, "hipGetSymbolAddress(scratch) on device %d failed: %s", device,
+                        hipGetErrorString(e));
+        it = g_scratch_dev.emplace(device, (uint8_t*)p).first;
+    }
+    *out = it->second;
+    return ZGR_OK;
+}
 
 // host memcpy of large staged outputs, split over a few threads (one thread moves about
 // 10 GB/s; the encoded data is 32 MB per coset)
@@ -836,6 +862,8 @@ int zgr_ctx_create(int device, int log_col, int log_row, int coset_n, zgr_ctx** 
     HIPCHK(hipGetDeviceCount(&ndev));
     if (device < 0 || device >= ndev) return fail(ZGR_ERR_INVALID_ARG, "device %d not present", device);
     HIPCHK(hipSetDevice(device));
+    uint8_t* scratch_dev = nullptr;  // set-up of zgr_merkle_noalloc's scratch on this device
+    CHECK(noalloc_scratch(device, &scratch_dev));
     zgr_ctx* c = new zgr_ctx();
     c->device = device;
     c->log_col = log_col;
@@ -1520,6 +1548,52 @@ int zgr_merkle(zgr_ctx* c, const uint8_t* data, size_t n_elems, int levels, uint
     (void)hipStreamSynchronize(st);
     for (auto* p : layers) (void)hipFree(p);
     return rc;
+}
+
+int zgr_merkle_noalloc(zgr_ctx* c, const void* data_dev, size_t n_elems, int levels, void* out_layers_dev,
+                       void* file_root_dev) {
+    if (!c || !data_dev || !file_root_dev || (levels >= 0 && !out_layers_dev) || levels < -1)
+        return fail(ZGR_ERR_INVALID_ARG, "null argument");
+    if (n_elems == 0 || n_elems % 8) return fail(ZGR_ERR_INVALID_ARG, "n_elems %zu unsupported", n_elems);
+    // the layers below the root: n/8 leaves, halving while the count is even and above
+    // coset_n; at most n/4 nodes of 32 B in all
+    std::vector<size_t> lens;
+    size_t total = 0;
+    for (size_t len = n_elems / 8;; len /= 2) {
+        lens.push_back(len);
+        total += len * 32;
+        if (len <= (size_t)c->coset_n || (len & 1)) break;
+    }
+    if (total > ZGR_MERKLE_SCRATCH_BYTES)
+        return fail(ZGR_ERR_INVALID_ARG, "n_elems %zu needs %zu B of Merkle layers, the no-alloc scratch holds %zu B",
+                    n_elems, total, (size_t)ZGR_MERKLE_SCRATCH_BYTES);
+    const int L = (int)lens.size();  // tree layers 1..L = lens[0..L-1]
+    if (levels >= L) return fail(ZGR_ERR_INVALID_ARG, "tree has only %d layers", L);
+    std::lock_guard<std::mutex> lk(c->mu);
+    HIPCHK(hipSetDevice(c->device));
+    uint8_t* base = nullptr;
+    CHECK(noalloc_scratch(c->device, &base));  // resolved when the context was created
+    hipStream_t st = c->lanes[0].st;
+    std::vector<uint32_t*> layer(L);
+    size_t off = 0;
+    for (int i = 0; i < L; i++) {
+        layer[i] = (uint32_t*)(base + off);
+        off += lens[i] * 32;
+    }
+    // every node is written before it is read: nothing to clear
+    LAUNCH(c, st, "merkle_leaves", k_merkle_leaves_flat, blocks(lens[0], 128), 128, 0, (const uint32_t*)data_dev,
+           layer[0], (uint32_t)lens[0]);
+    for (int i = 1; i < L; i++)
+        LAUNCH(c, st, "merkle_level", k_merkle_level, blocks(lens[i], 128), 128, 0, layer[i - 1], layer[i],
+               (uint32_t)lens[i]);
+    LAUNCH(c, st, "file_root", k_file_root, 1, 1, 0, layer[L - 1], (int)lens[L - 1], (uint32_t*)file_root_dev);
+    size_t out = 0;
+    for (int k = 0; k <= levels; k++) {  // tree[k] = layer L - 1 - k
+        HIPCHK(hipMemcpyAsync((uint8_t*)out_layers_dev + out, layer[L - 1 - k], lens[L - 1 - k] * 32,
+                              hipMemcpyDeviceToDevice, st));
+        out += lens[L - 1 - k] * 32;
+    }
+    return sync_all(c);
 }
 
 int zgr_verify_msm(zgr_ctx* c, const uint32_t* row_ids, const uint8_t* rows, const uint8_t* alphas, size_t n_rows,

@@ -51,6 +51,7 @@ SIGNATURES = {
     "zgr_msm": (I, [P, P, P, S, P]),
     "zgr_ecfft": (I, [P, P, I, I, P]),
     "zgr_merkle": (I, [P, P, S, I, P, P]),
+    "zgr_merkle_noalloc": (I, [P, P, S, I, P, P]),
     "zgr_server_create": (I, [P, S, I, ctypes.POINTER(P)]),
     "zgr_server_destroy": (None, [P]),
     "zgr_server_submit": (I, [P, P, S, P, ctypes.POINTER(ctypes.c_uint64)]),

@@ -173,6 +173,20 @@ int zgr_ecfft(zgr_ctx *ctx, const uint8_t *points, int log_n, int inverse, uint8
  * tree[0..=levels] and the file root (merkle/blob.rs:15-34, encoder/blob.rs:64-72). */
 int zgr_merkle(zgr_ctx *ctx, const uint8_t *data, size_t n_elems, int levels,
                uint8_t *out_layers, uint8_t *file_root);
+/* No counterpart in the reference.  For callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc):
+ * zgr_merkle with every buffer in device memory -- data (n_elems * 32 B), out_layers
+ * (tree[0..=levels], as zgr_merkle lays them out) and file_root (32 B) -- and the
+ * intermediate layers in a fixed scratch array of ZGR_MERKLE_SCRATCH_BYTES that each GPU
+ * holds for the life of the process, instead of per-call allocations.  The call itself
+ * allocates nothing.  n_elems * 8 must not exceed ZGR_MERKLE_SCRATCH_BYTES (n_elems <=
+ * 4 Mi, a 1024 x 1024 blob's 3 Mi encoded elements fit); larger inputs get
+ * ZGR_ERR_INVALID_ARG.  Synchronous on the context's stream.  Every call on one GPU uses
+ * that GPU's one scratch array: calls that overlap on one GPU (from different contexts
+ * or threads) race. */
+#define ZGR_MERKLE_SCRATCH_BYTES ((size_t)32 << 20)
+int zgr_merkle_noalloc(zgr_ctx *ctx, const void *data_dev, size_t n_elems, int levels,
+                       void *out_layers_dev, void *file_root_dev);
 
 /* ---- deferred AMT verification (the GPU half of DeferredVerifier) ---- */
 /* DeferredVerifier::fast_check_msm_gpu (crates/amt/src/deferred_verification.rs:203-222)
