@@ -191,10 +191,10 @@ struct zgr_ctx {
     uint32_t* consts = nullptr;   // K_COUNT Fr
     uint32_t* tw = nullptr;       // n/2 Fr: omega^k
     uint32_t* tw_inv = nullptr;   // n/2 Fr: omega^-k
-    // NTT tables in the 2^261 form (k_ntt_tables): omega^+-k (n/2 each), the stage
-    // twiddles of the in-LDS sub-transforms, and 2^k / n (k <= log_n)
-    uint32_t* tw29 = nullptr;
-    uint32_t* tw29_inv = nullptr;
+    // NTT tables in the 2^261 form (k_ntt_tables): omega^+-x split over two small
+    // tables (pw_lo, pw_hi), the stage twiddles of the in-LDS sub-transforms, and 2^k / n
+    uint32_t* pw_lo[2] = {nullptr, nullptr};  // [0] forward, [1] inverse
+    uint32_t* pw_hi[2] = {nullptr, nullptr};
     uint32_t* stw = nullptr;
     uint32_t* stw_inv = nullptr;
     uint32_t* ninv29 = nullptr;
@@ -288,7 +288,9 @@ int ntt_launch(zgr_ctx* c, hipStream_t st, const NttPass& p, uint32_t groups, ui
 
 NttPass ntt_pass_base(const zgr_ctx* c, bool inverse, int n) {
     NttPass p{};
-    p.tw = inverse ? c->tw29_inv : c->tw29;
+    p.pw_lo = c->pw_lo[inverse ? 1 : 0];
+    p.pw_hi = c->pw_hi[inverse ? 1 : 0];
+    p.pw_s = ntt_pw_bits(c->log_n);
     p.stw = inverse ? c->stw_inv : c->stw;
     p.log_ctx = c->log_n;
     p.row_len = 1ull << n;
@@ -893,8 +895,10 @@ int zgr_ctx_create(int device, int log_col, int log_row, int coset_n, zgr_ctx** 
         DALLOC(c->consts, K_COUNT * 32);
         DALLOC(c->tw, c->n / 2 * 32);
         DALLOC(c->tw_inv, c->n / 2 * 32);
-        DALLOC(c->tw29, c->n / 2 * 32);
-        DALLOC(c->tw29_inv, c->n / 2 * 32);
+        for (int d = 0; d < 2; d++) {
+            DALLOC(c->pw_lo[d], ((size_t)32) << ntt_pw_bits(c->log_n));
+            DALLOC(c->pw_hi[d], ((size_t)32) << (c->log_n - ntt_pw_bits(c->log_n)));
+        }
         DALLOC(c->stw, 9 * NTT_STAGE_TW * 4);
         DALLOC(c->stw_inv, 9 * NTT_STAGE_TW * 4);
         DALLOC(c->ninv29, (c->log_n + 1) * 32);
@@ -955,7 +959,7 @@ int zgr_ctx_create(int device, int log_col, int log_row, int coset_n, zgr_ctx** 
         LAUNCH(c, st, "setup", k_powers, blocks(c->n / 2, 256), 256, 0, c->tw_inv, c->consts, (int)K_OMEGA_INV,
                (uint32_t)(c->n / 2));
         LAUNCH(c, st, "setup", k_ntt_tables, blocks(c->n / 2, 256), 256, 0, c->tw, c->tw_inv, c->consts, c->log_n,
-               c->tw29, c->tw29_inv, c->stw, c->stw_inv, c->ninv29);
+               c->pw_lo[0], c->pw_lo[1], c->pw_hi[0], c->pw_hi[1], c->stw, c->stw_inv, c->ninv29);
         for (int cs = 1; cs < coset_n; cs++) {
             DALLOC(c->scale[cs], c->n * 32);
             LAUNCH(c, st, "setup", k_coset_scale_table, blocks(c->n, 256), 256, 0, c->scale[cs], c->consts, cs,
@@ -981,8 +985,8 @@ void zgr_ctx_destroy(zgr_ctx* c) {
         if (l.st) (void)hipStreamSynchronize(l.st);
     for (auto& s : c->slots)
         if (s.st) (void)hipStreamSynchronize(s.st);
-    void* ptrs[] = {c->consts, c->tw, c->tw_inv, c->tw29, c->tw29_inv, c->stw, c->stw_inv, c->ninv29, c->flag,
-                    c->work[0], c->work[1]};
+    void* ptrs[] = {c->consts, c->tw,        c->tw_inv,   c->pw_lo[0], c->pw_lo[1], c->pw_hi[0], c->pw_hi[1],
+                    c->stw,    c->stw_inv, c->ninv29,   c->flag,     c->work[0],  c->work[1]};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     for (void* p : c->rec_buf)

@@ -58,38 +58,70 @@ def roofline_scalars(geo):
 
 
 def cpu_baseline(ctx, geo, raw, threads):
-    """Oracle (CPU restatement of the reference parallel path) on a bounded sample:
-    one coset (coset 1) of one blob -- its coset extension and its gen_all_proofs over
-    12 x 2^20 pairs -- plus the Merkle tree over the blob's encoded data, scaled to
-    one blob (x3 cosets for extension + prove)."""
+    """The oracle -- a CPU restatement of the reference `parallel` path (to_coset_blob,
+    gen_all_proofs, EncodedBlobMerkle::build) -- on bounded samples of one 1024x1024 blob.
+
+    All host cores: coset 0 (identity extension + gen_all_proofs over 12 x 2^20 pairs)
+    and coset 1 (iFFT / scale / FFT of 2^20 + gen_all_proofs) timed separately, per blob
+    = coset 0 + 2 x coset 1 + the Merkle tree over 3 x 2^20 elements.
+    One core: coset 1's extension, a quarter of its basis line (256 chunk MSMs of 1024
+    points, 2^18 pairs) scaled to 12 lines x 2^20 pairs per coset, and the Merkle tree
+    over an eighth of the elements, scaled x8."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
 
-    O.set_threads(threads)
-    bases = ctx.params_download(1)
     blob = O.raw_to_blob(raw, geo.log_n).reshape(geo.row_n, geo.col_n, 4)
     pts = np.ascontiguousarray(blob.transpose(1, 0, 2)).reshape(-1, 4)
-    t0 = time.time()
-    y = O.to_coset_blob(pts, geo.log_n, 1)
     idx = np.arange(geo.n)
     rev = np.zeros(geo.n, dtype=np.int64)
     for b in range(geo.log_n):
         rev |= ((idx >> b) & 1) << (geo.log_n - 1 - b)
-    ri = np.ascontiguousarray(y.reshape(-1, 4)[rev])
-    O.amt_prove(bases, geo.log_n, geo.h, ri)
-    t_coset = time.time() - t0
     data = np.random.default_rng(1).integers(0, 256, geo.coset_n * geo.n * 32, dtype=np.uint8)
+
+    O.set_threads(threads)
+    t_c = []
+    for c in (0, 1):
+        bases = ctx.params_download(c)
+        t0 = time.time()
+        y = pts if c == 0 else O.to_coset_blob(pts, geo.log_n, c)
+        O.amt_prove(bases, geo.log_n, geo.h, np.ascontiguousarray(y.reshape(-1, 4)[rev]))
+        t_c.append(time.time() - t0)
     t0 = time.time()
     O.merkle(data, geo.coset_n, geo.log_row)
     t_merkle = time.time() - t0
-    per_blob = geo.coset_n * t_coset + t_merkle
+    per_blob = t_c[0] + (geo.coset_n - 1) * t_c[1] + t_merkle
+
+    O.set_threads(1)
+    t0 = time.time()
+    y = O.to_coset_blob(pts, geo.log_n, 1)
+    t_ext1 = time.time() - t0
+    bases = ctx.params_download(1).reshape(geo.h + 2, geo.n, 8)[geo.h]
+    canon = np.ascontiguousarray(O.fr_canonical_array(np.ascontiguousarray(y.reshape(-1, 4)[rev]))).reshape(-1, 4)
+    chunks, bs = 256, geo.n >> geo.h
+    t0 = time.time()
+    for b in range(chunks):
+        O.msm(bases[b * bs:(b + 1) * bs], canon[b * bs:(b + 1) * bs])
+    t_chunks = time.time() - t0
+    eighth = geo.coset_n * geo.n // 8
+    t0 = time.time()
+    O.merkle(data[:eighth * 32], geo.coset_n, geo.log_row - 3)
+    t_m1 = time.time() - t0
+    prove1 = t_chunks * (geo.n // (chunks * bs)) * (geo.h + 2)
+    per_blob_1 = (geo.coset_n - 1) * t_ext1 + geo.coset_n * prove1 + 8 * t_m1
+    O.set_threads(threads)
     return {
         "value": 1.0 / per_blob,
         "unit": "blobs/s",
         "cores": threads,
         "kind": "port",
-        "sample": "coset 1 of one 1024x1024 blob (coset extension + gen_all_proofs, 12 x 2^20 pairs) "
-                  "%.1f s, x3 cosets, + Merkle over 3 x 2^20 elements %.1f s" % (t_coset, t_merkle),
+        "label": "CPU restatement of the reference parallel path (the oracle, C + OpenMP)",
+        "sample": "one 1024x1024 blob: coset 0 (gen_all_proofs, 12 x 2^20 pairs) %.1f s, coset 1 (extension + "
+                  "gen_all_proofs) %.1f s, Merkle over 3 x 2^20 elements %.1f s; per blob = coset 0 + 2 x coset 1 "
+                  "+ Merkle" % (t_c[0], t_c[1], t_merkle),
+        "one_core": {"value": 1.0 / per_blob_1, "unit": "blobs/s", "cores": 1,
+                     "sample": "coset 1 extension %.1f s; 256 of the basis line's 1024-point chunk MSMs %.1f s, "
+                               "scaled to 12 lines x 2^20 pairs per coset (%.0f s per coset); Merkle over 1/8 of "
+                               "the elements %.1f s, x8" % (t_ext1, t_chunks, prove1, t_m1)},
     }
 
 
@@ -125,6 +157,14 @@ def main():
         scal = roofline_scalars(geo)
         for _ in range(3):
             ctx.amt_prove(1, scal)
+        # and three of each NTT shape of the bench (k_ntt_pass, k_ntt_unrev)
+        nrows = np.random.default_rng(DATA_SEED + 11).integers(0, 2**63, (1024, 1 << 10, 4), dtype=np.uint64)
+        nrows[:, :, 3] &= np.uint64(0x0FFFFFFFFFFFFFFF)
+        ndev = Z.DeviceBuffer(nrows.tobytes())
+        for log_n, batch in ((10, 1024), (20, 1)):
+            for _ in range(3):
+                ctx.ntt_device(ndev.address, log_n, batch)
+        del ndev
         import ctypes
         pa, pb = ctypes.c_void_p(), ctypes.c_void_p()
         Z.hip().hipMalloc(ctypes.byref(pa), 1 << 30)
@@ -134,7 +174,8 @@ def main():
         Z.hip().hipFree(pa)
         Z.hip().hipFree(pb)
         ctx.close()
-        print(json.dumps({"roofline_only": True, "msm_launches": 3, "copy_bytes": 1 << 30}))
+        print(json.dumps({"roofline_only": True, "msm_launches": 3, "ntt_calls": {"1024x2^10": 3, "2^20": 3},
+                          "copy_bytes": 1 << 30}))
         return
 
     def resident(raws):
@@ -270,10 +311,96 @@ def main():
     ec_ms, ec_cnt = ctx.kernel_time_ms("ecfft")
     ctx.set_profiling(False)
     butterflies = geo.log_n * geo.n // 2
+    ec_s = ec_ms * ec_cnt * 1e-3
     ecfft = {"log_n": geo.log_n, "wall_ms": round(ec_wall * 1e3, 1), "kernel_ms": round(ec_ms * ec_cnt, 1),
-             "butterflies_per_s": round(butterflies / (ec_ms * ec_cnt * 1e-3)) if ec_cnt else None,
+             "butterflies_per_s": round(butterflies / ec_s) if ec_cnt else None,
+             # reference-equivalent work: a 254-bit double-and-add scalar multiplication per
+             # butterfly (254 doublings of 6 Fq multiplications, 127 additions of 14), 136
+             # MAC32 per Fq multiplication, against the 39.3 T MAC32/s issue peak
+             "roofline": {"bound": "valu", "mac32_per_butterfly": (254 * 6 + 127 * 14) * 136,
+                          "frac": round(butterflies * (254 * 6 + 127 * 14) * 136 / ec_s / PEAK_MAC32, 4)
+                          if ec_cnt else None},
              "note": "one forward DFT of 2^20 G1 points (Pairing::fast_fft); 20 x 2^19 butterflies, "
                      "each a scalar multiplication by a GLV-split twiddle"}
+
+    # config 3 primitives, inputs resident in HBM where the entry point allows it.
+    # NTT: the batched 1024 rows x 2^10 (one launch, each row in LDS) and one 2^20-point
+    # transform (two passes over HBM + the return to natural order); k_ntt_pass is the
+    # "ntt" timer, the bit-reversal copy "ntt_unrev"
+    nrng = np.random.default_rng(DATA_SEED + 11)
+    nrows = nrng.integers(0, 2**63, (1024, 1 << 10, 4), dtype=np.uint64)
+    nrows[:, :, 3] &= np.uint64(0x0FFFFFFFFFFFFFFF)
+    ndev = Z.DeviceBuffer(nrows.tobytes())
+    ntt = {}
+    for label, log_n, batch in (("batch_1024_rows_2pow10", 10, 1024), ("one_2pow20", 20, 1)):
+        ctx.ntt_device(ndev.address, log_n, batch)  # warm
+        ctx.set_profiling(True)
+        ctx.kernel_time_reset()
+        Z.hip().hipDeviceSynchronize()
+        reps = 10
+        s0 = time.perf_counter()
+        for _ in range(reps):
+            ctx.ntt_device(ndev.address, log_n, batch)
+        wall = (time.perf_counter() - s0) / reps
+        ctx.set_profiling(False)
+        k_ms, k_cnt = ctx.kernel_time_ms("ntt")
+        u_ms, u_cnt = ctx.kernel_time_ms("ntt_unrev")
+        pass_ms = k_ms * k_cnt / reps  # k_ntt_pass time per call (1 or 2 launches)
+        elems = batch << log_n
+        butterflies = batch * log_n * (1 << (log_n - 1))
+        ntt[label] = {
+            "log_n": log_n, "rows": batch, "wall_ms": round(wall * 1e3, 4), "pass_kernel_ms": round(pass_ms, 4),
+            "passes": k_cnt // reps, "unrev_ms": round(u_ms * u_cnt / reps, 4) if u_cnt else 0.0,
+            # compulsory bytes of the passes: each pass reads and writes every element once
+            "pass_GB_s": round(2 * 32 * elems * (k_cnt // reps) / (pass_ms * 1e-3) / 1e9, 1),
+            "butterflies_per_s": round(butterflies / (pass_ms * 1e-3)),
+            # VALU bound: one Fr Montgomery product (9 x 9 + 9 x 9 = 162 v_mad_u64_u32 in the
+            # 29-bit form) per butterfly, against the 39.3 T lane-op/s issue peak
+            "valu_mad_frac": round(butterflies * 162 / (pass_ms * 1e-3) / PEAK_MAC32, 4),
+        }
+    del ndev
+    ntt["note"] = ("zgr_ntt_device (natural order in and out, forward), mean of 10 calls; the encode path runs "
+                   "the same passes without the return to natural order (DIF iNTT, then DIT per coset)")
+
+    # MSM of 2^20 (zgr_msm: host bases and scalars in, the windowed kernels) and the
+    # 2^10-point EC-FFT (zgr_ecfft)
+    mbases = ctx.params_download(0).reshape(geo.h + 2, geo.n, 8)[geo.h].copy()
+    mscal = roofline_scalars(geo)
+    ctx.msm(mbases, mscal)
+    mt = []
+    for _ in range(3):
+        s0 = time.perf_counter()
+        ctx.msm(mbases, mscal)
+        mt.append(time.perf_counter() - s0)
+    msm = {"n": geo.n, "wall_ms": round(float(np.median(mt)) * 1e3, 2),
+           "note": "zgr_msm, one MSM of 2^20 (basis line bases x dense scalars); the call uploads 64 MB of bases "
+                   "and 32 MB of scalars and runs the windowed bucket kernels; median of 3"}
+    ec10 = ec_in[:(1 << 10) * 64]
+    ctx.ecfft(ec10, 10, inverse=False)
+    ctx.set_profiling(True)
+    ctx.kernel_time_reset()
+    et = []
+    for _ in range(3):
+        s0 = time.perf_counter()
+        ctx.ecfft(ec10, 10, inverse=False)
+        et.append(time.perf_counter() - s0)
+    e_ms, e_cnt = ctx.kernel_time_ms("ecfft")
+    ctx.set_profiling(False)
+    ecfft["log_n_10"] = {"wall_ms": round(float(np.median(et)) * 1e3, 3), "kernel_ms": round(e_ms * e_cnt / 3, 3)}
+
+    # plain encode from host memory (zgr_encode: the 32.5 MB raw blob crosses PCIe inside
+    # the call) -- the PCIe-inclusive rate, never the headline
+    out_h = Z.EncodedBlobOutputs(geo)
+    ctx.encode(raws[0])
+    ht = []
+    for _ in range(max(3, args.steps // 2)):
+        s0 = time.perf_counter()
+        lib_out = out_h.cstruct()
+        Z._check(Z.lib().zgr_encode(ctx.h, Z._p(np.frombuffer(raws[0], dtype=np.uint8)), len(raws[0]),
+                                    Z.ctypes.byref(lib_out)))
+        ht.append(time.perf_counter() - s0)
+    host_input = {"value": round(len(ht) / sum(ht), 4), "unit": "blobs/s", "ms_p50": round(float(np.median(ht)) * 1e3, 2),
+                  "note": "zgr_encode from host memory, outputs to host: the PCIe-inclusive plain rate"}
 
     # batched path: args.batch full-size blobs per call (two blobs in flight)
     nb_ = max(2, args.batch)
@@ -402,6 +529,9 @@ def main():
         "recovery": recovery,
         "verify_msm": verify,
         "ecfft": ecfft,
+        "ntt": ntt,
+        "msm_2pow20": msm,
+        "plain_host_input": host_input,
     }
     if not args.no_cpu:
         res["cpu_baseline"] = cpu_baseline(ctx, geo, raws[0], args.cpu_threads)
