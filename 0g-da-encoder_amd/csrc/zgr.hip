@@ -39,6 +39,7 @@
 #include "verify.h"
 #include "params_pp.h"
 #include "ntt.h"
+#include "params_io.h"
 
 using namespace zgr;
 
@@ -205,6 +206,20 @@ struct zgr_ctx {
     // resident params
     uint32_t* bases[3] = {nullptr, nullptr, nullptr};
     bool loaded[3] = {false, false, false};
+    // verifier side of each coset's params: the vanishes (2^(h+1) - 2 G2 points, affine
+    // Montgomery in the bamt order), then g2 and high_g2 (canonical affine G2)
+    uint32_t* van[3] = {nullptr, nullptr, nullptr};
+    uint32_t* vg2[3] = {nullptr, nullptr, nullptr};
+    bool van_ready[3] = {false, false, false};
+    // powers of tau (zgr_power_tau_setup / zgr_power_tau_load), canonical affine
+    uint32_t* pt_g1 = nullptr;       // n x 16 u32: [tau^i] G1
+    uint32_t* pt_high = nullptr;     // n x 16 u32: [tau^(3n + i)] G1
+    uint32_t* pt_g2 = nullptr;       // n x 32 u32: [tau^i] G2
+    uint32_t* pt_high_g2 = nullptr;  // 32 u32: [tau^(3n)] G2
+    bool pt_ready = false;
+    // fixed-base tables of the generators, built on first use
+    uint32_t* g1tab = nullptr;  // 32 x 256 affine G1 (k_fixed_base_table)
+    uint32_t* g2tab = nullptr;  // 32 x 256 affine G2 (k_g2_fixed_base_table)
     // fixed-base tables (shifted copies of the bases); fb == false: windowed MSM
     bool fb = false;
     uint32_t* fbtab[3] = {nullptr, nullptr, nullptr};
@@ -846,6 +861,130 @@ void destroy_streams(zgr_ctx* c) {
     }
 }
 
+// ---------------------------------------------------------------------------
+// public-parameter pipeline (params_pp.h, params_io.h)
+// ---------------------------------------------------------------------------
+size_t van_points(const zgr_ctx* c) { return ((size_t)2 << c->h) - 2; }
+
+// fixed-base tables of the G1 and G2 generators, built once per context
+int ensure_gen_tables(zgr_ctx* c, hipStream_t st) {
+    if (!c->g1tab) {
+        DALLOC(c->g1tab, 32 * 256 * 64);
+        LAUNCH(c, st, "params_table", k_fixed_base_table, blocks(32 * 256, 64), 64, 0, c->g1tab);
+    }
+    if (!c->g2tab) {
+        DALLOC(c->g2tab, 32 * 256 * G2_WORDS * 4);
+        LAUNCH(c, st, "params_table", k_g2_fixed_base_table, blocks(32 * 256, 64), 64, 0, c->g2tab);
+    }
+    return ZGR_OK;
+}
+
+int ensure_van(zgr_ctx* c, int coset) {
+    if (!c->van[coset]) DALLOC(c->van[coset], van_points(c) * G2_WORDS * 4);
+    if (!c->vg2[coset]) DALLOC(c->vg2[coset], 2 * G2_WORDS * 4);
+    return ZGR_OK;
+}
+
+// AMTParams::from_pp's G1 half for one coset (generate.rs:152-195, 204-232) from
+// canonical powers already in device memory: coset scaling, the EC-FFTs of gen_basis /
+// gen_quotients and enact, straight into the resident layout, then the shifted-copy
+// tables.  The caller holds the context lock and has synchronised the streams.
+int pp_bases(zgr_ctx* c, int coset, const uint32_t* g1pp, const uint32_t* high_g1pp) {
+    CHECK(alloc_bases(c, coset));
+    c->loaded[coset] = false;
+    hipStream_t st = c->lanes[0].st;
+    const size_t n = c->n;
+    const int log_n = c->log_n;
+    uint32_t *gp = nullptr, *hp = nullptr, *pts = nullptr, *w = nullptr;
+    auto body = [&]() -> int {
+        DALLOC(gp, n * X29_WORDS * 4);
+        DALLOC(hp, n * X29_WORDS * 4);
+        DALLOC(pts, n * X29_WORDS * 4);
+        DALLOC(w, 32);
+        LAUNCH(c, st, "params_pp", k_pp_coset_w, 1, 64, 0, c->consts, coset, w);
+        // coset-scaled powers (generate.rs:152-170); coset 0 is unscaled
+        LAUNCH(c, st, "params_pp", k_pp_scale, blocks(n, 64), 64, 0, g1pp, w, coset != 0, (uint32_t)n, gp);
+        LAUNCH(c, st, "params_pp", k_pp_scale, blocks(n, 64), 64, 0, high_g1pp, w, coset != 0, (uint32_t)n, hp);
+        const uint32_t half_n = (uint32_t)(n / 2);
+        auto transform = [&](const uint32_t* src, uint32_t m, bool inverse, int line) -> int {
+            LAUNCH(c, st, "params_pp", k_pp_load, blocks(n, 256), 256, 0, src, pts, log_n, m);
+            const uint32_t* tw = inverse ? c->tw_inv : c->tw;
+            for (int s = 1; s <= log_n; s++)
+                LAUNCH(c, st, "params_pp_fft", k_ec_dit_stage, blocks(half_n, 64), 64, 0, pts, tw, s, log_n - s, half_n);
+            // both directions end with size_inv: the iDFT's own, and gen_quotients' explicit one
+            LAUNCH(c, st, "params_pp", k_pp_store, blocks(n, 64), 64, 0, pts, c->consts, 1, log_n,
+                   c->bases[coset] + (size_t)line * n * 16);
+            return ZGR_OK;
+        };
+        for (int d = 1; d <= c->h; d++) CHECK(transform(gp, (uint32_t)(n >> d), false, d - 1));  // quotients
+        CHECK(transform(gp, 0, true, c->h));                                                    // basis
+        CHECK(transform(hp, 0, true, c->h + 1));                                                // high_basis
+        CHECK(build_fb(c, coset));
+        return sync_all(c);
+    };
+    int rc = body();
+    (void)hipStreamSynchronize(st);
+    for (void* p : {(void*)gp, (void*)hp, (void*)pts, (void*)w})
+        if (p) (void)hipFree(p);
+    if (rc == ZGR_OK) c->loaded[coset] = true;
+    return rc;
+}
+
+// AMTParams::from_pp's G2 half for one coset: vanishes[d - 1] = enact(gen_vanishes(g2pp',
+// d)) for d = 1..h (generate.rs:177-179, 234-263), g2 = g2pp[0], high_g2 (both unscaled:
+// w^0 = 1), from canonical powers in device memory.  Caller holds the lock.
+int pp_vanishes(zgr_ctx* c, int coset, const uint32_t* g2pp, const uint32_t* high_g2) {
+    CHECK(ensure_van(c, coset));
+    c->van_ready[coset] = false;
+    hipStream_t st = c->lanes[0].st;
+    uint32_t *work = nullptr, *w = nullptr;
+    auto body = [&]() -> int {
+        DALLOC(work, ((size_t)G2X_WORDS * 4) << c->h);
+        DALLOC(w, 32);
+        LAUNCH(c, st, "params_vanish", k_rec_clear, 1, 64, 0, c->flag);
+        LAUNCH(c, st, "params_vanish", k_pp_coset_w, 1, 64, 0, c->consts, coset, w);
+        for (int d = 1; d <= c->h; d++) {
+            LAUNCH(c, st, "params_vanish", k_vanish_load, blocks((size_t)1 << d, 64), 64, 0, g2pp, w, coset != 0,
+                   c->log_n, d, work);
+            for (int s = 1; s <= d; s++)
+                LAUNCH(c, st, "params_vanish", k_vanish_stage, blocks((size_t)1 << (d - 1), 64), 64, 0, work, c->tw_inv,
+                       s, c->log_n, 1u << (d - 1));
+            LAUNCH(c, st, "params_vanish", k_vanish_store, blocks((size_t)1 << d, 64), 64, 0, work, d, c->van[coset],
+                   c->flag);
+        }
+        HIPCHK(hipMemcpyAsync(c->vg2[coset], g2pp, G2_WORDS * 4, hipMemcpyDeviceToDevice, st));
+        HIPCHK(hipMemcpyAsync(c->vg2[coset] + G2_WORDS, high_g2, G2_WORDS * 4, hipMemcpyDeviceToDevice, st));
+        int bad = 0;
+        HIPCHK(hipMemcpyAsync(&bad, c->flag, sizeof(int), hipMemcpyDeviceToHost, st));
+        CHECK(sync_all(c));
+        if (bad) return fail(ZGR_ERR_INVALID_ARG, "a vanish point is infinity (tau' on a level's domain)");
+        return ZGR_OK;
+    };
+    int rc = body();
+    (void)hipStreamSynchronize(st);
+    for (void* p : {(void*)work, (void*)w})
+        if (p) (void)hipFree(p);
+    if (rc == ZGR_OK) c->van_ready[coset] = true;
+    return rc;
+}
+
+// little-endian writer of a parameter file image (counts bytes when out is null)
+struct ImageWriter {
+    uint8_t* out;
+    size_t cap, len = 0;
+    void bytes(const void* p, size_t n) {
+        if (out && len + n <= cap) memcpy(out + len, p, n);
+        len += n;
+    }
+    void u64(uint64_t v) { bytes(&v, 8); }
+    void u8(uint8_t v) { bytes(&v, 1); }
+    uint8_t* at(size_t n) {  // the next n bytes, to be filled by a device copy
+        uint8_t* p = out && len + n <= cap ? out + len : nullptr;
+        len += n;
+        return p;
+    }
+};
+
 }  // namespace
 
 extern "C" {
@@ -991,7 +1130,12 @@ void zgr_ctx_destroy(zgr_ctx* c) {
         if (p) (void)hipFree(p);
     for (void* p : c->rec_buf)
         if (p) (void)hipFree(p);
+    for (void* p : {(void*)c->pt_g1, (void*)c->pt_high, (void*)c->pt_g2, (void*)c->pt_high_g2, (void*)c->g1tab,
+                    (void*)c->g2tab})
+        if (p) (void)hipFree(p);
     for (int i = 0; i < 3; i++) {
+        if (c->van[i]) (void)hipFree(c->van[i]);
+        if (c->vg2[i]) (void)hipFree(c->vg2[i]);
         if (c->scale[i]) (void)hipFree(c->scale[i]);
         if (c->bases[i]) (void)hipFree(c->bases[i]);
         if (c->fbtab[i]) (void)hipFree(c->fbtab[i]);
@@ -1051,8 +1195,7 @@ int zgr_params_load_bamt(zgr_ctx* c, int coset, const uint8_t* bamt, size_t nbyt
         return fail(ZGR_ERR_INVALID_ARG, "bamt degree %d / sub-degree %d, context has %d / %d", degree, sub,
                     c->log_n, c->h);
     const size_t line = c->n * 64;
-    size_t vanish = 0;
-    for (int d = 0; d < sub; d++) vanish += ((size_t)2 << d) * 128;
+    const size_t vanish = van_points(c) * 128;
     // g2 / high_g2: ark uncompressed G2 affine = x.c0 x.c1 y.c0 y.c1, 32 B each, the
     // infinity / sign flags in the spare top bits of the last byte
     const size_t g2 = 128;
@@ -1062,19 +1205,33 @@ int zgr_params_load_bamt(zgr_ctx* c, int coset, const uint8_t* bamt, size_t nbyt
     HIPCHK(hipSetDevice(c->device));
     CHECK(sync_all(c));  // blobs still in flight read the bases and tables about to be replaced
     CHECK(alloc_bases(c, coset));
+    CHECK(ensure_van(c, coset));
     c->loaded[coset] = false;
+    c->van_ready[coset] = false;
     hipStream_t st = c->lanes[0].st;
     uint8_t* dev = (uint8_t*)c->bases[coset];
+    // the verifier half: g2 and high_g2 as canonical points (the flag bits cleared; the
+    // infinity flag gives zeros), the vanishes verbatim (Montgomery, the device layout)
+    uint8_t vg2[2][128];
+    for (int k = 0; k < 2; k++) {
+        memcpy(vg2[k], bamt + 6 + k * g2, g2);
+        const bool inf = vg2[k][127] & 0x40;
+        vg2[k][127] &= 0x3f;
+        if (inf) memset(vg2[k], 0, 128);
+    }
+    HIPCHK(hipMemcpy(c->vg2[coset], vg2, sizeof(vg2), hipMemcpyHostToDevice));
     const uint8_t* p = bamt + 6 + 2 * g2;
     HIPCHK(hipMemcpyAsync(dev + (size_t)c->h * line, p, line, hipMemcpyHostToDevice, st));  // basis
     p += line;
     for (int d = 0; d < sub; d++, p += line)  // quotients
         HIPCHK(hipMemcpyAsync(dev + (size_t)d * line, p, line, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(c->van[coset], p, vanish, hipMemcpyHostToDevice, st));  // vanishes
     p += vanish;
     HIPCHK(hipMemcpyAsync(dev + (size_t)(c->h + 1) * line, p, line, hipMemcpyHostToDevice, st));  // high_basis
     CHECK(build_fb(c, coset));
     CHECK(sync_all(c));
     c->loaded[coset] = true;
+    c->van_ready[coset] = true;
     return ZGR_OK;
 }
 
@@ -1086,46 +1243,20 @@ int zgr_params_from_pp(zgr_ctx* c, int coset, const uint8_t* g1pp, const uint8_t
         return fail(ZGR_ERR_INVALID_ARG, "%zu powers of tau, the context's domain has %zu", n_points, c->n);
     HIPCHK(hipSetDevice(c->device));
     CHECK(sync_all(c));  // blobs still in flight read the bases and tables about to be replaced
-    CHECK(alloc_bases(c, coset));
-    c->loaded[coset] = false;
     hipStream_t st = c->lanes[0].st;
-    const size_t n = c->n;
-    const int log_n = c->log_n;
-    uint32_t *din = nullptr, *gp = nullptr, *hp = nullptr, *pts = nullptr, *w = nullptr;
+    uint32_t *din = nullptr, *hin = nullptr;
     auto body = [&]() -> int {
-        DALLOC(din, n * 64);
-        DALLOC(gp, n * X29_WORDS * 4);
-        DALLOC(hp, n * X29_WORDS * 4);
-        DALLOC(pts, n * X29_WORDS * 4);
-        DALLOC(w, 32);
-        LAUNCH(c, st, "params_pp", k_pp_coset_w, 1, 64, 0, c->consts, coset, w);
-        // coset-scaled powers (generate.rs:152-170); coset 0 is unscaled
-        HIPCHK(hipMemcpyAsync(din, g1pp, n * 64, hipMemcpyHostToDevice, st));
-        LAUNCH(c, st, "params_pp", k_pp_scale, blocks(n, 64), 64, 0, din, w, coset != 0, (uint32_t)n, gp);
-        HIPCHK(hipMemcpyAsync(din, high_g1pp, n * 64, hipMemcpyHostToDevice, st));
-        LAUNCH(c, st, "params_pp", k_pp_scale, blocks(n, 64), 64, 0, din, w, coset != 0, (uint32_t)n, hp);
-        const uint32_t half_n = (uint32_t)(n / 2);
-        auto transform = [&](const uint32_t* src, uint32_t m, bool inverse, int line) -> int {
-            LAUNCH(c, st, "params_pp", k_pp_load, blocks(n, 256), 256, 0, src, pts, log_n, m);
-            const uint32_t* tw = inverse ? c->tw_inv : c->tw;
-            for (int s = 1; s <= log_n; s++)
-                LAUNCH(c, st, "params_pp_fft", k_ec_dit_stage, blocks(half_n, 64), 64, 0, pts, tw, s, log_n - s, half_n);
-            // both directions end with size_inv: the iDFT's own, and gen_quotients' explicit one
-            LAUNCH(c, st, "params_pp", k_pp_store, blocks(n, 64), 64, 0, pts, c->consts, 1, log_n,
-                   c->bases[coset] + (size_t)line * n * 16);
-            return ZGR_OK;
-        };
-        for (int d = 1; d <= c->h; d++) CHECK(transform(gp, (uint32_t)(n >> d), false, d - 1));  // quotients
-        CHECK(transform(gp, 0, true, c->h));                                                    // basis
-        CHECK(transform(hp, 0, true, c->h + 1));                                                // high_basis
-        CHECK(build_fb(c, coset));
-        return sync_all(c);
+        DALLOC(din, c->n * 64);
+        DALLOC(hin, c->n * 64);
+        HIPCHK(hipMemcpyAsync(din, g1pp, c->n * 64, hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(hin, high_g1pp, c->n * 64, hipMemcpyHostToDevice, st));
+        c->van_ready[coset] = false;  // the G2 half is not given here
+        return pp_bases(c, coset, din, hin);
     };
     int rc = body();
     (void)hipStreamSynchronize(st);
-    for (void* p : {(void*)din, (void*)gp, (void*)hp, (void*)pts, (void*)w})
+    for (void* p : {(void*)din, (void*)hin})
         if (p) (void)hipFree(p);
-    if (rc == ZGR_OK) c->loaded[coset] = true;
     return rc;
 }
 
@@ -1135,19 +1266,27 @@ int zgr_params_generate(zgr_ctx* c, const uint8_t tau[32]) {
     HIPCHK(hipSetDevice(c->device));
     CHECK(sync_all(c));
     hipStream_t st = c->lanes[0].st;
-    uint32_t *table = nullptr, *tau_d = nullptr, *pc = nullptr;
+    uint32_t *tau_d = nullptr, *pc = nullptr;
     auto body = [&]() -> int {
-        DALLOC(table, 32 * 256 * 64);
         DALLOC(tau_d, 32);
         DALLOC(pc, (3 + c->h) * 32);
         HIPCHK(hipMemcpyAsync(tau_d, tau, 32, hipMemcpyHostToDevice, st));
-        HIPCHK(hipMemsetAsync(c->flag, 0, sizeof(int), st));
-        LAUNCH(c, st, "params_table", k_fixed_base_table, blocks(32 * 256, 64), 64, 0, table);
+        LAUNCH(c, st, "params_gen", k_rec_clear, 1, 64, 0, c->flag);
+        CHECK(ensure_gen_tables(c, st));
         for (int cs = 0; cs < c->coset_n; cs++) {
             CHECK(alloc_bases(c, cs));
+            CHECK(ensure_van(c, cs));
+            c->loaded[cs] = c->van_ready[cs] = false;
             LAUNCH(c, st, "params_consts", k_params_consts, 1, 1, 0, tau_d, c->consts, cs, c->log_n, c->h, pc);
-            LAUNCH(c, st, "params_gen", k_params_gen, blocks(c->n, 64), 64, 0, c->bases[cs], c->consts, pc, table,
+            LAUNCH(c, st, "params_gen", k_params_gen, blocks(c->n, 64), 64, 0, c->bases[cs], c->consts, pc, c->g1tab,
                    c->log_n, c->h, c->flag);
+            // the verifier half in closed form: vanishes, g2 = [1] G2, high_g2 = [tau^(3N)] G2
+            LAUNCH(c, st, "params_vanish", k_vanish_closed, blocks(van_points(c), 64), 64, 0, pc, c->consts, c->log_n,
+                   c->h, c->g2tab, c->van[cs], c->flag);
+            LAUNCH(c, st, "params_gen", k_power_tau, 1, 64, 0, tau_d, (uint64_t)0, 1u, (const uint32_t*)nullptr,
+                   c->g2tab, (uint32_t*)nullptr, c->vg2[cs], c->flag);
+            LAUNCH(c, st, "params_gen", k_power_tau, 1, 64, 0, tau_d, (uint64_t)3 * c->n, 1u, (const uint32_t*)nullptr,
+                   c->g2tab, (uint32_t*)nullptr, c->vg2[cs] + G2_WORDS, c->flag);
         }
         int bad = 0;
         HIPCHK(hipMemcpyAsync(&bad, c->flag, sizeof(int), hipMemcpyDeviceToHost, st));
@@ -1155,13 +1294,247 @@ int zgr_params_generate(zgr_ctx* c, const uint8_t tau[32]) {
         if (bad) return fail(ZGR_ERR_INVALID_ARG, "tau hits the evaluation domain (or a base is infinity)");
         for (int cs = 0; cs < c->coset_n; cs++) CHECK(build_fb(c, cs));
         CHECK(sync_all(c));
-        for (int cs = 0; cs < c->coset_n; cs++) c->loaded[cs] = true;
+        for (int cs = 0; cs < c->coset_n; cs++) c->loaded[cs] = c->van_ready[cs] = true;
         return ZGR_OK;
     };
     int rc = body();
     (void)hipStreamSynchronize(st);
-    for (void* p : {(void*)table, (void*)tau_d, (void*)pc})
+    for (void* p : {(void*)tau_d, (void*)pc})
         if (p) (void)hipFree(p);
+    return rc;
+}
+
+int zgr_power_tau_setup(zgr_ctx* c, const uint8_t tau[32]) {
+    if (!c || !tau) return fail(ZGR_ERR_INVALID_ARG, "null argument");
+    std::lock_guard<std::mutex> lk(c->mu);
+    HIPCHK(hipSetDevice(c->device));
+    CHECK(sync_all(c));
+    hipStream_t st = c->lanes[0].st;
+    const size_t n = c->n;
+    uint32_t* tau_d = nullptr;
+    auto body = [&]() -> int {
+        c->pt_ready = false;
+        if (!c->pt_g1) DALLOC(c->pt_g1, n * 64);
+        if (!c->pt_high) DALLOC(c->pt_high, n * 64);
+        if (!c->pt_g2) DALLOC(c->pt_g2, n * G2_WORDS * 4);
+        if (!c->pt_high_g2) DALLOC(c->pt_high_g2, G2_WORDS * 4);
+        DALLOC(tau_d, 32);
+        HIPCHK(hipMemcpyAsync(tau_d, tau, 32, hipMemcpyHostToDevice, st));
+        LAUNCH(c, st, "power_tau", k_rec_clear, 1, 64, 0, c->flag);
+        CHECK(ensure_gen_tables(c, st));
+        // H = 2^(log_n + 2) - 2^log_n = 3N (power_tau.rs:66-67)
+        LAUNCH(c, st, "power_tau", k_power_tau, blocks(n, 64), 64, 0, tau_d, (uint64_t)0, (uint32_t)n, c->g1tab,
+               c->g2tab, c->pt_g1, c->pt_g2, c->flag);
+        LAUNCH(c, st, "power_tau", k_power_tau, blocks(n, 64), 64, 0, tau_d, (uint64_t)3 * n, (uint32_t)n, c->g1tab,
+               (const uint32_t*)nullptr, c->pt_high, (uint32_t*)nullptr, c->flag);
+        LAUNCH(c, st, "power_tau", k_power_tau, 1, 64, 0, tau_d, (uint64_t)3 * n, 1u, (const uint32_t*)nullptr, c->g2tab,
+               (uint32_t*)nullptr, c->pt_high_g2, c->flag);
+        int bad = 0;
+        HIPCHK(hipMemcpyAsync(&bad, c->flag, sizeof(int), hipMemcpyDeviceToHost, st));
+        CHECK(sync_all(c));
+        if (bad) return fail(ZGR_ERR_INVALID_ARG, "tau = 0 gives points at infinity");
+        c->pt_ready = true;
+        return ZGR_OK;
+    };
+    int rc = body();
+    (void)hipStreamSynchronize(st);
+    if (tau_d) (void)hipFree(tau_d);
+    return rc;
+}
+
+int zgr_power_tau_load(zgr_ctx* c, const uint8_t* g1pp, const uint8_t* g2pp, const uint8_t* high_g1pp,
+                       const uint8_t* high_g2, size_t n_points) {
+    if (!c || !g1pp || !g2pp || !high_g1pp || !high_g2) return fail(ZGR_ERR_INVALID_ARG, "null argument");
+    if (n_points != c->n)
+        return fail(ZGR_ERR_INVALID_ARG, "%zu powers of tau, the context's domain has %zu", n_points, c->n);
+    std::lock_guard<std::mutex> lk(c->mu);
+    HIPCHK(hipSetDevice(c->device));
+    CHECK(sync_all(c));
+    const size_t n = c->n;
+    c->pt_ready = false;
+    if (!c->pt_g1) DALLOC(c->pt_g1, n * 64);
+    if (!c->pt_high) DALLOC(c->pt_high, n * 64);
+    if (!c->pt_g2) DALLOC(c->pt_g2, n * G2_WORDS * 4);
+    if (!c->pt_high_g2) DALLOC(c->pt_high_g2, G2_WORDS * 4);
+    HIPCHK(hipMemcpy(c->pt_g1, g1pp, n * 64, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->pt_high, high_g1pp, n * 64, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->pt_g2, g2pp, n * G2_WORDS * 4, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->pt_high_g2, high_g2, G2_WORDS * 4, hipMemcpyHostToDevice));
+    c->pt_ready = true;
+    return ZGR_OK;
+}
+
+int zgr_power_tau_download(zgr_ctx* c, uint8_t* g1pp, uint8_t* g2pp, uint8_t* high_g1pp, uint8_t* high_g2) {
+    if (!c) return fail(ZGR_ERR_INVALID_ARG, "null argument");
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (!c->pt_ready) return fail(ZGR_ERR_NOT_LOADED, "no powers of tau in this context");
+    HIPCHK(hipSetDevice(c->device));
+    const size_t n = c->n;
+    if (g1pp) HIPCHK(hipMemcpy(g1pp, c->pt_g1, n * 64, hipMemcpyDeviceToHost));
+    if (g2pp) HIPCHK(hipMemcpy(g2pp, c->pt_g2, n * G2_WORDS * 4, hipMemcpyDeviceToHost));
+    if (high_g1pp) HIPCHK(hipMemcpy(high_g1pp, c->pt_high, n * 64, hipMemcpyDeviceToHost));
+    if (high_g2) HIPCHK(hipMemcpy(high_g2, c->pt_high_g2, G2_WORDS * 4, hipMemcpyDeviceToHost));
+    return ZGR_OK;
+}
+
+int zgr_params_from_power_tau(zgr_ctx* c, int coset) {
+    if (!c) return fail(ZGR_ERR_INVALID_ARG, "null argument");
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (coset < 0 || coset >= c->coset_n) return fail(ZGR_ERR_INVALID_ARG, "coset %d out of range", coset);
+    if (!c->pt_ready) return fail(ZGR_ERR_NOT_LOADED, "no powers of tau in this context");
+    HIPCHK(hipSetDevice(c->device));
+    CHECK(sync_all(c));  // blobs still in flight read the bases and tables about to be replaced
+    CHECK(pp_bases(c, coset, c->pt_g1, c->pt_high));
+    return pp_vanishes(c, coset, c->pt_g2, c->pt_high_g2);
+}
+
+int zgr_params_g2(zgr_ctx* c, int coset, uint8_t* vanishes, uint8_t* g2, uint8_t* high_g2) {
+    if (!c) return fail(ZGR_ERR_INVALID_ARG, "null argument");
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (coset < 0 || coset >= c->coset_n) return fail(ZGR_ERR_INVALID_ARG, "coset %d out of range", coset);
+    if (!c->van_ready[coset]) return fail(ZGR_ERR_NOT_LOADED, "no verifier params (vanishes) for coset %d", coset);
+    HIPCHK(hipSetDevice(c->device));
+    hipStream_t st = c->lanes[0].st;
+    uint32_t* tmp = nullptr;
+    auto body = [&]() -> int {
+        if (vanishes) {  // Montgomery in the device -> canonical
+            DALLOC(tmp, van_points(c) * G2_WORDS * 4);
+            LAUNCH(c, st, "params_io", k_fq_convert, blocks(van_points(c) * 4, 256), 256, 0, c->van[coset], tmp,
+                   (uint64_t)van_points(c) * 4, 0);
+            HIPCHK(hipMemcpyAsync(vanishes, tmp, van_points(c) * G2_WORDS * 4, hipMemcpyDeviceToHost, st));
+        }
+        if (g2) HIPCHK(hipMemcpyAsync(g2, c->vg2[coset], G2_WORDS * 4, hipMemcpyDeviceToHost, st));
+        if (high_g2) HIPCHK(hipMemcpyAsync(high_g2, c->vg2[coset] + G2_WORDS, G2_WORDS * 4, hipMemcpyDeviceToHost, st));
+        return sync_all(c);
+    };
+    int rc = body();
+    (void)hipStreamSynchronize(st);
+    if (tmp) (void)hipFree(tmp);
+    return rc;
+}
+
+int zgr_write_params_file(zgr_ctx* c, int kind, int coset, int sub_depth, uint8_t* out, size_t cap, size_t* len) {
+    if (!c || !len) return fail(ZGR_ERR_INVALID_ARG, "null argument");
+    std::lock_guard<std::mutex> lk(c->mu);
+    const bool amt = kind == ZGR_FILE_AMT_MONT || kind == ZGR_FILE_AMT || kind == ZGR_FILE_AMT_VERIFY;
+    const bool ptau = kind == ZGR_FILE_PTAU || kind == ZGR_FILE_PTAU_MONT;
+    if (!amt && !ptau) return fail(ZGR_ERR_INVALID_ARG, "unknown parameter file kind %d", kind);
+    if (amt && (coset < 0 || coset >= c->coset_n)) return fail(ZGR_ERR_INVALID_ARG, "coset %d out of range", coset);
+    if (kind == ZGR_FILE_AMT_VERIFY && (sub_depth < 0 || sub_depth > c->h))
+        return fail(ZGR_ERR_INVALID_ARG, "verify depth %d outside [0, %d]", sub_depth, c->h);
+    if (amt && !c->loaded[coset]) return fail(ZGR_ERR_NOT_LOADED, "params for coset %d are not resident", coset);
+    if (amt && !c->van_ready[coset])
+        return fail(ZGR_ERR_NOT_LOADED, "no verifier params (vanishes, g2) for coset %d", coset);
+    if (ptau && !c->pt_ready) return fail(ZGR_ERR_NOT_LOADED, "no powers of tau in this context");
+    HIPCHK(hipSetDevice(c->device));
+    CHECK(sync_all(c));
+    const size_t n = c->n;
+    const int h = c->h;
+    // one pass counts the image, a second one fills it (when out has room)
+    ImageWriter w{out, cap};
+    hipStream_t st = c->lanes[0].st;
+    uint32_t* stage = nullptr;  // device staging of one converted line (at most n G2 points)
+    auto emit = [&](bool fill) -> int {
+        w.len = 0;
+        // ark form of n_pts points from device memory (g2: G2 else G1; mont: Montgomery input)
+        auto ark = [&](const uint32_t* src, size_t n_pts, bool g2, bool mont, bool compressed) -> int {
+            const size_t bytes = n_pts * (g2 ? 64 : 32) * (compressed ? 1 : 2);
+            uint8_t* dst = w.at(bytes);
+            if (!fill || !dst) return ZGR_OK;
+            if (g2)
+                LAUNCH(c, st, "params_io", k_g2_ark, blocks(n_pts, 256), 256, 0, src, (uint32_t)n_pts, (int)mont,
+                       (int)compressed, stage);
+            else
+                LAUNCH(c, st, "params_io", k_g1_ark, blocks(n_pts, 256), 256, 0, src, (uint32_t)n_pts, (int)mont,
+                       (int)compressed, stage);
+            HIPCHK(hipMemcpyAsync(dst, stage, bytes, hipMemcpyDeviceToHost, st));
+            return sync_all(c);  // the staging buffer is reused by the next section
+        };
+        // raw copy (Montgomery in the device layout), or canonical -> Montgomery first
+        auto raw = [&](const uint32_t* src, size_t bytes, bool to_mont) -> int {
+            uint8_t* dst = w.at(bytes);
+            if (!fill || !dst) return ZGR_OK;
+            if (to_mont) {
+                LAUNCH(c, st, "params_io", k_fq_convert, blocks(bytes / 32, 256), 256, 0, src, stage,
+                       (uint64_t)(bytes / 32), 1);
+                src = stage;
+            }
+            HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, st));
+            return sync_all(c);
+        };
+        const uint32_t* line = amt ? c->bases[coset] : nullptr;
+        auto lineptr = [&](int L) { return line + (size_t)L * n * 16; };
+        const uint32_t* van = amt ? c->van[coset] : nullptr;
+        const uint32_t* vg2 = amt ? c->vg2[coset] : nullptr;
+        switch (kind) {
+            case ZGR_FILE_AMT_MONT:  // write_amt_params (fast_serde_bn254.rs:20-55)
+                w.bytes("bamt", 4);
+                w.u8((uint8_t)c->log_n);
+                w.u8((uint8_t)h);
+                CHECK(ark(vg2, 1, true, false, false));             // g2
+                CHECK(ark(vg2 + G2_WORDS, 1, true, false, false));  // high_g2
+                CHECK(raw(lineptr(h), n * 64, false));              // basis
+                for (int d = 0; d < h; d++) CHECK(raw(lineptr(d), n * 64, false));  // quotients
+                CHECK(raw(van, van_points(c) * 128, false));        // vanishes
+                CHECK(raw(lineptr(h + 1), n * 64, false));          // high_basis
+                break;
+            case ZGR_FILE_AMT:  // serialize_uncompressed of AMTParams (prove_params/serde.rs)
+                w.u64(n);
+                CHECK(ark(lineptr(h), n, false, true, false));
+                w.u64((uint64_t)h);
+                for (int d = 0; d < h; d++) {
+                    w.u64(n);
+                    CHECK(ark(lineptr(d), n, false, true, false));
+                }
+                w.u64((uint64_t)h);
+                for (int d = 1; d <= h; d++) {
+                    w.u64((uint64_t)1 << d);
+                    CHECK(ark(van + G2_WORDS * (((size_t)1 << d) - 2), (size_t)1 << d, true, true, false));
+                }
+                CHECK(ark(vg2, 1, true, false, false));
+                w.u64(n);
+                CHECK(ark(lineptr(h + 1), n, false, true, false));
+                CHECK(ark(vg2 + G2_WORDS, 1, true, false, false));
+                break;
+            case ZGR_FILE_AMT_VERIFY:  // serialize_uncompressed of AMTVerifyParams (verify_params.rs:22-28, 81-98)
+                w.u64(n);
+                CHECK(ark(lineptr(h), n, false, true, false));
+                w.u64((uint64_t)sub_depth);
+                for (int d = 1; d <= sub_depth; d++) {
+                    w.u64((uint64_t)1 << d);
+                    CHECK(ark(van + G2_WORDS * (((size_t)1 << d) - 2), (size_t)1 << d, true, true, false));
+                }
+                CHECK(ark(vg2, 1, true, false, false));
+                CHECK(ark(vg2 + G2_WORDS, 1, true, false, false));
+                break;
+            case ZGR_FILE_PTAU:  // serialize_compressed of PowerTau (power_tau.rs:20-26)
+                w.u64(n);
+                CHECK(ark(c->pt_g1, n, false, false, true));
+                w.u64(n);
+                CHECK(ark(c->pt_g2, n, true, false, true));
+                w.u64(n);
+                CHECK(ark(c->pt_high, n, false, false, true));
+                CHECK(ark(c->pt_high_g2, 1, true, false, true));
+                break;
+            case ZGR_FILE_PTAU_MONT:  // write_power_tau (fast_serde_bn254.rs:90-113)
+                w.bytes("ptau", 4);
+                w.u8((uint8_t)c->log_n);
+                CHECK(ark(c->pt_high_g2, 1, true, false, false));
+                CHECK(raw(c->pt_g1, n * 64, true));
+                CHECK(raw(c->pt_g2, n * G2_WORDS * 4, true));
+                CHECK(raw(c->pt_high, n * 64, true));
+                break;
+        }
+        return ZGR_OK;
+    };
+    CHECK(emit(false));
+    *len = w.len;
+    if (!out) return ZGR_OK;
+    if (cap < w.len) return fail(ZGR_ERR_INVALID_ARG, "output holds %zu bytes, the file image needs %zu", cap, w.len);
+    int rc = dmalloc((void**)&stage, n * G2_WORDS * 4);
+    if (!rc) rc = emit(true);
+    (void)hipStreamSynchronize(st);
+    if (stage) (void)hipFree(stage);
     return rc;
 }
 

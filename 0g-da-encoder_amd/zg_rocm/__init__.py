@@ -27,6 +27,16 @@ ERR_DEVICE = -4
 ERR_NOT_LOADED = -5
 ERR_INCONSISTENT = -6
 
+# parameter file kinds of zgr_write_params_file
+FILE_AMT_MONT = 0
+FILE_AMT = 1
+FILE_AMT_VERIFY = 2
+FILE_PTAU = 3
+FILE_PTAU_MONT = 4
+# type_hash::<Bn254>()[..6] of the reference's file names (crates/amt/src/utils.rs:11-52,
+# README.md:73-82)
+BN254_HASH = "5DWgDV"
+
 # exported symbols and their ctypes signatures (kept in step with include/zgr.h)
 P = ctypes.c_void_p
 I = ctypes.c_int
@@ -40,6 +50,12 @@ SIGNATURES = {
     "zgr_params_generate": (I, [P, P]),
     "zgr_params_from_pp": (I, [P, I, P, P, S]),
     "zgr_params_download": (I, [P, I, P, S]),
+    "zgr_power_tau_setup": (I, [P, P]),
+    "zgr_power_tau_load": (I, [P, P, P, P, P, S]),
+    "zgr_power_tau_download": (I, [P, P, P, P, P]),
+    "zgr_params_from_power_tau": (I, [P, I]),
+    "zgr_params_g2": (I, [P, I, P, P, P]),
+    "zgr_write_params_file": (I, [P, I, I, I, P, S, ctypes.POINTER(S)]),
     "zgr_amt_prove": (I, [P, I, P, S, P, P, P, P]),
     "zgr_multiple_multiexp": (I, [P, I, P, S, P]),
     "zgr_encode": (I, [P, P, S, P]),
@@ -303,6 +319,42 @@ class Context:
         b = np.ascontiguousarray(high_g1pp, dtype=np.uint8).reshape(-1)
         _check(lib().zgr_params_from_pp(self.h, coset, _p(a), _p(b), a.size // 64))
 
+    # ---- powers of tau, verifier params, parameter files ----
+    def power_tau_setup(self, tau):
+        """PowerTau::setup_inner with a known tau, on the device (kept in the context)."""
+        t = np.frombuffer((tau % R_MOD).to_bytes(32, "little"), dtype=np.uint8).copy()
+        _check(lib().zgr_power_tau_setup(self.h, _p(t)))
+
+    def power_tau_load(self, g1pp, g2pp, high_g1pp, high_g2):
+        a = [np.ascontiguousarray(x, dtype=np.uint8).reshape(-1) for x in (g1pp, g2pp, high_g1pp, high_g2)]
+        _check(lib().zgr_power_tau_load(self.h, _p(a[0]), _p(a[1]), _p(a[2]), _p(a[3]), a[0].size // 64))
+
+    def power_tau_download(self):
+        """-> g1pp (n, 64), g2pp (n, 128), high_g1pp (n, 64), high_g2 (128,): canonical affine."""
+        n = self.geo.n
+        g1, g2, hi, hg2 = (np.zeros(k, dtype=np.uint8) for k in (64 * n, 128 * n, 64 * n, 128))
+        _check(lib().zgr_power_tau_download(self.h, _p(g1), _p(g2), _p(hi), _p(hg2)))
+        return g1.reshape(n, 64), g2.reshape(n, 128), hi.reshape(n, 64), hg2
+
+    def params_from_power_tau(self, coset):
+        _check(lib().zgr_params_from_power_tau(self.h, coset))
+
+    def params_g2(self, coset):
+        """-> vanishes ((2^(h+1) - 2), 128), g2 (128,), high_g2 (128,): canonical affine G2."""
+        v = np.zeros((((2 << self.geo.h) - 2), 128), dtype=np.uint8)
+        g2, hg2 = np.zeros(128, dtype=np.uint8), np.zeros(128, dtype=np.uint8)
+        _check(lib().zgr_params_g2(self.h, coset, _p(v), _p(g2), _p(hg2)))
+        return v, g2, hg2
+
+    def write_params_file(self, kind, coset=0, sub_depth=None):
+        """The image (bytes) of one of the reference's parameter files (FILE_*)."""
+        sd = self.geo.h if sub_depth is None else sub_depth
+        n = ctypes.c_size_t(0)
+        _check(lib().zgr_write_params_file(self.h, kind, coset, sd, None, 0, ctypes.byref(n)))
+        out = np.empty(max(n.value, 1), dtype=np.uint8)
+        _check(lib().zgr_write_params_file(self.h, kind, coset, sd, _p(out), out.size, ctypes.byref(n)))
+        return out[:n.value].tobytes()
+
     def params_download(self, coset):
         out = np.zeros(self.geo.params_bytes() // 8, dtype=np.uint64)
         _check(lib().zgr_params_download(self.h, coset, _p(out), out.nbytes))
@@ -448,6 +500,50 @@ class Context:
 
     def kernel_time_reset(self):
         lib().zgr_kernel_time_reset(self.h)
+
+
+def params_file_name(kind, log_n, h, coset=0):
+    """The reference's file names (ptau_file_name, amtp_file_name, amtp_verify_file_name,
+    crates/amt/src/utils.rs:11-52)."""
+    if kind == FILE_PTAU:
+        return "power-tau-%s-%02d.bin" % (BN254_HASH, log_n)
+    if kind == FILE_PTAU_MONT:
+        return "power-tau-mont-%s-%02d.bin" % (BN254_HASH, log_n)
+    prefix = {FILE_AMT_MONT: "amt-prove-coset%d-mont", FILE_AMT: "amt-prove-coset%d",
+              FILE_AMT_VERIFY: "amt-verify-coset%d"}[kind] % coset
+    return "%s-%s-%02d-%02d.bin" % (prefix, BN254_HASH, h, log_n)
+
+
+def build_params(ctx, out_dir, tau=None, verify_depth=None):
+    """bin/build_params.rs:24-53 in create mode on the device: powers of tau (a fresh
+    random tau unless one is given), then for every coset the prove bases and vanishes
+    (AMTParams::from_pp) and the reference's files -- the compressed powers of tau, the
+    unmont and mont (bamt) AMT params and the verify params.  The prove depth is the
+    context's log_row and must equal verify_depth (build_params passes its verify depth
+    as the prove depth).  Returns the written paths."""
+    import secrets
+
+    g = ctx.geo
+    vd = g.h if verify_depth is None else verify_depth
+    if vd != g.h:
+        raise ValueError("build_params uses the verify depth as the prove depth: %d != log_row %d" % (vd, g.h))
+    os.makedirs(out_dir, exist_ok=True)
+    ctx.power_tau_setup(secrets.randbelow(R_MOD - 1) + 1 if tau is None else tau)
+    written = []
+
+    def put(kind, coset=0):
+        path = os.path.join(out_dir, params_file_name(kind, g.log_n, vd, coset))
+        with open(path, "wb") as f:
+            f.write(ctx.write_params_file(kind, coset, vd))
+        written.append(path)
+
+    put(FILE_PTAU)
+    for c in range(g.coset_n):
+        ctx.params_from_power_tau(c)
+        put(FILE_AMT, c)
+        put(FILE_AMT_MONT, c)
+        put(FILE_AMT_VERIFY, c)
+    return written
 
 
 def make_context(geo=PRODUCTION, fixed_base=True, group_plan=None, device=0):

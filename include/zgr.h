@@ -101,6 +101,56 @@ int zgr_params_from_pp(zgr_ctx *ctx, int coset, const uint8_t *g1pp, const uint8
 /* Copy one coset's resident bases back to host memory (layout above). */
 int zgr_params_download(zgr_ctx *ctx, int coset, void *out, size_t nbytes);
 
+/* ---- the rest of the parameter pipeline: powers of tau, the verifier's G2 data and the
+ *      reference's parameter files (bin/build_params.rs:24-53) ----
+ * G2 points in memory are canonical affine x.c0 || x.c1 || y.c0 || y.c1 (128 B; 128 zero
+ * bytes = infinity), G1 points canonical affine x || y (64 B).
+ * zgr_params_generate also sets every coset's verifier data (closed form, from tau);
+ * zgr_params_load_bamt keeps the file's; zgr_params_from_pp (G1 only) clears it. */
+/* PowerTau::setup_inner with a known tau (crates/amt/src/power_tau.rs:54-80), on the device,
+ * kept in the context: g1pp[i] = [tau^i] G1, g2pp[i] = [tau^i] G2, high_g1pp[i] =
+ * [tau^(H+i)] G1 (i < 2^log_n, H = 3 * 2^log_n, power_tau.rs:66-67), high_g2 = [tau^H] G2. */
+int zgr_power_tau_setup(zgr_ctx *ctx, const uint8_t tau[32]);
+/* Powers of tau from elsewhere (a ceremony), kept in the context; n_points = 2^log_n. */
+int zgr_power_tau_load(zgr_ctx *ctx, const uint8_t *g1pp, const uint8_t *g2pp, const uint8_t *high_g1pp,
+                       const uint8_t *high_g2, size_t n_points);
+/* Copy the context's powers out (NULL = not wanted); ZGR_ERR_NOT_LOADED without any. */
+int zgr_power_tau_download(zgr_ctx *ctx, uint8_t *g1pp, uint8_t *g2pp, uint8_t *high_g1pp, uint8_t *high_g2);
+/* AMTParams::from_pp (crates/amt/src/prove_params/generate.rs:139-195) for one coset from
+ * the context's powers: the prove bases (as zgr_params_from_pp) and the verifier half --
+ * vanishes[d-1] = enact(gen_vanishes(g2pp', d)) for d = 1..log_row (generate.rs:234-263,
+ * a G2 DFT of 2^d points per level with the inverse root, unscaled), g2 = g2pp[0] and
+ * high_g2. */
+int zgr_params_from_power_tau(zgr_ctx *ctx, int coset);
+/* One coset's verifier data: vanishes ((2^(log_row+1) - 2) G2, level d = 1..log_row
+ * holding 2^d points at point offset 2^d - 2, in AMTParams.vanishes order), g2, high_g2. */
+int zgr_params_g2(zgr_ctx *ctx, int coset, uint8_t *vanishes, uint8_t *g2, uint8_t *high_g2);
+
+/* Images of the reference's parameter files, written from what the context holds:
+ *   ZGR_FILE_AMT_MONT    amt-prove-coset{c}-mont-{hash}-{h:02}-{n:02}.bin: write_amt_params
+ *                        (crates/amt/src/prove_params/fast_serde_bn254.rs:20-55)
+ *   ZGR_FILE_AMT         amt-prove-coset{c}-{hash}-{h:02}-{n:02}.bin: serialize_uncompressed
+ *                        of AMTParams (crates/amt/src/prove_params/serde.rs)
+ *   ZGR_FILE_AMT_VERIFY  amt-verify-coset{c}-{hash}-{vd:02}-{n:02}.bin: serialize_uncompressed
+ *                        of AMTVerifyParams with sub_depth = vd levels (crates/amt/src/
+ *                        verify_params.rs:22-28, 81-98)
+ *   ZGR_FILE_PTAU        power-tau-{hash}-{n:02}.bin: serialize_compressed of PowerTau
+ *                        (power_tau.rs:20-26, 121-125); coset ignored
+ *   ZGR_FILE_PTAU_MONT   power-tau-mont-{hash}-{n:02}.bin: write_power_tau
+ *                        (fast_serde_bn254.rs:90-113); coset ignored
+ * ({hash} = 5DWgDV for BN254, README.md:73-82.)  out == NULL: *len receives the size.
+ * ZGR_ERR_INVALID_ARG when cap is smaller (with *len set); ZGR_ERR_NOT_LOADED when the
+ * context lacks the data (bases, verifier data, or powers). */
+enum {
+    ZGR_FILE_AMT_MONT = 0,
+    ZGR_FILE_AMT = 1,
+    ZGR_FILE_AMT_VERIFY = 2,
+    ZGR_FILE_PTAU = 3,
+    ZGR_FILE_PTAU_MONT = 4
+};
+int zgr_write_params_file(zgr_ctx *ctx, int kind, int coset, int sub_depth, uint8_t *out, size_t cap,
+                          size_t *len);
+
 /* ---- AMT prover seam: AMTProofs::gen_amt_proofs
  *      (crates/amt/src/prove_params/interfaces.rs:7-45) ---- */
 /* ri_mont: 2^log_n Fr (mont), already bit-reversed (HalfBlob::generate,
