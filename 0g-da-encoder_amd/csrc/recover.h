@@ -11,8 +11,11 @@
 // D_i by interpolation; the erased coset-0 rows (points x_l = w_ROW^l) come out of one
 // Lagrange matrix product, and a forward DFT_COL per row turns them back into data.
 // Extra rows are checked against the interpolant through a random combination of
-// their D_i values (Schwartz-Zippel), the case where the reference's degree assert
-// fires (data_poly.rs:112).
+// their D_i values (Schwartz-Zippel) and through their D_0 alone: rows on a polynomial
+// of degree exactly N, which the reference's degree assert admits (data_poly.rs:112),
+// differ from the interpolant only in D_0, by one constant times A(u), and that constant
+// is folded onto the coset-0 values as coeffs_to_evals does; anything else is the case
+// where the assert fires.
 #pragma once
 #include "field.h"
 #include "field29.h"
@@ -643,39 +646,70 @@ __global__ void k_rec_fold(const int* D, uint32_t tp, uint32_t cp, uint32_t nt, 
     f_store(Out + 8 * ((size_t)t * col + i), o);
 }
 
-// Consistency of the extra rows (indices m..nrows-1 of the received list): their
-// combination yv must equal the interpolant A(u_x) sum_r L[x][r] yv[r].  One workgroup
-// (256 threads) per extra row x.  flag bit 1.
+// One workgroup (256 threads) per extra row x (beyond the ROW interpolation rows): the
+// residuals of its random combination and of its D_0 against the interpolant I:
+//   e  = yv_x   - A(u_x) sum_r L[x][r] yv_r
+//   e0 = V_x[0] - A(u_x) sum_r L[x][r] V_r[0]
+// Rows on one polynomial of degree < N give e = e0 = 0.  The reference also accepts
+// degree exactly N (data_coeffs.len() <= RAW_BLOB_SIZE + 1, data_poly.rs:112): its x^N
+// term reaches only D_0, whose degree-ROW part over the interpolation points is
+// c A(u) for one constant c, so e = e0 = c A(u_x) on every extra row.  Flag bit 1 when
+// e != e0 (another D_i has a higher degree); cx[x] = e0 / A(u_x) for k_rec_fold_const.
 __global__ __launch_bounds__(256) void k_rec_check(const uint32_t* L, const uint32_t* A, const uint32_t* yv,
-                                                   uint32_t m, int* flag) {
-    __shared__ fr part[256];
+                                                   const uint32_t* V, uint32_t m, uint32_t col, uint32_t* cx,
+                                                   int* flag) {
+    __shared__ fr part[256], part0[256];
     const uint32_t x = blockIdx.x;
-    fr s = f_zero<FrP>();
-    for (uint32_t r = threadIdx.x; r < m; r += blockDim.x)
-        s = f_add(s, f_mul(f_load<FrP>(L + 8 * ((size_t)x * m + r)), f_load<FrP>(yv + 8 * (size_t)r)));
+    fr s = f_zero<FrP>(), s0 = f_zero<FrP>();
+    for (uint32_t r = threadIdx.x; r < m; r += blockDim.x) {
+        const fr l = f_load<FrP>(L + 8 * ((size_t)x * m + r));
+        s = f_add(s, f_mul(l, f_load<FrP>(yv + 8 * (size_t)r)));
+        s0 = f_add(s0, f_mul(l, f_load<FrP>(V + 8 * ((size_t)r * col))));
+    }
     part[threadIdx.x] = s;
+    part0[threadIdx.x] = s0;
     __syncthreads();
     for (uint32_t o = blockDim.x >> 1; o > 0; o >>= 1) {
-        if (threadIdx.x < o) part[threadIdx.x] = f_add(part[threadIdx.x], part[threadIdx.x + o]);
+        if (threadIdx.x < o) {
+            part[threadIdx.x] = f_add(part[threadIdx.x], part[threadIdx.x + o]);
+            part0[threadIdx.x] = f_add(part0[threadIdx.x], part0[threadIdx.x + o]);
+        }
         __syncthreads();
     }
     if (threadIdx.x == 0) {
-        const fr v = f_mul(part[0], f_load<FrP>(A + 8 * (size_t)x));
-        if (!f_eq(v, f_load<FrP>(yv + 8 * (size_t)(m + x)))) atomicOr(flag, 2);
+        const fr a = f_load<FrP>(A + 8 * (size_t)x);
+        const fr e = f_sub(f_load<FrP>(yv + 8 * (size_t)(m + x)), f_mul(part[0], a));
+        const fr e0 = f_sub(f_load<FrP>(V + 8 * ((size_t)(m + x) * col)), f_mul(part0[0], a));
+        if (!f_eq(e, e0)) atomicOr(flag, 2);
+        f_store(cx + 8 * (size_t)x, f_mul(e0, f_inv(a)));  // A(u_x) != 0: the points are distinct
     }
 }
 
+// The degree-N coefficient c = cx[0] when every extra row gives the same one (else flag
+// bit 1: degree above N); c = 0 for rows on a polynomial of degree < N.  One workgroup.
+__global__ __launch_bounds__(256) void k_rec_fold_const(const uint32_t* cx, uint32_t nx, uint32_t* c, int* flag) {
+    const fr c0 = f_load<FrP>(cx);
+    for (uint32_t x = threadIdx.x + 1; x < nx; x += blockDim.x)
+        if (!f_eq(f_load<FrP>(cx + 8 * (size_t)x), c0)) atomicOr(flag, 2);
+    if (threadIdx.x == 0) f_store(c, c0);
+}
+
 // One workgroup per erased coset-0 row l = erased[t]: w_i = A(x_l) Out[t][i] w_N^(l i),
-// forward DFT_COL, canonical 32 B into blob row l.
+// forward DFT_COL, canonical 32 B into blob row l.  cfold (null: none) is the degree-N
+// coefficient c of k_rec_fold_const: D_0(x_l) = A(x_l) (Out[t][0] + c), which at the
+// coset-0 points (x^N = 1) is coeffs_to_evals' fold of the top coefficient.
 __global__ __launch_bounds__(256) void k_rec_out(const uint32_t* Out, const uint32_t* A, const uint32_t* erased,
-                                                 const uint32_t* tw, int log_col, int log_row, uint32_t* blob) {
+                                                 const uint32_t* tw, int log_col, int log_row, const uint32_t* cfold,
+                                                 uint32_t* blob) {
     extern __shared__ fr rec_lds[];
     fr* lds = rec_lds;
     const uint32_t t = blockIdx.x, col = 1u << log_col, l = erased[t];
     const int log_n = log_col + log_row;
     const fr a = f_load<FrP>(A + 8 * (size_t)t);
     for (uint32_t i = threadIdx.x; i < col; i += blockDim.x) {
-        fr w = f_mul(f_load<FrP>(Out + 8 * ((size_t)t * col + i)), a);
+        fr o = f_load<FrP>(Out + 8 * ((size_t)t * col + i));
+        if (cfold && i == 0) o = f_add(o, f_load<FrP>(cfold));
+        fr w = f_mul(o, a);
         lds[bitrev_bits(i, log_col)] = f_mul(w, rec_wpow(tw, (uint64_t)l * i, log_n));
     }
     __syncthreads();

@@ -2194,10 +2194,19 @@ int zgr_recover(zgr_ctx* c, const uint32_t* row_ids, const uint8_t* rows, size_t
                        (uint32_t)COL, d_out);
             }
         }
-        if (nx) LAUNCH(c, st, "recover", k_rec_check, nx, 256, 0, d_L + 8 * (size_t)e0 * m, d_A + 8 * (size_t)e0, d_yv,
-                       m, flag);
+        uint32_t* d_c = nullptr;  // the degree-N coefficient, when there are extra rows
+        if (nx) {
+            uint32_t* d_cx;
+            CHECK(alloc(d_cx, (size_t)nx * 32));
+            CHECK(alloc(d_c, 32));
+            LAUNCH(c, st, "recover", k_rec_check, nx, 256, 0, d_L + 8 * (size_t)e0 * m, d_A + 8 * (size_t)e0, d_yv, d_V,
+                   m, (uint32_t)COL, d_cx, flag);
+            LAUNCH(c, st, "recover", k_rec_fold_const, 1, 256, 0, d_cx, nx, d_c, flag);
+        }
         if (n0) LAUNCH(c, st, "recover", k_rec_pass, blocks(n0 * COL, 256), 256, 0, d_rows, d_ids, (uint32_t)n0, lc, d_blob);
-        if (e0) LAUNCH(c, st, "recover_rows", k_rec_out, e0, 256, COL * 32, d_out, d_A, d_er, c->tw, lc, h, d_blob);
+        if (e0)
+            LAUNCH(c, st, "recover_rows", k_rec_out, e0, 256, COL * 32, d_out, d_A, d_er, c->tw, lc, h,
+                   (const uint32_t*)d_c, d_blob);
         LAUNCH(c, st, "recover", k_rec_pack, blocks(31 * n / 4, 256), 256, 0, d_blob, (uint32_t)n, (uint32_t*)d_raw,
                flag);
         LAUNCH(c, st, "recover", k_rec_padding, blocks(max_raw, 256), 256, 0, d_raw, (uint64_t)max_raw, flag);
@@ -2207,7 +2216,7 @@ int zgr_recover(zgr_ctx* c, const uint32_t* row_ids, const uint8_t* rows, size_t
         CHECK(sync_all(c));
         // the reference's order: the degree assert in data_poly, then RawBlob -> RawData
         if (hflag & 2)
-            return fail(ZGR_ERR_INCONSISTENT, "rows do not lie on one polynomial of degree < N (the reference panics)");
+            return fail(ZGR_ERR_INCONSISTENT, "rows do not lie on one polynomial of degree <= N (the reference panics)");
         if (hflag & 4) return fail(ZGR_ERR_INVALID_ARG, "Incorrect scalar");
         if (hflag & 8) return fail(ZGR_ERR_INVALID_ARG, "Incorrect length field");
         if (hflag & 16) return fail(ZGR_ERR_INVALID_ARG, "Non zero in padding range");

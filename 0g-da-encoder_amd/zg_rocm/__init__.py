@@ -752,7 +752,7 @@ class RecoveryError(ValueError):
 
 class RecoveryPanic(RuntimeError):
     """An input on which the reference panics (an element >= r, or rows off one
-    polynomial of degree < N)."""
+    polynomial of degree <= N)."""
 
 
 def recover_from_da_slice(ctx, slices):

@@ -467,6 +467,29 @@ def main():
                "p50_ms": pct(s_lat, 50), "p99_ms": pct(s_lat, 99),
                "plain_per_call_same_arrivals": {"value": round(n_req / p_wall, 4), "p50_ms": pct(p_lat, 50),
                                                 "p99_ms": pct(p_lat, 99)}}
+    # bin/build_params (crates/amt/src/bin/build_params.rs:24-53) in create mode on the
+    # device: powers of tau, then per coset the prove bases and vanishes from them
+    # (AMTParams::from_pp: 12 G1 EC-FFTs of 2^20 and the G2 vanishes DFTs) and the images
+    # of the reference's four files (in host memory: no disk in the timed region)
+    tb = {}
+    t0 = time.perf_counter()
+    ctx.power_tau_setup(Z.tau_from_seed(TAU_SEED))
+    tb["power_tau_s"] = round(time.perf_counter() - t0, 3)
+    img_bytes = 0
+    for c in range(geo.coset_n):
+        s0 = time.perf_counter()
+        ctx.params_from_power_tau(c)
+        tb.setdefault("from_pp_s", []).append(round(time.perf_counter() - s0, 3))
+        s0 = time.perf_counter()
+        for kind in (Z.FILE_AMT, Z.FILE_AMT_MONT, Z.FILE_AMT_VERIFY) + ((Z.FILE_PTAU,) if c == 0 else ()):
+            img_bytes += len(ctx.write_params_file(kind, c))
+        tb.setdefault("file_images_s", []).append(round(time.perf_counter() - s0, 3))
+    tb["total_s"] = round(time.perf_counter() - t0, 3)
+    tb["file_bytes"] = img_bytes
+    tb["note"] = ("depth 20, prove/verify depth 10, 3 cosets; the reference quotes about 30 minutes for its "
+                  "params build on an AWS g5.4xlarge (README.md:121)")
+    params_build = tb
+
     # The encode pipeline overlaps kernels of different cosets on two streams, which
     # stretches event-timed durations; the roofline uses isolated launches of the same
     # bucket kernels (amt_prove of one coset: the encode path's multi-MSM over 12 lines
@@ -532,6 +555,7 @@ def main():
         "ntt": ntt,
         "msm_2pow20": msm,
         "plain_host_input": host_input,
+        "params_build": params_build,
     }
     if not args.no_cpu:
         res["cpu_baseline"] = cpu_baseline(ctx, geo, raws[0], args.cpu_threads)

@@ -264,9 +264,11 @@ int zgr_verify_msm(zgr_ctx *ctx, const uint32_t *row_ids, const uint8_t *rows, c
  *   ZGR_ERR_INVALID_ARG   an element >= r (the reference panics), "Not enough valid
  *                         lines", "Cannot recover data: RowIdOverflow", "Incorrect
  *                         scalar", "Incorrect length field", "Non zero in padding range";
- *   ZGR_ERR_INCONSISTENT  the rows do not lie on one polynomial of degree < N (the
+ *   ZGR_ERR_INCONSISTENT  the rows do not lie on one polynomial of degree <= N (the
  *                         reference's degree assert, data_poly.rs:112).
- * Needs no params; runs synchronously on the context's stream. */
+ * Rows on a polynomial of degree exactly N decode as the reference's do: its top
+ * coefficient is added to every coset-0 value (coeffs_to_evals, crates/recovery/src/
+ * utils.rs:76-110).  * Needs no params; runs synchronously on the context's stream. */
 int zgr_recover(zgr_ctx *ctx, const uint32_t *row_ids, const uint8_t *rows, size_t n_rows,
                 uint8_t *raw_out, size_t raw_cap, size_t *raw_len);
 

@@ -72,6 +72,37 @@ def test_params_from_pp_production(ctx):
     assert np.array_equal(ctx.params_download(2), before)
 
 
+def test_build_params_production(ctx):
+    """bin/build_params at 1024 x 1024 on the device, without tau after the powers: the
+    powers of tau in G1 and G2, then for every coset the prove bases and the vanishes from
+    them (AMTParams::from_pp), each equal to the closed-form generator's; the bamt image
+    of coset 0 loads back into the same bases."""
+    tau = seeded_tau()
+    ctx.params_generate(tau)  # the closed form, verifier half included (a G1-only from_pp clears it)
+    closed = [ctx.params_g2(c) for c in range(GEO.coset_n)]
+    before = [ctx.params_download(c) for c in range(GEO.coset_n)]
+    t = time.time()
+    ctx.power_tau_setup(tau)
+    log("power tau (2^20 G1, 2^20 high G1, 2^20 G2) on device: %.2f s" % (time.time() - t))
+    for c in range(GEO.coset_n):
+        t = time.time()
+        ctx.params_from_power_tau(c)
+        log("params from powers of tau, coset %d (12 EC-FFTs of 2^20 + vanishes): %.2f s" % (c, time.time() - t))
+        assert np.array_equal(ctx.params_download(c), before[c]), c
+        for a, b in zip(ctx.params_g2(c), closed[c]):
+            assert np.array_equal(a, b), c
+    for kind, name in ((Z.FILE_AMT_MONT, "bamt"), (Z.FILE_AMT, "ark AMTParams"), (Z.FILE_AMT_VERIFY, "verify"),
+                       (Z.FILE_PTAU, "power tau")):
+        t = time.time()
+        img = ctx.write_params_file(kind, 0)
+        log("%s image of coset 0: %.1f MB in %.2f s" % (name, len(img) / 1e6, time.time() - t))
+        if kind == Z.FILE_AMT_MONT:
+            ctx.params_load_bamt(0, img)
+            assert np.array_equal(ctx.params_download(0), before[0])
+            assert ctx.write_params_file(Z.FILE_AMT_MONT, 0) == img
+        del img
+
+
 def test_encode_closed_form(blob):
     raw, out = blob
     O.set_threads(CPU_THREADS)
@@ -202,6 +233,37 @@ def test_batch_equals_single(ctx, blob):
     for a, b in zip(outs, singles):
         for f in ("commitment", "file_root", "proof_tree", "comm_tree", "high", "merkle", "data"):
             assert np.array_equal(getattr(a, f), getattr(b, f)), f
+
+
+def test_uneven_mix_32(ctx):
+    """config 4: the seeded 32-blob mix (raw lengths log-uniform in [1 KB, 32.5 MB], the
+    same seeds bench.py times) in one batched call.  Every blob: commitment = [f(tau)] G1
+    and high = [tau^(3N) f(tau)] G1 for the interpolant f of its column-major data,
+    coset 0 equal to its raw blob, and every output equal to the same blob encoded alone."""
+    tau = seeded_tau()
+    rng = np.random.default_rng(DATA_SEED)
+    lens = np.exp(rng.uniform(np.log(1024), np.log(GEO.max_raw_data_size), 32)).astype(np.int64)
+    raws = [Z.seeded_bytes(DATA_SEED + 100 + i, int(L)) for i, L in enumerate(lens)]
+    dev = [Z.DeviceBuffer(r) for r in raws]
+    outs = [Z.EncodedBlobOutputs(GEO) for _ in raws]
+    t = time.time()
+    ctx.encode_batch_device([d.address for d in dev], [len(r) for r in raws], outs)
+    log("32-blob uneven mix (%.1f MB raw): %.3f s" % (lens.sum() / 1e6, time.time() - t))
+    del dev
+    O.set_threads(CPU_THREADS)
+    t3n = pow(tau, 3 * GEO.n, O.R_MOD)
+    for k, (raw, out) in enumerate(zip(raws, outs)):
+        rowmaj = O.raw_to_blob(raw, GEO.log_n).reshape(GEO.row_n, GEO.col_n, 4)
+        colmaj = np.ascontiguousarray(rowmaj.transpose(1, 0, 2)).reshape(-1, 4)
+        ftau = O.eval_interp(colmaj, GEO.log_n, tau)
+        assert O.point_to_ints(out.commitment) == O.point_to_ints(O.g1_mul_gen(ftau)), k
+        high = O.g1_mul_gen(ftau * t3n % O.R_MOD)
+        assert all(np.array_equal(out.high[64 * c:64 * c + 64], high) for c in range(GEO.coset_n)), k
+        canon = O.fr_canonical_array(rowmaj.reshape(-1, 4)).view(np.uint8).reshape(-1)
+        assert np.array_equal(out.data[:GEO.n * 32], canon), k
+        single = ctx.encode(raw)
+        for f in ("commitment", "file_root", "proof_tree", "comm_tree", "high", "merkle", "data"):
+            assert np.array_equal(getattr(out, f), getattr(single, f)), (k, f)
 
 
 def test_served_equals_single(ctx, blob):

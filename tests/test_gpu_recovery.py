@@ -9,7 +9,7 @@ import pytest
 import oracle as O
 import zg_rocm as Z
 from helpers import DATA_SEED, oracle_params
-from test_recovery_oracle import _blob_with, encoded_rows, reference_row_sets, rows_of_blob
+from test_recovery_oracle import _blob_with, degree_n_rows, encoded_rows, reference_row_sets, rows_of_blob
 
 pytestmark = pytest.mark.gpu
 
@@ -178,3 +178,22 @@ def test_gemm_paths(ctx, blobs):
             os.environ.pop("ZGR_REC_GEMM", None)
         else:
             os.environ["ZGR_REC_GEMM"] = old
+
+
+def test_degree_n_rows_match_oracle(ctx, blobs):
+    """Rows on a polynomial of degree exactly N (the reference folds its top coefficient
+    onto coset 0) decode to the oracle's bytes; degree N + 1, or extra rows that disagree
+    on the coefficient, give the reference's panic."""
+    raw, rows = blobs[3]
+    for coeff in (1, 5):
+        bent = degree_n_rows(rows, coeff)
+        for ids in (list(range(R, RE)), list(range(2 * R, RE)) + [R + 5], sorted(set(range(0, RE, 2)) | {1}),
+                    list(range(RE))):
+            assert same_outcome(ctx, {i: bent[i] for i in ids}) in (0, 3, 4, 5), (coeff, len(ids))
+    bent = degree_n_rows(rows, 1)
+    assert same_outcome(ctx, {i: bent[i] for i in range(R, RE)}) == 0
+    above = degree_n_rows(rows, 1, extra=1)
+    assert same_outcome(ctx, {i: above[i] for i in range(R, RE)}) == 7
+    mixed = degree_n_rows(rows, 1)
+    mixed.update({i: v for i, v in degree_n_rows(rows, 2).items() if i == RE - 1})
+    assert same_outcome(ctx, {i: mixed[i] for i in range(R, RE)}) == 7

@@ -152,6 +152,32 @@ def test_params_closed_form_equals_ecfft_restatement(coset):
         assert np.array_equal(a, b)
 
 
+@pytest.mark.parametrize("start", [0, 3 * 16, 12345])
+def test_power_tau_textbook(start):
+    # power_tau (power_tau.rs:28-42): out[i] = [tau^(start + i)] G1, against the textbook
+    # affine group law; start 3N is the high powers of setup_inner (power_tau.rs:66-71)
+    tau = 0x0BADC0FFEE0DDF00D1234567
+    log_n = 4
+    out = O.power_tau(tau, start, log_n).reshape(-1, 64)
+    for i in range(1 << log_n):
+        x = int.from_bytes(bytes(out[i, :32]), "little")
+        y = int.from_bytes(bytes(out[i, 32:]), "little")
+        assert (x, y) == pyref.g1_mul(pyref.G1, pow(tau, start + i, R)), i
+
+
+@pytest.mark.parametrize("log_n,h", [(3, 2), (4, 3), (5, 2)])
+def test_params_point_equals_params_from_tau(log_n, h):
+    # the single-point closed form (used to sample production params) against the whole
+    # buffer, at every position of every line of every coset
+    tau = 0x77777777777777777777777777777
+    n = 1 << log_n
+    for coset in range(3):
+        full = O.params_from_tau(tau, log_n, h, coset, 0).reshape(h + 2, n, 8)
+        for line in range(h + 2):
+            for pos in range(n):
+                assert np.array_equal(O.params_point(tau, log_n, h, coset, line, pos), full[line, pos]), (coset, line, pos)
+
+
 def test_gen_basis_brute_force():
     # generate.rs:276-290: basis_t == MSM(g1pp, ifft(one-hot t)) -- here [L_t(tau)] G1
     tau = 987654321987654321

@@ -157,3 +157,56 @@ def test_malformed_raw_data_outcomes():
     vals[7] = 1 << 250
     rows = rows_of_blob(vals)
     assert O.recover({i: rows[i] for i in ids}, GEO.log_col, GEO.log_row)[0] == 3
+
+
+def degree_n_rows(rows, coeff, extra=0):
+    """The encoded rows of D'(x) = D(x) + coeff x^(N + extra): row l of coset k holds
+    D(x) at x = cf_k w_N^(l + ROW j), j < COL, so every element gains coeff x^(N + extra)."""
+    g = GEO
+    w = O.domain_root(g.log_n)
+    out = {}
+    for i, row in rows.items():
+        k, l = divmod(i, g.row_n)
+        cf = O.coset_factor(g.log_n, k)
+        vals = []
+        for j in range(g.col_n):
+            x = cf * pow(w, l + g.row_n * j, O.R_MOD) % O.R_MOD
+            v = int.from_bytes(row[32 * j:32 * j + 32], "little")
+            vals.append(((v + coeff * pow(x, g.n + extra, O.R_MOD)) % O.R_MOD).to_bytes(32, "little"))
+        out[i] = b"".join(vals)
+    return out
+
+
+def test_degree_n_rows_fold_onto_coset0(blobs):
+    """Rows on a polynomial of degree exactly N pass the reference's degree assert
+    (data_coeffs.len() <= RAW_BLOB_SIZE + 1, data_poly.rs:112) and coeffs_to_evals folds the
+    top coefficient c onto coset 0 (every coset-0 value gains c, utils.rs:81-105: the
+    2N-point FFT taken at every second point).  With c = 1 and a full-length blob the
+    recovered bytes are the original 31-byte chunks each plus one."""
+    raw, rows = blobs[2]
+    assert len(raw) == GEO.max_raw_data_size
+    padded = raw + len(raw).to_bytes(4, "little")
+    chunks = [int.from_bytes(padded[31 * i:31 * i + 31], "little") + 1 for i in range(GEO.n)]
+    assert all(ch < 1 << 248 for ch in chunks)
+    want = b"".join(ch.to_bytes(31, "little") for ch in chunks)[:len(raw)]
+    bent = degree_n_rows(rows, 1)
+    for ids in (list(range(R, RE)), list(range(2 * R, RE)) + [R + 5], sorted(set(range(0, RE, 2)) | {1})):
+        rc, out = O.recover({i: bent[i] for i in ids}, GEO.log_col, GEO.log_row)
+        assert (rc, out) == (0, want), len(ids)
+    # exactly ROW rows fix a polynomial of degree < N: the bent rows of coset 2 alone
+    # decode to other data (no fold is visible from ROW rows)
+    rc, out = O.recover({i: bent[i] for i in range(2 * R, RE)}, GEO.log_col, GEO.log_row)
+    assert out != want
+
+
+def test_degree_above_n_panics(blobs):
+    """Degree N + 1, or rows whose extra terms do not agree on one coefficient: the
+    degree assert fires (the reference panics)."""
+    raw, rows = blobs[2]
+    bent = degree_n_rows(rows, 1, extra=1)
+    rc, _ = O.recover({i: bent[i] for i in range(R, RE)}, GEO.log_col, GEO.log_row)
+    assert rc == 7
+    mixed = degree_n_rows(rows, 1)
+    mixed.update({i: v for i, v in degree_n_rows(rows, 2).items() if i == RE - 1})
+    rc, _ = O.recover({i: mixed[i] for i in range(R, RE)}, GEO.log_col, GEO.log_row)
+    assert rc == 7
