@@ -180,6 +180,23 @@ struct Slot {
     bool busy = false;
 };
 
+// A chunk-group plan of the fixed-base bucket kernels: per AMT line the log2 of the
+// chunks that share one bucket set, and the work lists built from it ([0] every line,
+// [1] without the high-basis line).
+struct FbPlan {
+    std::string name;
+    uint32_t* work[2] = {nullptr, nullptr};
+    uint32_t nwork[2] = {0, 0};
+    uint64_t plan[2] = {0, 0};  // 2 bits per line
+    int gl_used = 0;            // deepest chunk-group level any line uses
+};
+
+// Density classes of a coset's scalars for the plan choice: the share of non-zero
+// chunks (coset 0 of a blob shorter than MAX_RAW_DATA_SIZE has zero chunks; cosets 1
+// and 2 are dense): at most 1/16, 1/4, 1/2, more.
+constexpr int ZGR_DENSITY_CLASSES = 4;
+constexpr int ZGR_DENSE = ZGR_DENSITY_CLASSES - 1;
+
 }  // namespace
 
 struct zgr_ctx {
@@ -223,11 +240,13 @@ struct zgr_ctx {
     // fixed-base tables (shifted copies of the bases); fb == false: windowed MSM
     bool fb = false;
     uint32_t* fbtab[3] = {nullptr, nullptr, nullptr};
-    // work lists of the grouped bucket kernels: [0] every line, [1] without the high line
-    uint32_t* work[2] = {nullptr, nullptr};
-    uint32_t nwork[2] = {0, 0};
-    uint64_t plan[2] = {0, 0};  // chunk-group level per line, 2 bits each
-    int gl_used = 0;            // deepest chunk-group level any line uses
+    // chunk-group plans of the bucket kernels (plans[0] the default), the plan each
+    // density class of a coset's scalars uses (set by zgr_autotune), the tuned split of
+    // the log_n-point NTT (0: ceil(log_n / 2)) and the tuner's report
+    std::vector<FbPlan> plans;
+    int plan_of_class[ZGR_DENSITY_CLASSES] = {};
+    int ntt_n1 = 0;
+    std::string tune_report;
 
     Lane lanes[2];
     Slot slots[2];
@@ -301,6 +320,10 @@ int ntt_launch(zgr_ctx* c, hipStream_t st, const NttPass& p, uint32_t groups, ui
     return ZGR_OK;
 }
 
+// high-bit part n1 of a two-pass 2^n transform: the tuned one for the context's own
+// size, else ceil(n / 2)
+int ntt_split(const zgr_ctx* c, int n) { return n == c->log_n && c->ntt_n1 ? c->ntt_n1 : (n + 1) / 2; }
+
 NttPass ntt_pass_base(const zgr_ctx* c, bool inverse, int n) {
     NttPass p{};
     p.pw_lo = c->pw_lo[inverse ? 1 : 0];
@@ -324,7 +347,7 @@ int ntt_dif(zgr_ctx* c, hipStream_t st, const uint32_t* src, uint32_t* dst, int 
         p.log_m = n;
         return ntt_launch(c, st, p, 1, rows);
     }
-    const int n1 = (n + 1) / 2, n2 = n - n1;
+    const int n1 = ntt_split(c, n), n2 = n - n1;
     p.gstride = 1;  // pass A: columns
     p.estride = 1u << n2;
     p.log_m = n1;
@@ -354,7 +377,7 @@ int ntt_dit(zgr_ctx* c, hipStream_t st, const uint32_t* src, uint32_t* dst, int 
         p.log_m = n;
         return ntt_launch(c, st, p, 1, rows);
     }
-    const int n1 = (n + 1) / 2, n2 = n - n1;
+    const int n1 = ntt_split(c, n), n2 = n - n1;
     p.gstride = 1u << n2;  // pass 1: blocks
     p.estride = 1;
     p.log_m = n2;
@@ -404,13 +427,14 @@ int heap_windows(const zgr_ctx* c) { return c->fb ? 1 : MSM_W; }
 
 // multi-line MSM of one coset on a lane: scalars `src` -> chunk sums in the lane heap
 // (one tree per line, or one per (line, window)), for the first `lines` lines
-// (quotients[0..h), basis, high_basis).  grouped (fixed-base only, lines == nlines or
-// nlines - 1): each line gets only the chunk-group sums its AMT outputs read
-// (fb_group_log), at heap level h - fb_group_log(L, h); else every chunk of every line.
-int msm_lines(zgr_ctx* c, Lane& ln, int coset, const uint32_t* src, int from_y, int lines, bool grouped) {
+// (quotients[0..h), basis, high_basis).  With a plan (fixed-base only, lines == nlines or
+// nlines - 1) each line gets only the chunk-group sums its plan gives it (at most what
+// its AMT outputs read, fb_group_log), at heap level h - gl; without one every chunk of
+// every line.
+int msm_lines(zgr_ctx* c, Lane& ln, int coset, const uint32_t* src, int from_y, int lines, const FbPlan* grouped) {
     const int log_bs = c->log_n - c->h;
     if (c->fb) {
-        const int gl_used = grouped ? c->gl_used : 0;
+        const int gl_used = grouped ? grouped->gl_used : 0;
         FbLevels lv{};
         FbOut fo{};
         lv.ent[0] = fo.ent[0] = ln.entries;
@@ -429,8 +453,8 @@ int msm_lines(zgr_ctx* c, Lane& ln, int coset, const uint32_t* src, int from_y, 
         uint32_t waves = (uint32_t)(c->nb * lines);
         if (grouped) {
             const int v = lines == c->nlines ? 0 : 1;
-            work = c->work[v];
-            waves = c->nwork[v];
+            work = grouped->work[v];
+            waves = grouped->nwork[v];
         }
         LAUNCH(c, ln.st, "msm_accum", k_fb_sums, waves, FB_LANES, 0, c->fbtab[coset], lv, work, ln.bkt, c->log_n,
                c->h, lines);
@@ -450,9 +474,9 @@ int msm_lines(zgr_ctx* c, Lane& ln, int coset, const uint32_t* src, int from_y, 
 // trees over the chunks (+ Horner over the windows): AMT outputs of one coset into
 // `out` (out_slots points).  With lines == h + 1 the high-basis line was not computed
 // and its output slot (the last one) is left for the caller to fill.
-int amt_outputs(zgr_ctx* c, Lane& ln, uint32_t* out, int lines, bool grouped) {
+int amt_outputs(zgr_ctx* c, Lane& ln, uint32_t* out, int lines, const FbPlan* grouped) {
     const int nw = heap_windows(c);
-    const uint64_t plan = grouped ? c->plan[lines == c->nlines ? 0 : 1] : 0;
+    const uint64_t plan = grouped ? grouped->plan[lines == c->nlines ? 0 : 1] : 0;
     for (int d = c->h - 1; d >= 0; d--) {
         uint32_t total = (uint32_t)(lines * nw) << d;
         LAUNCH(c, ln.st, "msm_tree", k_msm_tree_level, blocks(total, 64), 64, 0, ln.heap, c->h, d, total, nw, plan);
@@ -465,59 +489,88 @@ int amt_outputs(zgr_ctx* c, Lane& ln, uint32_t* out, int lines, bool grouped) {
 // Work lists of the grouped bucket kernels (one u32 per wave: L | gl << 8 | S << 10),
 // deepest levels first so that the longest waves start first; within a level the lines
 // of one chunk group are adjacent (they read the same sorted entries).
-// Every line sits at the largest level its outputs allow (fb_group_log), with one
-// exception: when the waves of the deepest level outnumber the wave slots k_fb_sums has
-// on the device (4 SIMDs per CU, 2 waves each at its register count), the ones left over
-// would run last and alone, so the high-basis line (read only at the root) drops one
-// level.  At 2^20 x 12 lines that turns 2304 four-chunk waves into 2048 and measured 4 %
-// faster; other plans that keep lines at smaller groups measured slower.
-// ZGR_GROUP_PLAN=none keeps every line at level 0 (one bucket set per chunk, as
-// multiple_multiexp needs); a comma-separated list sets each line's level explicitly.
+// The default plan puts every line at the largest level its outputs allow
+// (fb_group_log), with one exception: when the waves of the deepest level outnumber the
+// wave slots k_fb_sums has on the device (4 SIMDs per CU, 2 waves each at its register
+// count), the ones left over would run last and alone, so the high-basis line (read only
+// at the root) drops one level.  At 2^20 x 12 lines that turns 2304 four-chunk waves into
+// 2048 and measured 4 % faster on dense scalars.  The other candidates of zgr_autotune:
+// "max" (no drop), "half" (at most 2-chunk groups) and "chunk" (one bucket set per chunk).
+// ZGR_GROUP_PLAN=none makes the default plan "chunk" and a comma-separated list sets each
+// line's level explicitly (each capped at fb_group_log); either one turns the tuner off.
 constexpr int FB_SUMS_WAVES_PER_SIMD = 2;
 
-int build_work_lists(zgr_ctx* c) {
+int build_plan(zgr_ctx* c, const std::string& kind, const std::vector<int>& given, FbPlan* out) {
     int cus = 0;
     HIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device));
     const size_t slots = (size_t)cus * 4 * FB_SUMS_WAVES_PER_SIMD;
-    const char* mode = getenv("ZGR_GROUP_PLAN");
-    const bool none = mode && !strcmp(mode, "none");
-    // an explicit plan: one level per line, comma-separated, each capped at fb_group_log
-    std::vector<int> given;
-    if (mode && !none)
-        for (const char* q = mode; *q;) {
-            given.push_back((int)strtol(q, (char**)&q, 10));
-            if (*q == ',') q++;
-            else if (*q) return fail(ZGR_ERR_INVALID_ARG, "ZGR_GROUP_PLAN=%s: expected none or levels like 2,2,1,0", mode);
-        }
+    out->name = kind;
     for (int v = 0; v < 2; v++) {
         const int lines = c->nlines - v;
         std::vector<int> gl(lines, 0);
-        if (!none)
+        if (kind != "chunk")
             for (int L = 0; L < lines; L++) {
                 gl[L] = fb_group_log(L, c->h);
-                if (L < (int)given.size()) gl[L] = std::max(0, std::min(gl[L], given[L]));
+                if (kind == "half") gl[L] = std::min(gl[L], 1);
+                if (kind == "given" && L < (int)given.size()) gl[L] = std::max(0, std::min(gl[L], given[L]));
             }
-        if (!none && given.empty() && lines == c->h + 2 && gl[c->h + 1] > 0) {
+        if (kind == "default" && lines == c->h + 2 && gl[c->h + 1] > 0) {
             const int top = *std::max_element(gl.begin(), gl.end());
             size_t deepest = 0;
             for (int L = 0; L < lines; L++) deepest += gl[L] == top ? c->nb >> top : 0;
             if (gl[c->h + 1] == top && deepest > slots && deepest - (c->nb >> top) <= slots) gl[c->h + 1]--;
         }
         std::vector<uint32_t> w;
-        c->plan[v] = 0;
+        out->plan[v] = 0;
         for (int L = 0; L < lines; L++) {
-            c->plan[v] |= (uint64_t)gl[L] << (2 * L);
-            c->gl_used = std::max(c->gl_used, gl[L]);
+            out->plan[v] |= (uint64_t)gl[L] << (2 * L);
+            out->gl_used = std::max(out->gl_used, gl[L]);
         }
         for (int g = FB_GL_MAX; g >= 0; g--)
             for (uint32_t S = 0; S < (uint32_t)(c->nb >> g); S++)
                 for (int L = 0; L < lines; L++)
                     if (gl[L] == g) w.push_back((uint32_t)L | (uint32_t)g << 8 | S << 10);
-        DALLOC(c->work[v], w.size() * 4);
-        HIPCHK(hipMemcpy(c->work[v], w.data(), w.size() * 4, hipMemcpyHostToDevice));
-        c->nwork[v] = (uint32_t)w.size();
+        DALLOC(out->work[v], w.size() * 4);
+        HIPCHK(hipMemcpy(out->work[v], w.data(), w.size() * 4, hipMemcpyHostToDevice));
+        out->nwork[v] = (uint32_t)w.size();
     }
     return ZGR_OK;
+}
+
+int build_work_lists(zgr_ctx* c) {
+    const char* mode = getenv("ZGR_GROUP_PLAN");
+    std::vector<int> given;
+    std::vector<std::string> kinds;
+    if (mode && !strcmp(mode, "none")) {
+        kinds = {"chunk"};
+    } else if (mode) {  // one level per line, comma-separated
+        for (const char* q = mode; *q;) {
+            given.push_back((int)strtol(q, (char**)&q, 10));
+            if (*q == ',') q++;
+            else if (*q) return fail(ZGR_ERR_INVALID_ARG, "ZGR_GROUP_PLAN=%s: expected none or levels like 2,2,1,0", mode);
+        }
+        kinds = {"given"};
+    } else {
+        kinds = {"default", "max", "half", "chunk"};
+    }
+    c->plans.resize(kinds.size());
+    for (size_t k = 0; k < kinds.size(); k++) CHECK(build_plan(c, kinds[k], given, &c->plans[k]));
+    return ZGR_OK;
+}
+
+// the plan for a coset's scalars of density class cls, or null without fixed-base tables
+const FbPlan* plan_for(const zgr_ctx* c, int cls) { return c->fb ? &c->plans[c->plan_of_class[cls]] : nullptr; }
+
+// density class of coset 0's scalars for a blob of len raw bytes: the raw rows that hold
+// data, plus the last row (it holds the u32 length footer unless len = 0)
+int density_class(const zgr_ctx* c, size_t len) {
+    const size_t row_bytes = 31 * ((size_t)1 << c->log_col), rows = (size_t)1 << c->log_row;
+    size_t nz = std::min(rows, (len + row_bytes - 1) / row_bytes);
+    if (len && nz < rows) nz++;
+    if (16 * nz <= rows) return 0;
+    if (4 * nz <= rows) return 1;
+    if (2 * nz <= rows) return 2;
+    return ZGR_DENSE;
 }
 
 // shifted-copy table of one coset from its resident bases (after every params upload)
@@ -578,8 +631,9 @@ int issue_blob(zgr_ctx* c, Slot& s, const uint8_t* raw_dev, size_t len, const zg
         // coset (coset c's evaluations interpolate f(cf_c X)), so the high-basis line
         // runs for coset 0 only and finish_blob copies its result to the others
         const int lines = cs == 0 ? c->nlines : c->nlines - 1;
-        CHECK(msm_lines(c, ln, cs, y, 1, lines, c->fb));
-        CHECK(amt_outputs(c, ln, s.outpts + (size_t)cs * slots * 16, lines, c->fb));
+        const FbPlan* plan = plan_for(c, cs == 0 ? density_class(c, len) : ZGR_DENSE);
+        CHECK(msm_lines(c, ln, cs, y, 1, lines, plan));
+        CHECK(amt_outputs(c, ln, s.outpts + (size_t)cs * slots * 16, lines, plan));
         LAUNCH(c, ln.st, "merkle_leaves", k_merkle_leaves, blocks(n / 8, 128), 128, 0, y, c->log_col, c->log_row,
                cs, s.data, s.mlayers[0]);
         HIPCHK(hipEventRecord(s.ev_coset[cs], ln.st));
@@ -731,122 +785,147 @@ struct Request {
     const uint8_t* raw = nullptr;
     size_t len = 0;
     const zgr_encoded* out = nullptr;
-    size_t off = 0;
-    hipEvent_t uploaded = nullptr;
+    size_t off = 0;  // raw bytes' place in the staging arena
 };
 
 }  // namespace
 
+// Serving over one context's two blob slots.  Three threads:
+//   worker    admits queued requests (staging-arena footprint, in-flight cap), and
+//             issues the next admitted one into each free slot: the raw bytes' upload
+//             on the slot's own stream, then the encode, then a host callback;
+//   callback  (HIP runtime thread, after the slot's last copy into pinned staging) marks
+//             the slot done and wakes the worker -- no polling;
+//   finisher  copies a done slot's staged outputs into the request's buffers and
+//             completes its ticket, off the issuing thread; the slot is free after that.
+// The context's four streams (two coset lanes, two slots) are all there is: an extra
+// upload stream would share a hardware queue with a lane (four queues per process) and
+// wait behind its bucket kernels.
 struct zgr_server {
     zgr_ctx* c = nullptr;
     Arena arena;
     int max_inflight = 1;
-    hipStream_t upload = nullptr;
     std::mutex m;
-    std::condition_variable cv_done, cv_work;
+    std::condition_variable cv_done, cv_work, cv_finish;
     std::deque<Request*> queue;   // submitted, waiting for admission
-    std::deque<Request*> staged;  // admitted and uploading / uploaded, waiting for a slot
-    Request* in_slot[2] = {nullptr, nullptr};
-    // ticket -> final status and, for a failure, the worker thread's error message
-    // (zgr_last_error is per thread), until polled
+    std::deque<Request*> staged;  // admitted (arena space reserved), waiting for a free slot
+    Request* in_slot[2] = {nullptr, nullptr};  // issued, until its outputs are copied out
+    bool gpu_done[2] = {false, false};         // set by the slot's host callback
+    std::deque<int> to_finish;                 // slots for the finisher
+    struct CbArg {
+        zgr_server* sv;
+        int slot;
+    } cb[2];
+    // ticket -> final status and, for a failure, the error message of the thread that
+    // failed it (zgr_last_error is per thread), until polled
     std::unordered_map<uint64_t, std::pair<int, std::string>> done;
     std::unordered_map<uint64_t, bool> pending;  // submitted tickets not yet completed
     uint64_t next_id = 1, n_completed = 0;
     int n_admitted = 0;
     bool stop = false;
-    std::thread worker;
+    bool worker_exited = false;  // set once the worker has completed every request
+    std::thread worker, finisher;
 };
 
 namespace {
+
+// complete a request (caller holds sv->m): its ticket, the arena, the in-flight count
+void server_complete(zgr_server* sv, Request* r, int st, const std::string& msg) {
+    sv->arena.release(r->off, r->len);
+    sv->n_admitted--;
+    sv->n_completed++;
+    sv->done[r->id] = {st, st ? msg : std::string()};
+    sv->pending.erase(r->id);
+    delete r;
+}
+
+// host callback queued after a slot's work: no HIP calls here
+void server_slot_done(void* arg) {
+    auto* a = (zgr_server::CbArg*)arg;
+    {
+        std::lock_guard<std::mutex> lk(a->sv->m);
+        a->sv->gpu_done[a->slot] = true;
+    }
+    a->sv->cv_work.notify_all();
+}
+
+void server_finisher(zgr_server* sv) {
+    zgr_ctx* c = sv->c;
+    (void)hipSetDevice(c->device);
+    std::unique_lock<std::mutex> lk(sv->m);
+    while (true) {
+        sv->cv_finish.wait(lk, [&] { return !sv->to_finish.empty() || sv->worker_exited; });
+        if (sv->to_finish.empty()) break;
+        const int k = sv->to_finish.front();
+        sv->to_finish.pop_front();
+        Request* r = sv->in_slot[k];
+        lk.unlock();
+        const int st = finish_blob(c, c->slots[k]);  // host copies of the staged outputs
+        const std::string msg = st ? g_err : std::string();
+        lk.lock();
+        server_complete(sv, r, st, msg);
+        sv->in_slot[k] = nullptr;  // the slot's staging may be written again
+        sv->cv_done.notify_all();
+        sv->cv_work.notify_all();
+    }
+}
 
 void server_loop(zgr_server* sv) {
     zgr_ctx* c = sv->c;
     std::lock_guard<std::mutex> own(c->mu);  // the server owns its context
     (void)hipSetDevice(c->device);
+    std::unique_lock<std::mutex> lk(sv->m);
     while (true) {
-        bool progressed = false;
-        std::vector<Request*> to_upload;
-        {
-            std::unique_lock<std::mutex> lk(sv->m);
-            // admission: by staging-arena footprint and in-flight count
-            while (!sv->queue.empty() && sv->n_admitted < sv->max_inflight) {
-                Request* r = sv->queue.front();
-                if (!sv->arena.alloc(r->len, &r->off)) break;
-                sv->queue.pop_front();
-                sv->n_admitted++;
-                to_upload.push_back(r);
-            }
-            bool busy = sv->in_slot[0] || sv->in_slot[1] || !sv->staged.empty() || !to_upload.empty();
-            if (sv->stop && sv->queue.empty() && !busy) break;
-            if (!busy && sv->queue.empty()) {
-                sv->cv_work.wait_for(lk, std::chrono::milliseconds(5));
-                continue;
-            }
-        }
-        for (Request* r : to_upload) {
-            (void)hipEventCreateWithFlags(&r->uploaded, hipEventDisableTiming);
-            if (r->len)
-                (void)hipMemcpyAsync(sv->arena.base + r->off, r->raw, r->len, hipMemcpyHostToDevice, sv->upload);
-            (void)hipEventRecord(r->uploaded, sv->upload);
-            std::lock_guard<std::mutex> lk(sv->m);
+        // admission: by staging-arena footprint and in-flight count
+        while (!sv->queue.empty() && sv->n_admitted < sv->max_inflight) {
+            Request* r = sv->queue.front();
+            if (!sv->arena.alloc(r->len, &r->off)) break;
+            sv->queue.pop_front();
+            sv->n_admitted++;
             sv->staged.push_back(r);
-            progressed = true;
         }
-        // retire finished slots
-        for (int k = 0; k < 2; k++) {
-            Request* r = sv->in_slot[k];
-            if (!r || hipEventQuery(c->slots[k].ev_done) != hipSuccess) continue;
-            int st = finish_blob(c, c->slots[k]);
-            (void)hipEventDestroy(r->uploaded);
-            {
-                std::lock_guard<std::mutex> lk(sv->m);
-                sv->arena.release(r->off, r->len);
-                sv->in_slot[k] = nullptr;
-                sv->n_admitted--;
-                sv->n_completed++;
-                sv->done[r->id] = {st, st ? g_err : std::string()};
-                sv->pending.erase(r->id);
+        // slots whose work is done go to the finisher
+        for (int k = 0; k < 2; k++)
+            if (sv->gpu_done[k]) {
+                sv->gpu_done[k] = false;
+                sv->to_finish.push_back(k);
+                sv->cv_finish.notify_all();
             }
-            sv->cv_done.notify_all();
-            delete r;
-            progressed = true;
-        }
-        // issue staged requests into free slots (upload ordered before the blob)
-        for (int k = 0; k < 2; k++) {
+        // issue into free slots
+        for (int k = 0; k < 2 && !sv->staged.empty(); k++) {
             if (sv->in_slot[k]) continue;
-            Request* r = nullptr;
-            {
-                std::lock_guard<std::mutex> lk(sv->m);
-                if (sv->staged.empty()) break;
-                r = sv->staged.front();
-                sv->staged.pop_front();
-            }
+            Request* r = sv->staged.front();
+            sv->staged.pop_front();
+            sv->in_slot[k] = r;
+            lk.unlock();  // no HIP call under the server lock (the callback takes it)
             Slot& s = c->slots[k];
             int st = ZGR_OK;
-            if (hipStreamWaitEvent(s.st, r->uploaded, 0) != hipSuccess)
-                st = fail(ZGR_ERR_DEVICE, "hipStreamWaitEvent failed");
+            if (r->len && hipMemcpyAsync(sv->arena.base + r->off, r->raw, r->len, hipMemcpyHostToDevice, s.st) != hipSuccess)
+                st = fail(ZGR_ERR_DEVICE, "upload of a request's raw bytes failed");
             if (!st) st = issue_blob(c, s, sv->arena.base + r->off, r->len, r->out);
-            if (st) {  // could not be issued: complete it with the error
+            if (!st && hipLaunchHostFunc(s.st, server_slot_done, &sv->cb[k]) != hipSuccess)
+                st = fail(ZGR_ERR_DEVICE, "hipLaunchHostFunc failed");
+            if (st) {  // could not be issued: complete it with the error, slot free again
                 s.busy = false;
+                s.out = nullptr;
                 (void)hipStreamSynchronize(s.st);
-                (void)hipEventDestroy(r->uploaded);
-                {
-                    std::lock_guard<std::mutex> lk(sv->m);
-                    sv->arena.release(r->off, r->len);
-                    sv->n_admitted--;
-                    sv->n_completed++;
-                    sv->done[r->id] = {st, g_err};
-                    sv->pending.erase(r->id);
-                }
-                sv->cv_done.notify_all();
-                delete r;
-            } else {
-                sv->in_slot[k] = r;
             }
-            progressed = true;
+            const std::string msg = st ? g_err : std::string();
+            lk.lock();
+            if (st) {
+                sv->in_slot[k] = nullptr;
+                server_complete(sv, r, st, msg);
+                sv->cv_done.notify_all();
+            }
         }
-        if (!progressed) std::this_thread::sleep_for(std::chrono::microseconds(100));
+        const bool busy = sv->in_slot[0] || sv->in_slot[1] || !sv->staged.empty() || !sv->to_finish.empty();
+        if (sv->stop && sv->queue.empty() && !busy) break;
+        // sleep until a submission, a slot callback or a finished copy-out wakes us (the
+        // timeout only guards against a lost wake-up)
+        const bool can_issue = !sv->staged.empty() && (!sv->in_slot[0] || !sv->in_slot[1]);
+        if (!can_issue && !sv->gpu_done[0] && !sv->gpu_done[1]) sv->cv_work.wait_for(lk, std::chrono::milliseconds(50));
     }
+    lk.unlock();
     (void)sync_all(c);
 }
 
@@ -1124,8 +1203,11 @@ void zgr_ctx_destroy(zgr_ctx* c) {
         if (l.st) (void)hipStreamSynchronize(l.st);
     for (auto& s : c->slots)
         if (s.st) (void)hipStreamSynchronize(s.st);
-    void* ptrs[] = {c->consts, c->tw,        c->tw_inv,   c->pw_lo[0], c->pw_lo[1], c->pw_hi[0], c->pw_hi[1],
-                    c->stw,    c->stw_inv, c->ninv29,   c->flag,     c->work[0],  c->work[1]};
+    void* ptrs[] = {c->consts, c->tw,      c->tw_inv, c->pw_lo[0], c->pw_lo[1], c->pw_hi[0],
+                    c->pw_hi[1], c->stw,   c->stw_inv, c->ninv29,  c->flag};
+    for (auto& pl : c->plans)
+        for (void* p : {(void*)pl.work[0], (void*)pl.work[1]})
+            if (p) (void)hipFree(p);
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     for (void* p : c->rec_buf)
@@ -1558,7 +1640,7 @@ int zgr_multiple_multiexp(zgr_ctx* c, int coset, const uint8_t* ri_mont, size_t 
     Lane& ln = c->lanes[0];
     uint32_t* opts = c->slots[0].outpts;
     HIPCHK(hipMemcpyAsync(ln.y, ri_mont, n * 32, hipMemcpyHostToDevice, ln.st));
-    CHECK(msm_lines(c, ln, coset, ln.y, 0, c->nlines, false));  // every chunk of every line
+    CHECK(msm_lines(c, ln, coset, ln.y, 0, c->nlines, nullptr));  // every chunk of every line
     uint32_t total = (uint32_t)(c->nlines * c->nb);
     LAUNCH(c, ln.st, "msm_leaves_out", k_msm_leaves_out, blocks(total, 64), 64, 0, ln.heap, c->h, heap_windows(c),
            c->nlines, opts);
@@ -1576,8 +1658,8 @@ int zgr_amt_prove(zgr_ctx* c, int coset, const uint8_t* ri_mont, size_t n, uint8
     Lane& ln = c->lanes[0];
     uint32_t* opts = c->slots[0].outpts;
     HIPCHK(hipMemcpyAsync(ln.y, ri_mont, n * 32, hipMemcpyHostToDevice, ln.st));
-    CHECK(msm_lines(c, ln, coset, ln.y, 0, c->nlines, c->fb));
-    CHECK(amt_outputs(c, ln, opts, c->nlines, c->fb));
+    CHECK(msm_lines(c, ln, coset, ln.y, 0, c->nlines, plan_for(c, ZGR_DENSE)));
+    CHECK(amt_outputs(c, ln, opts, c->nlines, plan_for(c, ZGR_DENSE)));
     const size_t slots = out_slots(c), T = tree_pts(c);
     std::vector<uint8_t> pts(slots * 64);
     HIPCHK(hipMemcpyAsync(pts.data(), opts, pts.size(), hipMemcpyDeviceToHost, ln.st));
@@ -1633,16 +1715,15 @@ int zgr_server_create(zgr_ctx* c, size_t hbm_budget, int max_inflight, zgr_serve
     sv->c = c;
     sv->max_inflight = max_inflight;
     sv->arena.size = Arena::round(hbm_budget ? hbm_budget : 1);
+    for (int k = 0; k < 2; k++) sv->cb[k] = {sv, k};
     int rc = dmalloc((void**)&sv->arena.base, sv->arena.size);
-    if (!rc && hipStreamCreateWithFlags(&sv->upload, hipStreamNonBlocking) != hipSuccess)
-        rc = fail(ZGR_ERR_DEVICE, "hipStreamCreateWithFlags failed");
     if (rc) {
-        if (sv->arena.base) (void)hipFree(sv->arena.base);
         delete sv;
         return rc;
     }
     sv->arena.free_[0] = sv->arena.size;
     sv->worker = std::thread(server_loop, sv);
+    sv->finisher = std::thread(server_finisher, sv);
     *out = sv;
     return ZGR_OK;
 }
@@ -1654,12 +1735,14 @@ void zgr_server_destroy(zgr_server* sv) {
         sv->stop = true;
     }
     sv->cv_work.notify_all();
-    if (sv->worker.joinable()) sv->worker.join();
-    (void)hipSetDevice(sv->c->device);
-    if (sv->upload) {
-        (void)hipStreamSynchronize(sv->upload);
-        (void)hipStreamDestroy(sv->upload);
+    if (sv->worker.joinable()) sv->worker.join();  // every request completed
+    {
+        std::lock_guard<std::mutex> lk(sv->m);
+        sv->worker_exited = true;
     }
+    sv->cv_finish.notify_all();
+    if (sv->finisher.joinable()) sv->finisher.join();
+    (void)hipSetDevice(sv->c->device);
     if (sv->arena.base) (void)hipFree(sv->arena.base);
     delete sv;
 }
@@ -2228,6 +2311,126 @@ int zgr_recover(zgr_ctx* c, const uint32_t* row_ids, const uint8_t* rows, size_t
     int rc = body();
     (void)hipStreamSynchronize(st);
     return rc;
+}
+
+int zgr_autotune(zgr_ctx* c, char* report, size_t cap) {
+    if (!c) return fail(ZGR_ERR_INVALID_ARG, "null argument");
+    std::lock_guard<std::mutex> lk(c->mu);
+    CHECK(ensure_params(c, 0));  // the bucket kernels read coset 0's tables
+    HIPCHK(hipSetDevice(c->device));
+    CHECK(sync_all(c));
+    Lane& ln = c->lanes[0];
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    const bool prof = c->profiling;
+    c->profiling = false;
+    // mean device time of `reps` runs of f on lane 0's stream, after one warm-up run
+    auto time_ms = [&](auto&& f, int reps, double* ms) -> int {
+        CHECK(f());
+        HIPCHK(hipEventRecord(e0, ln.st));
+        for (int r = 0; r < reps; r++) CHECK(f());
+        HIPCHK(hipEventRecord(e1, ln.st));
+        HIPCHK(hipEventSynchronize(e1));
+        float t = 0;
+        HIPCHK(hipEventElapsedTime(&t, e0, e1));
+        *ms = t / reps;
+        return ZGR_OK;
+    };
+    std::string rep = "{";
+    auto body = [&]() -> int {
+        char buf[256];
+        // 1. chunk-group plan per density class: scalars whose non-zero chunks are those
+        //    of coset 0 of a blob filling a share of the raw rows (chunk b holds raw row
+        //    bitrev(b)), plus the footer row; 12 lines, sort + bucket kernels
+        if (c->fb && c->plans.size() > 1) {
+            const size_t rows = c->nb;
+            const double share[ZGR_DENSITY_CLASSES] = {1.0 / 32, 1.0 / 8, 3.0 / 8, 1.0};
+            std::mt19937_64 rng(20260);
+            std::vector<uint64_t> sc(c->n * 4);
+            rep += "\"msm_plan\": {";
+            for (int cls = 0; cls < ZGR_DENSITY_CLASSES; cls++) {
+                const size_t nz = (size_t)(share[cls] * rows);
+                for (size_t b = 0; b < c->nb; b++) {
+                    size_t row = 0;
+                    for (int k = 0; k < c->h; k++) row |= ((b >> k) & 1) << (c->h - 1 - k);
+                    const bool on = row < nz || row == rows - 1;
+                    for (size_t i = 0; i < c->bs; i++) {
+                        uint64_t* v = &sc[4 * (b * c->bs + i)];
+                        for (int q = 0; q < 4; q++) v[q] = on ? rng() : 0;
+                        v[3] &= 0x0fffffffffffffffull;  // below 2^252 < r
+                    }
+                }
+                HIPCHK(hipMemcpy(ln.y, sc.data(), c->n * 32, hipMemcpyHostToDevice));
+                int best = 0;
+                double best_ms = 1e30;
+                snprintf(buf, sizeof(buf), "%s\"nonzero_rows_%zu_of_%zu\": {", cls ? ", " : "", nz == rows ? rows : nz + 1,
+                         rows);
+                rep += buf;
+                for (size_t k = 0; k < c->plans.size(); k++) {
+                    double ms = 0;
+                    CHECK(time_ms([&] { return msm_lines(c, ln, 0, ln.y, 0, c->nlines, &c->plans[k]); }, 2, &ms));
+                    snprintf(buf, sizeof(buf), "%s\"%s\": %.3f", k ? ", " : "", c->plans[k].name.c_str(), ms);
+                    rep += buf;
+                    if (ms < best_ms) {
+                        best_ms = ms;
+                        best = (int)k;
+                    }
+                }
+                c->plan_of_class[cls] = best;
+                snprintf(buf, sizeof(buf), ", \"chosen\": \"%s\"}", c->plans[best].name.c_str());
+                rep += buf;
+            }
+            rep += "}";
+        }
+        // 2. split of the context's two-pass NTT: the iNTT and one coset's DIT, as the
+        //    encode path runs them
+        if (c->log_n > NTT_MAX_LOG && c->coset_n > 1) {
+            const int n = c->log_n;
+            std::vector<int> cand;
+            for (int n1 : {(n + 1) / 2, n / 2, (n + 1) / 2 + 1})
+                if (n1 <= NTT_MAX_LOG && n - n1 <= NTT_MAX_LOG &&
+                    std::find(cand.begin(), cand.end(), n1) == cand.end())
+                    cand.push_back(n1);
+            uint32_t *a = c->slots[0].pts, *b = c->slots[0].coeff;
+            HIPCHK(hipMemsetAsync(a, 0, c->n * 32, ln.st));
+            int best = 0;
+            double best_ms = 1e30;
+            rep += std::string(rep.size() > 1 ? ", " : "") + "\"ntt_split\": {";
+            for (size_t k = 0; k < cand.size(); k++) {
+                c->ntt_n1 = cand[k];
+                double ms = 0;
+                CHECK(time_ms([&] {
+                    CHECK(ntt_dif(c, ln.st, a, b, n, 1, true));
+                    return ntt_dit(c, ln.st, b, ln.y, n, 1, false, c->scale[1]);
+                }, 3, &ms));
+                snprintf(buf, sizeof(buf), "%s\"%d+%d\": %.4f", k ? ", " : "", cand[k], n - cand[k], ms);
+                rep += buf;
+                if (ms < best_ms) {
+                    best_ms = ms;
+                    best = cand[k];
+                }
+            }
+            c->ntt_n1 = best;
+            snprintf(buf, sizeof(buf), ", \"chosen\": \"%d+%d\"}", best, n - best);
+            rep += buf;
+        }
+        rep += "}";
+        return sync_all(c);
+    };
+    int rc = body();
+    c->profiling = prof;
+    (void)hipStreamSynchronize(ln.st);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) return rc;
+    c->tune_report = rep;
+    if (report && cap) {
+        const size_t k = std::min(cap - 1, rep.size());
+        memcpy(report, rep.data(), k);
+        report[k] = 0;
+    }
+    return ZGR_OK;
 }
 
 double zgr_kernel_time_ms(zgr_ctx* c, const char* name, int* count) {

@@ -76,6 +76,7 @@ SIGNATURES = {
                                 ctypes.POINTER(ctypes.c_uint64)]),
     "zgr_recover": (I, [P, P, P, S, P, S, ctypes.POINTER(S)]),
     "zgr_verify_msm": (I, [P, P, P, P, S, P]),
+    "zgr_autotune": (I, [P, ctypes.c_char_p, S]),
     "zgr_kernel_time_ms": (ctypes.c_double, [P, ctypes.c_char_p, ctypes.POINTER(I)]),
     "zgr_kernel_time_reset": (None, [P]),
     "zgr_set_profiling": (None, [P, I]),
@@ -488,6 +489,16 @@ class Context:
         _check(lib().zgr_verify_msm(self.h, _p(ids if ids.size else np.zeros(1, dtype=np.uint32)), _p(arr), _p(al),
                                     ids.size, _p(out)))
         return bytes(out)
+
+    # ---- autotuning ----
+    def autotune(self):
+        """zgr_autotune: time the bucket kernels' chunk-group plans per density class and
+        the NTT split, keep the fastest; returns the tuner's report (a dict)."""
+        import json
+
+        buf = ctypes.create_string_buffer(8192)
+        _check(lib().zgr_autotune(self.h, buf, len(buf)))
+        return json.loads(buf.value.decode())
 
     # ---- profiling ----
     def set_profiling(self, on):

@@ -272,6 +272,17 @@ int zgr_verify_msm(zgr_ctx *ctx, const uint32_t *row_ids, const uint8_t *rows, c
 int zgr_recover(zgr_ctx *ctx, const uint32_t *row_ids, const uint8_t *rows, size_t n_rows,
                 uint8_t *raw_out, size_t raw_cap, size_t *raw_len);
 
+/* ---- autotuning (no counterpart in the reference, whose GPU path hard-codes its window,
+ *      prove_gpu.rs:16-17) ----
+ * Times, on the device, the candidate chunk-group plans of the fixed-base bucket kernels
+ * (how many chunks of an AMT line share one bucket set) for each density class of a
+ * coset's scalars (coset 0 of a short blob has zero chunks; cosets 1 and 2 are dense),
+ * and the split of the context's two-pass NTT; keeps the fastest of each for the encode
+ * path and zgr_amt_prove.  Every choice gives the same outputs.  Needs coset 0's params.
+ * report (may be NULL) receives a JSON object of the timings and choices (truncated to
+ * cap - 1 bytes).  Without a call the defaults hold. */
+int zgr_autotune(zgr_ctx *ctx, char *report, size_t cap);
+
 /* ---- timing hooks for the benchmark (HIP events on the library's own stream) ---- */
 /* Average device time in ms of the named kernel over the calls since the last reset
  * (name: "msm_accum", "ntt", "merkle", ...); count receives the launch count. */

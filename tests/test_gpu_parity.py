@@ -242,3 +242,26 @@ def test_encode_batch_matches_oracle(ctx, params):
         assert np.array_equal(got.high, exp.high)
         assert np.array_equal(got.merkle, exp.merkle)
         assert np.array_equal(got.data, exp.data)
+
+
+def test_autotune_keeps_outputs(params):
+    """zgr_autotune picks a chunk-group plan per density class of a coset's scalars; the
+    encode outputs stay bit-exact with the oracle for blobs of every class."""
+    c = Z.make_context(GEO)
+    try:
+        c.params_generate(seeded_tau())
+        rep = c.autotune()
+        classes = rep["msm_plan"]
+        assert len(classes) == 4
+        for v in classes.values():
+            assert v["chosen"] in ("default", "max", "half", "chunk") and v["chosen"] in v
+        for length in (0, 100, GEO.max_raw_data_size // 9, GEO.max_raw_data_size // 3, GEO.max_raw_data_size):
+            raw = Z.seeded_bytes(DATA_SEED + 17, length)
+            got = c.encode(raw)
+            exp = O.encode(list(params), GEO.log_col, GEO.log_row, raw)
+            assert np.array_equal(got.commitment, exp.commitment), length
+            assert np.array_equal(got.proof_tree, exp.proof_tree), length
+            assert np.array_equal(got.comm_tree, exp.comm_tree), length
+            assert np.array_equal(got.file_root, exp.root), length
+    finally:
+        c.close()

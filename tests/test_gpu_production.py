@@ -266,6 +266,19 @@ def test_uneven_mix_32(ctx):
             assert np.array_equal(getattr(out, f), getattr(single, f)), (k, f)
 
 
+def test_autotune_production(ctx, blob):
+    """The tuner at 1024 x 1024: a plan per density class and the NTT split; the full
+    blob encodes to the same outputs afterwards."""
+    raw, out = blob
+    t = time.time()
+    rep = ctx.autotune()
+    log("autotune %.2f s: %s" % (time.time() - t, rep))
+    assert len(rep["msm_plan"]) == 4 and rep["ntt_split"]["chosen"] in ("10+10", "11+9")
+    got = ctx.encode(raw)
+    for f in ("commitment", "file_root", "proof_tree", "comm_tree", "high", "merkle", "data"):
+        assert np.array_equal(getattr(got, f), getattr(out, f)), f
+
+
 def test_served_equals_single(ctx, blob):
     # config 5 shape: requests of mixed sizes submitted together to the serving
     # layer; each result equals the same blob encoded alone
