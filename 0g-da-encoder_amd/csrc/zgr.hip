@@ -252,6 +252,11 @@ struct zgr_ctx {
     Slot slots[2];
     std::vector<size_t> mlens;
     int next_lane = 0;
+    // the MSM stage of one coset (sort, bucket kernels, trees, outputs) captured once per
+    // (lane, slot, coset, plan) and replayed on the lane's stream; ZGR_GRAPHS=0 launches
+    // the kernels one by one instead
+    bool graphs = true;
+    std::map<std::tuple<int, int, int, int>, hipGraphExec_t> msm_graphs;
     int* flag = nullptr;
 
     // RS recovery workspace, grown on demand and kept until the context ends
@@ -632,8 +637,35 @@ int issue_blob(zgr_ctx* c, Slot& s, const uint8_t* raw_dev, size_t len, const zg
         // runs for coset 0 only and finish_blob copies its result to the others
         const int lines = cs == 0 ? c->nlines : c->nlines - 1;
         const FbPlan* plan = plan_for(c, cs == 0 ? density_class(c, len) : ZGR_DENSE);
-        CHECK(msm_lines(c, ln, cs, y, 1, lines, plan));
-        CHECK(amt_outputs(c, ln, s.outpts + (size_t)cs * slots * 16, lines, plan));
+        uint32_t* outp = s.outpts + (size_t)cs * slots * 16;
+        auto stage = [&] {
+            CHECK(msm_lines(c, ln, cs, y, 1, lines, plan));
+            return amt_outputs(c, ln, outp, lines, plan);
+        };
+        if (c->graphs && plan && !c->profiling) {  // (event timers cannot be captured)
+            const auto key = std::make_tuple((int)(&ln - c->lanes), (int)(&s - c->slots), cs,
+                                             (int)(plan - c->plans.data()));
+            auto it = c->msm_graphs.find(key);
+            if (it == c->msm_graphs.end()) {
+                hipGraph_t g = nullptr;
+                hipGraphExec_t ex = nullptr;
+                HIPCHK(hipStreamBeginCapture(ln.st, hipStreamCaptureModeThreadLocal));
+                const int rc = stage();
+                const hipError_t ec = hipStreamEndCapture(ln.st, &g);
+                if (rc) {
+                    if (g) (void)hipGraphDestroy(g);
+                    return rc;
+                }
+                if (ec != hipSuccess) return fail(ZGR_ERR_DEVICE, "hipStreamEndCapture: %s", hipGetErrorString(ec));
+                const hipError_t ei = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(g);
+                if (ei != hipSuccess) return fail(ZGR_ERR_DEVICE, "hipGraphInstantiate: %s", hipGetErrorString(ei));
+                it = c->msm_graphs.emplace(key, ex).first;
+            }
+            HIPCHK(hipGraphLaunch(it->second, ln.st));
+        } else {
+            CHECK(stage());
+        }
         LAUNCH(c, ln.st, "merkle_leaves", k_merkle_leaves, blocks(n / 8, 128), 128, 0, y, c->log_col, c->log_row,
                cs, s.data, s.mlayers[0]);
         HIPCHK(hipEventRecord(s.ev_coset[cs], ln.st));
@@ -1097,6 +1129,8 @@ int zgr_ctx_create(int device, int log_col, int log_row, int coset_n, zgr_ctx** 
     c->bs = c->n / c->nb;
     // fixed-base tables unless the chunk is too large for their entry format, or
     // ZGR_FIXED_BASE=0 asks for the windowed MSM (same results, far less memory)
+    const char* genv = getenv("ZGR_GRAPHS");
+    c->graphs = !(genv && genv[0] == '0');
     const char* fbenv = getenv("ZGR_FIXED_BASE");
     c->fb = (c->log_n - c->h) <= FB_MAX_LOG_BS && !(fbenv && fbenv[0] == '0');
     auto init = [&]() -> int {
@@ -1243,6 +1277,7 @@ void zgr_ctx_destroy(zgr_ctx* c) {
         (void)hipEventDestroy(t.a);
         (void)hipEventDestroy(t.b);
     }
+    for (auto& g : c->msm_graphs) (void)hipGraphExecDestroy(g.second);
     destroy_streams(c);
     delete c;
 }

@@ -210,10 +210,12 @@ def main():
     raws = [Z.seeded_bytes(DATA_SEED, geo.max_raw_data_size)]
     keep, ptrs, lens = resident(raws)
     out1 = Z.EncodedBlobOutputs(geo)
-    elapsed, lat = timed_steps(lambda: ctx.encode_device(ptrs[0], lens[0], out=out1), args.steps, args.warmup,
-                               profile=True)
+    elapsed, lat = timed_steps(lambda: ctx.encode_device(ptrs[0], lens[0], out=out1), args.steps, args.warmup)
     value = args.steps / elapsed
-    # per-kernel event times inside the plain pipeline (overlapping streams)
+    # per-kernel event times inside the plain pipeline (overlapping streams), from a
+    # separate pass: the event timers sit between the launches (and keep the MSM stage
+    # out of its captured graphs), so they are not in the timed steps above
+    timed_steps(lambda: ctx.encode_device(ptrs[0], lens[0], out=out1), 3, 0, profile=True)
     kernels = {}
     for name in ("pack", "ntt", "coset_scale", "msm_sort", "msm_scatter", "msm_group_scatter", "msm_accum",
                  "msm_reduce", "msm_tree", "msm_horner", "merkle_leaves", "merkle_level", "file_root"):

@@ -265,3 +265,38 @@ def test_autotune_keeps_outputs(params):
             assert np.array_equal(got.file_root, exp.root), length
     finally:
         c.close()
+
+
+def test_graph_replay_equals_launches(params):
+    """The MSM stage replayed from captured graphs (the default) and launched kernel by
+    kernel (ZGR_GRAPHS=0) give the oracle's outputs, over repeated encodes of blobs of
+    different density (several captured graphs, each replayed more than once)."""
+    import os
+
+    ctxs = []
+    try:
+        for mode in ("1", "0"):
+            old = os.environ.get("ZGR_GRAPHS")
+            os.environ["ZGR_GRAPHS"] = mode
+            try:
+                c = Z.Context(GEO)
+            finally:
+                if old is None:
+                    os.environ.pop("ZGR_GRAPHS", None)
+                else:
+                    os.environ["ZGR_GRAPHS"] = old
+            c.params_generate(seeded_tau())
+            ctxs.append(c)
+        for rep in range(3):
+            for length in (0, 5000, GEO.max_raw_data_size):
+                raw = Z.seeded_bytes(DATA_SEED + 40 + rep, length)
+                a, b = ctxs[0].encode(raw), ctxs[1].encode(raw)
+                exp = O.encode(list(params), GEO.log_col, GEO.log_row, raw)
+                for got in (a, b):
+                    assert np.array_equal(got.commitment, exp.commitment), (rep, length)
+                    assert np.array_equal(got.proof_tree, exp.proof_tree), (rep, length)
+                    assert np.array_equal(got.comm_tree, exp.comm_tree), (rep, length)
+                    assert np.array_equal(got.data, exp.data), (rep, length)
+    finally:
+        for c in ctxs:
+            c.close()
