@@ -8,6 +8,7 @@
 //   The MSM scalar vector of a coset is ri[i] = y[bitrev(i)] (index_reverse,
 //   crates/amt/src/utils.rs:70-88), gathered on the fly by k_msm_digits.
 #pragma once
+#include "common.h"
 #include "curve29.h"
 #include "keccak.h"
 
@@ -29,10 +30,6 @@ __host__ __device__ inline int fb_group_log(int L, int h) {
     return g < FB_GL_MAX ? g : FB_GL_MAX;
 }
 
-__device__ __forceinline__ uint32_t bitrev_bits(uint32_t x, int bits) {
-    return bits ? (__builtin_bitreverse32(x) >> (32 - bits)) : 0u;
-}
-
 // Spread consecutive workgroup ids over the 8 XCDs so that each XCD works on a
 // contiguous range of logical ids (and so on one L2-resident set of chunks).
 __device__ __forceinline__ uint32_t xcd_remap(uint32_t g, uint32_t total) {
@@ -51,15 +48,6 @@ __device__ __forceinline__ int wave_max(int v) {
 // ---------------------------------------------------------------------------
 // constants
 // ---------------------------------------------------------------------------
-// consts buffer (8 u32 per entry, Fr mont)
-enum {
-    K_OMEGA = 0,
-    K_OMEGA_INV = 1,
-    K_NINV = 2,
-    K_CF = 3,  // K_CF + c, c < 3
-    K_COUNT = 8
-};
-
 // TWO_ADIC_ROOT_OF_UNITY of Fr = 5^((r-1)/2^28), canonical
 __device__ __forceinline__ fr g28_canon() {
     fr g;
@@ -1132,18 +1120,6 @@ __global__ __launch_bounds__(64) void k_fixed_base_table(uint32_t* table) {
         a.y = f_zero<FqP>();
     }
     aff_store(table + 16 * (size_t)t, a);
-}
-
-// [s] G1 for a canonical scalar s, via the fixed-base table
-__device__ __forceinline__ g1_xyzz fixed_base_mul(const uint32_t* table, const fr& s) {
-    g1_xyzz acc = xyzz_inf();
-    for (int w = 0; w < 32; w++) {
-        uint32_t d = (s.v[w >> 2] >> (8 * (w & 3))) & 0xffu;
-        g1_aff P = aff_load(table + 16 * (size_t)(w * 256 + (d ? d : 1)));
-        g1_xyzz nxt = xyzz_madd(acc, P);
-        acc = xyzz_select(d != 0, nxt, acc);
-    }
-    return acc;
 }
 
 // per-coset scalars: pc[0] = tau' = tau / cf, pc[1] = tau'^N - 1, pc[2] = tau^(3N),

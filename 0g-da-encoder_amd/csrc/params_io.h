@@ -2,17 +2,18 @@
 // tau in G1 and G2 (PowerTau::setup_inner, crates/amt/src/power_tau.rs:54-80), the G2
 // vanishes of each coset (gen_vanishes, crates/amt/src/prove_params/generate.rs:234-263),
 // and the conversion of points to arkworks' serialized forms for the parameter files
-// (ark-serialize 0.4 CanonicalSerialize of G1Affine / G2Affine).  Included once, by
-// zgr.hip, after kernels.h (fixed_base_mul, k_params_consts) and params_pp.h.
+// (ark-serialize 0.4 CanonicalSerialize of G1Affine / G2Affine).  Compiled once, in
+// zgr_params.hip; zgr.hip launches these kernels through params_kernels.h.
 #pragma once
+#include "common.h"
 #include "g2.h"
+#include "params_kernels.h"
 
 namespace zgr {
 
 // ---------------------------------------------------------------------------
 // fixed-base multiplication in G2
 // ---------------------------------------------------------------------------
-constexpr int G2_WORDS = 32;  // one affine G2 point in memory: x.c0 x.c1 y.c0 y.c1
 
 // table[w * 256 + j] = [j 2^(8 w)] G2 (affine Montgomery), j > 0
 __global__ __launch_bounds__(64) void k_g2_fixed_base_table(uint32_t* table) {
@@ -114,7 +115,6 @@ __global__ __launch_bounds__(64) void k_vanish_closed(const uint32_t* pc, const 
     g2_store(out + G2_WORDS * (size_t)idx, a);
 }
 
-constexpr int G2X_WORDS = 64;  // one XYZZ G2 point in the DFT workspace
 
 __device__ __forceinline__ void g2x_store(uint32_t* p, const g2_xyzz& v) {
     const fq f[8] = {v.X.c0, v.X.c1, v.Y.c0, v.Y.c1, v.ZZ.c0, v.ZZ.c1, v.ZZZ.c0, v.ZZZ.c1};

@@ -39,7 +39,7 @@
 #include "verify.h"
 #include "params_pp.h"
 #include "ntt.h"
-#include "params_io.h"
+#include "params_kernels.h"
 
 using namespace zgr;
 
