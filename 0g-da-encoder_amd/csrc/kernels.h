@@ -306,7 +306,7 @@ __global__ __launch_bounds__(ACC_THREADS, 2) void k_msm_accum(const uint32_t* ba
 }
 
 // One level of the per-(line, window) binary trees over the chunks:
-// heap[i] = heap[2i] + heap[2i+1] for i in [2^d, 2^(d+1)).  plan: 2 bits per line, the
+// heap[i] = heap[2i] + heap[2i+1] for i in [2^d, 2^(d+1)).  plan: 4 bits per line, the
 // log2 chunk-group size gl of its bucket sets, which already produced its level h - gl
 // (left as it is); 0 = every line from the chunks.
 __global__ __launch_bounds__(64) void k_msm_tree_level(uint32_t* heap, int h, int d, uint32_t total, int nw,
@@ -315,7 +315,7 @@ __global__ __launch_bounds__(64) void k_msm_tree_level(uint32_t* heap, int h, in
     if (t >= total) return;
     const uint32_t nb = 1u << h;
     uint32_t lw = t >> d;
-    if (d >= h - (int)((plan >> (2 * (lw / (uint32_t)nw))) & 3u)) return;
+    if (d >= h - (int)((plan >> (4 * (lw / (uint32_t)nw))) & 15u)) return;
     uint32_t i = (1u << d) + (t & ((1u << d) - 1));
     uint32_t* base = heap + (size_t)lw * 2 * nb * X29_WORDS;
     x29 a = x29_load(base + (size_t)(2 * i) * X29_WORDS);
@@ -382,56 +382,104 @@ __global__ __launch_bounds__(64) void k_msm_root_out(const uint32_t* heap, int h
 // Fixed-base MSM over the resident prove bases
 // ---------------------------------------------------------------------------
 // The prove bases do not change from blob to blob, so every base P is kept with its
-// shifted copies P_j = 2^(12 j) P, j < FB_W.  A scalar with signed 12-bit digits
-// s = sum_j d_j 2^(12 j) contributes sum_j |d_j| (sign(d_j) P_j): every chunk MSM
-// (line L, chunk b) becomes ONE bucket sum over FB_W * bs (point, digit) pairs into
-// 2048 buckets -- about 24.6k additions per 1024-point chunk instead of 32 windows x
-// (1024 + 128) = 36.9k, and no window recombination (no doublings, one tree per line).
-// The digits, and so the sorted entries, are shared by every line of a chunk.
+// shifted copies P_j = 2^(c j) P, j < W (W c-bit windows cover a scalar).  A scalar with
+// signed c-bit digits s = sum_j d_j 2^(c j) contributes sum_j |d_j| (sign(d_j) P_j): a
+// chunk MSM (line L, chunk b) becomes ONE bucket sum over W * bs (point, digit) pairs
+// into 2^(c-1) buckets, with no window recombination (no doublings, one tree per line).
+// The digits, and so the sorted entries, are shared by every line.
+//
+// Two window widths:
+//  * narrow, c = 12 (2048 buckets, 22 digits): one wave sums one (line, group of up to
+//    4 chunks); about 24.6k additions per 1024-point chunk instead of 32 windows x
+//    (1024 + 128) = 36.9k for windowed Pippenger;
+//  * wide, c = 16 (32768 buckets, 16 digits): for the lines whose AMT outputs are sums
+//    over groups of at least 16 chunks (quotient lines 0..h-5 and the high line).  A
+//    bucket set per group then costs 2 x 32768 additions against 16 x 1024 x G entries,
+//    and 16 digits instead of 22 cut the entries by 27 %.  Their waves take bucket
+//    segments (ranges of the sorted list that never split a (bucket, group) run), so a
+//    wave's length does not grow with the group.
 //
 // Table layout (16 u32 per point: x, y as 256-bit integers holding the 2^261-
-// Montgomery value, canonical): point (L, b, j, k) at ((L * nb + b) * FB_W + j) * bs + k,
-// i.e. a chunk's FB_W * bs points are contiguous.
+// Montgomery value, canonical): point (slot, b, j, k) at ((slot * nb + b) * W + j) * bs
+// + k, i.e. a chunk's W * bs points are contiguous.  The narrow table holds every line
+// (slot = L), the wide table the wide lines (slot = their order).
 constexpr int FB_C = 12;
 constexpr int FB_W = 22;  // ceil(255 / 12): the top digit (bits 252..253 + carry) never carries out
 constexpr int FB_BUCKETS = 1 << (FB_C - 1);  // |digit| in 1..2048
-constexpr int FB_LANES = 64;                 // one wave per (line, chunk group)
+constexpr int FB_LANES = 64;                 // one wave per work item
 constexpr int FB_MAX_LOG_BS = 10;
-// Sorted entry: point index within the chunk group's table region (g * FB_W + j) * bs + k
-// (< 4 * 22 * 1024 < 2^18) | sign << 18 | |digit| << 19.
+constexpr int FW_C = 16;
+constexpr int FW_W = 16;  // 16 x 16 bits: the top digit (bits 240..253 + carry) never carries out
+constexpr int FW_W_LOG = 4;
+constexpr int FW_BUCKETS = 1 << (FW_C - 1);  // |digit| in 1..32768
+constexpr int FW_BLK_LOG = 2;                // chunks per block of the wide sort: 4
+constexpr int FW_MIN_GL = 4;                 // a line goes wide when its groups hold >= 16 chunks
+constexpr int FW_MAX_LINES = 16;
+constexpr int FW_RSEG = 2048;                // buckets per reduction wave
+constexpr int FW_RSEGS = FW_BUCKETS / FW_RSEG;
+constexpr int FW_SORT_T = 256;   // threads per block of the wide sort (one 4-chunk block each)
+constexpr int FW_TILE = 128;     // buckets per block of the offset scan
+constexpr int FW_TILES = FW_BUCKETS / FW_TILE;
+// Sorted entry (both widths, 8 bytes): x = point index within the wave's table region
+// (narrow: (g * FB_W + j) * bs + k for chunk g of the group; wide: (b * FW_W + j) * bs +
+// k) | sign << 31; y = |digit| << 16 | the block of the wide sort (0 in narrow lists).
+// An entry's key (its bucket within its group) is y with the low bits below the group's
+// block granularity cleared: every key is >= 2^16, never 0.
+constexpr uint32_t FB_SIGN_BIT = 0x80000000u;
+constexpr int FB_BSTART = FB_BUCKETS + 2;  // bucket starts 1..FB_BUCKETS, then the count
+// Narrow entry as staged in k_fb_scatter's LDS: index (< 4 * 22 * 1024 < 2^18) | sign
+// << 18 | |digit| << 19.
 constexpr int FB_SIGN_SHIFT = 18;
 constexpr int FB_MAG_SHIFT = 19;
 constexpr uint32_t FB_IDX_MASK = (1u << FB_SIGN_SHIFT) - 1;
-constexpr int FB_BSTART = FB_BUCKETS + 2;  // bucket starts 1..FB_BUCKETS, then the count
 
-// per-level sorted entries: level gl groups 2^gl consecutive chunks
+// per-level narrow sorted entries: level gl groups 2^gl consecutive chunks
 struct FbLevels {
-    const uint32_t* ent[FB_GL_MAX + 1];  // (nb >> gl) groups x 2^gl * FB_W * bs entries
+    const uint2* ent[FB_GL_MAX + 1];     // (nb >> gl) groups x 2^gl * FB_W * bs entries
     const uint32_t* cnt[FB_GL_MAX + 1];  // entries per group
     const uint32_t* bst[FB_GL_MAX + 1];  // bucket starts per group (FB_BSTART each)
 };
+
+// The wide lines of one launch: the shared sorted list, its bucket offsets, and where
+// each wide line w keeps its segments, bucket sums and reduction partials.
+struct FbWide {
+    const uint2* ent;       // entries sorted by (|digit|, sort block)
+    const uint32_t* off;    // off[blk * FW_BUCKETS + m - 1]: first entry of (m, blk)
+    const uint32_t* tot;    // tot[0]: number of entries
+    const uint32_t* seg;    // line w's segment boundaries at seg[segbase[w] + w + s], s <= nseg
+    const uint32_t* table;  // wide shifted copies, slot w
+    uint32_t* bkt;          // bucket sums: (gbase[w] + S) * FW_BUCKETS + m - 1
+    uint32_t* hand;         // per wide sums wave: 2 x 64 lane partials
+    uint32_t* part;         // reduction partials: (gbase[w] + S) * FW_RSEGS + q
+    uint32_t nblk;          // blocks of the wide sort: nb >> FW_BLK_LOG
+    int nw;
+    // Per-line fields, indexed by a wave's line w: 32-bit each, so that every scalar load
+    // of one has a dword-aligned base (a byte array here let the compiler fold a byte
+    // offset into the base of a dword load, whose low address bits the scalar unit drops).
+    uint32_t segbase[FW_MAX_LINES + 1];  // prefix of the lines' segment counts
+    uint32_t gbase[FW_MAX_LINES + 1];    // prefix of the lines' group counts
+    uint32_t line[FW_MAX_LINES];         // AMT line of wide line w
+    uint32_t sh[FW_MAX_LINES];           // log2(sort blocks per group) = gl - FW_BLK_LOG
+};
+
+// Work items of k_fb_sums / k_fb_reduce (x; y = the wave's narrow bucket region or wide
+// hand-off slot): narrow L | gl << 8 | S << 10; wide sums FB_WIDE | w | s << 8; wide
+// reduction FB_WIDE | w | q << 4 | S << 8.
+constexpr uint32_t FB_WIDE = 0x80000000u;
 
 // Wave id: without a work list every wave has the same work, and consecutive ids (the
 // lines of one chunk, which read the same sorted entries) are kept on one XCD; a work
 // list is sorted longest first, so its entries are dealt round-robin to the XCDs as
 // dispatched, giving each XCD the same mix of wave lengths.
-__device__ __forceinline__ uint32_t fb_wave_id(const uint32_t* work) {
+__device__ __forceinline__ uint32_t fb_wave_id(const uint2* work) {
     return work ? blockIdx.x : xcd_remap(blockIdx.x, gridDim.x);
 }
 
-// wave -> (line, level, group): from the work list, or (no list) every line at level 0
-__device__ __forceinline__ void fb_work(const uint32_t* work, uint32_t wid, int lines, uint32_t* L, uint32_t* gl,
-                                        uint32_t* S) {
-    if (work) {
-        const uint32_t u = work[wid];
-        *L = u & 0xffu;
-        *gl = (u >> 8) & 3u;
-        *S = u >> 10;
-    } else {
-        *L = wid % (uint32_t)lines;
-        *gl = 0;
-        *S = wid / (uint32_t)lines;
-    }
+// wave -> (item, region): from the work list, or (no list) every line at level 0 with
+// the wave's own region
+__device__ __forceinline__ uint2 fb_item(const uint2* work, uint32_t wid, int lines) {
+    if (work) return work[wid];
+    return make_uint2((wid % (uint32_t)lines) | ((wid / (uint32_t)lines) << 10), wid);
 }
 
 __device__ __forceinline__ fq29 fb_load_coord(const uint32_t* s) {
@@ -449,24 +497,25 @@ __device__ __forceinline__ void fb_store29(uint32_t* d, const fq29& v) {  // 256
     o[1] = make_uint4(w[4], w[5], w[6], w[7]);
 }
 
-// Build the shifted copies of `count` consecutive bases starting at global base index
-// `first` (= L * n + i), one thread per base: P_0 = P, P_(j+1) = 2^12 P_j, each stored
-// affine.  The chain runs in XYZZ; the 21 projective copies and the running products of
-// their ZZZ go to `scratch` (count * (FB_W - 1) * 40 u32), so that one inversion per base
+// Build the shifted copies of bases first .. first + count - 1 of line src_line into
+// table slot dst_slot, one thread per base: P_0 = P, P_(j+1) = 2^C P_j, each stored
+// affine.  The chain runs in XYZZ; the W - 1 projective copies and the running products
+// of their ZZZ go to `scratch` (count * (W - 1) * 40 u32), so that one inversion per base
 // (Montgomery's trick over its copies) converts all of them.  ZZ^3 = ZZZ^2 holds along
 // the chain (ZZ = Z^2, ZZZ = Z^3), so 1/ZZ = (ZZ / ZZZ)^2.
+template <int C, int W>
 __global__ __launch_bounds__(64) void k_fb_build(const uint32_t* bases, uint32_t* table, uint32_t* scratch,
-                                                 int log_n, int h, uint32_t first, uint32_t count) {
+                                                 int log_n, int h, uint32_t src_line, uint32_t dst_slot,
+                                                 uint32_t first, uint32_t count) {
     uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= count) return;
-    const uint32_t p = first + t;
+    const uint32_t i = first + t;
     const int log_bs = log_n - h;
-    const uint32_t L = p >> log_n, i = p & ((1u << log_n) - 1);
     const uint32_t b = i >> log_bs, k = i & ((1u << log_bs) - 1);
-    uint32_t* dst = table + ((((((size_t)L << h) + b) * FB_W) << log_bs) + k) * 16;
+    uint32_t* dst = table + ((((((size_t)dst_slot << h) + b) * W) << log_bs) + k) * 16;
     const size_t stride = (size_t)16 << log_bs;  // next copy of the same point
     auto slot = [&](int j) { return scratch + ((size_t)(j - 1) * count + t) * 40; };  // X, Y, ZZ, ZZZ, prefix
-    const g1_aff a = aff_load(bases + (size_t)p * 16);
+    const g1_aff a = aff_load(bases + (((size_t)src_line << log_n) + i) * 16);
     a29 P;
     P.x = q_canon(q_from_fq(a.x));
     P.y = q_canon(q_from_fq(a.y));
@@ -474,8 +523,8 @@ __global__ __launch_bounds__(64) void k_fb_build(const uint32_t* bases, uint32_t
     fb_store29(dst + 8, P.y);
     x29 q = x29_from_aff(P);
     fq29 acc = q_const(Q29::ONE);
-    for (int j = 1; j < FB_W; j++) {
-        for (int s = 0; s < FB_C; s++) q = x29_dbl(q);  // never infinity: 2^(12j) is not a multiple of r
+    for (int j = 1; j < W; j++) {
+        for (int s = 0; s < C; s++) q = x29_dbl(q);  // never infinity: 2^(C j) is not a multiple of r
         acc = q_mul(acc, q.ZZZ);
         uint32_t* sl = slot(j);
         fb_store29(sl, q.X);
@@ -484,8 +533,8 @@ __global__ __launch_bounds__(64) void k_fb_build(const uint32_t* bases, uint32_t
         fb_store29(sl + 24, q.ZZZ);
         fb_store29(sl + 32, acc);  // prod_{1..j} ZZZ
     }
-    fq29 inv = q_inv(acc);  // 1 / prod_{1..FB_W-1} ZZZ
-    for (int j = FB_W - 1; j >= 1; j--) {
+    fq29 inv = q_inv(acc);  // 1 / prod_{1..W-1} ZZZ
+    for (int j = W - 1; j >= 1; j--) {
         const uint32_t* sl = slot(j);
         const fq29 X = fb_load_coord(sl), Y = fb_load_coord(sl + 8), ZZ = fb_load_coord(sl + 16),
                    ZZZ = fb_load_coord(sl + 24);
@@ -498,31 +547,32 @@ __global__ __launch_bounds__(64) void k_fb_build(const uint32_t* bases, uint32_t
     }
 }
 
-// Signed 12-bit digits of scalar i (ri[i] = y[bitrev(i)] with from_y, else y[i]),
-// converted out of Montgomery form: s = sum_w d_w 2^(12 w), d_w in [-2047, 2048].
-__device__ __forceinline__ void fb_digits(const uint32_t* y, int log_n, int from_y, uint32_t i, int dig[FB_W]) {
+// Signed C-bit digits of scalar i (ri[i] = y[bitrev(i)] with from_y, else y[i]),
+// converted out of Montgomery form: s = sum_w d_w 2^(C w), d_w in [-2^(C-1) + 1, 2^(C-1)].
+template <int C, int W>
+__device__ __forceinline__ void fb_digits(const uint32_t* y, int log_n, int from_y, uint32_t i, int dig[W]) {
     fr s = f_from_mont(f_load<FrP>(y + 8 * (size_t)(from_y ? bitrev_bits(i, log_n) : i)));
     int carry = 0;
 #pragma unroll
-    for (int w = 0; w < FB_W; w++) {
-        const int bit = FB_C * w, wi = bit >> 5, sh = bit & 31;
+    for (int w = 0; w < W; w++) {
+        const int bit = C * w, wi = bit >> 5, sh = bit & 31;
         uint32_t v = s.v[wi] >> sh;
-        if (sh > 32 - FB_C && wi + 1 < 8) v |= s.v[wi + 1] << (32 - sh);
-        int d = (int)(v & ((1u << FB_C) - 1)) + carry;  // [0, 4096]
-        carry = d > FB_BUCKETS;
-        dig[w] = d - (carry << FB_C);
+        if (sh > 32 - C && wi + 1 < 8) v |= s.v[wi + 1] << (32 - sh);
+        int d = (int)(v & ((1u << C) - 1)) + carry;  // [0, 2^C]
+        carry = d > (1 << (C - 1));
+        dig[w] = d - (carry << C);
     }
 }
 
-// Counting sort of the digits by |digit|, in two passes over each chunk b (one thread
-// per scalar, blockDim == bs <= 1024):
+// Counting sort of the narrow digits by |digit|, in two passes over each chunk b (one
+// thread per scalar, blockDim == bs <= 1024):
 //   k_fb_count    bucket sizes -> bstart[b * FB_BSTART + m] = first entry of bucket m
 //                 (m = 1..FB_BUCKETS), [.. + FB_BUCKETS + 1] = count, counts[b] = count;
-//   k_fb_scatter  entries (point index j * bs + k, sign, |digit|) at level 0 (the
-//                 chunk's list, ascending |digit|) and at every chunk-group level
-//                 gl <= gl_used: group S = b >> gl lists bucket m of its chunks one after
-//                 the other (chunk g's indices offset by g * FB_W * bs, its table region),
-//                 so the group's bucket start is the sum of its chunks' bucket starts.
+//   k_fb_scatter  entries at level 0 (the chunk's list, ascending |digit|) and at every
+//                 chunk-group level gl <= gl_used: group S = b >> gl lists bucket m of
+//                 its chunks one after the other (chunk g's indices offset by g * FB_W *
+//                 bs, its table region), so the group's bucket start is the sum of its
+//                 chunks' bucket starts.
 //                 The chunk's list is assembled in LDS, FB_STAGE slots at a time (a
 //                 window of the sorted order; the slots stay in registers meanwhile),
 //                 and each window is copied out: at level 0 as one contiguous block, at
@@ -538,7 +588,7 @@ __global__ __launch_bounds__(1024) void k_fb_count(const uint32_t* y, int log_n,
     for (uint32_t q = k; q <= FB_BUCKETS; q += T) hist[q] = 0;
     __syncthreads();
     int dig[FB_W];
-    fb_digits(y, log_n, from_y, (b << log_bs) + k, dig);
+    fb_digits<FB_C, FB_W>(y, log_n, from_y, (b << log_bs) + k, dig);
 #pragma unroll
     for (int w = 0; w < FB_W; w++)
         if (dig[w]) atomicAdd(&hist[dig[w] < 0 ? -dig[w] : dig[w]], 1u);
@@ -579,10 +629,16 @@ constexpr uint32_t FB_NO_SLOT = 0xffffffffu;
 
 // level-gl outputs of k_fb_scatter (level 0: ent only; cnt / bst come from k_fb_count)
 struct FbOut {
-    uint32_t* ent[FB_GL_MAX + 1];
+    uint2* ent[FB_GL_MAX + 1];
     uint32_t* cnt[FB_GL_MAX + 1];
     uint32_t* bst[FB_GL_MAX + 1];
 };
+
+// staged narrow entry -> sorted entry, its index moved by `offset` (chunk g of a group)
+__device__ __forceinline__ uint2 fb_entry(uint32_t e, uint32_t offset) {
+    return make_uint2(((e & FB_IDX_MASK) + offset) | (((e >> FB_SIGN_SHIFT) & 1u) << 31),
+                      (e >> FB_MAG_SHIFT) << 16);
+}
 
 __global__ __launch_bounds__(1024) void k_fb_scatter(const uint32_t* y, int log_n, int log_bs, int from_y,
                                                      const uint32_t* bstart, int gl_used, FbOut out) {
@@ -616,7 +672,7 @@ __global__ __launch_bounds__(1024) void k_fb_scatter(const uint32_t* y, int log_
     }
     __syncthreads();
     int dig[FB_W];
-    fb_digits(y, log_n, from_y, (b << log_bs) + k, dig);
+    fb_digits<FB_C, FB_W>(y, log_n, from_y, (b << log_bs) + k, dig);
     uint32_t pos[FB_W];  // level-0 slot of each digit's entry, or FB_NO_SLOT
 #pragma unroll
     for (int w = 0; w < FB_W; w++) {
@@ -638,83 +694,248 @@ __global__ __launch_bounds__(1024) void k_fb_scatter(const uint32_t* y, int log_
         }
         __syncthreads();
         const uint32_t hi = min(count, lo + (uint32_t)FB_STAGE);
-        for (uint32_t i = lo + k; i < hi; i += T) out.ent[0][(size_t)b * cap + i] = staged[i - lo];
+        for (uint32_t i = lo + k; i < hi; i += T) out.ent[0][(size_t)b * cap + i] = fb_entry(staged[i - lo], 0);
         for (int gl = 1; gl <= gl_used; gl++) {
             const uint32_t S = b >> gl, g = b & ((1u << gl) - 1);
-            uint32_t* dst = out.ent[gl] + ((size_t)S << gl) * cap;
+            uint2* dst = out.ent[gl] + ((size_t)S << gl) * cap;
             const uint32_t* bs_gl = base[gl - 1];
             for (uint32_t i = lo + k; i < hi; i += T) {
                 const uint32_t e = staged[i - lo];
-                // the index field absorbs g * cap: j * bs + k + g * cap < 2^gl * cap < 2^18
-                dst[bs_gl[e >> FB_MAG_SHIFT] + i] = e + g * (uint32_t)cap;
+                // the index absorbs g * cap: j * bs + k + g * cap < 2^gl * cap < 2^18
+                dst[bs_gl[e >> FB_MAG_SHIFT] + i] = fb_entry(e, g * (uint32_t)cap);
             }
         }
     }
 }
 
-// Bucket sum of one (line L, chunk group) over its shifted copies, one wave, two phases:
-//  1. bucket sums: lane s takes an even share of the sorted entries and adds each
-//     bucket's points with mixed additions (R += +-P_e).  A bucket belongs to the lane
-//     holding its first entry, which writes the bucket sum B_m to `bkt` (the chunk's
-//     FB_BUCKETS slots; empty buckets stay unwritten); a lane whose share starts inside
-//     a bucket hands that partial sum to the owner through LDS;
-//  2. bucket reduction: lane s folds buckets m in (32 s, 32 s + 32] from the top with
-//     the running sum (R += B_m; T += R), its share being T + 32 s R; the shares are
-//     summed through LDS into the group's heap node of line L.
-// Splitting the work this way keeps every step of a phase the same addition on every
-// lane: phase 1 never adds two projective points, phase 2 never touches a base.
-// Both phases keep X unreduced in [0, 8p) (x29_madd_unchecked, x29_add_lazy); phase 2
-// normalises only the group's result.
-constexpr int FB_PER_LANE = FB_BUCKETS / FB_LANES;  // buckets per lane in phase 2 (32)
+// ---- the wide sort: entries by (|digit|, block of 4 chunks) over the whole coset ----
+// k_fw_count     cnt[blk * FW_BUCKETS + m - 1] = entries of block blk in bucket m
+// k_fw_tile_sums / k_fw_tile_scan / k_fw_offsets
+//                exclusive prefix of the counts in (m, blk) order -> off (same layout),
+//                tot[0] = number of entries
+// k_fw_scatter   entries to their slots, one LDS cursor per bucket
+// One block per sort block (FW_SORT_T threads, 4 * bs scalars), the bucket counters or
+// cursors in LDS (128 KB).
+__device__ __forceinline__ uint32_t fw_sort_first(uint32_t blk, int log_bs) {
+    return blk << (FW_BLK_LOG + log_bs);
+}
+
+__global__ __launch_bounds__(FW_SORT_T) void k_fw_count(const uint32_t* y, int log_n, int log_bs, int from_y,
+                                                        uint32_t* cnt) {
+    extern __shared__ uint32_t hist[];  // FW_BUCKETS
+    const uint32_t blk = blockIdx.x, t = threadIdx.x;
+    for (uint32_t q = t; q < FW_BUCKETS; q += FW_SORT_T) hist[q] = 0;
+    __syncthreads();
+    const uint32_t first = fw_sort_first(blk, log_bs), ns = 1u << (FW_BLK_LOG + log_bs);
+    for (uint32_t u = t; u < ns; u += FW_SORT_T) {
+        int dig[FW_W];
+        fb_digits<FW_C, FW_W>(y, log_n, from_y, first + u, dig);
+#pragma unroll
+        for (int j = 0; j < FW_W; j++)
+            if (dig[j]) atomicAdd(&hist[(dig[j] < 0 ? -dig[j] : dig[j]) - 1], 1u);
+    }
+    __syncthreads();
+    for (uint32_t q = t; q < FW_BUCKETS; q += FW_SORT_T) cnt[(size_t)blk * FW_BUCKETS + q] = hist[q];
+}
+
+// bucket m - 1 = q's total over the blocks
+__device__ __forceinline__ uint32_t fw_bucket_total(const uint32_t* cnt, uint32_t nblk, uint32_t q) {
+    uint32_t s = 0;
+    for (uint32_t blk = 0; blk < nblk; blk++) s += cnt[(size_t)blk * FW_BUCKETS + q];
+    return s;
+}
+
+// inclusive prefix over a block of FW_TILE threads (2 waves) through LDS
+__device__ __forceinline__ uint32_t fw_block_incl(uint32_t v, uint32_t* wtot) {
+    const uint32_t t = threadIdx.x, lane = t & 63;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t u = __shfl_up(v, o);
+        if (lane >= (uint32_t)o) v += u;
+    }
+    if (lane == 63) wtot[t >> 6] = v;
+    __syncthreads();
+    if (t >= 64) v += wtot[0];
+    return v;
+}
+
+__global__ __launch_bounds__(FW_TILE) void k_fw_tile_sums(const uint32_t* cnt, uint32_t nblk, uint32_t* tile_tot) {
+    __shared__ uint32_t wtot[2];
+    const uint32_t q = blockIdx.x * FW_TILE + threadIdx.x;
+    const uint32_t v = fw_block_incl(fw_bucket_total(cnt, nblk, q), wtot);
+    if (threadIdx.x == FW_TILE - 1) tile_tot[blockIdx.x] = v;
+}
+
+__global__ __launch_bounds__(FW_TILES) void k_fw_tile_scan(const uint32_t* tile_tot, uint32_t* tile_base,
+                                                           uint32_t* tot) {
+    __shared__ uint32_t wtot[FW_TILES / 64];
+    const uint32_t t = threadIdx.x, lane = t & 63;
+    const uint32_t x = tile_tot[t];
+    uint32_t v = x;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t u = __shfl_up(v, o);
+        if (lane >= (uint32_t)o) v += u;
+    }
+    if (lane == 63) wtot[t >> 6] = v;
+    __syncthreads();
+    for (uint32_t w = 0; w < (t >> 6); w++) v += wtot[w];
+    tile_base[t] = v - x;
+    if (t == FW_TILES - 1) tot[0] = v;
+}
+
+__global__ __launch_bounds__(FW_TILE) void k_fw_offsets(const uint32_t* cnt, uint32_t nblk, const uint32_t* tile_base,
+                                                        uint32_t* off) {
+    __shared__ uint32_t wtot[2];
+    const uint32_t q = blockIdx.x * FW_TILE + threadIdx.x;
+    const uint32_t s = fw_bucket_total(cnt, nblk, q);
+    uint32_t r = tile_base[blockIdx.x] + fw_block_incl(s, wtot) - s;
+    for (uint32_t blk = 0; blk < nblk; blk++) {
+        const size_t a = (size_t)blk * FW_BUCKETS + q;
+        const uint32_t c = cnt[a];
+        off[a] = r;
+        r += c;
+    }
+}
+
+__global__ __launch_bounds__(FW_SORT_T) void k_fw_scatter(const uint32_t* y, int log_n, int log_bs, int from_y,
+                                                          const uint32_t* off, uint2* ent) {
+    extern __shared__ uint32_t cursor[];  // FW_BUCKETS
+    const uint32_t blk = blockIdx.x, t = threadIdx.x;
+    for (uint32_t q = t; q < FW_BUCKETS; q += FW_SORT_T) cursor[q] = off[(size_t)blk * FW_BUCKETS + q];
+    __syncthreads();
+    const uint32_t first = fw_sort_first(blk, log_bs), ns = 1u << (FW_BLK_LOG + log_bs);
+    for (uint32_t u = t; u < ns; u += FW_SORT_T) {
+        const uint32_t i = first + u, b = i >> log_bs, k = i & ((1u << log_bs) - 1);
+        int dig[FW_W];
+        fb_digits<FW_C, FW_W>(y, log_n, from_y, i, dig);
+#pragma unroll
+        for (int j = 0; j < FW_W; j++) {
+            const int d = dig[j];
+            if (!d) continue;
+            const uint32_t mag = d < 0 ? -d : d;
+            const uint32_t pos = atomicAdd(&cursor[mag - 1], 1u);
+            ent[pos] = make_uint2(((((b << FW_W_LOG) | j) << log_bs) | k) | ((uint32_t)(d < 0) << 31),
+                                  (mag << 16) | blk);
+        }
+    }
+}
+
+// first entry of (bucket m0 + 1, block blk) in the (m, blk) order; blk == nblk is the
+// next bucket's first block, and past the last bucket the entry count
+__device__ __forceinline__ uint32_t fw_pos(const FbWide& wd, uint32_t m0, uint32_t blk) {
+    if (blk >= wd.nblk) {
+        m0++;
+        blk = 0;
+    }
+    return m0 >= (uint32_t)FW_BUCKETS ? wd.tot[0] : wd.off[(size_t)blk * FW_BUCKETS + m0];
+}
+
+// Segment boundaries of the wide lines' sums waves: boundary s of line w is the first
+// (bucket, group) run starting at or after entry s * tot / nseg_w, so no run is split
+// between waves.  One thread per boundary.
+__global__ void k_fw_segments(FbWide wd, uint32_t* seg) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    int w = 0;
+    while (w < wd.nw && t >= wd.segbase[w + 1] + w + 1) w++;
+    if (w >= wd.nw) return;
+    const uint32_t s = t - wd.segbase[w] - w, nseg = wd.segbase[w + 1] - wd.segbase[w];
+    const uint32_t sh = wd.sh[w], ng = wd.nblk >> sh;  // groups of the line
+    const uint32_t target = (uint32_t)(((uint64_t)s * wd.tot[0]) / nseg);
+    // smallest key index i in [0, FW_BUCKETS * ng] with start(i) >= target
+    uint32_t lo = 0, hi = (uint32_t)FW_BUCKETS * ng;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (fw_pos(wd, mid / ng, (mid % ng) << sh) >= target) hi = mid;
+        else lo = mid + 1;
+    }
+    seg[t] = lo == (uint32_t)FW_BUCKETS * ng ? wd.tot[0] : fw_pos(wd, lo / ng, (lo % ng) << sh);
+}
+
+// Bucket sums, one wave per work item, two kinds of item in one launch:
+//  * narrow (line L, chunk group S of level gl): the group's sorted list; its 2^gl
+//    chunks share one bucket set, region y of `bkt`;
+//  * wide (wide line w, segment s): entries [seg_s, seg_(s+1)) of the shared wide list,
+//    whose (bucket, group) runs go to the line's bucket sets in FbWide::bkt.
+// Lane s takes an even share of the wave's entries and adds each run's points with
+// mixed additions (R += +-P_e).  A run belongs to the lane holding its first entry,
+// which writes the run's sum to its bucket slot (empty buckets stay unwritten); a lane
+// whose share starts inside a run hands that partial sum to the owner through global
+// memory (the wave's first / last partials), so LDS does not bound occupancy.
+// Splitting the work this way keeps every step the same addition on every lane.  X stays
+// unreduced in [0, 8p) (x29_madd_unchecked).  The reduction (k_fb_reduce) is a kernel
+// of its own so that the waves on a CU run one loop body at a time: the two bodies
+// together overflow the instruction cache.
+constexpr int FB_PER_LANE = FB_BUCKETS / FB_LANES;  // buckets per lane in the reduction (32)
 constexpr uint32_t FB_NO_SHARE = 0xffffffffu;        // lane with no entries
-// XYZZ slots per (line, chunk) in `bkt`: the buckets, then each lane's first- and
-// last-bucket partials (kept in global memory so that LDS does not bound occupancy)
+// XYZZ slots per narrow region in `bkt`: the buckets, then each lane's first- and
+// last-bucket partials
 constexpr int FB_REGION = FB_BUCKETS + 2 * FB_LANES;
 
-// The two phases are separate kernels (k_fb_sums, k_fb_reduce) so that the waves on a
-// CU run one loop body at a time: the two bodies together overflow the instruction cache.
-// One wave per work item (line L, chunk group S of level gl): the group's 2^gl chunks
-// share one bucket set, whose result lands in heap node (nb >> gl) + S of line L.
-__global__ __launch_bounds__(FB_LANES, 2) void k_fb_sums(const uint32_t* table, FbLevels lv, const uint32_t* work,
-                                                         uint32_t* bkt, int log_n, int h, int lines) {
-    __shared__ uint32_t head_k[FB_LANES];               // bucket continued from an earlier lane, 0 if none
-    __shared__ uint32_t one_bucket[FB_LANES];           // the share lies inside one bucket
+__global__ __launch_bounds__(FB_LANES, 2) void k_fb_sums(const uint32_t* table, FbLevels lv, FbWide wd,
+                                                         const uint2* work, uint32_t* bkt, int log_n, int h,
+                                                         int lines) {
+    __shared__ uint32_t head_k[FB_LANES];               // run continued from an earlier lane, 0 if none
+    __shared__ uint32_t one_bucket[FB_LANES];           // the share lies inside one run
     const int log_bs = log_n - h;
-    const uint32_t wid = fb_wave_id(work);
-    uint32_t L, gl, S;
-    fb_work(work, wid, lines, &L, &gl, &S);
+    const uint2 it = fb_item(work, fb_wave_id(work), lines);
     const int s = threadIdx.x;
-    const int cnt = (int)lv.cnt[gl][S];
+    const uint2* ent;
+    const uint32_t* tab;
+    uint32_t* bk;       // bucket slots: key m << 16 | blk -> bk + (((blk >> sh) << 15) + m - 1) * X29_WORDS
+    uint32_t* first_p;  // lanes' continued-first partials, then their last partials
+    uint32_t kmask, sh;
+    int cnt;
+    if (it.x & FB_WIDE) {
+        const uint32_t w = it.x & 0xffu, sg = (it.x >> 8) & 0x7fffffu;
+        const uint32_t* b0 = wd.seg + wd.segbase[w] + w + sg;
+        const uint32_t lo = b0[0];
+        cnt = (int)(b0[1] - lo);
+        ent = wd.ent + lo;
+        tab = wd.table + ((size_t)w << (h + FW_W_LOG + log_bs)) * 16;  // slot w: nb * FW_W * bs points
+        sh = wd.sh[w];
+        kmask = 0xffff0000u | (0xffffu & ~((1u << sh) - 1));
+        bk = wd.bkt + (size_t)wd.gbase[w] * FW_BUCKETS * X29_WORDS;
+        first_p = wd.hand + (size_t)it.y * 2 * FB_LANES * X29_WORDS;
+    } else {
+        const uint32_t L = it.x & 0xffu, gl = (it.x >> 8) & 3u, S = it.x >> 10;
+        cnt = (int)lv.cnt[gl][S];
+        const size_t cap = (size_t)FB_W << log_bs;  // entries (and table points) of one chunk
+        ent = lv.ent[gl] + ((size_t)S << gl) * cap;
+        tab = table + ((((size_t)L << h) + ((size_t)S << gl)) * cap) * 16;
+        sh = 0;
+        kmask = 0xffff0000u;
+        bk = bkt + (size_t)it.y * FB_REGION * X29_WORDS;
+        first_p = bk + FB_BUCKETS * X29_WORDS;
+    }
     if (cnt == 0) return;  // zero group: k_fb_reduce writes infinity
-    const size_t cap = (size_t)FB_W << log_bs;  // entries (and table points) of one chunk
-    const uint32_t* ent = lv.ent[gl] + ((size_t)S << gl) * cap;
-    const uint32_t* tab = table + ((((size_t)L << h) + ((size_t)S << gl)) * cap) * 16;
-    uint32_t* bk = bkt + (size_t)wid * FB_REGION * X29_WORDS;  // slot m - 1 for bucket m
+    uint32_t* last_p = first_p + FB_LANES * X29_WORDS;
+    auto slot = [&](uint32_t key) {
+        return bk + (((size_t)((key & 0xffffu) >> sh) << (FW_C - 1)) + (key >> 16) - 1) * X29_WORDS;
+    };
 
-    // phase 1: even share [begin, end) of the sorted entries
-    uint32_t* first_p = bk + FB_BUCKETS * X29_WORDS;  // partial of a continued first bucket
-    uint32_t* last_p = first_p + FB_LANES * X29_WORDS;  // partial of the share's last bucket
+    // even share [begin, end) of the wave's entries
     const int begin = (s * cnt) / FB_LANES, end = ((s + 1) * cnt) / FB_LANES;
     const int steps = end - begin;
-    const uint32_t k_first = steps > 0 ? ent[begin] >> FB_MAG_SHIFT : 0u;
-    const uint32_t k_last = steps > 0 ? ent[end - 1] >> FB_MAG_SHIFT : 0u;
-    const bool cont = steps > 0 && begin > 0 && (ent[begin - 1] >> FB_MAG_SHIFT) == k_first;
+    const uint32_t k_first = steps > 0 ? ent[begin].y & kmask : 0u;
+    const uint32_t k_last = steps > 0 ? ent[end - 1].y & kmask : 0u;
+    const bool cont = steps > 0 && begin > 0 && (ent[begin - 1].y & kmask) == k_first;
     const int maxsteps = wave_max(steps);
     x29 R = x29_inf();
-    uint32_t cur = 0;   // bucket of R (0: none yet)
-    bool head = true;   // R still holds the share's first bucket
+    uint32_t cur = 0;   // run key of R (0: none yet)
+    bool head = true;   // R still holds the share's first run
     // software pipeline: the entry two steps ahead and the point one step ahead are in
     // flight while a step computes.  Past the end of its share a lane re-reads the
     // list's last entry (an index clamp, not a select on the loaded value, so the load
     // need not complete before the next one is issued); its results are discarded.
-    auto entry_at = [&](int st) -> uint32_t { return ent[min(begin + st, cnt - 1)]; };
-    uint32_t ev_next = entry_at(0), ev_after = entry_at(1);
-    const uint4* pn = (const uint4*)(tab + (size_t)(ev_next & FB_IDX_MASK) * 16);
+    auto entry_at = [&](int st) -> uint2 { return ent[min(begin + st, cnt - 1)]; };
+    uint2 ev_next = entry_at(0), ev_after = entry_at(1);
+    const uint4* pn = (const uint4*)(tab + (size_t)(ev_next.x & ~FB_SIGN_BIT) * 16);
     uint4 p0 = pn[0], p1 = pn[1], p2 = pn[2], p3 = pn[3];
     for (int st = 0; st < maxsteps; st++) {
         const bool active = st < steps;
-        const uint32_t ev = ev_next;
-        const uint32_t m = ev >> FB_MAG_SHIFT;
+        const uint2 ev = ev_next;
+        const uint32_t m = ev.y & kmask;
         const uint32_t wx[8] = {p0.x, p0.y, p0.z, p0.w, p1.x, p1.y, p1.z, p1.w};
         const uint32_t wy[8] = {p2.x, p2.y, p2.z, p2.w, p3.x, p3.y, p3.z, p3.w};
         a29 P;
@@ -723,12 +944,12 @@ __global__ __launch_bounds__(FB_LANES, 2) void k_fb_sums(const uint32_t* table, 
         // issue the next step's loads before this step's arithmetic
         ev_next = ev_after;
         ev_after = entry_at(st + 2);
-        pn = (const uint4*)(tab + (size_t)(ev_next & FB_IDX_MASK) * 16);
+        pn = (const uint4*)(tab + (size_t)(ev_next.x & ~FB_SIGN_BIT) * 16);
         p0 = pn[0];
         p1 = pn[1];
         p2 = pn[2];
         p3 = pn[3];
-        P.y = q_cneg_canon(P.y, (ev >> FB_SIGN_SHIFT) & 1u);  // table points are canonical
+        P.y = q_cneg_canon(P.y, ev.x >> 31);  // table points are canonical
         const bool same = m == cur;
         bool rare;
         x29 M = x29_madd_unchecked(R, P, &rare);
@@ -736,22 +957,22 @@ __global__ __launch_bounds__(FB_LANES, 2) void k_fb_sums(const uint32_t* table, 
         // infinity: the exact formula, on a wave-uniform branch that is almost never taken
         if (__builtin_expect(__any(active && same && rare), 0))
             if (active && same && rare) M = x29_madd(x29_norm(R), P);
-        if (active && !same && cur != 0) {  // bucket cur is complete in this share
+        if (active && !same && cur != 0) {  // run cur is complete in this share
             if (head && cont)
                 x29_store(first_p + s * X29_WORDS, R);
             else
-                x29_store(bk + (size_t)(cur - 1) * X29_WORDS, R);
+                x29_store(slot(cur), R);
             head = false;
         }
         R = x29_select(same, M, x29_from_aff(P));
         cur = m;
         if (active && st == steps - 1) x29_store(last_p + s * X29_WORDS, R);
     }
-    const bool single = steps > 0 && head;  // never left the first bucket
+    const bool single = steps > 0 && head;  // never left the first run
     head_k[s] = steps == 0 ? FB_NO_SHARE : (cont ? k_first : 0u);
     one_bucket[s] = single;
     __syncthreads();
-    // the owner of the share's last bucket adds the partials of the lanes that continue it
+    // the owner of the share's last run adds the partials of the lanes that continue it
     if (steps > 0 && !(single && cont)) {
         x29 W = x29_load(last_p + s * X29_WORDS);
         for (int t = s + 1; t < FB_LANES; t++) {
@@ -761,51 +982,91 @@ __global__ __launch_bounds__(FB_LANES, 2) void k_fb_sums(const uint32_t* table, 
             W = x29_add(W, x29_load((one_bucket[t] ? last_p : first_p) + t * X29_WORDS));
             if (!one_bucket[t]) break;
         }
-        x29_store(bk + (size_t)(k_last - 1) * X29_WORDS, W);
+        x29_store(slot(k_last), W);
     }
 }
 
-__global__ __launch_bounds__(FB_LANES, 2) void k_fb_reduce(FbLevels lv, const uint32_t* work, const uint32_t* bkt,
-                                                           uint32_t* heap, int h, int lines) {
+// Bucket reduction, one wave per work item: 2048 consecutive buckets m of one bucket
+// set (a narrow group's whole set; segment q of a wide group's), lane s folding buckets
+// (32 s, 32 s + 32] of them from the top with the running sum (R += B_m; T += R).  Its
+// share of sum_m m B_m is T + (first bucket - 1 + 32 s) R; the shares are summed through
+// LDS into the group's heap node of line L (narrow) or the segment's partial (wide,
+// summed by k_fw_combine).  Empty buckets count as infinity.
+__global__ __launch_bounds__(FB_LANES, 2) void k_fb_reduce(FbLevels lv, FbWide wd, const uint2* work,
+                                                           const uint32_t* bkt, uint32_t* heap, int h, int lines) {
     __shared__ uint32_t red[FB_LANES * X29_WORDS];
     const uint32_t nb = 1u << h;
-    const uint32_t wid = fb_wave_id(work);
-    uint32_t L, gl, S;
-    fb_work(work, wid, lines, &L, &gl, &S);
+    const uint2 it = fb_item(work, fb_wave_id(work), lines);
     const int s = threadIdx.x;
-    uint32_t* leaf = heap + ((size_t)L * 2 * nb + (nb >> gl) + S) * X29_WORDS;
-    if (lv.cnt[gl][S] == 0) {  // zero group
-        if (s == 0) x29_store(leaf, x29_inf());
-        return;
+    uint32_t* outp;
+    const uint32_t* bk;  // the lane's first bucket
+    uint32_t mul;        // the lane's bucket offset / 32
+    uint32_t filled = 0;  // bit i: bucket i of the lane's 32 has entries
+    if (it.x & FB_WIDE) {
+        const uint32_t w = it.x & 0xfu, q = (it.x >> 4) & 0xfu, S = (it.x >> 8) & 0x7fffffu;
+        const uint32_t g = wd.gbase[w] + S, sh = wd.sh[w];
+        mul = q * (FW_RSEG / FB_PER_LANE) + (uint32_t)s;
+        const uint32_t m0 = mul * FB_PER_LANE;  // bucket m0 + 1 is the lane's first
+        bk = wd.bkt + ((size_t)g * FW_BUCKETS + m0) * X29_WORDS;
+        outp = wd.part + ((size_t)g * FW_RSEGS + q) * X29_WORDS;
+        for (int i = 0; i < FB_PER_LANE; i++)
+            filled |= (uint32_t)(fw_pos(wd, m0 + i, (S + 1) << sh) > fw_pos(wd, m0 + i, S << sh)) << i;
+    } else {
+        const uint32_t L = it.x & 0xffu, gl = (it.x >> 8) & 3u, S = it.x >> 10;
+        outp = heap + ((size_t)L * 2 * nb + (nb >> gl) + S) * X29_WORDS;
+        if (lv.cnt[gl][S] == 0) {  // zero group
+            if (s == 0) x29_store(outp, x29_inf());
+            return;
+        }
+        const uint32_t* bst = lv.bst[gl] + (size_t)S * FB_BSTART + s * FB_PER_LANE + 1;  // bucket 32 s + 1
+        bk = bkt + ((size_t)it.y * FB_REGION + s * FB_PER_LANE) * X29_WORDS;
+        mul = (uint32_t)s;
+        for (int i = 0; i < FB_PER_LANE; i++) filled |= (uint32_t)(bst[i + 1] > bst[i]) << i;
     }
-    const uint32_t* bst = lv.bst[gl] + (size_t)S * FB_BSTART;
-    const uint32_t* bk = bkt + (size_t)wid * FB_REGION * X29_WORDS;
-
-    // phase 2: buckets (32 s, 32 s + 32], top down
     x29 Rr = x29_inf(), T = x29_inf();
-    for (int q = FB_PER_LANE; q >= 1; q--) {
-        const uint32_t m = (uint32_t)(s * FB_PER_LANE + q);
-        const bool filled = bst[m + 1] > bst[m];
+    for (int i = FB_PER_LANE - 1; i >= 0; i--) {
         x29 B = x29_inf();
-        if (filled) B = x29_load(bk + (size_t)(m - 1) * X29_WORDS);
+        if ((filled >> i) & 1u) B = x29_load(bk + (size_t)i * X29_WORDS);
         Rr = x29_add_lazy(Rr, B);
         T = x29_add_lazy(T, Rr);
     }
-    // T + 32 s Rr = (s Rr) doubled 5 times, plus T
-    x29 off = x29_mul_small_uniform(Rr, (uint32_t)s, 6);
+    // T + 32 mul Rr = (mul Rr) doubled 5 times, plus T (mul < 2^10, uniform per lane)
+    x29 off = x29_mul_small_uniform(Rr, mul, (it.x & FB_WIDE) ? 10 : 6);
     for (int k = 0; k < 5; k++) off = x29_dbl(off);
     x29 W = x29_add_lazy(T, off);
 
     x29_store(red + s * X29_WORDS, W);
     __syncthreads();
-    for (int off = 1; off < FB_LANES; off <<= 1) {
-        if ((s & (2 * off - 1)) == 0) {
-            W = x29_add_lazy(W, x29_load(red + (s + off) * X29_WORDS));
+    for (int o = 1; o < FB_LANES; o <<= 1) {
+        if ((s & (2 * o - 1)) == 0) {
+            W = x29_add_lazy(W, x29_load(red + (s + o) * X29_WORDS));
             x29_store(red + s * X29_WORDS, W);
         }
         __syncthreads();
     }
-    if (s == 0) x29_store(leaf, x29_norm(W));
+    if (s == 0) x29_store(outp, x29_norm(W));
+}
+
+// Group results of the wide lines: heap node (nb >> gl) + S of line L = the sum of the
+// group's FW_RSEGS reduction partials.  One wave per group, in group order.
+__global__ __launch_bounds__(FB_LANES) void k_fw_combine(FbWide wd, uint32_t* heap, int h) {
+    __shared__ uint32_t red[FW_RSEGS * X29_WORDS];
+    const uint32_t g = blockIdx.x, s = threadIdx.x;
+    int w = 0;
+    while (w + 1 < wd.nw && g >= wd.gbase[w + 1]) w++;
+    const uint32_t S = g - wd.gbase[w], gl = wd.sh[w] + FW_BLK_LOG, nb = 1u << h;
+    if (s < (uint32_t)FW_RSEGS)
+        x29_store(red + s * X29_WORDS, x29_load(wd.part + ((size_t)g * FW_RSEGS + s) * X29_WORDS));
+    __syncthreads();
+    for (uint32_t o = 1; o < (uint32_t)FW_RSEGS; o <<= 1) {
+        if (s < (uint32_t)FW_RSEGS && (s & (2 * o - 1)) == 0) {
+            const x29 W = x29_add(x29_load(red + s * X29_WORDS), x29_load(red + (s + o) * X29_WORDS));
+            x29_store(red + s * X29_WORDS, W);
+        }
+        __syncthreads();
+    }
+    if (s == 0)
+        x29_store(heap + ((size_t)wd.line[w] * 2 * nb + (nb >> gl) + S) * X29_WORDS, x29_load(red));
 }
 
 // ---------------------------------------------------------------------------

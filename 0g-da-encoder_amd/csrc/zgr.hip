@@ -147,15 +147,26 @@ struct Lane {
     hipStream_t st = nullptr;
     uint32_t* y = nullptr;        // n Fr: coset evaluations / scalars
     int8_t* digits = nullptr;     // 32 n
-    uint32_t* entries = nullptr;  // 32 n u32
+    uint32_t* entries = nullptr;  // 32 n u32 (windowed) / FB_W n uint2 (fixed-base level 0)
     uint32_t* counts = nullptr;   // 32 nb
     uint32_t* heap = nullptr;     // nlines * 32 * 2nb XYZZ (36 u32 each)
     uint32_t* bstart = nullptr;   // fixed-base: nb * FB_BSTART bucket starts
     uint32_t* bkt = nullptr;      // fixed-base: nlines * nb * FB_REGION bucket sums (XYZZ)
     // fixed-base chunk groups, levels 1..FB_GL_MAX (level 0: entries / counts / bstart)
-    uint32_t* gent[FB_GL_MAX + 1] = {};
+    uint2* gent[FB_GL_MAX + 1] = {};
     uint32_t* gcnt[FB_GL_MAX + 1] = {};
     uint32_t* gbst[FB_GL_MAX + 1] = {};
+    // wide lines (FbWide): the sort's counts and offsets (nblk x FW_BUCKETS each), its
+    // tile sums / bases and entry count, the sorted list (FW_W n), segment boundaries,
+    // bucket sums, hand-off partials and reduction partials
+    uint32_t* fw_cnt = nullptr;
+    uint32_t* fw_off = nullptr;
+    uint32_t* fw_tile = nullptr;
+    uint2* fw_ent = nullptr;
+    uint32_t* fw_seg = nullptr;
+    uint32_t* fw_bkt = nullptr;
+    uint32_t* fw_hand = nullptr;
+    uint32_t* fw_part = nullptr;
 };
 
 struct Slot {
@@ -183,12 +194,17 @@ struct Slot {
 // A chunk-group plan of the fixed-base bucket kernels: per AMT line the log2 of the
 // chunks that share one bucket set, and the work lists built from it ([0] every line,
 // [1] without the high-basis line).
+// A plan of the bucket kernels for one coset, in two variants: v = 0 every line, v = 1
+// without the high-basis line (cosets > 0, whose high commitment is coset 0's).
 struct FbPlan {
     std::string name;
-    uint32_t* work[2] = {nullptr, nullptr};
+    uint2* work[2] = {nullptr, nullptr};   // k_fb_sums items, longest first
     uint32_t nwork[2] = {0, 0};
-    uint64_t plan[2] = {0, 0};  // 2 bits per line
-    int gl_used = 0;            // deepest chunk-group level any line uses
+    uint2* rwork[2] = {nullptr, nullptr};  // k_fb_reduce items
+    uint32_t nrwork[2] = {0, 0};
+    uint64_t plan[2] = {0, 0};  // 4 bits per line: log2 chunks per bucket set
+    int gl_used = 0;            // deepest narrow chunk-group level any line uses
+    FbWide wide[2] = {};        // the wide lines (host fields only: counts, bases, lines)
 };
 
 // Density classes of a coset's scalars for the plan choice: the share of non-zero
@@ -240,6 +256,11 @@ struct zgr_ctx {
     // fixed-base tables (shifted copies of the bases); fb == false: windowed MSM
     bool fb = false;
     uint32_t* fbtab[3] = {nullptr, nullptr, nullptr};
+    // wide lines (16-bit windows, fb_wide_lines) and their shifted copies: slot w of
+    // fwtab holds line wide_lines[w]
+    std::vector<int> wide_lines;
+    uint32_t* fwtab[3] = {nullptr, nullptr, nullptr};
+    uint32_t fw_nseg = 1;  // sums waves per wide line
     // chunk-group plans of the bucket kernels (plans[0] the default), the plan each
     // density class of a coset's scalars uses (set by zgr_autotune), the tuned split of
     // the log_n-point NTT (0: ceil(log_n / 2)) and the tuner's report
@@ -440,9 +461,10 @@ int msm_lines(zgr_ctx* c, Lane& ln, int coset, const uint32_t* src, int from_y, 
     const int log_bs = c->log_n - c->h;
     if (c->fb) {
         const int gl_used = grouped ? grouped->gl_used : 0;
+        const int v = lines == c->nlines ? 0 : 1;
         FbLevels lv{};
         FbOut fo{};
-        lv.ent[0] = fo.ent[0] = ln.entries;
+        lv.ent[0] = fo.ent[0] = (uint2*)ln.entries;
         lv.cnt[0] = ln.counts;
         lv.bst[0] = ln.bstart;
         for (int gl = 1; gl <= gl_used; gl++) {
@@ -450,20 +472,56 @@ int msm_lines(zgr_ctx* c, Lane& ln, int coset, const uint32_t* src, int from_y, 
             lv.cnt[gl] = fo.cnt[gl] = ln.gcnt[gl];
             lv.bst[gl] = fo.bst[gl] = ln.gbst[gl];
         }
-        LAUNCH(c, ln.st, "msm_sort", k_fb_count, (uint32_t)c->nb, (uint32_t)c->bs, 0, src, c->log_n, log_bs, from_y,
-               ln.counts, ln.bstart);
-        LAUNCH(c, ln.st, gl_used ? "msm_group_scatter" : "msm_scatter", k_fb_scatter, (uint32_t)c->nb,
-               (uint32_t)c->bs, 0, src, c->log_n, log_bs, from_y, ln.bstart, gl_used, fo);
-        const uint32_t* work = nullptr;
-        uint32_t waves = (uint32_t)(c->nb * lines);
+        FbWide wd = grouped ? grouped->wide[v] : FbWide{};
+        bool narrow = true;  // any narrow line
         if (grouped) {
-            const int v = lines == c->nlines ? 0 : 1;
+            narrow = false;
+            for (int L = 0; L < lines; L++)
+                narrow |= std::find(wd.line, wd.line + wd.nw, (uint32_t)L) == wd.line + wd.nw;
+        }
+        if (narrow) {
+            LAUNCH(c, ln.st, "msm_sort", k_fb_count, (uint32_t)c->nb, (uint32_t)c->bs, 0, src, c->log_n, log_bs,
+                   from_y, ln.counts, ln.bstart);
+            LAUNCH(c, ln.st, gl_used ? "msm_group_scatter" : "msm_scatter", k_fb_scatter, (uint32_t)c->nb,
+                   (uint32_t)c->bs, 0, src, c->log_n, log_bs, from_y, ln.bstart, gl_used, fo);
+        }
+        if (wd.nw) {  // the wide sort, segment boundaries
+            wd.ent = ln.fw_ent;
+            wd.off = ln.fw_off;
+            wd.tot = ln.fw_tile + 2 * FW_TILES;
+            wd.seg = ln.fw_seg;
+            wd.table = c->fwtab[coset];
+            wd.bkt = ln.fw_bkt;
+            wd.hand = ln.fw_hand;
+            wd.part = ln.fw_part;
+            const size_t shm = FW_BUCKETS * 4;
+            LAUNCH(c, ln.st, "msm_wide_sort", k_fw_count, wd.nblk, FW_SORT_T, shm, src, c->log_n, log_bs, from_y,
+                   ln.fw_cnt);
+            LAUNCH(c, ln.st, "msm_wide_sort", k_fw_tile_sums, FW_TILES, FW_TILE, 0, ln.fw_cnt, wd.nblk, ln.fw_tile);
+            LAUNCH(c, ln.st, "msm_wide_sort", k_fw_tile_scan, 1, FW_TILES, 0, ln.fw_tile, ln.fw_tile + FW_TILES,
+                   ln.fw_tile + 2 * FW_TILES);
+            LAUNCH(c, ln.st, "msm_wide_sort", k_fw_offsets, FW_TILES, FW_TILE, 0, ln.fw_cnt, wd.nblk,
+                   ln.fw_tile + FW_TILES, ln.fw_off);
+            LAUNCH(c, ln.st, "msm_wide_sort", k_fw_scatter, wd.nblk, FW_SORT_T, shm, src, c->log_n, log_bs, from_y,
+                   ln.fw_off, ln.fw_ent);
+            const uint32_t nbnd = wd.segbase[wd.nw] + (uint32_t)wd.nw;
+            LAUNCH(c, ln.st, "msm_wide_sort", k_fw_segments, blocks(nbnd, 64), 64, 0, wd, ln.fw_seg);
+        }
+        const uint2* work = nullptr;
+        uint32_t waves = (uint32_t)(c->nb * lines), rwaves = waves;
+        const uint2* rwork = nullptr;
+        if (grouped) {
             work = grouped->work[v];
             waves = grouped->nwork[v];
+            rwork = grouped->rwork[v];
+            rwaves = grouped->nrwork[v];
         }
-        LAUNCH(c, ln.st, "msm_accum", k_fb_sums, waves, FB_LANES, 0, c->fbtab[coset], lv, work, ln.bkt, c->log_n,
-               c->h, lines);
-        LAUNCH(c, ln.st, "msm_reduce", k_fb_reduce, waves, FB_LANES, 0, lv, work, ln.bkt, ln.heap, c->h, lines);
+        LAUNCH(c, ln.st, "msm_accum", k_fb_sums, waves, FB_LANES, 0, c->fbtab[coset], lv, wd, work, ln.bkt,
+               c->log_n, c->h, lines);
+        LAUNCH(c, ln.st, "msm_reduce", k_fb_reduce, rwaves, FB_LANES, 0, lv, wd, rwork, ln.bkt, ln.heap, c->h,
+               lines);
+        if (wd.nw)
+            LAUNCH(c, ln.st, "msm_reduce", k_fw_combine, wd.gbase[wd.nw], FB_LANES, 0, wd, ln.heap, c->h);
         return ZGR_OK;
     }
     LAUNCH(c, ln.st, "msm_digits", k_msm_digits, blocks(c->n, 256), 256, 0, src, c->log_n, log_bs, ln.digits,
@@ -491,53 +549,121 @@ int amt_outputs(zgr_ctx* c, Lane& ln, uint32_t* out, int lines, const FbPlan* gr
     return ZGR_OK;
 }
 
-// Work lists of the grouped bucket kernels (one u32 per wave: L | gl << 8 | S << 10),
-// deepest levels first so that the longest waves start first; within a level the lines
-// of one chunk group are adjacent (they read the same sorted entries).
-// The default plan puts every line at the largest level its outputs allow
-// (fb_group_log), with one exception: when the waves of the deepest level outnumber the
+// Lines that go wide (16-bit windows): those whose AMT outputs are sums over groups of
+// at least 2^FW_MIN_GL chunks -- quotient lines 0..h-1-FW_MIN_GL and the high-basis line
+// -- when the chunks split into whole blocks of the wide sort.  ZGR_WIDE=0 keeps every
+// line narrow.
+std::vector<int> fb_wide_lines(const zgr_ctx* c) {
+    std::vector<int> w;
+    const char* env = getenv("ZGR_WIDE");
+    if ((env && env[0] == '0') || c->h < FW_MIN_GL || c->nlines > FW_MAX_LINES) return w;
+    for (int L = 0; L < c->nlines; L++) {
+        const int g = L < c->h ? c->h - 1 - L : (L == c->h ? 0 : c->h);
+        if (g >= FW_MIN_GL) w.push_back(L);
+    }
+    return w;
+}
+
+// log2 of the chunks per bucket set of wide line L (its outputs' group, uncapped)
+int fw_group_log(const zgr_ctx* c, int L) { return L < c->h ? c->h - 1 - L : c->h; }
+
+// Work lists of the bucket kernels, one item per wave (FB_WIDE and the narrow / wide
+// item layouts in kernels.h).  k_fb_sums takes them longest first (dense estimate:
+// entries / 64 per lane), so that the long waves start first; within a narrow level the
+// lines of one chunk group are adjacent (they read the same sorted entries), and the
+// wide lines of one segment are adjacent (they read nearly the same entry range).
+// The default plan puts the wide lines (fb_wide_lines) in 16-bit windows and every
+// other line at the largest narrow level its outputs allow (fb_group_log).  With no
+// wide lines it has one exception: when the waves of the deepest level outnumber the
 // wave slots k_fb_sums has on the device (4 SIMDs per CU, 2 waves each at its register
 // count), the ones left over would run last and alone, so the high-basis line (read only
 // at the root) drops one level.  At 2^20 x 12 lines that turns 2304 four-chunk waves into
 // 2048 and measured 4 % faster on dense scalars.  The other candidates of zgr_autotune:
-// "max" (no drop), "half" (at most 2-chunk groups) and "chunk" (one bucket set per chunk).
-// ZGR_GROUP_PLAN=none makes the default plan "chunk" and a comma-separated list sets each
-// line's level explicitly (each capped at fb_group_log); either one turns the tuner off.
+// "narrow" (the default without wide lines), "max" (no drop), "half" (at most 2-chunk
+// groups) and "chunk" (one bucket set per chunk).  ZGR_GROUP_PLAN=none makes the default
+// plan "chunk" and a comma-separated list sets each line's narrow level explicitly (each
+// capped at fb_group_log); either one turns the tuner off.
 constexpr int FB_SUMS_WAVES_PER_SIMD = 2;
 
 int build_plan(zgr_ctx* c, const std::string& kind, const std::vector<int>& given, FbPlan* out) {
     int cus = 0;
     HIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device));
     const size_t slots = (size_t)cus * 4 * FB_SUMS_WAVES_PER_SIMD;
+    const int log_bs = c->log_n - c->h;
     out->name = kind;
     for (int v = 0; v < 2; v++) {
         const int lines = c->nlines - v;
         std::vector<int> gl(lines, 0);
+        std::vector<bool> wide(lines, false);
+        if (kind == "default")
+            for (int L : c->wide_lines)
+                if (L < lines) wide[L] = true;
         if (kind != "chunk")
             for (int L = 0; L < lines; L++) {
                 gl[L] = fb_group_log(L, c->h);
                 if (kind == "half") gl[L] = std::min(gl[L], 1);
                 if (kind == "given" && L < (int)given.size()) gl[L] = std::max(0, std::min(gl[L], given[L]));
             }
-        if (kind == "default" && lines == c->h + 2 && gl[c->h + 1] > 0) {
-            const int top = *std::max_element(gl.begin(), gl.end());
+        if ((kind == "default" || kind == "narrow") && lines == c->h + 2 && gl[c->h + 1] > 0 && !wide[c->h + 1]) {
+            int top = 0;
+            for (int L = 0; L < lines; L++)
+                if (!wide[L]) top = std::max(top, gl[L]);
             size_t deepest = 0;
-            for (int L = 0; L < lines; L++) deepest += gl[L] == top ? c->nb >> top : 0;
+            for (int L = 0; L < lines; L++) deepest += (!wide[L] && gl[L] == top) ? c->nb >> top : 0;
             if (gl[c->h + 1] == top && deepest > slots && deepest - (c->nb >> top) <= slots) gl[c->h + 1]--;
         }
-        std::vector<uint32_t> w;
+        FbWide& wd = out->wide[v];
+        wd = FbWide{};
+        wd.nblk = (uint32_t)(c->nb >> FW_BLK_LOG);
+        for (int L = 0; L < lines; L++) {
+            if (!wide[L]) continue;
+            const int w = wd.nw++;
+            const int g = fw_group_log(c, L);
+            wd.line[w] = (uint32_t)L;
+            wd.sh[w] = (uint32_t)(g - FW_BLK_LOG);
+            wd.segbase[w + 1] = wd.segbase[w] + c->fw_nseg;
+            wd.gbase[w + 1] = wd.gbase[w] + (uint32_t)(c->nb >> g);
+            gl[L] = g;
+        }
         out->plan[v] = 0;
         for (int L = 0; L < lines; L++) {
-            out->plan[v] |= (uint64_t)gl[L] << (2 * L);
-            out->gl_used = std::max(out->gl_used, gl[L]);
+            out->plan[v] |= (uint64_t)gl[L] << (4 * L);
+            if (!wide[L]) out->gl_used = std::max(out->gl_used, gl[L]);
         }
+        // (dense length, item, region) of every sums wave; narrow regions in list order
+        struct Item {
+            uint64_t len;
+            uint32_t x, y;
+        };
+        std::vector<Item> sums, red, red_n;
+        uint32_t region = 0;
         for (int g = FB_GL_MAX; g >= 0; g--)
             for (uint32_t S = 0; S < (uint32_t)(c->nb >> g); S++)
                 for (int L = 0; L < lines; L++)
-                    if (gl[L] == g) w.push_back((uint32_t)L | (uint32_t)g << 8 | S << 10);
-        DALLOC(out->work[v], w.size() * 4);
-        HIPCHK(hipMemcpy(out->work[v], w.data(), w.size() * 4, hipMemcpyHostToDevice));
-        out->nwork[v] = (uint32_t)w.size();
+                    if (!wide[L] && gl[L] == g) {
+                        const uint32_t x = (uint32_t)L | (uint32_t)g << 8 | S << 10;
+                        sums.push_back({((uint64_t)FB_W << (g + log_bs)) / FB_LANES, x, region});
+                        red_n.push_back({0, x, region++});
+                    }
+        for (uint32_t s = 0; s < c->fw_nseg; s++)
+            for (int w = 0; w < wd.nw; w++)
+                sums.push_back({((uint64_t)FW_W * c->n / c->fw_nseg) / FB_LANES, FB_WIDE | (uint32_t)w | s << 8,
+                                wd.segbase[w] + s});
+        for (int w = 0; w < wd.nw; w++)  // wide reductions first, then the narrow ones
+            for (uint32_t S = 0; S < wd.gbase[w + 1] - wd.gbase[w]; S++)
+                for (uint32_t q = 0; q < (uint32_t)FW_RSEGS; q++)
+                    red.push_back({0, FB_WIDE | (uint32_t)w | q << 4 | S << 8, 0});
+        red.insert(red.end(), red_n.begin(), red_n.end());
+        std::stable_sort(sums.begin(), sums.end(), [](const Item& a, const Item& b) { return a.len > b.len; });
+        std::vector<uint2> ws, wr;
+        for (const Item& it : sums) ws.push_back(make_uint2(it.x, it.y));
+        for (const Item& it : red) wr.push_back(make_uint2(it.x, it.y));
+        DALLOC(out->work[v], ws.size() * sizeof(uint2));
+        HIPCHK(hipMemcpy(out->work[v], ws.data(), ws.size() * sizeof(uint2), hipMemcpyHostToDevice));
+        out->nwork[v] = (uint32_t)ws.size();
+        DALLOC(out->rwork[v], wr.size() * sizeof(uint2));
+        HIPCHK(hipMemcpy(out->rwork[v], wr.data(), wr.size() * sizeof(uint2), hipMemcpyHostToDevice));
+        out->nrwork[v] = (uint32_t)wr.size();
     }
     return ZGR_OK;
 }
@@ -557,6 +683,7 @@ int build_work_lists(zgr_ctx* c) {
         kinds = {"given"};
     } else {
         kinds = {"default", "max", "half", "chunk"};
+        if (!c->wide_lines.empty()) kinds.insert(kinds.begin() + 1, "narrow");
     }
     c->plans.resize(kinds.size());
     for (size_t k = 0; k < kinds.size(); k++) CHECK(build_plan(c, kinds[k], given, &c->plans[k]));
@@ -578,22 +705,31 @@ int density_class(const zgr_ctx* c, size_t len) {
     return ZGR_DENSE;
 }
 
-// shifted-copy table of one coset from its resident bases (after every params upload)
+// shifted-copy tables of one coset from its resident bases (after every params upload):
+// the narrow table of every line, and the wide table of the wide lines
 int build_fb(zgr_ctx* c, int coset) {
     if (!c->fb) return ZGR_OK;
+    const int nw = (int)c->wide_lines.size();
     if (!c->fbtab[coset]) DALLOC(c->fbtab[coset], (size_t)c->nlines * c->n * FB_W * 64);
+    if (nw && !c->fwtab[coset]) DALLOC(c->fwtab[coset], (size_t)nw * c->n * FW_W * 64);
     // lane 0's bucket-sum buffer is the chain scratch: its stream runs the build and
     // nothing else uses it meanwhile (the context lock is held)
     hipStream_t st = c->lanes[0].st;
     const size_t bkt_bytes = (size_t)c->nlines * c->nb * FB_REGION * X29_WORDS * 4;
-    const size_t per_base = (size_t)(FB_W - 1) * 40 * 4;
-    const size_t total = (size_t)c->nlines * c->n;
-    const size_t step = std::min(total, bkt_bytes / per_base);
-    for (size_t first = 0; first < total; first += step) {
-        const uint32_t count = (uint32_t)std::min(step, total - first);
-        LAUNCH(c, st, "fb_build", k_fb_build, blocks(count, 64), 64, 0, c->bases[coset], c->fbtab[coset],
-               c->lanes[0].bkt, c->log_n, c->h, (uint32_t)first, count);
-    }
+    auto build = [&](auto kern, int W, uint32_t* table, uint32_t src, uint32_t dst) -> int {
+        const size_t per_base = (size_t)(W - 1) * 40 * 4;
+        const size_t step = std::min(c->n, bkt_bytes / per_base);
+        for (size_t first = 0; first < c->n; first += step) {
+            const uint32_t count = (uint32_t)std::min(step, c->n - first);
+            LAUNCH(c, st, "fb_build", kern, blocks(count, 64), 64, 0, c->bases[coset], table, c->lanes[0].bkt,
+                   c->log_n, c->h, src, dst, (uint32_t)first, count);
+        }
+        return ZGR_OK;
+    };
+    for (int L = 0; L < c->nlines; L++)
+        CHECK(build(k_fb_build<FB_C, FB_W>, FB_W, c->fbtab[coset], (uint32_t)L, (uint32_t)L));
+    for (int w = 0; w < nw; w++)
+        CHECK(build(k_fb_build<FW_C, FW_W>, FW_W, c->fwtab[coset], (uint32_t)c->wide_lines[w], (uint32_t)w));
     return ZGR_OK;
 }
 
@@ -1132,7 +1268,14 @@ int zgr_ctx_create(int device, int log_col, int log_row, int coset_n, zgr_ctx** 
     const char* genv = getenv("ZGR_GRAPHS");
     c->graphs = !(genv && genv[0] == '0');
     const char* fbenv = getenv("ZGR_FIXED_BASE");
-    c->fb = (c->log_n - c->h) <= FB_MAX_LOG_BS && !(fbenv && fbenv[0] == '0');
+    c->fb = (c->log_n - c->h) <= FB_MAX_LOG_BS && c->nlines <= 16 && !(fbenv && fbenv[0] == '0');
+    if (c->fb) {
+        c->wide_lines = fb_wide_lines(c);
+        // about 1024 dense steps per lane in each wide sums wave (ZGR_WIDE_SEGS overrides)
+        const char* segenv = getenv("ZGR_WIDE_SEGS");
+        const size_t nseg = segenv ? (size_t)strtoul(segenv, nullptr, 10) : ((size_t)FW_W * c->n) / (FB_LANES * 1024);
+        c->fw_nseg = (uint32_t)std::max<size_t>(1, std::min<size_t>(nseg, 1u << 20));
+    }
     auto init = [&]() -> int {
         int max_shm = 0;
         HIPCHK(hipDeviceGetAttribute(&max_shm, hipDeviceAttributeMaxSharedMemoryPerBlock, device));
@@ -1144,6 +1287,12 @@ int zgr_ctx_create(int device, int log_col, int log_row, int coset_n, zgr_ctx** 
         if (ntt_lds > (size_t)max_shm)
             return fail(ZGR_ERR_INVALID_ARG, "NTT pass needs %zu B of LDS, device allows %d", ntt_lds, max_shm);
         HIPCHK(hipFuncSetAttribute((const void*)k_ntt_pass, hipFuncAttributeMaxDynamicSharedMemorySize, (int)ntt_lds));
+        if (!c->wide_lines.empty()) {  // the wide sort's LDS counters / cursors
+            if (FW_BUCKETS * 4 > max_shm) c->wide_lines.clear();
+            else
+                for (const void* k : {(const void*)k_fw_count, (const void*)k_fw_scatter})
+                    HIPCHK(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, FW_BUCKETS * 4));
+        }
         DALLOC(c->consts, K_COUNT * 32);
         DALLOC(c->tw, c->n / 2 * 32);
         DALLOC(c->tw_inv, c->n / 2 * 32);
@@ -1165,16 +1314,29 @@ int zgr_ctx_create(int device, int log_col, int log_row, int coset_n, zgr_ctx** 
             HIPCHK(hipStreamCreateWithFlags(&l.st, hipStreamNonBlocking));
             DALLOC(l.y, c->n * 32);
             DALLOC(l.digits, c->n * MSM_W);
-            DALLOC(l.entries, c->n * MSM_W * 4);
+            DALLOC(l.entries, std::max(c->n * MSM_W * 4, c->fb ? c->n * FB_W * sizeof(uint2) : 0));
             DALLOC(l.counts, c->nb * MSM_W * 4);
             DALLOC(l.heap, (size_t)c->nlines * MSM_W * 2 * c->nb * X29_WORDS * 4);
             if (c->fb) {
                 DALLOC(l.bstart, c->nb * FB_BSTART * 4);
                 DALLOC(l.bkt, (size_t)c->nlines * c->nb * FB_REGION * X29_WORDS * 4);
                 for (int gl = 1; gl <= FB_GL_MAX && gl <= c->h; gl++) {
-                    DALLOC(l.gent[gl], c->n * FB_W * 4);
+                    DALLOC(l.gent[gl], c->n * FB_W * sizeof(uint2));
                     DALLOC(l.gcnt[gl], (c->nb >> gl) * 4);
                     DALLOC(l.gbst[gl], (c->nb >> gl) * FB_BSTART * 4);
+                }
+                if (!c->wide_lines.empty()) {
+                    const size_t nw = c->wide_lines.size(), nblk = c->nb >> FW_BLK_LOG;
+                    size_t groups = 0;
+                    for (int L : c->wide_lines) groups += c->nb >> fw_group_log(c, L);
+                    DALLOC(l.fw_cnt, nblk * FW_BUCKETS * 4);
+                    DALLOC(l.fw_off, nblk * FW_BUCKETS * 4);
+                    DALLOC(l.fw_tile, (2 * FW_TILES + 1) * 4);
+                    DALLOC(l.fw_ent, c->n * FW_W * sizeof(uint2));
+                    DALLOC(l.fw_seg, nw * (c->fw_nseg + 1) * 4);
+                    DALLOC(l.fw_bkt, groups * FW_BUCKETS * X29_WORDS * 4);
+                    DALLOC(l.fw_hand, nw * c->fw_nseg * 2 * FB_LANES * X29_WORDS * 4);
+                    DALLOC(l.fw_part, groups * FW_RSEGS * X29_WORDS * 4);
                 }
             }
         }
@@ -1240,7 +1402,7 @@ void zgr_ctx_destroy(zgr_ctx* c) {
     void* ptrs[] = {c->consts, c->tw,      c->tw_inv, c->pw_lo[0], c->pw_lo[1], c->pw_hi[0],
                     c->pw_hi[1], c->stw,   c->stw_inv, c->ninv29,  c->flag};
     for (auto& pl : c->plans)
-        for (void* p : {(void*)pl.work[0], (void*)pl.work[1]})
+        for (void* p : {(void*)pl.work[0], (void*)pl.work[1], (void*)pl.rwork[0], (void*)pl.rwork[1]})
             if (p) (void)hipFree(p);
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
@@ -1255,9 +1417,11 @@ void zgr_ctx_destroy(zgr_ctx* c) {
         if (c->scale[i]) (void)hipFree(c->scale[i]);
         if (c->bases[i]) (void)hipFree(c->bases[i]);
         if (c->fbtab[i]) (void)hipFree(c->fbtab[i]);
+        if (c->fwtab[i]) (void)hipFree(c->fwtab[i]);
     }
     for (auto& l : c->lanes) {
-        void* lp[] = {l.y, l.digits, l.entries, l.counts, l.heap, l.bstart, l.bkt};
+        void* lp[] = {l.y,      l.digits, l.entries, l.counts,  l.heap,    l.bstart,  l.bkt,
+                      l.fw_cnt, l.fw_off, l.fw_tile, l.fw_ent, l.fw_seg, l.fw_bkt, l.fw_hand, l.fw_part};
         for (void* p : lp)
             if (p) (void)hipFree(p);
         for (int gl = 0; gl <= FB_GL_MAX; gl++)
@@ -1913,7 +2077,7 @@ int msm_fb_line(zgr_ctx* c, Lane& ln, int coset, int L, uint32_t* out) {
     const int log_bs = c->log_n - c->h;
     FbLevels lv{};
     FbOut fo{};
-    lv.ent[0] = fo.ent[0] = ln.entries;
+    lv.ent[0] = fo.ent[0] = (uint2*)ln.entries;
     lv.cnt[0] = ln.counts;
     lv.bst[0] = ln.bstart;
     LAUNCH(c, ln.st, "msm_sort", k_fb_count, (uint32_t)c->nb, (uint32_t)c->bs, 0, ln.y, c->log_n, log_bs, 0, ln.counts,
@@ -1923,10 +2087,10 @@ int msm_fb_line(zgr_ctx* c, Lane& ln, int coset, int L, uint32_t* out) {
     const size_t cap = (size_t)FB_W << log_bs;
     const uint32_t* table = c->fbtab[coset] + ((size_t)L << c->h) * cap * 16;  // line L's chunks
     uint32_t* heap = ln.heap + (size_t)L * 2 * c->nb * X29_WORDS;               // line L's tree
-    LAUNCH(c, ln.st, "msm_accum", k_fb_sums, (uint32_t)c->nb, FB_LANES, 0, table, lv, (const uint32_t*)nullptr,
-           ln.bkt, c->log_n, c->h, 1);
-    LAUNCH(c, ln.st, "msm_reduce", k_fb_reduce, (uint32_t)c->nb, FB_LANES, 0, lv, (const uint32_t*)nullptr, ln.bkt,
-           heap, c->h, 1);
+    LAUNCH(c, ln.st, "msm_accum", k_fb_sums, (uint32_t)c->nb, FB_LANES, 0, table, lv, FbWide{},
+           (const uint2*)nullptr, ln.bkt, c->log_n, c->h, 1);
+    LAUNCH(c, ln.st, "msm_reduce", k_fb_reduce, (uint32_t)c->nb, FB_LANES, 0, lv, FbWide{}, (const uint2*)nullptr,
+           ln.bkt, heap, c->h, 1);
     for (int d = c->h - 1; d >= 0; d--)
         LAUNCH(c, ln.st, "msm_tree", k_msm_tree_level, blocks((size_t)1 << d, 64), 64, 0, heap, c->h, d,
                (uint32_t)1 << d, 1, (uint64_t)0);

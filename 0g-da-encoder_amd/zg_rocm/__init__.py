@@ -557,7 +557,7 @@ def build_params(ctx, out_dir, tau=None, verify_depth=None):
     return written
 
 
-def make_context(geo=PRODUCTION, fixed_base=True, group_plan=None, device=0):
+def make_context(geo=PRODUCTION, fixed_base=True, group_plan=None, device=0, wide=True, wide_segs=None):
     """Context with its MSM path chosen explicitly.  zgr_ctx_create reads
     ZGR_FIXED_BASE: the default keeps shifted copies of the resident bases (one bucket
     set per chunk, FB_W = 22 copies: 22 x the params' HBM); "0" selects the windowed
@@ -565,9 +565,13 @@ def make_context(geo=PRODUCTION, fixed_base=True, group_plan=None, device=0):
     comma-separated list ("2,2,1,0,...") sets each line's chunk-group level (log2 of the
     chunks sharing one bucket set), capped at what the line's outputs allow (default:
     the largest such groups, up to 4 chunks, with the high-basis line one level lower
-    when that fits the longest waves into the device's wave slots).  All give identical
-    results."""
-    want = {"ZGR_FIXED_BASE": "1" if fixed_base else "0", "ZGR_GROUP_PLAN": group_plan}
+    when that fits the longest waves into the device's wave slots).  ZGR_WIDE="0" keeps
+    every line in 12-bit windows (default: the lines whose outputs are sums over groups
+    of at least 16 chunks use 16-bit windows and one bucket set per group), and
+    ZGR_WIDE_SEGS sets the number of bucket-segment waves per wide line.  All give
+    identical results."""
+    want = {"ZGR_FIXED_BASE": "1" if fixed_base else "0", "ZGR_GROUP_PLAN": group_plan,
+            "ZGR_WIDE": None if wide else "0", "ZGR_WIDE_SEGS": None if wide_segs is None else str(wide_segs)}
     old = {k: os.environ.get(k) for k in want}
     try:
         for k, v in want.items():

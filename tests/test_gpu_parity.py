@@ -12,19 +12,23 @@ pytestmark = pytest.mark.gpu
 GEO = Z.TEST_SETTING
 
 
-PLANS = {"fixed_base": None, "fixed_base_per_chunk": "none", "fixed_base_mixed_plan": "1,0,2,1,0,2,1,0,2,1,0,2",
-         "windowed": None}
+PLANS = {"fixed_base": None, "fixed_base_narrow": None, "fixed_base_wide_segments": None,
+         "fixed_base_per_chunk": "none", "fixed_base_mixed_plan": "1,0,2,1,0,2,1,0,2,1,0,2", "windowed": None}
 
 
 @pytest.fixture(scope="module", params=list(PLANS))
 def ctx(request):
     """Every MSM path over the resident bases: the shifted-copy tables (default: the
-    AMT lines share bucket sets over chunk groups), the same with one bucket set per
-    chunk on every line (ZGR_GROUP_PLAN=none) and with an explicit per-line plan that
+    AMT lines share bucket sets over chunk groups; quotient lines 0 and 1 and the high
+    line, whose groups hold at least 16 chunks, in 16-bit windows), the same with 12-bit
+    windows on every line (ZGR_WIDE=0), with the wide lines split into 7 bucket-segment
+    waves each (ZGR_WIDE_SEGS=7; the default at this size is one), with one bucket set
+    per chunk on every line (ZGR_GROUP_PLAN=none) and with an explicit per-line plan that
     mixes the group levels (each capped at what the line's outputs allow), and the
-    windowed kernels (ZGR_FIXED_BASE=0); both variables are read at context creation."""
+    windowed kernels (ZGR_FIXED_BASE=0); the variables are read at context creation."""
     fixed = request.param != "windowed"
-    c = Z.make_context(GEO, fixed_base=fixed, group_plan=PLANS[request.param])
+    c = Z.make_context(GEO, fixed_base=fixed, group_plan=PLANS[request.param], wide=request.param != "fixed_base_narrow",
+                       wide_segs=7 if request.param == "fixed_base_wide_segments" else None)
     c.set_profiling(True)
     c.params_generate(seeded_tau())
     # the table build runs only on the fixed-base path
@@ -44,7 +48,19 @@ def test_msm_path_in_use(ctx):
     merges = ctx.kernel_time_ms("msm_group_scatter")[1]
     ctx.set_profiling(False)
     ctx.kernel_time_reset()
-    assert (merges > 0) == (ctx.mode in ("fixed_base", "fixed_base_mixed_plan"))
+    assert (merges > 0) == (ctx.mode in ("fixed_base", "fixed_base_narrow", "fixed_base_wide_segments",
+                                         "fixed_base_mixed_plan"))
+
+
+def test_wide_lines_in_use(ctx):
+    # the 16-bit-window sort runs exactly when the default plan has wide lines
+    ctx.set_profiling(True)
+    ctx.kernel_time_reset()
+    ctx.encode(Z.seeded_bytes(DATA_SEED, 1000))
+    wide = ctx.kernel_time_ms("msm_wide_sort")[1]
+    ctx.set_profiling(False)
+    ctx.kernel_time_reset()
+    assert (wide > 0) == (ctx.mode in ("fixed_base", "fixed_base_wide_segments"))
 
 
 def test_group_plan_rejected():
@@ -254,7 +270,7 @@ def test_autotune_keeps_outputs(params):
         classes = rep["msm_plan"]
         assert len(classes) == 4
         for v in classes.values():
-            assert v["chosen"] in ("default", "max", "half", "chunk") and v["chosen"] in v
+            assert v["chosen"] in ("default", "narrow", "max", "half", "chunk") and v["chosen"] in v
         for length in (0, 100, GEO.max_raw_data_size // 9, GEO.max_raw_data_size // 3, GEO.max_raw_data_size):
             raw = Z.seeded_bytes(DATA_SEED + 17, length)
             got = c.encode(raw)
