@@ -521,7 +521,7 @@ int msm_lines(zgr_ctx* c, Lane& ln, int coset, const uint32_t* src, int from_y, 
         LAUNCH(c, ln.st, "msm_reduce", k_fb_reduce, rwaves, FB_LANES, 0, lv, wd, rwork, ln.bkt, ln.heap, c->h,
                lines);
         if (wd.nw)
-            LAUNCH(c, ln.st, "msm_reduce", k_fw_combine, wd.gbase[wd.nw], FB_LANES, 0, wd, ln.heap, c->h);
+            LAUNCH(c, ln.st, "msm_combine", k_fw_combine, wd.gbase[wd.nw], FB_LANES, 0, wd, ln.heap, c->h);
         return ZGR_OK;
     }
     LAUNCH(c, ln.st, "msm_digits", k_msm_digits, blocks(c->n, 256), 256, 0, src, c->log_n, log_bs, ln.digits,
@@ -552,14 +552,17 @@ int amt_outputs(zgr_ctx* c, Lane& ln, uint32_t* out, int lines, const FbPlan* gr
 // Lines that go wide (16-bit windows): those whose AMT outputs are sums over groups of
 // at least 2^FW_MIN_GL chunks -- quotient lines 0..h-1-FW_MIN_GL and the high-basis line
 // -- when the chunks split into whole blocks of the wide sort.  ZGR_WIDE=0 keeps every
-// line narrow.
+// line narrow; ZGR_WIDE_MIN_GL raises the threshold (log2 chunks per group, at least
+// FW_MIN_GL).
 std::vector<int> fb_wide_lines(const zgr_ctx* c) {
     std::vector<int> w;
     const char* env = getenv("ZGR_WIDE");
     if ((env && env[0] == '0') || c->h < FW_MIN_GL || c->nlines > FW_MAX_LINES) return w;
+    const char* genv = getenv("ZGR_WIDE_MIN_GL");
+    const int min_gl = std::max(FW_MIN_GL, genv ? atoi(genv) : FW_MIN_GL);
     for (int L = 0; L < c->nlines; L++) {
         const int g = L < c->h ? c->h - 1 - L : (L == c->h ? 0 : c->h);
-        if (g >= FW_MIN_GL) w.push_back(L);
+        if (g >= min_gl) w.push_back(L);
     }
     return w;
 }
@@ -1271,9 +1274,11 @@ int zgr_ctx_create(int device, int log_col, int log_row, int coset_n, zgr_ctx** 
     c->fb = (c->log_n - c->h) <= FB_MAX_LOG_BS && c->nlines <= 16 && !(fbenv && fbenv[0] == '0');
     if (c->fb) {
         c->wide_lines = fb_wide_lines(c);
-        // about 1024 dense steps per lane in each wide sums wave (ZGR_WIDE_SEGS overrides)
+        // about 512 dense steps per lane in each wide sums wave (ZGR_WIDE_SEGS overrides):
+        // at 2^20, 512 segments per line measured 1 % faster than 256 or 768 and 12 %
+        // faster than 128 (the long waves' tail)
         const char* segenv = getenv("ZGR_WIDE_SEGS");
-        const size_t nseg = segenv ? (size_t)strtoul(segenv, nullptr, 10) : ((size_t)FW_W * c->n) / (FB_LANES * 1024);
+        const size_t nseg = segenv ? (size_t)strtoul(segenv, nullptr, 10) : ((size_t)FW_W * c->n) / (FB_LANES * 512);
         c->fw_nseg = (uint32_t)std::max<size_t>(1, std::min<size_t>(nseg, 1u << 20));
     }
     auto init = [&]() -> int {

@@ -35,14 +35,14 @@ MAC32_PER_PAIR_WINDOW = 10 * 136
 PEAK_MAC32 = 256 * 4 * 16 * 2.4e9
 # dense int8 MFMA peak of MI355X: 5.0 POP/s = 2.5 x 10^15 multiply-accumulates per second
 PEAK_INT8_MAC = 2.5e15
-# HBM bytes per launch pair (k_fb_sums + k_fb_reduce) from two rocprofv3 counter passes
-# over `bench.py --roofline-only`: FETCH_SIZE (22,002,553 + 1,230,987) KiB x 2.0 (a 1 GiB
-# device copy reads as 524,322 KiB) + WRITE_SIZE (1,739,867 + 769) KiB x 1.0 (the copy
-# writes as 1,048,576 KiB).
-MSM_TRAFFIC_BYTES = ((22002553 + 1230987) * 2.0 + 1739867 + 769) * 1024
-# VALU lane-instructions per launch pair: rocprofv3 SQ_INSTS_VALU (wave instructions)
-# of k_fb_sums + k_fb_reduce over the same command, x 64 lanes.
-VALU_LANE_OPS = (10150435430 + 1430224644) * 64.0
+# HBM bytes of one coset's bucket stage (the sorts, k_fb_sums, k_fb_reduce, k_fw_combine)
+# from two rocprofv3 counter passes over `bench.py --roofline-only`: FETCH_SIZE
+# 19,913,726 KiB x 2.0 (a 1 GiB device copy reads as 524,316 KiB) + WRITE_SIZE
+# 3,002,260 KiB x 1.0 (the copy writes as 1,048,576 KiB).
+MSM_TRAFFIC_BYTES = (19913726 * 2.0 + 3002260) * 1024
+# VALU lane-instructions of the same stage: rocprofv3 SQ_INSTS_VALU (wave instructions)
+# over the same command, x 64 lanes.
+VALU_LANE_OPS = 10129364584 * 64.0
 
 
 def log(*a):
@@ -149,16 +149,11 @@ def main():
     ctx.params_generate(Z.tau_from_seed(TAU_SEED))
     t_params = time.time() - t0
     log("params (3 cosets x 12 x 2^20 bases) generated on device in %.2f s" % t_params)
-    # the in-project tuner: chunk-group plan per scalar-density class, NTT split
-    t0 = time.time()
-    tune = ctx.autotune()
-    tune["seconds"] = round(time.time() - t0, 2)
-    log("autotune: %s" % tune)
-
     if args.roofline_only:
         # workload for the counter passes: three isolated launches of the bucket kernels
-        # (one coset of amt_prove each) and one 1 GiB device-to-device copy whose
-        # FETCH_SIZE / WRITE_SIZE calibrate the counters
+        # (one coset of amt_prove each, the default plan: the tuner does not run, since
+        # its timings under a profiler could pick another plan) and one 1 GiB
+        # device-to-device copy whose FETCH_SIZE / WRITE_SIZE calibrate the counters
         scal = roofline_scalars(geo)
         for _ in range(3):
             ctx.amt_prove(1, scal)
@@ -182,6 +177,11 @@ def main():
         print(json.dumps({"roofline_only": True, "msm_launches": 3, "ntt_calls": {"1024x2^10": 3, "2^20": 3},
                           "copy_bytes": 1 << 30}))
         return
+    # the in-project tuner: chunk-group plan per scalar-density class, NTT split
+    t0 = time.time()
+    tune = ctx.autotune()
+    tune["seconds"] = round(time.time() - t0, 2)
+    log("autotune: %s" % tune)
 
     def resident(raws):
         dev = [Z.DeviceBuffer(r) for r in raws]
@@ -217,8 +217,9 @@ def main():
     # out of its captured graphs), so they are not in the timed steps above
     timed_steps(lambda: ctx.encode_device(ptrs[0], lens[0], out=out1), 3, 0, profile=True)
     kernels = {}
-    for name in ("pack", "ntt", "coset_scale", "msm_sort", "msm_scatter", "msm_group_scatter", "msm_accum",
-                 "msm_reduce", "msm_tree", "msm_horner", "merkle_leaves", "merkle_level", "file_root"):
+    for name in ("pack", "ntt", "coset_scale", "msm_sort", "msm_scatter", "msm_group_scatter", "msm_wide_sort",
+                 "msm_accum", "msm_reduce", "msm_combine", "msm_tree", "msm_horner", "merkle_leaves", "merkle_level",
+                 "file_root"):
         ms, cnt = ctx.kernel_time_ms(name)
         if cnt:
             kernels[name] = {"avg_ms": round(ms, 4), "launches": cnt}
@@ -521,9 +522,13 @@ def main():
     for _ in range(3):
         ctx.amt_prove(1, scal)
     ctx.set_profiling(False)
-    sums_ms, acc_cnt = ctx.kernel_time_ms("msm_accum")
-    red_ms, _ = ctx.kernel_time_ms("msm_reduce")
-    acc_ms = sums_ms + red_ms
+    def per_call(name):  # total time of the kernels named `name` per amt_prove call
+        ms, cnt = ctx.kernel_time_ms(name)
+        return ms * cnt / 3
+    sort_ms = per_call("msm_sort") + per_call("msm_group_scatter") + per_call("msm_wide_sort")
+    sums_ms, red_ms, comb_ms = per_call("msm_accum"), per_call("msm_reduce"), per_call("msm_combine")
+    acc_cnt = 3
+    acc_ms = sort_ms + sums_ms + red_ms + comb_ms
     work = 12 * geo.n * 32 * MAC32_PER_PAIR_WINDOW  # per launch pair (one coset)
     achieved = work / (acc_ms * 1e-3) / 1e12 if acc_ms else None
     peak = PEAK_MAC32 / 1e12
@@ -559,12 +564,15 @@ def main():
             "unit": "TOP/s (32x32->64 integer MAC)",
             "frac": round(achieved / peak, 4) if achieved else None,
             "traffic": round(MSM_TRAFFIC_BYTES),
-            "traffic_unit": "bytes per launch pair (rocprofv3 FETCH_SIZE + WRITE_SIZE, calibrated)",
-            "kernel": "k_fb_sums + k_fb_reduce (bucket sums, bucket reduction: one coset's AMT multi-MSM)",
+            "traffic_unit": "bytes per bucket stage (rocprofv3 FETCH_SIZE + WRITE_SIZE, calibrated)",
+            "kernel": "bucket stage of one coset's AMT multi-MSM: the digit sorts (k_fb_count, k_fb_scatter, "
+                      "k_fw_*), bucket sums (k_fb_sums), bucket reductions (k_fb_reduce) and wide-group "
+                      "combine (k_fw_combine)",
             "avg_ms": round(acc_ms, 3),
-            "kernel_ms": {"k_fb_sums": round(sums_ms, 3), "k_fb_reduce": round(red_ms, 3)},
+            "kernel_ms": {"sorts": round(sort_ms, 3), "k_fb_sums": round(sums_ms, 3), "k_fb_reduce": round(red_ms, 3),
+                          "k_fw_combine": round(comb_ms, 3)},
             "launches": acc_cnt,
-            # VALU lane-instructions the pair actually issues (rocprofv3 SQ_INSTS_VALU x 64)
+            # VALU lane-instructions the stage actually issues (rocprofv3 SQ_INSTS_VALU x 64)
             # over the time, against the same 39.3 T lane-op/s issue peak
             "valu_issue_frac": round(VALU_LANE_OPS / (acc_ms * 1e-3) / PEAK_MAC32, 4) if acc_ms else None,
         },
