@@ -518,8 +518,8 @@ int msm_lines(zgr_ctx* c, Lane& ln, int coset, const uint32_t* src, int from_y, 
         }
         LAUNCH(c, ln.st, "msm_accum", k_fb_sums, waves, FB_LANES, 0, c->fbtab[coset], lv, wd, work, ln.bkt,
                c->log_n, c->h, lines);
-        LAUNCH(c, ln.st, "msm_reduce", k_fb_reduce, rwaves, FB_LANES, 0, lv, wd, rwork, ln.bkt, ln.heap, c->h,
-               lines);
+        LAUNCH(c, ln.st, "msm_reduce", k_fb_reduce, (rwaves + 1) / 2, FB_LANES, 0, lv, wd, rwork, rwaves, ln.bkt,
+               ln.heap, c->h, lines);
         if (wd.nw)
             LAUNCH(c, ln.st, "msm_combine", k_fw_combine, wd.gbase[wd.nw], FB_LANES, 0, wd, ln.heap, c->h);
         return ZGR_OK;
@@ -2094,8 +2094,8 @@ int msm_fb_line(zgr_ctx* c, Lane& ln, int coset, int L, uint32_t* out) {
     uint32_t* heap = ln.heap + (size_t)L * 2 * c->nb * X29_WORDS;               // line L's tree
     LAUNCH(c, ln.st, "msm_accum", k_fb_sums, (uint32_t)c->nb, FB_LANES, 0, table, lv, FbWide{},
            (const uint2*)nullptr, ln.bkt, c->log_n, c->h, 1);
-    LAUNCH(c, ln.st, "msm_reduce", k_fb_reduce, (uint32_t)c->nb, FB_LANES, 0, lv, FbWide{}, (const uint2*)nullptr,
-           ln.bkt, heap, c->h, 1);
+    LAUNCH(c, ln.st, "msm_reduce", k_fb_reduce, ((uint32_t)c->nb + 1) / 2, FB_LANES, 0, lv, FbWide{},
+           (const uint2*)nullptr, (uint32_t)c->nb, ln.bkt, heap, c->h, 1);
     for (int d = c->h - 1; d >= 0; d--)
         LAUNCH(c, ln.st, "msm_tree", k_msm_tree_level, blocks((size_t)1 << d, 64), 64, 0, heap, c->h, d,
                (uint32_t)1 << d, 1, (uint64_t)0);
