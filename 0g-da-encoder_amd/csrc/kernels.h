@@ -866,9 +866,7 @@ __global__ void k_fw_segments(FbWide wd, uint32_t* seg) {
 // unreduced in [0, 8p) (x29_madd_unchecked).  The reduction (k_fb_reduce) is a kernel
 // of its own so that the waves on a CU run one loop body at a time: the two bodies
 // together overflow the instruction cache.
-constexpr int FB_RED_LANES = 32;                         // lanes per reduction item (half a wave)
-constexpr int FB_PER_LANE = FB_BUCKETS / FB_RED_LANES;  // buckets per lane in the reduction (64)
-constexpr uint32_t FB_NO_SHARE = 0xffffffffu;           // lane with no entries
+constexpr uint32_t FB_NO_SHARE = 0xffffffffu;  // lane with no entries
 // XYZZ slots per narrow region in `bkt`: the buckets, then each lane's first- and
 // last-bucket partials
 constexpr int FB_REGION = FB_BUCKETS + 2 * FB_LANES;
@@ -987,28 +985,34 @@ __global__ __launch_bounds__(FB_LANES, 2) void k_fb_sums(const uint32_t* table, 
     }
 }
 
-// Bucket reduction, one item per half wave: 2048 consecutive buckets m of one bucket set
-// (a narrow group's whole set; segment q of a wide group's), lane s of the half folding
-// buckets (64 s, 64 s + 64] of them from the top with the running sum (R += B_m; T += R).
-// Its share of sum_m m B_m is T + (first bucket - 1 + 64 s) R; the 32 shares are summed
-// through LDS into the group's heap node of line L (narrow) or the segment's partial
-// (wide, summed by k_fw_combine).  Empty buckets count as infinity.  64 buckets per lane
-// (not 32 over a whole wave) halve the per-lane offset multiplication and tree relative
-// to the running sums: 13 % fewer field multiplications per bucket set.  `items` bounds
-// the list (a last odd item leaves the second half idle).
+// Bucket reduction, RL lanes per item (RL = 64: one item per wave; RL = 32: two per
+// wave): 2048 consecutive buckets m of one bucket set (a narrow group's whole set;
+// segment q of a wide group's), lane s of the item folding buckets (P s, P s + P] of
+// them, P = 2048 / RL, from the top with the running sum (R += B_m; T += R).  Its share
+// of sum_m m B_m is T + (first bucket - 1 + P s) R; the RL shares are summed through LDS
+// into the group's heap node of line L (narrow) or the segment's partial (wide, summed by
+// k_fw_combine).  Empty buckets count as infinity, and a wave whose buckets are all
+// empty (zero groups of sparse scalars) does no work.  With RL = 32 the per-lane offset
+// multiplication and tree are spread over twice the buckets: 13 % fewer field
+// multiplications per bucket set, for dense scalars; RL = 64 halves an item's latency,
+// for sparse ones, whose few non-zero groups leave most of the device idle.  `items`
+// bounds the list (a last odd item leaves the second half idle).
+template <int RL>
 __global__ __launch_bounds__(FB_LANES, 2) void k_fb_reduce(FbLevels lv, FbWide wd, const uint2* work,
                                                            uint32_t items, const uint32_t* bkt, uint32_t* heap,
                                                            int h, int lines) {
+    constexpr int FB_PER_LANE = FB_BUCKETS / RL;
+    constexpr int PL_LOG = RL == 32 ? 6 : 5;  // log2 FB_PER_LANE
     __shared__ uint32_t red[FB_LANES * X29_WORDS];
     const uint32_t nb = 1u << h;
-    const int half = (int)(threadIdx.x / FB_RED_LANES), s = (int)(threadIdx.x % FB_RED_LANES);
-    const uint32_t idx = 2 * fb_wave_id(work) + (uint32_t)half;
+    const int half = (int)(threadIdx.x / RL), s = (int)(threadIdx.x % RL);
+    const uint32_t idx = (FB_LANES / RL) * fb_wave_id(work) + (uint32_t)half;
     const bool live = idx < items;
     const uint2 it = live ? fb_item(work, idx, lines) : make_uint2(0u, 0u);
     uint32_t* outp = nullptr;
     const uint32_t* bk = bkt;  // the lane's first bucket
-    uint32_t mul = 0;          // the lane's bucket offset / 64
-    uint64_t filled = 0;       // bit i: bucket i of the lane's 64 has entries
+    uint32_t mul = 0;          // the lane's bucket offset / FB_PER_LANE
+    uint64_t filled = 0;       // bit i: bucket i of the lane's FB_PER_LANE has entries
     const bool wide = (it.x & FB_WIDE) != 0;
     if (live && wide) {
         const uint32_t w = it.x & 0xfu, q = (it.x >> 4) & 0xfu, S = (it.x >> 8) & 0x7fffffu;
@@ -1023,34 +1027,38 @@ __global__ __launch_bounds__(FB_LANES, 2) void k_fb_reduce(FbLevels lv, FbWide w
         const uint32_t L = it.x & 0xffu, gl = (it.x >> 8) & 3u, S = it.x >> 10;
         outp = heap + ((size_t)L * 2 * nb + (nb >> gl) + S) * X29_WORDS;
         if (lv.cnt[gl][S]) {  // a zero group leaves `filled` empty: its sum is infinity
-            const uint32_t* bst = lv.bst[gl] + (size_t)S * FB_BSTART + s * FB_PER_LANE + 1;  // bucket 64 s + 1
+            const uint32_t* bst = lv.bst[gl] + (size_t)S * FB_BSTART + s * FB_PER_LANE + 1;  // bucket P s + 1
             bk = bkt + ((size_t)it.y * FB_REGION + s * FB_PER_LANE) * X29_WORDS;
             mul = (uint32_t)s;
             for (int i = 0; i < FB_PER_LANE; i++) filled |= (uint64_t)(bst[i + 1] > bst[i]) << i;
         }
     }
-    x29 Rr = x29_inf(), T = x29_inf();
-    for (int i = FB_PER_LANE - 1; i >= 0; i--) {
-        x29 B = x29_inf();
-        if ((filled >> i) & 1u) B = x29_load(bk + (size_t)i * X29_WORDS);
-        Rr = x29_add_lazy(Rr, B);
-        T = x29_add_lazy(T, Rr);
-    }
-    // T + 64 mul Rr = (mul Rr) doubled 6 times, plus T (mul < 2^9; the bit count is
-    // uniform over the wave, the multiplier per lane)
-    const bool any_wide = __any(wide);
-    x29 off = x29_mul_small_uniform(Rr, mul, any_wide ? 9 : 5);
-    for (int k = 0; k < 6; k++) off = x29_dbl(off);
-    x29 W = x29_add_lazy(T, off);
-
-    x29_store(red + threadIdx.x * X29_WORDS, W);
-    __syncthreads();
-    for (int o = 1; o < FB_RED_LANES; o <<= 1) {
-        if ((s & (2 * o - 1)) == 0) {
-            W = x29_add_lazy(W, x29_load(red + (threadIdx.x + o) * X29_WORDS));
-            x29_store(red + threadIdx.x * X29_WORDS, W);
+    // a wave whose buckets are all empty (zero groups of sparse scalars) skips the work:
+    // its sums are infinity
+    x29 W = x29_inf();
+    if (__any(filled != 0)) {  // uniform over the wave (one wave per block)
+        x29 Rr = x29_inf(), T = x29_inf();
+        for (int i = FB_PER_LANE - 1; i >= 0; i--) {
+            x29 B = x29_inf();
+            if ((filled >> i) & 1u) B = x29_load(bk + (size_t)i * X29_WORDS);
+            Rr = x29_add_lazy(Rr, B);
+            T = x29_add_lazy(T, Rr);
         }
+        // T + P mul Rr = (mul Rr) doubled log2 P times, plus T (mul < 16 RL; the bit
+        // count is uniform over the wave, the multiplier per lane)
+        x29 off = x29_mul_small_uniform(Rr, mul, (RL == 32 ? 5 : 6) + (__any(wide) ? 4 : 0));
+        for (int k = 0; k < PL_LOG; k++) off = x29_dbl(off);
+        W = x29_add_lazy(T, off);
+
+        x29_store(red + threadIdx.x * X29_WORDS, W);
         __syncthreads();
+        for (int o = 1; o < RL; o <<= 1) {
+            if ((s & (2 * o - 1)) == 0) {
+                W = x29_add_lazy(W, x29_load(red + (threadIdx.x + o) * X29_WORDS));
+                x29_store(red + threadIdx.x * X29_WORDS, W);
+            }
+            __syncthreads();
+        }
     }
     if (s == 0 && live) x29_store(outp, x29_norm(W));
 }

@@ -204,6 +204,8 @@ struct FbPlan {
     uint32_t nrwork[2] = {0, 0};
     uint64_t plan[2] = {0, 0};  // 4 bits per line: log2 chunks per bucket set
     int gl_used = 0;            // deepest narrow chunk-group level any line uses
+    uint32_t nseg = 0;          // bucket-segment waves per wide line
+    int red_lanes = 64;         // lanes per k_fb_reduce item (32: two items per wave)
     FbWide wide[2] = {};        // the wide lines (host fields only: counts, bases, lines)
 };
 
@@ -260,7 +262,8 @@ struct zgr_ctx {
     // fwtab holds line wide_lines[w]
     std::vector<int> wide_lines;
     uint32_t* fwtab[3] = {nullptr, nullptr, nullptr};
-    uint32_t fw_nseg = 1;  // sums waves per wide line
+    uint32_t fw_nseg = 1;  // sums waves per wide line in the default plan
+    uint32_t fw_nseg_max = 1;  // the most any plan uses (sizes the segment buffers)
     // chunk-group plans of the bucket kernels (plans[0] the default), the plan each
     // density class of a coset's scalars uses (set by zgr_autotune), the tuned split of
     // the log_n-point NTT (0: ceil(log_n / 2)) and the tuner's report
@@ -518,8 +521,12 @@ int msm_lines(zgr_ctx* c, Lane& ln, int coset, const uint32_t* src, int from_y, 
         }
         LAUNCH(c, ln.st, "msm_accum", k_fb_sums, waves, FB_LANES, 0, c->fbtab[coset], lv, wd, work, ln.bkt,
                c->log_n, c->h, lines);
-        LAUNCH(c, ln.st, "msm_reduce", k_fb_reduce, (rwaves + 1) / 2, FB_LANES, 0, lv, wd, rwork, rwaves, ln.bkt,
-               ln.heap, c->h, lines);
+        if (grouped && grouped->red_lanes == 32)
+            LAUNCH(c, ln.st, "msm_reduce", k_fb_reduce<32>, (rwaves + 1) / 2, FB_LANES, 0, lv, wd, rwork, rwaves,
+                   ln.bkt, ln.heap, c->h, lines);
+        else
+            LAUNCH(c, ln.st, "msm_reduce", k_fb_reduce<64>, rwaves, FB_LANES, 0, lv, wd, rwork, rwaves, ln.bkt,
+                   ln.heap, c->h, lines);
         if (wd.nw)
             LAUNCH(c, ln.st, "msm_combine", k_fw_combine, wd.gbase[wd.nw], FB_LANES, 0, wd, ln.heap, c->h);
         return ZGR_OK;
@@ -588,17 +595,23 @@ int fw_group_log(const zgr_ctx* c, int L) { return L < c->h ? c->h - 1 - L : c->
 // capped at fb_group_log); either one turns the tuner off.
 constexpr int FB_SUMS_WAVES_PER_SIMD = 2;
 
-int build_plan(zgr_ctx* c, const std::string& kind, const std::vector<int>& given, FbPlan* out) {
+int build_plan(zgr_ctx* c, const std::string& kind, const std::vector<int>& given, uint32_t nseg, FbPlan* out) {
     int cus = 0;
     HIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device));
     const size_t slots = (size_t)cus * 4 * FB_SUMS_WAVES_PER_SIMD;
     const int log_bs = c->log_n - c->h;
     out->name = kind;
+    out->nseg = nseg;
+    const bool wide_plan = kind == "default" || kind.compare(0, 5, "wide-") == 0;
+    // the dense-scalar plans reduce with two items per wave (fewer multiplications); the
+    // sparse-oriented ones ("max", "half", "chunk", "given") one item per wave (shorter
+    // waves: their few non-zero groups leave the device idle)
+    out->red_lanes = (wide_plan || kind == "narrow") ? 32 : 64;
     for (int v = 0; v < 2; v++) {
         const int lines = c->nlines - v;
         std::vector<int> gl(lines, 0);
         std::vector<bool> wide(lines, false);
-        if (kind == "default")
+        if (wide_plan)
             for (int L : c->wide_lines)
                 if (L < lines) wide[L] = true;
         if (kind != "chunk")
@@ -624,7 +637,7 @@ int build_plan(zgr_ctx* c, const std::string& kind, const std::vector<int>& give
             const int g = fw_group_log(c, L);
             wd.line[w] = (uint32_t)L;
             wd.sh[w] = (uint32_t)(g - FW_BLK_LOG);
-            wd.segbase[w + 1] = wd.segbase[w] + c->fw_nseg;
+            wd.segbase[w + 1] = wd.segbase[w] + nseg;
             wd.gbase[w + 1] = wd.gbase[w] + (uint32_t)(c->nb >> g);
             gl[L] = g;
         }
@@ -648,9 +661,9 @@ int build_plan(zgr_ctx* c, const std::string& kind, const std::vector<int>& give
                         sums.push_back({((uint64_t)FB_W << (g + log_bs)) / FB_LANES, x, region});
                         red_n.push_back({0, x, region++});
                     }
-        for (uint32_t s = 0; s < c->fw_nseg; s++)
+        for (uint32_t s = 0; s < nseg; s++)
             for (int w = 0; w < wd.nw; w++)
-                sums.push_back({((uint64_t)FW_W * c->n / c->fw_nseg) / FB_LANES, FB_WIDE | (uint32_t)w | s << 8,
+                sums.push_back({((uint64_t)FW_W * c->n / nseg) / FB_LANES, FB_WIDE | (uint32_t)w | s << 8,
                                 wd.segbase[w] + s});
         for (int w = 0; w < wd.nw; w++)  // wide reductions first, then the narrow ones
             for (uint32_t S = 0; S < wd.gbase[w + 1] - wd.gbase[w]; S++)
@@ -686,10 +699,17 @@ int build_work_lists(zgr_ctx* c) {
         kinds = {"given"};
     } else {
         kinds = {"default", "max", "half", "chunk"};
-        if (!c->wide_lines.empty()) kinds.insert(kinds.begin() + 1, "narrow");
+        // with wide lines the tuner also weighs the window width (every line narrow) and
+        // the wide sort's wave granularity (half and twice the default segments per line)
+        if (!c->wide_lines.empty()) kinds.insert(kinds.begin() + 1, {"narrow", "wide-coarse", "wide-fine"});
     }
     c->plans.resize(kinds.size());
-    for (size_t k = 0; k < kinds.size(); k++) CHECK(build_plan(c, kinds[k], given, &c->plans[k]));
+    for (size_t k = 0; k < kinds.size(); k++) {
+        uint32_t nseg = c->fw_nseg;
+        if (kinds[k] == "wide-coarse") nseg = std::max(1u, c->fw_nseg / 2);
+        if (kinds[k] == "wide-fine") nseg = c->fw_nseg_max;
+        CHECK(build_plan(c, kinds[k], given, nseg, &c->plans[k]));
+    }
     return ZGR_OK;
 }
 
@@ -1274,12 +1294,14 @@ int zgr_ctx_create(int device, int log_col, int log_row, int coset_n, zgr_ctx** 
     c->fb = (c->log_n - c->h) <= FB_MAX_LOG_BS && c->nlines <= 16 && !(fbenv && fbenv[0] == '0');
     if (c->fb) {
         c->wide_lines = fb_wide_lines(c);
-        // about 512 dense steps per lane in each wide sums wave (ZGR_WIDE_SEGS overrides):
-        // at 2^20, 512 segments per line measured 1 % faster than 256 or 768 and 12 %
-        // faster than 128 (the long waves' tail)
+        // about 256 dense steps per lane in each wide sums wave (ZGR_WIDE_SEGS overrides):
+        // at 2^20, 1024 segments per line measured 2-3 % faster than 512 in the tuner's
+        // dense timings, 512 1 % faster than 256 or 768, and 128 12 % slower (the long
+        // waves' tail)
         const char* segenv = getenv("ZGR_WIDE_SEGS");
-        const size_t nseg = segenv ? (size_t)strtoul(segenv, nullptr, 10) : ((size_t)FW_W * c->n) / (FB_LANES * 512);
+        const size_t nseg = segenv ? (size_t)strtoul(segenv, nullptr, 10) : ((size_t)FW_W * c->n) / (FB_LANES * 256);
         c->fw_nseg = (uint32_t)std::max<size_t>(1, std::min<size_t>(nseg, 1u << 20));
+        c->fw_nseg_max = 2 * c->fw_nseg;  // the tuner's finer-segment plan
     }
     auto init = [&]() -> int {
         int max_shm = 0;
@@ -1338,9 +1360,9 @@ int zgr_ctx_create(int device, int log_col, int log_row, int coset_n, zgr_ctx** 
                     DALLOC(l.fw_off, nblk * FW_BUCKETS * 4);
                     DALLOC(l.fw_tile, (2 * FW_TILES + 1) * 4);
                     DALLOC(l.fw_ent, c->n * FW_W * sizeof(uint2));
-                    DALLOC(l.fw_seg, nw * (c->fw_nseg + 1) * 4);
+                    DALLOC(l.fw_seg, nw * (c->fw_nseg_max + 1) * 4);
                     DALLOC(l.fw_bkt, groups * FW_BUCKETS * X29_WORDS * 4);
-                    DALLOC(l.fw_hand, nw * c->fw_nseg * 2 * FB_LANES * X29_WORDS * 4);
+                    DALLOC(l.fw_hand, nw * c->fw_nseg_max * 2 * FB_LANES * X29_WORDS * 4);
                     DALLOC(l.fw_part, groups * FW_RSEGS * X29_WORDS * 4);
                 }
             }
@@ -2094,7 +2116,7 @@ int msm_fb_line(zgr_ctx* c, Lane& ln, int coset, int L, uint32_t* out) {
     uint32_t* heap = ln.heap + (size_t)L * 2 * c->nb * X29_WORDS;               // line L's tree
     LAUNCH(c, ln.st, "msm_accum", k_fb_sums, (uint32_t)c->nb, FB_LANES, 0, table, lv, FbWide{},
            (const uint2*)nullptr, ln.bkt, c->log_n, c->h, 1);
-    LAUNCH(c, ln.st, "msm_reduce", k_fb_reduce, ((uint32_t)c->nb + 1) / 2, FB_LANES, 0, lv, FbWide{},
+    LAUNCH(c, ln.st, "msm_reduce", k_fb_reduce<64>, (uint32_t)c->nb, FB_LANES, 0, lv, FbWide{},
            (const uint2*)nullptr, (uint32_t)c->nb, ln.bkt, heap, c->h, 1);
     for (int d = c->h - 1; d >= 0; d--)
         LAUNCH(c, ln.st, "msm_tree", k_msm_tree_level, blocks((size_t)1 << d, 64), 64, 0, heap, c->h, d,

@@ -270,7 +270,7 @@ def test_autotune_keeps_outputs(params):
         classes = rep["msm_plan"]
         assert len(classes) == 4
         for v in classes.values():
-            assert v["chosen"] in ("default", "narrow", "max", "half", "chunk") and v["chosen"] in v
+            assert v["chosen"] in ("default", "narrow", "wide-coarse", "wide-fine", "max", "half", "chunk") and v["chosen"] in v
         for length in (0, 100, GEO.max_raw_data_size // 9, GEO.max_raw_data_size // 3, GEO.max_raw_data_size):
             raw = Z.seeded_bytes(DATA_SEED + 17, length)
             got = c.encode(raw)
