@@ -590,9 +590,11 @@ int fw_group_log(const zgr_ctx* c, int L) { return L < c->h ? c->h - 1 - L : c->
 // at the root) drops one level.  At 2^20 x 12 lines that turns 2304 four-chunk waves into
 // 2048 and measured 4 % faster on dense scalars.  The other candidates of zgr_autotune:
 // "narrow" (the default without wide lines), "max" (no drop), "half" (at most 2-chunk
-// groups) and "chunk" (one bucket set per chunk).  ZGR_GROUP_PLAN=none makes the default
-// plan "chunk" and a comma-separated list sets each line's narrow level explicitly (each
-// capped at fb_group_log); either one turns the tuner off.
+// groups) and "chunk" (one bucket set per chunk), plus "wide-coarse" / "wide-fine" (the
+// default with half / twice its wide segments).  ZGR_GROUP_PLAN=none makes the only plan
+// "chunk", a candidate's name makes it that candidate, and a comma-separated list sets
+// each line's narrow level explicitly (each capped at fb_group_log); each turns the tuner
+// off.
 constexpr int FB_SUMS_WAVES_PER_SIMD = 2;
 
 int build_plan(zgr_ctx* c, const std::string& kind, const std::vector<int>& given, uint32_t nseg, FbPlan* out) {
@@ -688,8 +690,15 @@ int build_work_lists(zgr_ctx* c) {
     const char* mode = getenv("ZGR_GROUP_PLAN");
     std::vector<int> given;
     std::vector<std::string> kinds;
+    static const char* named[] = {"default", "narrow", "wide-coarse", "wide-fine", "max", "half", "chunk"};
+    const bool is_named = mode && std::any_of(std::begin(named), std::end(named),
+                                              [&](const char* k) { return !strcmp(mode, k); });
     if (mode && !strcmp(mode, "none")) {
         kinds = {"chunk"};
+    } else if (is_named) {  // one of the tuner's candidates, alone
+        if (c->wide_lines.empty() && std::string(mode).compare(0, 4, "wide") == 0)
+            return fail(ZGR_ERR_INVALID_ARG, "ZGR_GROUP_PLAN=%s: this geometry has no wide lines", mode);
+        kinds = {mode};
     } else if (mode) {  // one level per line, comma-separated
         for (const char* q = mode; *q;) {
             given.push_back((int)strtol(q, (char**)&q, 10));

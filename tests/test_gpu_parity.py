@@ -13,7 +13,8 @@ GEO = Z.TEST_SETTING
 
 
 PLANS = {"fixed_base": None, "fixed_base_narrow": None, "fixed_base_wide_segments": None,
-         "fixed_base_per_chunk": "none", "fixed_base_mixed_plan": "1,0,2,1,0,2,1,0,2,1,0,2", "windowed": None}
+         "fixed_base_wide_fine": "wide-fine", "fixed_base_narrow_plan": "narrow", "fixed_base_per_chunk": "none",
+         "fixed_base_mixed_plan": "1,0,2,1,0,2,1,0,2,1,0,2", "windowed": None}
 
 
 @pytest.fixture(scope="module", params=list(PLANS))
@@ -22,7 +23,9 @@ def ctx(request):
     AMT lines share bucket sets over chunk groups; quotient lines 0 and 1 and the high
     line, whose groups hold at least 16 chunks, in 16-bit windows), the same with 12-bit
     windows on every line (ZGR_WIDE=0), with the wide lines split into 7 bucket-segment
-    waves each (ZGR_WIDE_SEGS=7; the default at this size is one), with one bucket set
+    waves each (ZGR_WIDE_SEGS=7; the default at this size is two), the tuner's candidates
+    "wide-fine" (twice the default segments) and "narrow" (12-bit windows everywhere, the
+    reduction two groups per wave) alone (ZGR_GROUP_PLAN=<name>), with one bucket set
     per chunk on every line (ZGR_GROUP_PLAN=none) and with an explicit per-line plan that
     mixes the group levels (each capped at what the line's outputs allow), and the
     windowed kernels (ZGR_FIXED_BASE=0); the variables are read at context creation."""
@@ -49,7 +52,7 @@ def test_msm_path_in_use(ctx):
     ctx.set_profiling(False)
     ctx.kernel_time_reset()
     assert (merges > 0) == (ctx.mode in ("fixed_base", "fixed_base_narrow", "fixed_base_wide_segments",
-                                         "fixed_base_mixed_plan"))
+                                         "fixed_base_wide_fine", "fixed_base_narrow_plan", "fixed_base_mixed_plan"))
 
 
 def test_wide_lines_in_use(ctx):
@@ -60,13 +63,16 @@ def test_wide_lines_in_use(ctx):
     wide = ctx.kernel_time_ms("msm_wide_sort")[1]
     ctx.set_profiling(False)
     ctx.kernel_time_reset()
-    assert (wide > 0) == (ctx.mode in ("fixed_base", "fixed_base_wide_segments"))
+    assert (wide > 0) == (ctx.mode in ("fixed_base", "fixed_base_wide_segments", "fixed_base_wide_fine"))
 
 
 def test_group_plan_rejected():
-    """A malformed ZGR_GROUP_PLAN fails context creation with the variable in the message."""
+    """A malformed ZGR_GROUP_PLAN fails context creation with the variable in the message,
+    and so does a wide candidate where no line is wide."""
     with pytest.raises(Z.ZgrError, match="ZGR_GROUP_PLAN"):
         Z.make_context(GEO, group_plan="2,x,1")
+    with pytest.raises(Z.ZgrError, match="ZGR_GROUP_PLAN=wide-fine"):
+        Z.make_context(GEO, group_plan="wide-fine", wide=False)
 
 
 @pytest.fixture(scope="module")
