@@ -8,6 +8,7 @@
 #include "common.h"
 #include "g2.h"
 #include "params_kernels.h"
+#include "ppot.h"
 
 namespace zgr {
 

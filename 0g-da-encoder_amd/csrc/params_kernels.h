@@ -22,4 +22,13 @@ __global__ void k_g1_ark(const uint32_t* pts, uint32_t n, int mont, int compress
 __global__ void k_g2_ark(const uint32_t* pts, uint32_t n, int mont, int compressed, uint32_t* out);
 __global__ void k_fq_convert(const uint32_t* in, uint32_t* out, uint64_t count, int to_mont);
 
+// ceremony import (ppot.h): decode and check count encoded points, writing canonical
+// affine points and one status per point (0 = accepted)
+constexpr int PPOT_G1_THREADS = 256;
+constexpr int PPOT_G2_THREADS = 64;
+__global__ void k_ppot_g1u(const uint32_t* src, size_t count, uint32_t* out, uint32_t* status);
+__global__ void k_ppot_g1c(const uint32_t* src, size_t count, uint32_t* out, uint32_t* status);
+__global__ void k_ppot_g2u(const uint32_t* src, size_t count, uint32_t* out, uint32_t* status);
+__global__ void k_ppot_g2c(const uint32_t* src, size_t count, uint32_t* out, uint32_t* status);
+
 }  // namespace zgr

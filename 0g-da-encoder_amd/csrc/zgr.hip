@@ -1681,6 +1681,59 @@ int zgr_power_tau_load(zgr_ctx* c, const uint8_t* g1pp, const uint8_t* g2pp, con
     return ZGR_OK;
 }
 
+int zgr_ppot_decode(zgr_ctx* c, int kind, const uint8_t* src, size_t count, uint8_t* out, size_t* first_bad) {
+    if (!c || (count && (!src || !out))) return fail(ZGR_ERR_INVALID_ARG, "null argument");
+    if (kind < ZGR_PPOT_G1_UNCOMPRESSED || kind > ZGR_PPOT_G2_COMPRESSED)
+        return fail(ZGR_ERR_INVALID_ARG, "point kind %d is not one of ZGR_PPOT_*", kind);
+    if (first_bad) *first_bad = count;
+    if (count == 0) return ZGR_OK;
+    const bool g2 = kind >= ZGR_PPOT_G2_UNCOMPRESSED, comp = (kind & 1) != 0;
+    const size_t in_b = (g2 ? 64 : 32) * (comp ? 1 : 2), out_b = g2 ? 128 : 64;
+    static const char* const reason[] = {"accepted",
+                                         "unexpected compression mode (0x80 on an uncompressed point)",
+                                         "unexpected information (infinity flag with other bits set)",
+                                         "point at infinity",
+                                         "coordinate not below p",
+                                         "not on the curve",
+                                         "not in the order-r subgroup"};
+    std::lock_guard<std::mutex> lk(c->mu);
+    HIPCHK(hipSetDevice(c->device));
+    CHECK(sync_all(c));
+    hipStream_t st = c->lanes[0].st;
+    const size_t chunk = std::min(count, (size_t)1 << 20);
+    uint32_t *d_src = nullptr, *d_out = nullptr, *d_st = nullptr;
+    std::vector<uint32_t> status(chunk);
+    auto body = [&]() -> int {
+        DALLOC(d_src, chunk * in_b);
+        DALLOC(d_out, chunk * out_b);
+        DALLOC(d_st, chunk * 4);
+        for (size_t off = 0; off < count; off += chunk) {
+            const size_t n = std::min(chunk, count - off);
+            HIPCHK(hipMemcpyAsync(d_src, src + off * in_b, n * in_b, hipMemcpyHostToDevice, st));
+            const uint32_t t = g2 ? PPOT_G2_THREADS : PPOT_G1_THREADS;
+            if (!g2 && !comp) LAUNCH(c, st, "ppot", k_ppot_g1u, blocks(n, t), t, 0, d_src, n, d_out, d_st);
+            if (!g2 && comp) LAUNCH(c, st, "ppot", k_ppot_g1c, blocks(n, t), t, 0, d_src, n, d_out, d_st);
+            if (g2 && !comp) LAUNCH(c, st, "ppot", k_ppot_g2u, blocks(n, t), t, 0, d_src, n, d_out, d_st);
+            if (g2 && comp) LAUNCH(c, st, "ppot", k_ppot_g2c, blocks(n, t), t, 0, d_src, n, d_out, d_st);
+            HIPCHK(hipMemcpyAsync(out + off * out_b, d_out, n * out_b, hipMemcpyDeviceToHost, st));
+            HIPCHK(hipMemcpyAsync(status.data(), d_st, n * 4, hipMemcpyDeviceToHost, st));
+            CHECK(sync_all(c));
+            for (size_t j = 0; j < n; j++)
+                if (status[j]) {
+                    if (first_bad) *first_bad = off + j;
+                    return fail(ZGR_ERR_INVALID_ARG, "point %zu: %s", off + j,
+                                status[j] < sizeof(reason) / sizeof(reason[0]) ? reason[status[j]] : "bad status");
+                }
+        }
+        return ZGR_OK;
+    };
+    int rc = body();
+    (void)hipStreamSynchronize(st);
+    for (void* p : {(void*)d_src, (void*)d_out, (void*)d_st})
+        if (p) (void)hipFree(p);
+    return rc;
+}
+
 int zgr_power_tau_download(zgr_ctx* c, uint8_t* g1pp, uint8_t* g2pp, uint8_t* high_g1pp, uint8_t* high_g2) {
     if (!c) return fail(ZGR_ERR_INVALID_ARG, "null argument");
     std::lock_guard<std::mutex> lk(c->mu);

@@ -33,6 +33,11 @@ FILE_AMT = 1
 FILE_AMT_VERIFY = 2
 FILE_PTAU = 3
 FILE_PTAU_MONT = 4
+# point encodings of zgr_ppot_decode (ceremony files)
+PPOT_G1_UNCOMPRESSED = 0
+PPOT_G1_COMPRESSED = 1
+PPOT_G2_UNCOMPRESSED = 2
+PPOT_G2_COMPRESSED = 3
 # type_hash::<Bn254>()[..6] of the reference's file names (crates/amt/src/utils.rs:11-52,
 # README.md:73-82)
 BN254_HASH = "5DWgDV"
@@ -53,6 +58,7 @@ SIGNATURES = {
     "zgr_power_tau_setup": (I, [P, P]),
     "zgr_power_tau_load": (I, [P, P, P, P, P, S]),
     "zgr_power_tau_download": (I, [P, P, P, P, P]),
+    "zgr_ppot_decode": (I, [P, I, P, S, P, ctypes.POINTER(S)]),
     "zgr_params_from_power_tau": (I, [P, I]),
     "zgr_params_g2": (I, [P, I, P, P, P]),
     "zgr_write_params_file": (I, [P, I, I, I, P, S, ctypes.POINTER(S)]),
@@ -329,6 +335,25 @@ class Context:
     def power_tau_load(self, g1pp, g2pp, high_g1pp, high_g2):
         a = [np.ascontiguousarray(x, dtype=np.uint8).reshape(-1) for x in (g1pp, g2pp, high_g1pp, high_g2)]
         _check(lib().zgr_power_tau_load(self.h, _p(a[0]), _p(a[1]), _p(a[2]), _p(a[3]), a[0].size // 64))
+
+    def ppot_decode(self, kind, src, count):
+        """zgr_ppot_decode: count encoded ceremony points (bytes-like, back to back) ->
+        canonical affine points, (count, 64) for G1 or (count, 128) for G2.  A rejected
+        point raises ZgrError with .first_bad set to its index."""
+        a = np.frombuffer(src, dtype=np.uint8) if not isinstance(src, np.ndarray) else src.reshape(-1)
+        a = np.ascontiguousarray(a, dtype=np.uint8)
+        width = 128 if kind >= PPOT_G2_UNCOMPRESSED else 64
+        per = (64 if kind >= PPOT_G2_UNCOMPRESSED else 32) * (1 if kind % 2 else 2)
+        if a.size < count * per:
+            raise ValueError("%d bytes hold fewer than %d encoded points of %d B" % (a.size, count, per))
+        out = np.zeros((count, width), dtype=np.uint8)
+        bad = ctypes.c_size_t(0)
+        rc = lib().zgr_ppot_decode(self.h, kind, _p(a), count, _p(out), ctypes.byref(bad))
+        if rc != OK:
+            err = ZgrError(rc, lib().zgr_last_error().decode(errors="replace"))
+            err.first_bad = bad.value
+            raise err
+        return out
 
     def power_tau_download(self):
         """-> g1pp (n, 64), g2pp (n, 128), high_g1pp (n, 64), high_g2 (128,): canonical affine."""

@@ -114,6 +114,25 @@ int zgr_power_tau_setup(zgr_ctx *ctx, const uint8_t tau[32]);
 /* Powers of tau from elsewhere (a ceremony), kept in the context; n_points = 2^log_n. */
 int zgr_power_tau_load(zgr_ctx *ctx, const uint8_t *g1pp, const uint8_t *g2pp, const uint8_t *high_g1pp,
                        const uint8_t *high_g2, size_t n_points);
+/* Ceremony import: decode and check count points of a perpetual-powers-of-tau file, as
+ * BatchedAccumulator::read_chunk with CheckForCorrectness::Yes does for
+ * crates/ppot2ark/src/main.rs:137-187 (from_ppot_file_inner; the powersoftau crate of
+ * kobigurk/phase2-bn254 rev dd6b966, pairing_ce bn256 encodings: big-endian coordinates,
+ * G2 as c1 || c0, flag bits 0x80 / 0x40 in the first byte).  src holds the count
+ * encodings back to back (64 / 32 B per G1 point uncompressed / compressed, 128 / 64 B
+ * per G2 point); out receives canonical affine points (G1 64 B, G2 128 B, the layouts
+ * above).  A point that a checked read rejects -- wrong compression mode, stray bits
+ * with the infinity flag, the point at infinity, a coordinate >= p, off the curve, or a
+ * G2 point outside the order-r subgroup -- fails the call with ZGR_ERR_INVALID_ARG,
+ * the reason and *first_bad = its index (count when all are accepted).  Decoding,
+ * square roots and the subgroup test run on the device. */
+enum {
+    ZGR_PPOT_G1_UNCOMPRESSED = 0,
+    ZGR_PPOT_G1_COMPRESSED = 1,
+    ZGR_PPOT_G2_UNCOMPRESSED = 2,
+    ZGR_PPOT_G2_COMPRESSED = 3
+};
+int zgr_ppot_decode(zgr_ctx *ctx, int kind, const uint8_t *src, size_t count, uint8_t *out, size_t *first_bad);
 /* Copy the context's powers out (NULL = not wanted); ZGR_ERR_NOT_LOADED without any. */
 int zgr_power_tau_download(zgr_ctx *ctx, uint8_t *g1pp, uint8_t *g2pp, uint8_t *high_g1pp, uint8_t *high_g2);
 /* AMTParams::from_pp (crates/amt/src/prove_params/generate.rs:139-195) for one coset from
