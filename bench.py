@@ -57,6 +57,30 @@ def roofline_scalars(geo):
     return scal
 
 
+_HALF_P = np.array([((Z.P_MOD - 1) // 2 >> (64 * k)) & (2**64 - 1) for k in range(4)], dtype=np.uint64)
+
+
+def ppot_encode(pts, g2):
+    """Synthetic ceremony input: canonical affine points (the library's memory form) in
+    the compressed encodings of a response file -- big-endian x (G2: x.c1 then x.c0),
+    0x80 in the first byte when y is the larger of y and -y (G2: by c1, then c0)."""
+    def larger(le):  # (n, 32) little-endian integers > (p - 1) / 2
+        w = np.ascontiguousarray(le).view("<u8")
+        gt, eq = np.zeros(len(w), dtype=bool), np.ones(len(w), dtype=bool)
+        for k in (3, 2, 1, 0):
+            gt |= eq & (w[:, k] > _HALF_P[k])
+            eq &= w[:, k] == _HALF_P[k]
+        return gt
+    if g2:
+        out = np.concatenate([pts[:, 32:64][:, ::-1], pts[:, 0:32][:, ::-1]], axis=1)
+        flag = np.where(pts[:, 96:128].any(axis=1), larger(pts[:, 96:128]), larger(pts[:, 64:96]))
+    else:
+        out = np.ascontiguousarray(pts[:, 0:32][:, ::-1])
+        flag = larger(pts[:, 32:64])
+    out[:, 0] |= np.where(flag, 0x80, 0).astype(np.uint8)
+    return out.tobytes()
+
+
 def cpu_baseline(ctx, geo, raw, threads):
     """The oracle -- a CPU restatement of the reference `parallel` path (to_coset_blob,
     gen_all_proofs, EncodedBlobMerkle::build) -- on bounded samples of one 1024x1024 blob.
@@ -511,6 +535,26 @@ def main():
                   "params build on an AWS g5.4xlarge (README.md:121)")
     params_build = tb
 
+    # ceremony import (crates/ppot2ark/src/main.rs:47-124): the production read of
+    # 2^20 G1 and 2^20 G2 powers from a response file (compressed points), decoded and
+    # checked on the device -- square roots, and [r] P == O for every G2 point
+    g1pp, g2pp, _, _ = ctx.power_tau_download()
+    ppot = {}
+    for label, kind, enc in (("g1_compressed_2pow20", Z.PPOT_G1_COMPRESSED, ppot_encode(g1pp, False)),
+                             ("g2_compressed_2pow20", Z.PPOT_G2_COMPRESSED, ppot_encode(g2pp, True))):
+        ctx.set_profiling(True)
+        ctx.kernel_time_reset()
+        s0 = time.perf_counter()
+        out = ctx.ppot_decode(kind, enc, geo.n)
+        wall = time.perf_counter() - s0
+        ctx.set_profiling(False)
+        k_ms, k_cnt = ctx.kernel_time_ms("ppot")
+        assert np.array_equal(out, g2pp if kind == Z.PPOT_G2_COMPRESSED else g1pp)
+        ppot[label] = {"wall_ms": round(wall * 1e3, 1), "kernel_ms": round(k_ms * k_cnt, 1)}
+    ppot["note"] = ("zgr_ppot_decode of 2^20 compressed points from host bytes (copies included in wall_ms); "
+                    "ppot2ark reads 4 x 2^20 G1 and 2 x 2^20 G2 points this way for a depth-20 PowerTau")
+    del g1pp, g2pp
+
     # The encode pipeline overlaps kernels of different cosets on two streams, which
     # stretches event-timed durations; the roofline uses isolated launches of the same
     # bucket kernels (amt_prove of one coset: the encode path's multi-MSM over 12 lines
@@ -584,6 +628,7 @@ def main():
         "msm_2pow20": msm,
         "plain_host_input": host_input,
         "params_build": params_build,
+        "ppot_import": ppot,
         "autotune": tune,
     }
     if not args.no_cpu:
