@@ -552,7 +552,7 @@ def main():
         assert np.array_equal(out, g2pp if kind == Z.PPOT_G2_COMPRESSED else g1pp)
         ppot[label] = {"wall_ms": round(wall * 1e3, 1), "kernel_ms": round(k_ms * k_cnt, 1)}
     ppot["note"] = ("zgr_ppot_decode of 2^20 compressed points from host bytes (copies included in wall_ms); "
-                    "ppot2ark reads 4 x 2^20 G1 and 2 x 2^20 G2 points this way for a depth-20 PowerTau")
+                    "ppot2ark reads 6 x 2^20 G1 and 2 x 2^20 G2 points this way for a depth-20 PowerTau")
     del g1pp, g2pp
 
     # The encode pipeline overlaps kernels of different cosets on two streams, which
