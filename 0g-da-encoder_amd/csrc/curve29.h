@@ -128,11 +128,8 @@ __device__ __forceinline__ x29 x29_madd(const x29& p, const a29& q) {
 __device__ __forceinline__ x29 x29_madd_unchecked(const x29& p, const a29& q, bool* rare) {
     fq29 H = q_sub_lazy8(q_mul(q.x, p.ZZ), p.X);
     fq29 R = q_sub_lazy(q_mul(q.y, p.ZZZ), p.Y);
-    const uint32_t h0 = H.l[0];
-    bool hit = false;
-#pragma unroll
-    for (int k = 0; k < 10; k++) hit |= h0 == Q29::KP_L0[k];
-    *rare = hit;
+    // H mod 2^29 == (k p) mod 2^29 for some k in 0..9, in one product: k = H.l[0] / p_0
+    *rare = ((H.l[0] * Q29::P0_INV) & M29) <= 9u;
     x29 r;
     fq29 PP = q_sqr(H);  // H^2 < 100 p^2: below 2p (q_mul's bound)
     r.ZZ = q_mul(p.ZZ, PP);

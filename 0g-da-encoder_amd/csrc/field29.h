@@ -29,9 +29,9 @@ struct Q29 {
                                        0x11238484u, 0x03e94786u, 0x1628e538u, 0x012259d6u};
     static constexpr uint32_t P8[9] = {0x03e7ea38u, 0x082305b6u, 0x03951a78u, 0x16a91687u, 0x0c2ecbc0u,
                                        0x16da0605u, 0x05370a08u, 0x12e131a0u, 0x01832273u};
-    // low limbs of k p, k = 0..9
-    static constexpr uint32_t KP_L0[10] = {0x00000000u, 0x187cfd47u, 0x10f9fa8eu, 0x0976f7d5u, 0x01f3f51cu,
-                                           0x1a70f263u, 0x12edefaau, 0x0b6aecf1u, 0x03e7ea38u, 0x1c64e77fu};
+    // p_0^-1 mod 2^29 (p_0 = P[0] is odd): a normalised value's low limb l equals that of
+    // k p (mod 2^29) iff (l * P0_INV) mod 2^29 == k
+    static constexpr uint32_t P0_INV = 0x1b799c77u;
     static constexpr uint32_t ONE[9] = {0x157ccc21u, 0x141c2758u, 0x185230d3u, 0x014c0419u, 0x0aa36fb9u,
                                         0x1d4240ceu, 0x11d54c07u, 0x052ac7a8u, 0x000dc836u};
     // 2^5 * R' mod p: mont29(v, C_TO) = v * 2^5, i.e. 2^256-Montgomery -> 2^261-Montgomery

@@ -419,6 +419,7 @@ constexpr int FW_RSEG = 2048;                // buckets per reduction wave
 constexpr int FW_RSEGS = FW_BUCKETS / FW_RSEG;
 constexpr int FW_SORT_T = 256;   // threads per block of the wide sort (one 4-chunk block each)
 constexpr int FW_TILE = 128;     // buckets per block of the offset scan
+constexpr int FW_PARTS = 4;      // threads per bucket there (each sums a quarter of the blocks)
 constexpr int FW_TILES = FW_BUCKETS / FW_TILE;
 // Sorted entry (both widths, 8 bytes): x = point index within the wave's table region
 // (narrow: (g * FB_W + j) * bs + k for chunk g of the group; wide: (b * FW_W + j) * bs +
@@ -738,31 +739,44 @@ __global__ __launch_bounds__(FW_SORT_T) void k_fw_count(const uint32_t* y, int l
     for (uint32_t q = t; q < FW_BUCKETS; q += FW_SORT_T) cnt[(size_t)blk * FW_BUCKETS + q] = hist[q];
 }
 
-// bucket m - 1 = q's total over the blocks
-__device__ __forceinline__ uint32_t fw_bucket_total(const uint32_t* cnt, uint32_t nblk, uint32_t q) {
+// part pp of bucket m - 1 = q's total: the blocks [pp nblk / FW_PARTS, (pp + 1) nblk / FW_PARTS)
+__device__ __forceinline__ uint32_t fw_part_total(const uint32_t* cnt, uint32_t nblk, uint32_t q, uint32_t pp) {
+    const uint32_t b0 = pp * nblk / FW_PARTS, b1 = (pp + 1) * nblk / FW_PARTS;
     uint32_t s = 0;
-    for (uint32_t blk = 0; blk < nblk; blk++) s += cnt[(size_t)blk * FW_BUCKETS + q];
+    for (uint32_t blk = b0; blk < b1; blk++) s += cnt[(size_t)blk * FW_BUCKETS + q];
     return s;
 }
 
-// inclusive prefix over a block of FW_TILE threads (2 waves) through LDS
-__device__ __forceinline__ uint32_t fw_block_incl(uint32_t v, uint32_t* wtot) {
-    const uint32_t t = threadIdx.x, lane = t & 63;
+// Inclusive prefix over a tile's FW_TILE buckets of their totals, from FW_PARTS partial
+// totals per bucket (thread t holds part t / FW_TILE of bucket t % FW_TILE).  Every
+// thread of the block calls it and gets its bucket's prefix; *total = the bucket's total.
+__device__ __forceinline__ uint32_t fw_tile_prefix(uint32_t partial, uint32_t (*part)[FW_TILE], uint32_t* wtot,
+                                                   uint32_t* total) {
+    const uint32_t t = threadIdx.x, tq = t % FW_TILE, lane = t & 63;
+    part[t / FW_TILE][tq] = partial;
+    __syncthreads();
+    uint32_t v = 0;
+#pragma unroll
+    for (int pp = 0; pp < FW_PARTS; pp++) v += part[pp][tq];
+    *total = v;
 #pragma unroll
     for (int o = 1; o < 64; o <<= 1) {
         const uint32_t u = __shfl_up(v, o);
         if (lane >= (uint32_t)o) v += u;
     }
-    if (lane == 63) wtot[t >> 6] = v;
+    if (t < FW_TILE && lane == 63) wtot[t >> 6] = v;
     __syncthreads();
-    if (t >= 64) v += wtot[0];
+    if (tq >= 64) v += wtot[0];
     return v;
 }
 
-__global__ __launch_bounds__(FW_TILE) void k_fw_tile_sums(const uint32_t* cnt, uint32_t nblk, uint32_t* tile_tot) {
+__global__ __launch_bounds__(FW_TILE * FW_PARTS) void k_fw_tile_sums(const uint32_t* cnt, uint32_t nblk,
+                                                                     uint32_t* tile_tot) {
+    __shared__ uint32_t part[FW_PARTS][FW_TILE];
     __shared__ uint32_t wtot[2];
-    const uint32_t q = blockIdx.x * FW_TILE + threadIdx.x;
-    const uint32_t v = fw_block_incl(fw_bucket_total(cnt, nblk, q), wtot);
+    const uint32_t tq = threadIdx.x % FW_TILE, pp = threadIdx.x / FW_TILE;
+    uint32_t total;
+    const uint32_t v = fw_tile_prefix(fw_part_total(cnt, nblk, blockIdx.x * FW_TILE + tq, pp), part, wtot, &total);
     if (threadIdx.x == FW_TILE - 1) tile_tot[blockIdx.x] = v;
 }
 
@@ -784,13 +798,20 @@ __global__ __launch_bounds__(FW_TILES) void k_fw_tile_scan(const uint32_t* tile_
     if (t == FW_TILES - 1) tot[0] = v;
 }
 
-__global__ __launch_bounds__(FW_TILE) void k_fw_offsets(const uint32_t* cnt, uint32_t nblk, const uint32_t* tile_base,
-                                                        uint32_t* off) {
+// thread (part pp, bucket q) writes the offsets of q's blocks in part pp: the bucket's
+// exclusive start, plus the counts of the earlier parts, then a running sum
+__global__ __launch_bounds__(FW_TILE * FW_PARTS) void k_fw_offsets(const uint32_t* cnt, uint32_t nblk,
+                                                                   const uint32_t* tile_base, uint32_t* off) {
+    __shared__ uint32_t part[FW_PARTS][FW_TILE];
     __shared__ uint32_t wtot[2];
-    const uint32_t q = blockIdx.x * FW_TILE + threadIdx.x;
-    const uint32_t s = fw_bucket_total(cnt, nblk, q);
-    uint32_t r = tile_base[blockIdx.x] + fw_block_incl(s, wtot) - s;
-    for (uint32_t blk = 0; blk < nblk; blk++) {
+    const uint32_t tq = threadIdx.x % FW_TILE, pp = threadIdx.x / FW_TILE;
+    const uint32_t q = blockIdx.x * FW_TILE + tq;
+    uint32_t total;
+    const uint32_t incl = fw_tile_prefix(fw_part_total(cnt, nblk, q, pp), part, wtot, &total);
+    uint32_t r = tile_base[blockIdx.x] + incl - total;
+    for (uint32_t k = 0; k < pp; k++) r += part[k][tq];
+    const uint32_t b0 = pp * nblk / FW_PARTS, b1 = (pp + 1) * nblk / FW_PARTS;
+    for (uint32_t blk = b0; blk < b1; blk++) {
         const size_t a = (size_t)blk * FW_BUCKETS + q;
         const uint32_t c = cnt[a];
         off[a] = r;

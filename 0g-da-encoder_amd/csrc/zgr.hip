@@ -500,10 +500,11 @@ int msm_lines(zgr_ctx* c, Lane& ln, int coset, const uint32_t* src, int from_y, 
             const size_t shm = FW_BUCKETS * 4;
             LAUNCH(c, ln.st, "msm_wide_sort", k_fw_count, wd.nblk, FW_SORT_T, shm, src, c->log_n, log_bs, from_y,
                    ln.fw_cnt);
-            LAUNCH(c, ln.st, "msm_wide_sort", k_fw_tile_sums, FW_TILES, FW_TILE, 0, ln.fw_cnt, wd.nblk, ln.fw_tile);
+            LAUNCH(c, ln.st, "msm_wide_sort", k_fw_tile_sums, FW_TILES, FW_TILE * FW_PARTS, 0, ln.fw_cnt, wd.nblk,
+                   ln.fw_tile);
             LAUNCH(c, ln.st, "msm_wide_sort", k_fw_tile_scan, 1, FW_TILES, 0, ln.fw_tile, ln.fw_tile + FW_TILES,
                    ln.fw_tile + 2 * FW_TILES);
-            LAUNCH(c, ln.st, "msm_wide_sort", k_fw_offsets, FW_TILES, FW_TILE, 0, ln.fw_cnt, wd.nblk,
+            LAUNCH(c, ln.st, "msm_wide_sort", k_fw_offsets, FW_TILES, FW_TILE * FW_PARTS, 0, ln.fw_cnt, wd.nblk,
                    ln.fw_tile + FW_TILES, ln.fw_off);
             LAUNCH(c, ln.st, "msm_wide_sort", k_fw_scatter, wd.nblk, FW_SORT_T, shm, src, c->log_n, log_bs, from_y,
                    ln.fw_off, ln.fw_ent);
