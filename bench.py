@@ -37,12 +37,12 @@ PEAK_MAC32 = 256 * 4 * 16 * 2.4e9
 PEAK_INT8_MAC = 2.5e15
 # HBM bytes of one coset's bucket stage (the sorts, k_fb_sums, k_fb_reduce, k_fw_combine)
 # from two rocprofv3 counter passes over `bench.py --roofline-only`: FETCH_SIZE
-# 20,057,249 KiB x 2.0 (a 1 GiB device copy reads as 524,316 KiB) + WRITE_SIZE
-# 3,063,956 KiB x 1.0 (the copy writes as 1,048,576 KiB).
-MSM_TRAFFIC_BYTES = (20057249 * 2.0 + 3063956) * 1024
+# 19,562,518 KiB x 2.0 (a 1 GiB device copy reads as 524,316 KiB) + WRITE_SIZE
+# 3,064,418 KiB x 1.0 (the copy writes as 1,048,576 KiB).
+MSM_TRAFFIC_BYTES = (19562518 * 2.0 + 3064418) * 1024
 # VALU lane-instructions of the same stage: rocprofv3 SQ_INSTS_VALU (wave instructions)
 # over the same command, x 64 lanes.
-VALU_LANE_OPS = 9940843602 * 64.0
+VALU_LANE_OPS = 9909242770 * 64.0
 
 
 def log(*a):
