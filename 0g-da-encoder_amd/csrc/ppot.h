@@ -1,5 +1,5 @@
 // Ceremony import: the points of a perpetual-powers-of-tau file decoded and checked on
-// the device.  crates/ppot2ark/src/main.rs:137-187 reads them through the powersoftau
+// the device.  crates/ppot2ark/src/main.rs:45-119 reads them through the powersoftau
 // crate of kobigurk/phase2-bn254 (rev dd6b966, BatchedAccumulator::read_chunk with
 // CheckForCorrectness::Yes; an absent dependency).  Its point encodings are those of
 // pairing_ce's bn256 EncodedPoint:

@@ -25,6 +25,7 @@
 #include <deque>
 #include <algorithm>
 #include <map>
+#include <atomic>
 #include <mutex>
 #include <random>
 #include <string>
@@ -222,6 +223,9 @@ struct zgr_ctx {
     int log_col = 0, log_row = 0, log_n = 0, h = 0, coset_n = 0, nlines = 0;
     size_t n = 0, nb = 0, bs = 0;
     std::mutex mu;
+    // zgr_test_inject_fault: countdown to an injected failure per site (0: none); atomic
+    // so that it can be armed while a server owns the context
+    std::atomic<int> fault[ZGR_FAULT_SITES] = {};
 
     // domain tables
     uint32_t* consts = nullptr;   // K_COUNT Fr
@@ -786,7 +790,16 @@ int check_len(const zgr_ctx* c, size_t len) {
 
 // Issue the whole encode of one blob (raw bytes in device memory) into slot s.
 // Returns once everything is queued; the outputs land in the slot's pinned staging.
+// an armed fault at `site` counts down and fires when it reaches zero
+bool fault_fires(zgr_ctx* c, int site) {
+    int v = c->fault[site].load();
+    while (v > 0 && !c->fault[site].compare_exchange_weak(v, v - 1)) {
+    }
+    return v == 1;
+}
+
 int issue_blob(zgr_ctx* c, Slot& s, const uint8_t* raw_dev, size_t len, const zgr_encoded* out) {
+    if (fault_fires(c, ZGR_FAULT_ISSUE)) return fail(ZGR_ERR_DEVICE, "injected fault: blob issue");
     const uint32_t n = (uint32_t)c->n;
     LAUNCH(c, s.st, "pack", k_pack, blocks(n, 256), 256, 0, raw_dev, len, c->log_col, c->log_row, s.pts);
     if (c->coset_n > 1) CHECK(ntt_dif(c, s.st, s.pts, s.coeff, c->log_n, 1, true));  // coefficients, bit-reversed
@@ -2074,6 +2087,45 @@ int zgr_server_poll(zgr_server* sv, uint64_t ticket, int timeout_ms) {
     sv->done.erase(ticket);
     if (res.first != ZGR_OK) g_err = res.second;  // the worker's message, on the polling thread
     return res.first;
+}
+
+int zgr_server_cancel(zgr_server* sv, uint64_t ticket) {
+    if (!sv) return fail(ZGR_ERR_INVALID_ARG, "null argument");
+    {
+        std::lock_guard<std::mutex> lk(sv->m);
+        if (!sv->pending.count(ticket))
+            return fail(ZGR_ERR_INVALID_ARG, "ticket %llu is not pending", (unsigned long long)ticket);
+        auto take = [&](std::deque<Request*>& q) -> Request* {
+            for (auto it = q.begin(); it != q.end(); ++it)
+                if ((*it)->id == ticket) {
+                    Request* r = *it;
+                    q.erase(it);
+                    return r;
+                }
+            return nullptr;
+        };
+        if (Request* r = take(sv->queue)) {  // not admitted: no arena space, no in-flight count
+            sv->n_completed++;
+            sv->done[r->id] = {ZGR_ERR_CANCELLED, "cancelled"};
+            sv->pending.erase(r->id);
+            delete r;
+        } else if (Request* r2 = take(sv->staged)) {  // admitted, not issued
+            server_complete(sv, r2, ZGR_ERR_CANCELLED, "cancelled");
+        } else {
+            return fail(ZGR_ERR_INVALID_ARG, "ticket %llu is already on the GPU", (unsigned long long)ticket);
+        }
+    }
+    sv->cv_done.notify_all();
+    sv->cv_work.notify_all();
+    return ZGR_OK;
+}
+
+int zgr_test_inject_fault(zgr_ctx* c, int site, int countdown) {
+    if (!c) return fail(ZGR_ERR_INVALID_ARG, "null argument");
+    if (site < 0 || site >= ZGR_FAULT_SITES) return fail(ZGR_ERR_INVALID_ARG, "fault site %d out of range", site);
+    if (countdown < 0) return fail(ZGR_ERR_INVALID_ARG, "negative countdown");
+    c->fault[site].store(countdown);
+    return ZGR_OK;
 }
 
 void zgr_server_stats(zgr_server* sv, uint64_t* queued, uint64_t* admitted, uint64_t* completed) {

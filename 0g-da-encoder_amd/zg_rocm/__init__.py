@@ -26,6 +26,8 @@ ERR_OOM = -3
 ERR_DEVICE = -4
 ERR_NOT_LOADED = -5
 ERR_INCONSISTENT = -6
+ERR_CANCELLED = -7
+FAULT_ISSUE = 0  # zgr_test_inject_fault sites
 
 # parameter file kinds of zgr_write_params_file
 FILE_AMT_MONT = 0
@@ -78,6 +80,8 @@ SIGNATURES = {
     "zgr_server_destroy": (None, [P]),
     "zgr_server_submit": (I, [P, P, S, P, ctypes.POINTER(ctypes.c_uint64)]),
     "zgr_server_poll": (I, [P, ctypes.c_uint64, I]),
+    "zgr_server_cancel": (I, [P, ctypes.c_uint64]),
+    "zgr_test_inject_fault": (I, [P, I, I]),
     "zgr_server_stats": (None, [P, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64),
                                 ctypes.POINTER(ctypes.c_uint64)]),
     "zgr_recover": (I, [P, P, P, S, P, S, ctypes.POINTER(S)]),
@@ -526,6 +530,10 @@ class Context:
         return json.loads(buf.value.decode())
 
     # ---- profiling ----
+    def inject_fault(self, site, countdown):
+        """zgr_test_inject_fault: the countdown-th next event at `site` fails (0 disarms)."""
+        _check(lib().zgr_test_inject_fault(self.h, site, countdown))
+
     def set_profiling(self, on):
         lib().zgr_set_profiling(self.h, int(on))
 
@@ -645,6 +653,17 @@ class Server:
         _, _, out = self._live.pop(ticket)
         _check(rc)
         return out
+
+    def cancel(self, ticket):
+        """zgr_server_cancel: True when the request was withdrawn before reaching the GPU
+        (its poll then raises ZgrError with ERR_CANCELLED), False when it is already on
+        the GPU or done (it completes as usual)."""
+        rc = lib().zgr_server_cancel(self.h, ticket)
+        if rc == OK:
+            return True
+        if rc == ERR_INVALID_ARG and ticket in self._live:
+            return False
+        raise ZgrError(rc, lib().zgr_last_error().decode(errors="replace"))
 
     def stats(self):
         q, a, c = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()

@@ -1,6 +1,6 @@
 """Ceremony import: powers of tau from a perpetual-powers-of-tau file, the host side of
-crates/ppot2ark/src/main.rs (from_ppot_file_inner :47-124, from_ppot_file :126-145,
-from_ppot_file_ldt :147-176, load_save_power_tau :178-202).
+crates/ppot2ark/src/main.rs (from_ppot_file_inner :45-119, from_ppot_file :121-135,
+from_ppot_file_ldt :137-164, load_save_power_tau :166-191).
 
 The reference reads the files through the powersoftau crate of kobigurk/phase2-bn254
 (rev dd6b966; an absent dependency): a challenge holds the BatchedAccumulator with
@@ -33,7 +33,7 @@ class PpotError(RuntimeError):
 
 
 class InputType(enum.Enum):
-    """crates/ppot2ark/src/main.rs:26-43."""
+    """crates/ppot2ark/src/main.rs:27-43."""
     Challenge = "challenge"
     Response = "response"
 
@@ -141,7 +141,7 @@ def read_chunk(ctx, input_map, params, start, size, compressed):
 
 def from_ppot_file_inner(ctx, input_path, input_type, file_size, read_from, read_size_pow, chunk_size_pow,
                          parameters):
-    """crates/ppot2ark/src/main.rs:47-124."""
+    """crates/ppot2ark/src/main.rs:45-119."""
     read_size = 1 << read_size_pow
     chunk_size = 1 << chunk_size_pow
     if read_from + read_size > (1 << file_size):
@@ -169,7 +169,7 @@ def from_ppot_file_inner(ctx, input_path, input_type, file_size, read_from, read
 
 
 def from_ppot_file(ctx, input_path, input_type, file_size_pow, read_from, read_size_pow, chunk_size_pow):
-    """crates/ppot2ark/src/main.rs:126-145: points [read_from, read_from + 2^read_size_pow)
+    """crates/ppot2ark/src/main.rs:121-135: points [read_from, read_from + 2^read_size_pow)
     of tau_powers_g1 and tau_powers_g2 of <input_path>/<challenge|response>_<file_size_pow>."""
     params = CeremonyParams(file_size_pow, file_size_pow)
     return from_ppot_file_inner(ctx, input_path, input_type, file_size_pow, read_from, read_size_pow,
@@ -178,7 +178,7 @@ def from_ppot_file(ctx, input_path, input_type, file_size_pow, read_from, read_s
 
 def from_ppot_file_ldt(ctx, input_path, input_type, file_size_pow, read_size_pow, high_read_size_pow,
                        chunk_size_pow):
-    """crates/ppot2ark/src/main.rs:147-176: g1pp / g2pp = the first 2^read_size_pow powers,
+    """crates/ppot2ark/src/main.rs:137-164: g1pp / g2pp = the first 2^read_size_pow powers,
     high_g1pp = the powers from H = 2^high_read_size_pow - 2^read_size_pow on, and
     high_g2 = [tau^H] G2."""
     g1pp, g2pp = from_ppot_file(ctx, input_path, input_type, file_size_pow, 0, read_size_pow, chunk_size_pow)
@@ -195,7 +195,7 @@ def ptau_file_name(depth, mont=False):
 
 def load_save_power_tau(ctx, input_path, input_type, file_size_pow, target_size_pow, high_read_size_pow,
                         chunk_size_pow, dir):
-    """crates/ppot2ark/src/main.rs:178-202: read the powers and write the compressed
+    """crates/ppot2ark/src/main.rs:166-191: read the powers and write the compressed
     PowerTau file <dir>/power-tau-5DWgDV-<target>.bin.  The powers stay loaded in ctx
     (zgr_power_tau_load), whose domain must be 2^target_size_pow, ready for
     ctx.params_from_power_tau.  Returns the file's path."""

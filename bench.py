@@ -535,7 +535,7 @@ def main():
                   "params build on an AWS g5.4xlarge (README.md:121)")
     params_build = tb
 
-    # ceremony import (crates/ppot2ark/src/main.rs:47-124): the production read of
+    # ceremony import (crates/ppot2ark/src/main.rs:45-119): the production read of
     # 2^20 G1 and 2^20 G2 powers from a response file (compressed points), decoded and
     # checked on the device -- square roots, and [r] P == O for every G2 point
     g1pp, g2pp, _, _ = ctx.power_tau_download()

@@ -41,8 +41,9 @@ enum {
     ZGR_ERR_OOM = -3,            /* device allocation / admission failure */
     ZGR_ERR_DEVICE = -4,         /* HIP runtime error */
     ZGR_ERR_NOT_LOADED = -5,     /* params for a coset are not resident */
-    ZGR_ERR_INCONSISTENT = -6    /* coset commitments differ (crates/encoder/src/amt/blob.rs:42-47);
+    ZGR_ERR_INCONSISTENT = -6,   /* coset commitments differ (crates/encoder/src/amt/blob.rs:42-47);
                                     recovery rows off one polynomial (crates/recovery/src/data_poly.rs:112) */
+    ZGR_ERR_CANCELLED = -7       /* a served request withdrawn by zgr_server_cancel */
 };
 
 typedef struct zgr_ctx zgr_ctx;
@@ -116,7 +117,7 @@ int zgr_power_tau_load(zgr_ctx *ctx, const uint8_t *g1pp, const uint8_t *g2pp, c
                        const uint8_t *high_g2, size_t n_points);
 /* Ceremony import: decode and check count points of a perpetual-powers-of-tau file, as
  * BatchedAccumulator::read_chunk with CheckForCorrectness::Yes does for
- * crates/ppot2ark/src/main.rs:137-187 (from_ppot_file_inner; the powersoftau crate of
+ * crates/ppot2ark/src/main.rs:45-119 (from_ppot_file_inner; the powersoftau crate of
  * kobigurk/phase2-bn254 rev dd6b966, pairing_ce bn256 encodings: big-endian coordinates,
  * G2 as c1 || c0, flag bits 0x80 / 0x40 in the first byte).  src holds the count
  * encodings back to back (64 / 32 B per G1 point uncompressed / compressed, 128 / 64 B
@@ -217,8 +218,26 @@ int zgr_server_submit(zgr_server *srv, const uint8_t *raw, size_t len, const zgr
 /* Wait up to timeout_ms (< 0: forever) for a ticket.  Returns 1 while pending, else
  * the request's final status (ZGR_OK or an error), once. */
 int zgr_server_poll(zgr_server *srv, uint64_t ticket, int timeout_ms);
-/* queued (not yet admitted), admitted (staged or on the GPU), completed counts */
+/* Withdraw a request that has not been issued to the GPU yet (queued, or admitted and
+ * waiting for a slot): its poll then returns ZGR_ERR_CANCELLED and its buffers are free.
+ * No counterpart in the reference, whose requests run to completion once received
+ * (grpc/src/service.rs:48-65).  ZGR_ERR_INVALID_ARG when the ticket is not pending or
+ * is already on the GPU (the request then completes as usual). */
+int zgr_server_cancel(zgr_server *srv, uint64_t ticket);
+/* queued (not yet admitted), admitted (staged or on the GPU), completed counts
+ * (completed includes cancelled requests) */
 void zgr_server_stats(zgr_server *srv, uint64_t *queued, uint64_t *admitted, uint64_t *completed);
+
+/* ---- failure injection, for tests of the error paths ----
+ * No counterpart in the reference (whose GPU errors panic, prove_gpu.rs:48,84).  Arms a
+ * host-side failure: the countdown-th next event at `site` fails with ZGR_ERR_DEVICE
+ * and the message "injected fault: ..." before it launches anything, so every kernel
+ * still completes normally.  countdown 0 disarms.  Safe to call while a server owns
+ * the context.
+ *   ZGR_FAULT_ISSUE  the issue of one blob into a pipeline slot (zgr_encode*, served
+ *                    requests) */
+enum { ZGR_FAULT_ISSUE = 0, ZGR_FAULT_SITES = 1 };
+int zgr_test_inject_fault(zgr_ctx *ctx, int site, int countdown);
 
 /* ---- primitives (reference call sites named per function) ---- */
 /* Radix2EvaluationDomain::fft / ifft of ark-poly over the 2^log_n domain, natural

@@ -2,7 +2,7 @@
 tests/ import this module; it writes synthetic ceremony files and checks the device
 decoder, and is never the thing measured or shipped).
 
-crates/ppot2ark/src/main.rs:47-124 reads ceremony files through the powersoftau crate of
+crates/ppot2ark/src/main.rs:45-119 reads ceremony files through the powersoftau crate of
 kobigurk/phase2-bn254 (rev dd6b966), whose point encodings are pairing_ce's bn256
 EncodedPoint (G1Uncompressed / G1Compressed / G2Uncompressed / G2Compressed):
   * coordinates as canonical integers, 32 bytes big-endian each; G2 coordinates as

@@ -1,7 +1,7 @@
 """Ceremony import on the device (crates/ppot2ark/src/main.rs): zgr_ppot_decode and the
 zg_rocm.ppot mirror of from_ppot_file / from_ppot_file_ldt / load_save_power_tau, over
 synthetic challenge and response files of 2^12 powers (the reference tests' size,
-main.rs:273-381) written by the oracle's restatement of the format (oracle/ppot.py).
+main.rs:285-381) written by the oracle's restatement of the format (oracle/ppot.py).
 
 The powers in the files are the device's own [tau^i] G1 / G2 for a seeded tau (pinned
 against the oracle and textbook multiples in test_gpu_params_io.py); what is checked
@@ -146,7 +146,7 @@ def test_rejections_g2(ctx, powers):
 
 
 def test_from_ppot_file_challenge(ctx, powers, files):
-    """test_load_from_challenge_12_nomal (main.rs:294-315)."""
+    """test_load_from_challenge_12_nomal (main.rs:307-331)."""
     g1, g2 = powers
     pot = ZP.from_ppot_file(ctx, str(files), ZP.InputType.Challenge, FILE_POW, 3840, 8, 10)
     assert len(pot.g1) == 256 and len(pot.g2) == 256
@@ -160,7 +160,7 @@ def test_from_ppot_file_too_long(ctx, files):
 
 
 def test_from_ppot_file_high_response(ctx, powers, files):
-    """test_load_from_high_deg_response_nomal (main.rs:338-361): compressed points."""
+    """test_load_from_high_deg_response_nomal (main.rs:355-381): compressed points."""
     g1, g2 = powers
     start = N - 256
     pot = ZP.from_ppot_file(ctx, str(files), ZP.InputType.Response, FILE_POW, start, 8, 6)
@@ -168,8 +168,8 @@ def test_from_ppot_file_high_response(ctx, powers, files):
 
 
 def test_from_ppot_file_ldt(ctx, powers, files):
-    """test_from_ppot_file_ldt (main.rs:272-292) with PowerTau::check_ldt
-    (crates/amt/src/power_tau.rs:209-225): e(g1pp[i], high_g2) = e(high_g1pp[i], g2pp[0]),
+    """test_from_ppot_file_ldt (main.rs:285-305) with PowerTau::check_ldt
+    (crates/amt/src/power_tau.rs:210-225): e(g1pp[i], high_g2) = e(high_g1pp[i], g2pp[0]),
     here through tau: high_g1pp[i] = [tau^H] g1pp[i], H = 2^11 - 2^8."""
     g1, g2 = powers
     pt = ZP.from_ppot_file_ldt(ctx, str(files), ZP.InputType.Challenge, FILE_POW, 8, 11, 10)
@@ -193,7 +193,7 @@ def test_corrupt_section_fails_the_read(ctx, files, tmp_path):
 
 
 def test_load_save_power_tau(ctx, powers, files, tmp_path):
-    """load_save_power_tau (main.rs:178-202) into a 2^11 context: the powers stay loaded
+    """load_save_power_tau (main.rs:166-191) into a 2^11 context: the powers stay loaded
     and the compressed PowerTau file is written under its reference name."""
     g1, g2 = powers
     path = ZP.load_save_power_tau(ctx, str(files), ZP.InputType.Response, FILE_POW, 11, 12, 9, str(tmp_path))

@@ -185,7 +185,7 @@ class _NoDevice:
 
 
 def test_too_long_to_read(tmp_path):
-    # the reference's test_load_from_challenge_12_too_long (main.rs:317-336)
+    # the reference's test_load_from_challenge_12_too_long (main.rs:333-353)
     with pytest.raises(ZP.PpotError, match="^too long to read$"):
         ZP.from_ppot_file(_NoDevice(), str(tmp_path), ZP.InputType.Challenge, 12, 3841, 8, 10)
 
