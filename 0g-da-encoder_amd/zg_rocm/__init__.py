@@ -102,6 +102,10 @@ class ZgrError(RuntimeError):
         self.msg = msg
 
 
+class RequestCancelled(RuntimeError):
+    """A served request withdrawn (zgr_server_cancel) because its caller went away."""
+
+
 class TooLargeBlob(ValueError):
     """EncoderError::TooLargeBlob (crates/encoder/src/encoder/error.rs:3-6)."""
 
@@ -975,7 +979,9 @@ class EncoderService:
         self.ctx = ctx
         self.server = server
 
-    def _build(self, data, require_data):
+    POLL_MS = 20  # how often a served request checks that its caller is still there
+
+    def _build(self, data, require_data, alive=None):
         g = self.ctx.geo
         if len(data) > g.max_raw_data_size:  # RawData::try_from (raw_data.rs:55-73)
             raise TooLargeBlob(len(data), g.max_raw_data_size)
@@ -983,7 +989,18 @@ class EncoderService:
             return EncodedBlob(g, self.ctx.encode(data))
         # the slices need the encoded rows even when the reply omits them
         t = self.server.submit(data, True)
-        return EncodedBlob(g, self.server.poll(t))
+        while True:
+            out = self.server.poll(t, self.POLL_MS if alive is not None else -1)
+            if out is not None:
+                return EncodedBlob(g, out)
+            # the caller went away: withdraw the request unless it is already on the GPU
+            if not alive() and self.server.cancel(t):
+                try:
+                    self.server.poll(t)
+                except ZgrError as e:
+                    if e.code != ERR_CANCELLED:
+                        raise
+                raise RequestCancelled("request withdrawn before it reached the GPU")
 
     def process_data(self, data, require_data):
         """-> dict with the EncodeBlobReply fields (encoded_slice as a list of bytes)
@@ -999,10 +1016,12 @@ class EncoderService:
             "blob": blob,
         }
 
-    def encode_blob(self, data, require_data):
+    def encode_blob(self, data, require_data, alive=None):
         """The EncodeBlobReply protobuf bytes for one request (the gRPC handler's
-        body), serialized straight from the gathered slice array."""
-        blob = self._build(data, require_data)
+        body), serialized straight from the gathered slice array.  alive (the gRPC
+        context's is_active) lets a served request be withdrawn when its client has
+        gone: RequestCancelled is raised then."""
+        blob = self._build(data, require_data, alive)
         return wire.encode_reply({
             "version": self.VERSION,
             "erasure_commitment": blob.get_affine_commitment(),

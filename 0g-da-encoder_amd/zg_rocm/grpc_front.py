@@ -6,7 +6,8 @@ The method is served through grpcio's generic handlers with the protobuf codec o
 directions.  A request that ``EncoderService`` rejects with TooLargeBlob ends with
 status UNKNOWN and the error's Debug text as its message, as tonic's
 ``Status::new(Code::Unknown, e)`` reports it (service.rs:59-61,
-crates/encoder/src/encoder/error.rs:23-25).  grpcio is imported when a server or
+crates/encoder/src/encoder/error.rs:23-25).  A served request whose client goes away
+before it reaches the GPU is withdrawn from the server's queue.  grpcio is imported when a server or
 client is made, not with the package.
 """
 from concurrent import futures
@@ -24,18 +25,20 @@ def _options(limit):
 
 def serve(service, address="127.0.0.1:0", max_workers=4, message_limit=wire.MESSAGE_SIZE_LIMIT):
     """Starts a server for ``service`` (an EncoderService, or anything with its
-    ``encode_blob(data, require_data) -> reply bytes``) on ``address``.  Returns
+    ``encode_blob(data, require_data, alive) -> reply bytes``) on ``address``.  Returns
     (server, bound port); stop it with ``server.stop(grace)``."""
     import grpc
 
-    from . import TooLargeBlob
+    from . import RequestCancelled, TooLargeBlob
 
     def encode_blob(request, context):
         data, require_data = wire.decode_request(request)
         try:
-            return service.encode_blob(data, require_data)
+            return service.encode_blob(data, require_data, alive=context.is_active)
         except TooLargeBlob as e:
             context.abort(grpc.StatusCode.UNKNOWN, str(e))
+        except RequestCancelled as e:  # the client is gone; the status is never read
+            context.abort(grpc.StatusCode.CANCELLED, str(e))
 
     handler = grpc.unary_unary_rpc_method_handler(encode_blob, request_deserializer=None, response_serializer=None)
     server = grpc.server(futures.ThreadPoolExecutor(max_workers=max_workers), options=_options(message_limit))
