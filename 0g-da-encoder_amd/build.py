@@ -76,7 +76,9 @@ def build(verbose=False, incremental=False):
         raise RuntimeError("\n".join(errors))
     tmp = LIB + ".tmp"
     objs = [os.path.join(OBJDIR, os.path.splitext(s)[0] + ".o") for s in SOURCES]
-    cmd = [HIPCC, "--offload-arch=" + ARCH, "-fPIC", "-shared", "-Wl,-z,defs"] + objs + ["-o", tmp]
+    # roctx: the host-side ranges of the entry points (rocprofv3 --marker-trace)
+    cmd = [HIPCC, "--offload-arch=" + ARCH, "-fPIC", "-shared", "-Wl,-z,defs"] + objs + \
+        ["-lrocprofiler-sdk-roctx", "-o", tmp]
     res = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
     if res.returncode != 0:
         raise RuntimeError("link failed:\n" + _filter(res.stdout))

@@ -34,6 +34,8 @@
 #include <unordered_map>
 #include <vector>
 
+#include <rocprofiler-sdk-roctx/roctx.h>
+
 #include "../../include/zgr.h"
 #include "kernels.h"
 #include "recover.h"
@@ -77,6 +79,15 @@ int fail(int code, const char* fmt, ...) {
         int rc_ = (expr);     \
         if (rc_) return rc_;  \
     } while (0)
+
+// Host-side range of an entry point or pipeline stage, shown by rocprofv3
+// --marker-trace (roctx; a cheap no-op when no tool is attached).
+struct Range {
+    explicit Range(const char* name) { roctxRangePushA(name); }
+    ~Range() { roctxRangePop(); }
+    Range(const Range&) = delete;
+    Range& operator=(const Range&) = delete;
+};
 
 inline uint32_t blocks(size_t n, uint32_t t) { return (uint32_t)((n + t - 1) / t); }
 
@@ -799,6 +810,7 @@ bool fault_fires(zgr_ctx* c, int site) {
 }
 
 int issue_blob(zgr_ctx* c, Slot& s, const uint8_t* raw_dev, size_t len, const zgr_encoded* out) {
+    Range range_("issue_blob");
     if (fault_fires(c, ZGR_FAULT_ISSUE)) return fail(ZGR_ERR_DEVICE, "injected fault: blob issue");
     const uint32_t n = (uint32_t)c->n;
     LAUNCH(c, s.st, "pack", k_pack, blocks(n, 256), 256, 0, raw_dev, len, c->log_col, c->log_row, s.pts);
@@ -886,6 +898,7 @@ int issue_blob(zgr_ctx* c, Slot& s, const uint8_t* raw_dev, size_t len, const zg
 // Wait for slot s, copy its staged outputs into the caller's buffers, check the
 // coset commitments (crates/encoder/src/amt/blob.rs:42-47).
 int finish_blob(zgr_ctx* c, Slot& s) {
+    Range range_("finish_blob");
     if (!s.busy) return ZGR_OK;
     s.busy = false;
     const zgr_encoded* out = s.out;
@@ -1497,6 +1510,7 @@ void zgr_ctx_destroy(zgr_ctx* c) {
 }
 
 int zgr_params_load(zgr_ctx* c, int coset, const void* bases_mont, size_t nbytes) {
+    Range range_("zgr_params_load");
     if (!c || !bases_mont) return fail(ZGR_ERR_INVALID_ARG, "null argument");
     std::lock_guard<std::mutex> lk(c->mu);
     if (coset < 0 || coset >= c->coset_n) return fail(ZGR_ERR_INVALID_ARG, "coset %d out of range", coset);
@@ -1514,6 +1528,7 @@ int zgr_params_load(zgr_ctx* c, int coset, const void* bases_mont, size_t nbytes
 }
 
 int zgr_params_load_bamt(zgr_ctx* c, int coset, const uint8_t* bamt, size_t nbytes) {
+    Range range_("zgr_params_load_bamt");
     if (!c || !bamt) return fail(ZGR_ERR_INVALID_ARG, "null argument");
     std::lock_guard<std::mutex> lk(c->mu);
     if (coset < 0 || coset >= c->coset_n) return fail(ZGR_ERR_INVALID_ARG, "coset %d out of range", coset);
@@ -1567,6 +1582,7 @@ int zgr_params_load_bamt(zgr_ctx* c, int coset, const uint8_t* bamt, size_t nbyt
 }
 
 int zgr_params_from_pp(zgr_ctx* c, int coset, const uint8_t* g1pp, const uint8_t* high_g1pp, size_t n_points) {
+    Range range_("zgr_params_from_pp");
     if (!c || !g1pp || !high_g1pp) return fail(ZGR_ERR_INVALID_ARG, "null argument");
     std::lock_guard<std::mutex> lk(c->mu);
     if (coset < 0 || coset >= c->coset_n) return fail(ZGR_ERR_INVALID_ARG, "coset %d out of range", coset);
@@ -1592,6 +1608,7 @@ int zgr_params_from_pp(zgr_ctx* c, int coset, const uint8_t* g1pp, const uint8_t
 }
 
 int zgr_params_generate(zgr_ctx* c, const uint8_t tau[32]) {
+    Range range_("zgr_params_generate");
     if (!c || !tau) return fail(ZGR_ERR_INVALID_ARG, "null argument");
     std::lock_guard<std::mutex> lk(c->mu);
     HIPCHK(hipSetDevice(c->device));
@@ -1636,6 +1653,7 @@ int zgr_params_generate(zgr_ctx* c, const uint8_t tau[32]) {
 }
 
 int zgr_power_tau_setup(zgr_ctx* c, const uint8_t tau[32]) {
+    Range range_("zgr_power_tau_setup");
     if (!c || !tau) return fail(ZGR_ERR_INVALID_ARG, "null argument");
     std::lock_guard<std::mutex> lk(c->mu);
     HIPCHK(hipSetDevice(c->device));
@@ -1675,6 +1693,7 @@ int zgr_power_tau_setup(zgr_ctx* c, const uint8_t tau[32]) {
 
 int zgr_power_tau_load(zgr_ctx* c, const uint8_t* g1pp, const uint8_t* g2pp, const uint8_t* high_g1pp,
                        const uint8_t* high_g2, size_t n_points) {
+    Range range_("zgr_power_tau_load");
     if (!c || !g1pp || !g2pp || !high_g1pp || !high_g2) return fail(ZGR_ERR_INVALID_ARG, "null argument");
     if (n_points != c->n)
         return fail(ZGR_ERR_INVALID_ARG, "%zu powers of tau, the context's domain has %zu", n_points, c->n);
@@ -1696,6 +1715,7 @@ int zgr_power_tau_load(zgr_ctx* c, const uint8_t* g1pp, const uint8_t* g2pp, con
 }
 
 int zgr_ppot_decode(zgr_ctx* c, int kind, const uint8_t* src, size_t count, uint8_t* out, size_t* first_bad) {
+    Range range_("zgr_ppot_decode");
     if (!c || (count && (!src || !out))) return fail(ZGR_ERR_INVALID_ARG, "null argument");
     if (kind < ZGR_PPOT_G1_UNCOMPRESSED || kind > ZGR_PPOT_G2_COMPRESSED)
         return fail(ZGR_ERR_INVALID_ARG, "point kind %d is not one of ZGR_PPOT_*", kind);
@@ -1762,6 +1782,7 @@ int zgr_power_tau_download(zgr_ctx* c, uint8_t* g1pp, uint8_t* g2pp, uint8_t* hi
 }
 
 int zgr_params_from_power_tau(zgr_ctx* c, int coset) {
+    Range range_("zgr_params_from_power_tau");
     if (!c) return fail(ZGR_ERR_INVALID_ARG, "null argument");
     std::lock_guard<std::mutex> lk(c->mu);
     if (coset < 0 || coset >= c->coset_n) return fail(ZGR_ERR_INVALID_ARG, "coset %d out of range", coset);
@@ -1773,6 +1794,7 @@ int zgr_params_from_power_tau(zgr_ctx* c, int coset) {
 }
 
 int zgr_params_g2(zgr_ctx* c, int coset, uint8_t* vanishes, uint8_t* g2, uint8_t* high_g2) {
+    Range range_("zgr_params_g2");
     if (!c) return fail(ZGR_ERR_INVALID_ARG, "null argument");
     std::lock_guard<std::mutex> lk(c->mu);
     if (coset < 0 || coset >= c->coset_n) return fail(ZGR_ERR_INVALID_ARG, "coset %d out of range", coset);
@@ -1798,6 +1820,7 @@ int zgr_params_g2(zgr_ctx* c, int coset, uint8_t* vanishes, uint8_t* g2, uint8_t
 }
 
 int zgr_write_params_file(zgr_ctx* c, int kind, int coset, int sub_depth, uint8_t* out, size_t cap, size_t* len) {
+    Range range_("zgr_write_params_file");
     if (!c || !len) return fail(ZGR_ERR_INVALID_ARG, "null argument");
     std::lock_guard<std::mutex> lk(c->mu);
     const bool amt = kind == ZGR_FILE_AMT_MONT || kind == ZGR_FILE_AMT || kind == ZGR_FILE_AMT_VERIFY;
@@ -1934,6 +1957,7 @@ int zgr_params_download(zgr_ctx* c, int coset, void* out, size_t nbytes) {
 }
 
 int zgr_multiple_multiexp(zgr_ctx* c, int coset, const uint8_t* ri_mont, size_t n, uint8_t* out_points) {
+    Range range_("zgr_multiple_multiexp");
     if (!c || !ri_mont || !out_points) return fail(ZGR_ERR_INVALID_ARG, "null argument");
     std::lock_guard<std::mutex> lk(c->mu);
     CHECK(ensure_params(c, coset));
@@ -1952,6 +1976,7 @@ int zgr_multiple_multiexp(zgr_ctx* c, int coset, const uint8_t* ri_mont, size_t 
 
 int zgr_amt_prove(zgr_ctx* c, int coset, const uint8_t* ri_mont, size_t n, uint8_t* commitment,
                   uint8_t* comm_tree, uint8_t* proof_tree, uint8_t* high) {
+    Range range_("zgr_amt_prove");
     if (!c || !ri_mont) return fail(ZGR_ERR_INVALID_ARG, "null argument");
     std::lock_guard<std::mutex> lk(c->mu);
     CHECK(ensure_params(c, coset));
@@ -1974,6 +1999,7 @@ int zgr_amt_prove(zgr_ctx* c, int coset, const uint8_t* ri_mont, size_t n, uint8
 }
 
 int zgr_encode_device(zgr_ctx* c, const void* raw_dev, size_t len, const zgr_encoded* out) {
+    Range range_("zgr_encode_device");
     if (!c || !out || (!raw_dev && len)) return fail(ZGR_ERR_INVALID_ARG, "null argument");
     std::lock_guard<std::mutex> lk(c->mu);
     CHECK(check_len(c, len));
@@ -1983,6 +2009,7 @@ int zgr_encode_device(zgr_ctx* c, const void* raw_dev, size_t len, const zgr_enc
 }
 
 int zgr_encode(zgr_ctx* c, const uint8_t* raw, size_t len, const zgr_encoded* out) {
+    Range range_("zgr_encode");
     if (!c || !out || (!raw && len)) return fail(ZGR_ERR_INVALID_ARG, "null argument");
     std::lock_guard<std::mutex> lk(c->mu);
     CHECK(check_len(c, len));
@@ -1995,6 +2022,7 @@ int zgr_encode(zgr_ctx* c, const uint8_t* raw, size_t len, const zgr_encoded* ou
 
 int zgr_encode_batch_device(zgr_ctx* c, int nblobs, const void* const* raw_dev, const size_t* lens,
                             const zgr_encoded* outs) {
+    Range range_("zgr_encode_batch_device");
     if (!c || nblobs < 0 || (nblobs && (!raw_dev || !lens || !outs)))
         return fail(ZGR_ERR_INVALID_ARG, "null argument");
     std::lock_guard<std::mutex> lk(c->mu);
@@ -2137,6 +2165,7 @@ void zgr_server_stats(zgr_server* sv, uint64_t* queued, uint64_t* admitted, uint
 }
 
 int zgr_ntt(zgr_ctx* c, uint8_t* data_mont, int log_n, int batch, int inverse) {
+    Range range_("zgr_ntt");
     if (!c || (!data_mont && batch) || batch < 0) return fail(ZGR_ERR_INVALID_ARG, "bad argument");
     if (log_n < 1 || log_n > c->log_n)
         return fail(ZGR_ERR_INVALID_ARG, "log_n %d outside [1, %d] of this context", log_n, c->log_n);
@@ -2157,6 +2186,7 @@ int zgr_ntt(zgr_ctx* c, uint8_t* data_mont, int log_n, int batch, int inverse) {
 }
 
 int zgr_ntt_device(zgr_ctx* c, void* data_dev, int log_n, int batch, int inverse) {
+    Range range_("zgr_ntt_device");
     if (!c || (!data_dev && batch) || batch < 0) return fail(ZGR_ERR_INVALID_ARG, "bad argument");
     if (log_n < 1 || log_n > c->log_n)
         return fail(ZGR_ERR_INVALID_ARG, "log_n %d outside [1, %d] of this context", log_n, c->log_n);
@@ -2173,6 +2203,7 @@ int zgr_ntt_device(zgr_ctx* c, void* data_dev, int log_n, int batch, int inverse
 }
 
 int zgr_coset_extend(zgr_ctx* c, const uint8_t* data_mont, int coset, uint8_t* out_mont) {
+    Range range_("zgr_coset_extend");
     if (!c || !data_mont || !out_mont) return fail(ZGR_ERR_INVALID_ARG, "null argument");
     if (coset < 0 || coset >= c->coset_n) return fail(ZGR_ERR_INVALID_ARG, "coset %d out of range", coset);
     std::lock_guard<std::mutex> lk(c->mu);
@@ -2241,6 +2272,7 @@ int msm_fb_line(zgr_ctx* c, Lane& ln, int coset, int L, uint32_t* out) {
 }
 
 int zgr_msm(zgr_ctx* c, const uint8_t* bases_mont, const uint8_t* scalars_mont, size_t n, uint8_t* out_point) {
+    Range range_("zgr_msm");
     if (!c || !bases_mont || !scalars_mont || !out_point) return fail(ZGR_ERR_INVALID_ARG, "null argument");
     if (n == 0 || n > c->n) return fail(ZGR_ERR_INVALID_ARG, "msm size %zu outside [1, %zu]", n, c->n);
     std::lock_guard<std::mutex> lk(c->mu);
@@ -2271,6 +2303,7 @@ int zgr_msm(zgr_ctx* c, const uint8_t* bases_mont, const uint8_t* scalars_mont, 
 }
 
 int zgr_ecfft(zgr_ctx* c, const uint8_t* points, int log_n, int inverse, uint8_t* out) {
+    Range range_("zgr_ecfft");
     if (!c || !points || !out) return fail(ZGR_ERR_INVALID_ARG, "null argument");
     if (log_n < 1 || log_n > c->log_n)
         return fail(ZGR_ERR_INVALID_ARG, "log_n %d outside [1, %d] of this context", log_n, c->log_n);
@@ -2302,6 +2335,7 @@ int zgr_ecfft(zgr_ctx* c, const uint8_t* points, int log_n, int inverse, uint8_t
 
 int zgr_merkle(zgr_ctx* c, const uint8_t* data, size_t n_elems, int levels, uint8_t* out_layers,
                uint8_t* file_root) {
+    Range range_("zgr_merkle");
     if (!c || !data || !file_root) return fail(ZGR_ERR_INVALID_ARG, "null argument");
     if (n_elems == 0 || n_elems % 8) return fail(ZGR_ERR_INVALID_ARG, "n_elems %zu unsupported", n_elems);
     std::lock_guard<std::mutex> lk(c->mu);
@@ -2353,6 +2387,7 @@ int zgr_merkle(zgr_ctx* c, const uint8_t* data, size_t n_elems, int levels, uint
 
 int zgr_merkle_noalloc(zgr_ctx* c, const void* data_dev, size_t n_elems, int levels, void* out_layers_dev,
                        void* file_root_dev) {
+    Range range_("zgr_merkle_noalloc");
     if (!c || !data_dev || !file_root_dev || (levels >= 0 && !out_layers_dev) || levels < -1)
         return fail(ZGR_ERR_INVALID_ARG, "null argument");
     if (n_elems == 0 || n_elems % 8) return fail(ZGR_ERR_INVALID_ARG, "n_elems %zu unsupported", n_elems);
@@ -2399,6 +2434,7 @@ int zgr_merkle_noalloc(zgr_ctx* c, const void* data_dev, size_t n_elems, int lev
 
 int zgr_verify_msm(zgr_ctx* c, const uint32_t* row_ids, const uint8_t* rows, const uint8_t* alphas, size_t n_rows,
                    uint8_t* out_point) {
+    Range range_("zgr_verify_msm");
     if (!c || !out_point || (n_rows && (!row_ids || !rows || !alphas))) return fail(ZGR_ERR_INVALID_ARG, "null argument");
     const size_t ROW = (size_t)1 << c->log_row, COL = (size_t)1 << c->log_col;
     for (size_t k = 0; k < n_rows; k++) {
@@ -2486,6 +2522,7 @@ int zgr_verify_msm(zgr_ctx* c, const uint32_t* row_ids, const uint8_t* rows, con
 
 int zgr_recover(zgr_ctx* c, const uint32_t* row_ids, const uint8_t* rows, size_t n_rows, uint8_t* raw_out,
                 size_t raw_cap, size_t* raw_len) {
+    Range range_("zgr_recover");
     if (!c || (n_rows && (!row_ids || !rows)) || !raw_out || !raw_len) return fail(ZGR_ERR_INVALID_ARG, "null argument");
     const int lc = c->log_col, h = c->log_row, log_n = c->log_n;
     const size_t ROW = (size_t)1 << h, COL = (size_t)1 << lc, n = c->n;
@@ -2655,6 +2692,7 @@ int zgr_recover(zgr_ctx* c, const uint32_t* row_ids, const uint8_t* rows, size_t
 }
 
 int zgr_autotune(zgr_ctx* c, char* report, size_t cap) {
+    Range range_("zgr_autotune");
     if (!c) return fail(ZGR_ERR_INVALID_ARG, "null argument");
     std::lock_guard<std::mutex> lk(c->mu);
     CHECK(ensure_params(c, 0));  // the bucket kernels read coset 0's tables
