@@ -16,7 +16,7 @@
 //
 // One lane per point: the big-endian words are byte-swapped into limbs, the square root
 // of a compressed point's x^3 + b is an exponentiation (p = 3 mod 4; over Fq2 through
-// the norm), and the G2 subgroup test is [r] P == O by double-and-add.  Output points
+// the norm), and the G2 subgroup test uses the endomorphism psi (ppot_g2_in_subgroup).  Output points
 // are canonical affine limbs, the library's memory form (G1 x || y, G2 x.c0 || x.c1 ||
 // y.c0 || y.c1), and every lane writes a status (PpotStatus).
 #pragma once
@@ -48,6 +48,17 @@ struct PpotConst {
                                          0x1be06ac3u, 0x81be1899u, 0xceb8aaaeu, 0x2b149d40u};
     static constexpr uint32_t B2_1[8] = {0x85c315d2u, 0xe4a2bd06u, 0xe52d1852u, 0xa74fa084u,
                                          0xeed8fdf4u, 0xcd2cafadu, 0x3af0fed4u, 0x009713b0u};
+    // psi(x, y) = (conj(x) xi^((p-1)/3), conj(y) xi^((p-1)/2)), xi = 9 + u: the untwist-
+    // Frobenius-twist endomorphism of the twist, canonical
+    static constexpr uint32_t PSI_X0[8] = {0x176f553du, 0x99e39557u, 0xc2c3330cu, 0xb78cc310u,
+                                           0xf559b143u, 0x4c0bec3cu, 0x4f7911f7u, 0x2fb34798u};
+    static constexpr uint32_t PSI_X1[8] = {0x640fcba2u, 0x1665d51cu, 0x0b7c9dceu, 0x32ae2a1du,
+                                           0xd75a0794u, 0x4ba4cc8bu, 0x61ebae20u, 0x16c9e550u};
+    static constexpr uint32_t PSI_Y0[8] = {0x71a0135au, 0xdc540146u, 0xa9c95998u, 0xdbaae0edu,
+                                           0xb6e2f9b9u, 0xdc5ec698u, 0x489af5dcu, 0x063cf305u};
+    static constexpr uint32_t PSI_Y1[8] = {0x2623b0e3u, 0x82d37f63u, 0x8fa25bd2u, 0x21807dc9u,
+                                           0xec796f2bu, 0x0704b5a7u, 0xac41049au, 0x07c03cbcu};
+    static constexpr uint64_t BN_X = 0x44e992b44a6909f1ull;  // the curve parameter x of BN254
 };
 
 // a > b for canonical limbs
@@ -176,6 +187,39 @@ __device__ __forceinline__ fq2 ppot_pick_y2(const fq2& y, bool greatest) {
     return larger == greatest ? y : q2_neg(y);
 }
 
+// psi on XYZZ coordinates: conjugation is a field automorphism, so it maps X / ZZ and
+// Y / ZZZ coordinate-wise
+__device__ __forceinline__ g2_xyzz ppot_psi(const g2_xyzz& p, const fq2& cx, const fq2& cy) {
+    auto conj = [](const fq2& a) { return fq2{a.c0, f_neg(a.c1)}; };
+    return {q2_mul(conj(p.X), cx), q2_mul(conj(p.Y), cy), conj(p.ZZ), conj(p.ZZZ)};
+}
+
+__device__ __forceinline__ bool ppot_g2_eq(const g2_xyzz& a, const g2_xyzz& b) {
+    const bool ia = g2_is_inf(a), ib = g2_is_inf(b);
+    if (ia || ib) return ia == ib;
+    const fq2 x1 = q2_mul(a.X, b.ZZ), x2 = q2_mul(b.X, a.ZZ);
+    const fq2 y1 = q2_mul(a.Y, b.ZZZ), y2 = q2_mul(b.Y, a.ZZZ);
+    return f_eq(x1.c0, x2.c0) && f_eq(x1.c1, x2.c1) && f_eq(y1.c0, y2.c0) && f_eq(y1.c1, y2.c1);
+}
+
+// Q in the order-r subgroup of the twist iff [x+1]Q + psi([x]Q) + psi^2([x]Q) =
+// psi^3([2x]Q) (El Housni, Guillevic and Piellard, "Co-factor clearing and subgroup
+// membership testing on pairing-friendly curves", 2022, for BN curves): a 63-bit scalar
+// multiplication instead of [r]Q
+__device__ __noinline__ bool ppot_g2_in_subgroup(const g2_aff& q) {
+    const fq2 cx = {ppot_mont(PpotConst::PSI_X0), ppot_mont(PpotConst::PSI_X1)};
+    const fq2 cy = {ppot_mont(PpotConst::PSI_Y0), ppot_mont(PpotConst::PSI_Y1)};
+    g2_xyzz xq = g2_inf();
+    for (int b = 62; b >= 0; b--) {
+        xq = g2_dbl(xq);
+        if ((PpotConst::BN_X >> b) & 1) xq = g2_madd(xq, q);
+    }
+    const g2_xyzz p1 = ppot_psi(xq, cx, cy), p2 = ppot_psi(p1, cx, cy);
+    const g2_xyzz lhs = g2_add(g2_add(g2_madd(xq, q), p1), p2);
+    const g2_xyzz rhs = ppot_psi(ppot_psi(ppot_psi(g2_dbl(xq), cx, cy), cx, cy), cx, cy);
+    return ppot_g2_eq(lhs, rhs);
+}
+
 // G1: y^2 = x^3 + 3
 template <bool COMPRESSED>
 __device__ __forceinline__ void ppot_g1(const uint32_t* src, size_t count, uint32_t* out, uint32_t* status) {
@@ -217,7 +261,7 @@ __device__ __forceinline__ void ppot_g1(const uint32_t* src, size_t count, uint3
     status[i] = st;
 }
 
-// G2: y^2 = x^3 + 3 / (9 + u) over Fq2, then [r] P == O
+// G2: y^2 = x^3 + 3 / (9 + u) over Fq2, then the subgroup test
 template <bool COMPRESSED>
 __device__ __forceinline__ void ppot_g2(const uint32_t* src, size_t count, uint32_t* out, uint32_t* status) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -248,12 +292,7 @@ __device__ __forceinline__ void ppot_g2(const uint32_t* src, size_t count, uint3
             if (!f_eq(y2.c0, rhs.c0) || !f_eq(y2.c1, rhs.c1)) st = PPOT_NOT_ON_CURVE;
         }
     }
-    if (st == PPOT_OK) {
-        fr r;
-#pragma unroll
-        for (int j = 0; j < 8; j++) r.v[j] = FrP::M[j];
-        if (!g2_is_inf(g2_mul(g2_from_aff(P), r))) st = PPOT_NOT_IN_SUBGROUP;
-    }
+    if (st == PPOT_OK && !ppot_g2_in_subgroup(P)) st = PPOT_NOT_IN_SUBGROUP;
     if (st != PPOT_OK) P = {q2_zero(), q2_zero()};
     uint32_t* o = out + i * 32;
     ppot_canon(P.x.c0, l[0]);
