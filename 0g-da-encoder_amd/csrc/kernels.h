@@ -1130,13 +1130,17 @@ __global__ void k_ec_load_bitrev(const uint32_t* in, uint32_t* pts, int log_n) {
 
 // k * P for a canonical scalar k < r by the GLV split of the BN254 endomorphism
 // phi(x, y) = (beta x, y) = lambda (x, y): k = k1 + lambda k2 (mod r) with |k1|, |k2| <
-// 2^127, and k P = k1 P + k2 phi(P).  Both halves share 2-bit windows over the multiples
-// P, 2P, 3P (phi of a multiple is one multiplication of its X): 64 x (2 doublings + 2
+// 2^127, and k P = k1 P + k2 phi(P).  Both halves share signed 3-bit (Booth) windows,
+// digits in [-4, 4], over the affine multiples P, 2P, 3P, 4P (phi of a multiple is one
+// multiplication of its x; a negative digit negates y): 43 x (3 doublings + 2 mixed
 // additions), the additions selected away on zero digits, where the plain 254-bit scalar
-// needs 127 x (2 doublings + 1 addition).  The multiples sit in LDS, thread-interleaved:
-// a block of EC_THREADS threads needs EC_LDS_WORDS u32.
+// needs 127 x (2 doublings + 1 addition).  The multiples are made affine with one
+// inversion and sit in LDS, thread-interleaved: a block of EC_THREADS threads needs
+// EC_LDS_WORDS u32.
 constexpr int EC_THREADS = 64;
-constexpr int EC_LDS_WORDS = 3 * X29_WORDS * EC_THREADS;
+constexpr int EC_TAB_POINTS = 4;
+constexpr int EC_AFF_WORDS = 18;  // x, y: 9 limbs each
+constexpr int EC_LDS_WORDS = EC_TAB_POINTS * EC_AFF_WORDS * EC_THREADS;
 
 struct Glv {
     // lambda = 0xb3c4d79d41a917585bfc41088d8daaa78b17ea66b99c90dd (a cube root of unity mod r),
@@ -1226,49 +1230,89 @@ __device__ __forceinline__ void glv_split(const fr& k, uint32_t* m1, bool* n1, u
     *n2 = glv_abs(y, m2);
 }
 
-__device__ __forceinline__ void ec_tab_store(uint32_t* tab, int k, const x29& v) {
-    uint32_t w[X29_WORDS];
-    x29_store(w, v);
+__device__ __forceinline__ void ec_tab_store(uint32_t* tab, int k, const fq29& x, const fq29& y) {
 #pragma unroll
-    for (int i = 0; i < X29_WORDS; i++) tab[(k * X29_WORDS + i) * EC_THREADS + threadIdx.x] = w[i];
+    for (int i = 0; i < 9; i++) {
+        tab[(k * EC_AFF_WORDS + i) * EC_THREADS + threadIdx.x] = x.l[i];
+        tab[(k * EC_AFF_WORDS + 9 + i) * EC_THREADS + threadIdx.x] = y.l[i];
+    }
 }
 
-__device__ __forceinline__ x29 ec_tab_load(const uint32_t* tab, int k) {
-    uint32_t w[X29_WORDS];
+__device__ __forceinline__ a29 ec_tab_load(const uint32_t* tab, int k) {
+    a29 a;
 #pragma unroll
-    for (int i = 0; i < X29_WORDS; i++) w[i] = tab[(k * X29_WORDS + i) * EC_THREADS + threadIdx.x];
-    return x29_load(w);
+    for (int i = 0; i < 9; i++) {
+        a.x.l[i] = tab[(k * EC_AFF_WORDS + i) * EC_THREADS + threadIdx.x];
+        a.y.l[i] = tab[(k * EC_AFF_WORDS + 9 + i) * EC_THREADS + threadIdx.x];
+    }
+    return a;
+}
+
+// Booth digit of a window from its nibble (b_{3i+2} b_{3i+1} b_{3i} b_{3i-1}):
+// -4 b_{3i+2} + 2 b_{3i+1} + b_{3i} + b_{3i-1}
+__device__ __forceinline__ int ec_booth(uint32_t nib) {
+    return (int)(((nib >> 1) & 2u) + ((nib >> 1) & 1u) + (nib & 1u)) - 4 * (int)(nib >> 3);
+}
+
+// s = m << 31 over 160 bits: window 42's nibble (b_128 .. b_125) is the top of s[4], and
+// each window after it comes up by a shift of 3
+__device__ __forceinline__ void ec_booth_init(const uint32_t (&m)[4], uint32_t (&s)[5]) {
+    s[0] = m[0] << 31;
+    s[1] = (m[1] << 31) | (m[0] >> 1);
+    s[2] = (m[2] << 31) | (m[1] >> 1);
+    s[3] = (m[3] << 31) | (m[2] >> 1);
+    s[4] = m[3] >> 1;
+}
+
+__device__ __forceinline__ uint32_t ec_booth_next(uint32_t (&s)[5]) {
+    const uint32_t nib = s[4] >> 28;
+#pragma unroll
+    for (int i = 4; i > 0; i--) s[i] = (s[i] << 3) | (s[i - 1] >> 29);
+    s[0] <<= 3;
+    return nib;
 }
 
 __device__ __forceinline__ x29 x29_mul_scalar(const x29& p, const fr& k, uint32_t* tab) {
     uint32_t m1[4], m2[4];
     bool n1, n2;
     glv_split(k, m1, &n1, m2, &n2);
-    ec_tab_store(tab, 0, p);
-    const x29 p2 = x29_dbl(p);
-    ec_tab_store(tab, 1, p2);
-    ec_tab_store(tab, 2, x29_add(p2, p));
+    // P, 2P, 3P, 4P, made affine with one inversion of the product of their ZZ ZZZ
+    const x29 p2 = x29_dbl(p), p3 = x29_add(p2, p), p4 = x29_dbl(p2);
+    const fq29 d0 = q_mul(p.ZZ, p.ZZZ), d1 = q_mul(p2.ZZ, p2.ZZZ), d2 = q_mul(p3.ZZ, p3.ZZZ),
+               d3 = q_mul(p4.ZZ, p4.ZZZ);
+    const fq29 c1 = q_mul(d0, d1), c2 = q_mul(c1, d2);
+    fq29 inv = q_inv(q_mul(c2, d3));  // 0 when P is infinity (then the result is replaced)
+    const fq29 i3 = q_mul(inv, c2);
+    inv = q_mul(inv, d3);
+    const fq29 i2 = q_mul(inv, c1);
+    inv = q_mul(inv, d2);
+    const fq29 i1 = q_mul(inv, d0), i0 = q_mul(inv, d1);
+    // 1 / (ZZ ZZZ) = i: x = X ZZZ i, y = Y ZZ i
+    ec_tab_store(tab, 0, q_mul(q_mul(p.X, p.ZZZ), i0), q_mul(q_mul(p.Y, p.ZZ), i0));
+    ec_tab_store(tab, 1, q_mul(q_mul(p2.X, p2.ZZZ), i1), q_mul(q_mul(p2.Y, p2.ZZ), i1));
+    ec_tab_store(tab, 2, q_mul(q_mul(p3.X, p3.ZZZ), i2), q_mul(q_mul(p3.Y, p3.ZZ), i2));
+    ec_tab_store(tab, 3, q_mul(q_mul(p4.X, p4.ZZZ), i3), q_mul(q_mul(p4.Y, p4.ZZ), i3));
     // r = |k1| P + (+-)|k2| phi(P), the second half negated when the signs differ; the
     // sign of k1 is applied at the end
     const bool flip = n1 != n2;
     const fq29 beta = q_const(Glv::BETA29);
+    uint32_t s1[5], s2[5];
+    ec_booth_init(m1, s1);
+    ec_booth_init(m2, s2);
     x29 r = x29_inf();
-#pragma unroll
-    for (int w = 3; w >= 0; w--) {  // magnitude words by constant index: no private array
-        const uint32_t a = m1[w], b = m2[w];
-        for (int j = 15; j >= 0; j--) {  // bits 2j, 2j + 1 of word w
-            r = x29_dbl(x29_dbl(r));
-            const uint32_t d1 = (a >> (2 * j)) & 3, d2 = (b >> (2 * j)) & 3;
-            const x29 t1 = ec_tab_load(tab, d1 ? (int)d1 - 1 : 0);
-            r = x29_select(d1 != 0, x29_add(r, t1), r);
-            x29 t2 = ec_tab_load(tab, d2 ? (int)d2 - 1 : 0);
-            t2.X = q_mul(t2.X, beta);
-            if (flip) t2.Y = q_sub(q_zero(), t2.Y);
-            r = x29_select(d2 != 0, x29_add(r, t2), r);
-        }
+    for (int w = 0; w < 43; w++) {
+        r = x29_dbl(x29_dbl(x29_dbl(r)));
+        const int e1 = ec_booth(ec_booth_next(s1)), e2 = ec_booth(ec_booth_next(s2));
+        a29 t1 = ec_tab_load(tab, e1 ? (e1 < 0 ? -e1 : e1) - 1 : 0);
+        if (e1 < 0) t1.y = q_neg(t1.y);
+        r = x29_select(e1 != 0, x29_madd(r, t1), r);
+        a29 t2 = ec_tab_load(tab, e2 ? (e2 < 0 ? -e2 : e2) - 1 : 0);
+        t2.x = q_mul(t2.x, beta);
+        if ((e2 < 0) != flip) t2.y = q_neg(t2.y);
+        r = x29_select(e2 != 0, x29_madd(r, t2), r);
     }
     if (n1) r.Y = q_sub(q_zero(), r.Y);
-    return r;
+    return x29_select(x29_is_inf(p), x29_inf(), r);
 }
 
 // one radix-2 DIT stage: (u, v) -> (u + w v, u - w v), w = tw[j << tw_shift]
