@@ -842,6 +842,72 @@ __global__ __launch_bounds__(FW_SORT_T) void k_fw_scatter(const uint32_t* y, int
     }
 }
 
+// ---- one MSM of arbitrary bases (zgr_msm) through the same sort and bucket kernels ----
+// The FW_W windows of a scalar are FW_W virtual scalars of one digit each: sort block blk
+// holds window blk >> sh over scalars [(blk mod 2^sh) ns, + ns), so each window's entries
+// form one group of the wide sort (bucket runs by (|digit|, window)), and an entry's
+// point index is the base's own index: no shifted copies.  The bucket kernels then give
+// one sum S_w per window, and k_pm_horner returns sum_w 2^(16 w) S_w.
+__device__ __forceinline__ int fwp_digit(const uint32_t* y, uint32_t i, uint32_t win) {
+    int dig[FW_W];
+    fb_digits<FW_C, FW_W>(y, 0, 0, i, dig);
+    int d = 0;
+#pragma unroll
+    for (int j = 0; j < FW_W; j++) d = j == (int)win ? dig[j] : d;  // no private array
+    return d;
+}
+
+__global__ __launch_bounds__(FW_SORT_T) void k_fwp_count(const uint32_t* y, int sh, uint32_t ns, uint32_t* cnt) {
+    extern __shared__ uint32_t hist[];  // FW_BUCKETS
+    const uint32_t blk = blockIdx.x, t = threadIdx.x;
+    for (uint32_t q = t; q < FW_BUCKETS; q += FW_SORT_T) hist[q] = 0;
+    __syncthreads();
+    const uint32_t win = blk >> sh, first = (blk & ((1u << sh) - 1)) * ns;
+    for (uint32_t u = t; u < ns; u += FW_SORT_T) {
+        const int d = fwp_digit(y, first + u, win);
+        if (d) atomicAdd(&hist[(d < 0 ? -d : d) - 1], 1u);
+    }
+    __syncthreads();
+    for (uint32_t q = t; q < FW_BUCKETS; q += FW_SORT_T) cnt[(size_t)blk * FW_BUCKETS + q] = hist[q];
+}
+
+__global__ __launch_bounds__(FW_SORT_T) void k_fwp_scatter(const uint32_t* y, int sh, uint32_t ns, const uint32_t* off,
+                                                           uint2* ent) {
+    extern __shared__ uint32_t cursor[];  // FW_BUCKETS
+    const uint32_t blk = blockIdx.x, t = threadIdx.x;
+    for (uint32_t q = t; q < FW_BUCKETS; q += FW_SORT_T) cursor[q] = off[(size_t)blk * FW_BUCKETS + q];
+    __syncthreads();
+    const uint32_t win = blk >> sh, first = (blk & ((1u << sh) - 1)) * ns;
+    for (uint32_t u = t; u < ns; u += FW_SORT_T) {
+        const int d = fwp_digit(y, first + u, win);
+        if (!d) continue;
+        const uint32_t mag = d < 0 ? -d : d;
+        const uint32_t pos = atomicAdd(&cursor[mag - 1], 1u);
+        ent[pos] = make_uint2((first + u) | ((uint32_t)(d < 0) << 31), (mag << 16) | blk);
+    }
+}
+
+// bases (x_mont || y_mont, 2^256 form) -> the table form of the bucket kernels (x || y as
+// 256-bit integers of the 2^261 form), in place, one thread per point
+__global__ __launch_bounds__(256) void k_pm_table(uint32_t* pts, uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const g1_aff a = aff_load(pts + 16 * (size_t)i);
+    fb_store29(pts + 16 * (size_t)i, q_canon(q_from_fq(a.x)));
+    fb_store29(pts + 16 * (size_t)i + 8, q_canon(q_from_fq(a.y)));
+}
+
+// sum_w 2^(FW_C w) S_w from the window sums at heap nodes 1 .. FW_W, canonical affine
+__global__ __launch_bounds__(64) void k_pm_horner(const uint32_t* heap, uint32_t* out) {
+    if (threadIdx.x) return;
+    x29 r = x29_load(heap + FW_W * X29_WORDS);
+    for (int w = FW_W - 2; w >= 0; w--) {
+        for (int k = 0; k < FW_C; k++) r = x29_dbl(r);
+        r = x29_add(r, x29_load(heap + (1 + w) * X29_WORDS));
+    }
+    x29_store_canon(out, r);
+}
+
 // first entry of (bucket m0 + 1, block blk) in the (m, blk) order; blk == nblk is the
 // next bucket's first block, and past the last bucket the entry count
 __device__ __forceinline__ uint32_t fw_pos(const FbWide& wd, uint32_t m0, uint32_t blk) {

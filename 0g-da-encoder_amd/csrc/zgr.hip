@@ -237,6 +237,18 @@ struct zgr_ctx {
     // zgr_test_inject_fault: countdown to an injected failure per site (0: none); atomic
     // so that it can be armed while a server owns the context
     std::atomic<int> fault[ZGR_FAULT_SITES] = {};
+    // zgr_msm's buffers for the sort and bucket kernels (allocated on its first call):
+    // entries, counts and offsets per (sort block, bucket), scan tiles, segment bounds,
+    // FW_W x FW_BUCKETS bucket slots, reduction partials, hand-off partials, window sums,
+    // and the work lists (sums waves, then reduction items)
+    struct {
+        uint2* ent = nullptr;
+        uint32_t *cnt = nullptr, *off = nullptr, *tile = nullptr, *seg = nullptr, *bkt = nullptr, *part = nullptr,
+                 *hand = nullptr, *heap = nullptr;
+        uint2* work = nullptr;
+        std::vector<uint2> host_work;
+        bool lds_ok = false;  // the sort kernels' 128 KB of LDS were granted
+    } pm;
 
     // domain tables
     uint32_t* consts = nullptr;   // K_COUNT Fr
@@ -1350,6 +1362,11 @@ int zgr_ctx_create(int device, int log_col, int log_row, int coset_n, zgr_ctx** 
         if (ntt_lds > (size_t)max_shm)
             return fail(ZGR_ERR_INVALID_ARG, "NTT pass needs %zu B of LDS, device allows %d", ntt_lds, max_shm);
         HIPCHK(hipFuncSetAttribute((const void*)k_ntt_pass, hipFuncAttributeMaxDynamicSharedMemorySize, (int)ntt_lds));
+        if (FW_BUCKETS * 4 <= max_shm) {  // zgr_msm's sort (otherwise it keeps the windowed kernels)
+            for (const void* k : {(const void*)k_fwp_count, (const void*)k_fwp_scatter})
+                HIPCHK(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, FW_BUCKETS * 4));
+            c->pm.lds_ok = true;
+        }
         if (!c->wide_lines.empty()) {  // the wide sort's LDS counters / cursors
             if (FW_BUCKETS * 4 > max_shm) c->wide_lines.clear();
             else
@@ -1470,6 +1487,9 @@ void zgr_ctx_destroy(zgr_ctx* c) {
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     for (void* p : c->rec_buf)
+        if (p) (void)hipFree(p);
+    for (void* p : {(void*)c->pm.ent, (void*)c->pm.cnt, (void*)c->pm.off, (void*)c->pm.tile, (void*)c->pm.seg,
+                    (void*)c->pm.bkt, (void*)c->pm.part, (void*)c->pm.hand, (void*)c->pm.heap, (void*)c->pm.work})
         if (p) (void)hipFree(p);
     for (void* p : {(void*)c->pt_g1, (void*)c->pt_high, (void*)c->pt_g2, (void*)c->pt_high_g2, (void*)c->g1tab,
                     (void*)c->g2tab})
@@ -2223,6 +2243,82 @@ int zgr_coset_extend(zgr_ctx* c, const uint8_t* data_mont, int coset, uint8_t* o
 
 // Windowed MSM sum_i s_i P_i over 2^log_np device-resident points (param layout) with
 // the scalars (Montgomery) already in ln.y; the canonical affine result goes to out.
+// zgr_msm through the wide sort and bucket kernels: the FW_W windows of each scalar as
+// virtual scalars of one digit (k_fwp_count / k_fwp_scatter), one group per window,
+// bucket-segment sums waves (k_fb_sums), the groups' segmented reductions and combine,
+// then Horner over the windows (k_pm_horner).  dbases (np points, zero-padded) is
+// converted to the table form in place.  Returns 1 without launching anything when the
+// device cannot give the sort its LDS; the caller then runs the windowed kernels.
+int msm_plain(zgr_ctx* c, Lane& ln, uint32_t* dbases, int log_np, uint32_t* out) {
+    if (!c->pm.lds_ok) return 1;
+    const uint32_t np = 1u << log_np;
+    const uint32_t ns = std::min(np, 1u << 16);  // virtual scalars per sort block
+    const uint32_t bpw = np / ns;                 // sort blocks per window
+    int sh = 0;
+    while ((1u << sh) < bpw) sh++;
+    const uint32_t nblk = FW_W * bpw;
+    const uint32_t nseg = std::max(1u, std::min(1024u, (uint32_t)((uint64_t)FW_W * np / 16384)));
+    auto& pm = c->pm;
+    if (!pm.ent) {  // sized for the context's largest MSM
+        const size_t blk_max = (size_t)FW_W * std::max<size_t>(1, c->n >> 16);
+        DALLOC(pm.ent, (size_t)FW_W * c->n * sizeof(uint2));
+        DALLOC(pm.cnt, blk_max * FW_BUCKETS * 4);
+        DALLOC(pm.off, blk_max * FW_BUCKETS * 4);
+        DALLOC(pm.tile, (2 * FW_TILES + 1) * 4);
+        DALLOC(pm.seg, (1024 + 1) * 4);
+        DALLOC(pm.bkt, (size_t)FW_W * FW_BUCKETS * X29_WORDS * 4);
+        DALLOC(pm.part, (size_t)FW_W * FW_RSEGS * X29_WORDS * 4);
+        DALLOC(pm.hand, (size_t)1024 * 2 * FB_LANES * X29_WORDS * 4);
+        DALLOC(pm.heap, (size_t)(FW_W + 1) * X29_WORDS * 4);
+        DALLOC(pm.work, (1024 + FW_W * FW_RSEGS) * sizeof(uint2));
+    }
+    FbWide wd{};
+    wd.ent = pm.ent;
+    wd.off = pm.off;
+    wd.tot = pm.tile + 2 * FW_TILES;
+    wd.seg = pm.seg;
+    wd.table = dbases;
+    wd.bkt = pm.bkt;
+    wd.hand = pm.hand;
+    wd.part = pm.part;
+    wd.nblk = nblk;
+    wd.nw = 1;
+    wd.segbase[0] = 0;
+    wd.segbase[1] = nseg;
+    wd.gbase[0] = 0;
+    wd.gbase[1] = FW_W;
+    wd.line[0] = 0;
+    wd.sh[0] = (uint32_t)sh;
+    // work lists: the sums waves (wide line 0, segment s, hand-off slot s), then the
+    // reduction items (window S, segment q)
+    pm.host_work.clear();
+    for (uint32_t sg = 0; sg < nseg; sg++) pm.host_work.push_back(make_uint2(FB_WIDE | (sg << 8), sg));
+    for (uint32_t S = 0; S < (uint32_t)FW_W; S++)
+        for (uint32_t q = 0; q < (uint32_t)FW_RSEGS; q++) pm.host_work.push_back(make_uint2(FB_WIDE | (q << 4) | (S << 8), 0));
+    HIPCHK(hipMemcpyAsync(pm.work, pm.host_work.data(), pm.host_work.size() * sizeof(uint2), hipMemcpyHostToDevice,
+                          ln.st));
+    const size_t shm = FW_BUCKETS * 4;
+    const uint32_t nred = (uint32_t)(FW_W * FW_RSEGS);
+    LAUNCH(c, ln.st, "msm_pm_table", k_pm_table, blocks(np, 256), 256, 0, dbases, np);
+    LAUNCH(c, ln.st, "msm_wide_sort", k_fwp_count, nblk, FW_SORT_T, shm, ln.y, sh, ns, pm.cnt);
+    LAUNCH(c, ln.st, "msm_wide_sort", k_fw_tile_sums, FW_TILES, FW_TILE * FW_PARTS, 0, pm.cnt, nblk, pm.tile);
+    LAUNCH(c, ln.st, "msm_wide_sort", k_fw_tile_scan, 1, FW_TILES, 0, pm.tile, pm.tile + FW_TILES,
+           pm.tile + 2 * FW_TILES);
+    LAUNCH(c, ln.st, "msm_wide_sort", k_fw_offsets, FW_TILES, FW_TILE * FW_PARTS, 0, pm.cnt, nblk,
+           pm.tile + FW_TILES, pm.off);
+    LAUNCH(c, ln.st, "msm_wide_sort", k_fwp_scatter, nblk, FW_SORT_T, shm, ln.y, sh, ns, pm.off, pm.ent);
+    LAUNCH(c, ln.st, "msm_wide_sort", k_fw_segments, blocks(nseg + 1, 64), 64, 0, wd, pm.seg);
+    LAUNCH(c, ln.st, "msm_accum", k_fb_sums, nseg, FB_LANES, 0, (const uint32_t*)dbases, FbLevels{}, wd, pm.work,
+           ln.bkt, c->log_n, c->h, 1);
+    // the groups' node level: heap node (2^h >> gl) + S with h = gl is node 1 + S
+    const int gl = sh + FW_BLK_LOG;
+    LAUNCH(c, ln.st, "msm_reduce", k_fb_reduce<64>, nred, FB_LANES, 0, FbLevels{}, wd, pm.work + nseg, nred, ln.bkt,
+           pm.heap, gl, 1);
+    LAUNCH(c, ln.st, "msm_combine", k_fw_combine, (uint32_t)FW_W, FB_LANES, 0, wd, pm.heap, gl);
+    LAUNCH(c, ln.st, "msm_root", k_pm_horner, 1, 64, 0, pm.heap, out);
+    return ZGR_OK;
+}
+
 int msm_windowed(zgr_ctx* c, Lane& ln, const uint32_t* dbases, int log_np, uint32_t* out) {
     const int log_bs = log_np < 10 ? log_np : 10;
     const int h = log_np - log_bs;
@@ -2292,7 +2388,12 @@ int zgr_msm(zgr_ctx* c, const uint8_t* bases_mont, const uint8_t* scalars_mont, 
         HIPCHK(hipMemcpyAsync(ln.y, scalars_mont, n * 32, hipMemcpyHostToDevice, ln.st));
         HIPCHK(hipMemcpyAsync(dbases, bases_mont, n * 64, hipMemcpyHostToDevice, ln.st));
         uint32_t* opts = c->slots[0].outpts;
-        CHECK(msm_windowed(c, ln, dbases, log_np, opts));
+        // ZGR_MSM=windowed keeps the per-chunk windowed kernels (same result)
+        const char* mode = getenv("ZGR_MSM");
+        int rc = 1;  // 1: the windowed kernels
+        if (!(mode && !strcmp(mode, "windowed"))) rc = msm_plain(c, ln, dbases, log_np, opts);
+        if (rc < 0) return rc;
+        if (rc == 1) CHECK(msm_windowed(c, ln, dbases, log_np, opts));
         HIPCHK(hipMemcpyAsync(out_point, opts, 64, hipMemcpyDeviceToHost, ln.st));
         return sync_all(c);
     };
