@@ -842,72 +842,6 @@ __global__ __launch_bounds__(FW_SORT_T) void k_fw_scatter(const uint32_t* y, int
     }
 }
 
-// ---- one MSM of arbitrary bases (zgr_msm) through the same sort and bucket kernels ----
-// The FW_W windows of a scalar are FW_W virtual scalars of one digit each: sort block blk
-// holds window blk >> sh over scalars [(blk mod 2^sh) ns, + ns), so each window's entries
-// form one group of the wide sort (bucket runs by (|digit|, window)), and an entry's
-// point index is the base's own index: no shifted copies.  The bucket kernels then give
-// one sum S_w per window, and k_pm_horner returns sum_w 2^(16 w) S_w.
-__device__ __forceinline__ int fwp_digit(const uint32_t* y, uint32_t i, uint32_t win) {
-    int dig[FW_W];
-    fb_digits<FW_C, FW_W>(y, 0, 0, i, dig);
-    int d = 0;
-#pragma unroll
-    for (int j = 0; j < FW_W; j++) d = j == (int)win ? dig[j] : d;  // no private array
-    return d;
-}
-
-__global__ __launch_bounds__(FW_SORT_T) void k_fwp_count(const uint32_t* y, int sh, uint32_t ns, uint32_t* cnt) {
-    extern __shared__ uint32_t hist[];  // FW_BUCKETS
-    const uint32_t blk = blockIdx.x, t = threadIdx.x;
-    for (uint32_t q = t; q < FW_BUCKETS; q += FW_SORT_T) hist[q] = 0;
-    __syncthreads();
-    const uint32_t win = blk >> sh, first = (blk & ((1u << sh) - 1)) * ns;
-    for (uint32_t u = t; u < ns; u += FW_SORT_T) {
-        const int d = fwp_digit(y, first + u, win);
-        if (d) atomicAdd(&hist[(d < 0 ? -d : d) - 1], 1u);
-    }
-    __syncthreads();
-    for (uint32_t q = t; q < FW_BUCKETS; q += FW_SORT_T) cnt[(size_t)blk * FW_BUCKETS + q] = hist[q];
-}
-
-__global__ __launch_bounds__(FW_SORT_T) void k_fwp_scatter(const uint32_t* y, int sh, uint32_t ns, const uint32_t* off,
-                                                           uint2* ent) {
-    extern __shared__ uint32_t cursor[];  // FW_BUCKETS
-    const uint32_t blk = blockIdx.x, t = threadIdx.x;
-    for (uint32_t q = t; q < FW_BUCKETS; q += FW_SORT_T) cursor[q] = off[(size_t)blk * FW_BUCKETS + q];
-    __syncthreads();
-    const uint32_t win = blk >> sh, first = (blk & ((1u << sh) - 1)) * ns;
-    for (uint32_t u = t; u < ns; u += FW_SORT_T) {
-        const int d = fwp_digit(y, first + u, win);
-        if (!d) continue;
-        const uint32_t mag = d < 0 ? -d : d;
-        const uint32_t pos = atomicAdd(&cursor[mag - 1], 1u);
-        ent[pos] = make_uint2((first + u) | ((uint32_t)(d < 0) << 31), (mag << 16) | blk);
-    }
-}
-
-// bases (x_mont || y_mont, 2^256 form) -> the table form of the bucket kernels (x || y as
-// 256-bit integers of the 2^261 form), in place, one thread per point
-__global__ __launch_bounds__(256) void k_pm_table(uint32_t* pts, uint32_t n) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const g1_aff a = aff_load(pts + 16 * (size_t)i);
-    fb_store29(pts + 16 * (size_t)i, q_canon(q_from_fq(a.x)));
-    fb_store29(pts + 16 * (size_t)i + 8, q_canon(q_from_fq(a.y)));
-}
-
-// sum_w 2^(FW_C w) S_w from the window sums at heap nodes 1 .. FW_W, canonical affine
-__global__ __launch_bounds__(64) void k_pm_horner(const uint32_t* heap, uint32_t* out) {
-    if (threadIdx.x) return;
-    x29 r = x29_load(heap + FW_W * X29_WORDS);
-    for (int w = FW_W - 2; w >= 0; w--) {
-        for (int k = 0; k < FW_C; k++) r = x29_dbl(r);
-        r = x29_add(r, x29_load(heap + (1 + w) * X29_WORDS));
-    }
-    x29_store_canon(out, r);
-}
-
 // first entry of (bucket m0 + 1, block blk) in the (m, blk) order; blk == nblk is the
 // next bucket's first block, and past the last bucket the entry count
 __device__ __forceinline__ uint32_t fw_pos(const FbWide& wd, uint32_t m0, uint32_t blk) {
@@ -1592,6 +1526,116 @@ __global__ __launch_bounds__(64) void k_params_gen(uint32_t* bases, const uint32
         if (inf) *bad = 1;
         aff_store(bases + (((size_t)line << log_n) + pos) * 16, a);
     }
+}
+
+// ---- one MSM of arbitrary bases (zgr_msm) through the wide sort and bucket kernels ----
+// Each scalar is split by GLV (glv_split: k = +-k1 +- lambda k2, |k1|, |k2| < 2^127), so
+// the MSM is one of 2n points -- P_i, then phi(P_i) -- with 127-bit scalars: 8 signed
+// 16-bit windows.  Each window of each point is a virtual scalar of one digit: sort block
+// blk holds window blk >> sh over points [(blk mod 2^sh) ns, + ns), so a window's entries
+// form one group of the wide sort (bucket runs by (|digit|, window)) and an entry's point
+// index is its table row: no shifted copies.  The bucket kernels give one sum S_w per
+// window, and k_pm_horner returns sum_w 2^(16 w) S_w: 112 doublings on one lane, the
+// latency that halving the windows by GLV halves.
+constexpr int PM_W = 8;           // 16-bit windows of a 127-bit half
+constexpr int PM_SPLIT_WORDS = 9;  // per scalar: |k1| (4 words), |k2| (4 words), signs
+
+// |k1|, |k2| and their signs of every scalar (Montgomery in), one thread per scalar
+__global__ __launch_bounds__(256) void k_pm_split(const uint32_t* y, uint32_t n, uint32_t* halves) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t m1[4], m2[4];
+    bool n1, n2;
+    glv_split(f_from_mont(f_load<FrP>(y + 8 * (size_t)i)), m1, &n1, m2, &n2);
+    uint32_t* o = halves + (size_t)PM_SPLIT_WORDS * i;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        o[k] = m1[k];
+        o[4 + k] = m2[k];
+    }
+    o[8] = (uint32_t)n1 | ((uint32_t)n2 << 1);
+}
+
+// signed digit of window w of virtual point j (j < n: |k1| of scalar j, else |k2| of
+// scalar j - n) in the Booth form -2^15 b_{16w+15} + sum_{k<15} 2^k b_{16w+k} + b_{16w-1},
+// |digit| <= 2^15, with the half's sign folded in; no carry chain across the windows
+__device__ __forceinline__ int pm_digit(const uint32_t* halves, uint32_t n, uint32_t j, uint32_t w) {
+    const uint32_t half = j >= n ? 1u : 0u, i = j - half * n;
+    const uint32_t* o = halves + (size_t)PM_SPLIT_WORDS * i + 4 * half;
+    const uint32_t word = o[w >> 1], v = (w & 1) ? word >> 16 : word & 0xffffu;
+    uint32_t low = 0;  // bit 16 w - 1
+    if (w) low = (w & 1) ? (word >> 15) & 1u : o[(w >> 1) - 1] >> 31;
+    int d = (int)(v & 0x7fffu) - (int)(v & 0x8000u) + (int)low;
+    const bool neg = (o[8 - 4 * half] >> half) & 1u;
+    return neg ? -d : d;
+}
+
+__global__ __launch_bounds__(FW_SORT_T) void k_fwp_count(const uint32_t* halves, uint32_t n, int sh, uint32_t ns,
+                                                         uint32_t* cnt) {
+    extern __shared__ uint32_t hist[];  // FW_BUCKETS
+    const uint32_t blk = blockIdx.x, t = threadIdx.x;
+    for (uint32_t q = t; q < FW_BUCKETS; q += FW_SORT_T) hist[q] = 0;
+    __syncthreads();
+    const uint32_t win = blk >> sh, first = (blk & ((1u << sh) - 1)) * ns;
+    for (uint32_t u = t; u < ns; u += FW_SORT_T) {
+        const int d = pm_digit(halves, n, first + u, win);
+        if (d) atomicAdd(&hist[(d < 0 ? -d : d) - 1], 1u);
+    }
+    __syncthreads();
+    for (uint32_t q = t; q < FW_BUCKETS; q += FW_SORT_T) cnt[(size_t)blk * FW_BUCKETS + q] = hist[q];
+}
+
+__global__ __launch_bounds__(FW_SORT_T) void k_fwp_scatter(const uint32_t* halves, uint32_t n, int sh, uint32_t ns,
+                                                           const uint32_t* off, uint2* ent) {
+    extern __shared__ uint32_t cursor[];  // FW_BUCKETS
+    const uint32_t blk = blockIdx.x, t = threadIdx.x;
+    for (uint32_t q = t; q < FW_BUCKETS; q += FW_SORT_T) cursor[q] = off[(size_t)blk * FW_BUCKETS + q];
+    __syncthreads();
+    const uint32_t win = blk >> sh, first = (blk & ((1u << sh) - 1)) * ns;
+    for (uint32_t u = t; u < ns; u += FW_SORT_T) {
+        const int d = pm_digit(halves, n, first + u, win);
+        if (!d) continue;
+        const uint32_t mag = d < 0 ? -d : d;
+        const uint32_t pos = atomicAdd(&cursor[mag - 1], 1u);
+        ent[pos] = make_uint2((first + u) | ((uint32_t)(d < 0) << 31), (mag << 16) | blk);
+    }
+}
+
+// A base of all zeros is the point at infinity (the oracle's and the reference's reading
+// of (0, 0)); the bucket kernels add affine points only, so its scalar is cleared first
+// and the base is never added.  One thread per point, scalars Montgomery (8 words).
+__global__ __launch_bounds__(256) void k_msm_clear_inf(const uint32_t* bases, uint32_t n, uint32_t* y) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const g1_aff a = aff_load(bases + 16 * (size_t)i);
+    if (f_is_zero(a.x) && f_is_zero(a.y)) {
+#pragma unroll
+        for (int k = 0; k < 8; k++) y[8 * (size_t)i + k] = 0;
+    }
+}
+
+// bases (x_mont || y_mont, 2^256 form) -> the table rows of the bucket kernels (x || y as
+// 256-bit integers of the 2^261 form): row i = P_i, row n + i = phi(P_i) = (beta x, y)
+__global__ __launch_bounds__(256) void k_pm_table(const uint32_t* bases, uint32_t n, uint32_t* table) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const g1_aff a = aff_load(bases + 16 * (size_t)i);
+    const fq29 x = q_from_fq(a.x), y = q_canon(q_from_fq(a.y));
+    fb_store29(table + 16 * (size_t)i, q_canon(x));
+    fb_store29(table + 16 * (size_t)i + 8, y);
+    fb_store29(table + 16 * ((size_t)n + i), q_canon(q_mul(x, q_const(Glv::BETA29))));
+    fb_store29(table + 16 * ((size_t)n + i) + 8, y);
+}
+
+// sum_w 2^(16 w) S_w from the window sums at heap nodes 1 .. PM_W, canonical affine
+__global__ __launch_bounds__(64) void k_pm_horner(const uint32_t* heap, uint32_t* out) {
+    if (threadIdx.x) return;
+    x29 r = x29_load(heap + PM_W * X29_WORDS);
+    for (int w = PM_W - 2; w >= 0; w--) {
+        for (int k = 0; k < FW_C; k++) r = x29_dbl(r);
+        r = x29_add(r, x29_load(heap + (1 + w) * X29_WORDS));
+    }
+    x29_store_canon(out, r);
 }
 
 }  // namespace zgr

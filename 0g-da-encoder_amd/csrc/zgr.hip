@@ -239,13 +239,16 @@ struct zgr_ctx {
     std::atomic<int> fault[ZGR_FAULT_SITES] = {};
     // zgr_msm's buffers for the sort and bucket kernels (allocated on its first call):
     // entries, counts and offsets per (sort block, bucket), scan tiles, segment bounds,
-    // FW_W x FW_BUCKETS bucket slots, reduction partials, hand-off partials, window sums,
-    // and the work lists (sums waves, then reduction items)
+    // PM_W x FW_BUCKETS bucket slots, reduction partials, hand-off partials, window sums,
+    // the work lists (sums waves, then reduction items), the scalars' GLV halves and the
+    // table rows of P_i and phi(P_i)
     struct {
         uint2* ent = nullptr;
         uint32_t *cnt = nullptr, *off = nullptr, *tile = nullptr, *seg = nullptr, *bkt = nullptr, *part = nullptr,
                  *hand = nullptr, *heap = nullptr;
         uint2* work = nullptr;
+        uint32_t *halves = nullptr, *table = nullptr;  // GLV halves of the scalars; P_i and phi(P_i) rows
+        uint32_t* bases = nullptr;                      // the uploaded bases (any zgr_msm path)
         std::vector<uint2> host_work;
         bool lds_ok = false;  // the sort kernels' 128 KB of LDS were granted
     } pm;
@@ -1489,7 +1492,8 @@ void zgr_ctx_destroy(zgr_ctx* c) {
     for (void* p : c->rec_buf)
         if (p) (void)hipFree(p);
     for (void* p : {(void*)c->pm.ent, (void*)c->pm.cnt, (void*)c->pm.off, (void*)c->pm.tile, (void*)c->pm.seg,
-                    (void*)c->pm.bkt, (void*)c->pm.part, (void*)c->pm.hand, (void*)c->pm.heap, (void*)c->pm.work})
+                    (void*)c->pm.bkt, (void*)c->pm.part, (void*)c->pm.hand, (void*)c->pm.heap, (void*)c->pm.work,
+                    (void*)c->pm.halves, (void*)c->pm.table, (void*)c->pm.bases})
         if (p) (void)hipFree(p);
     for (void* p : {(void*)c->pt_g1, (void*)c->pt_high, (void*)c->pt_g2, (void*)c->pt_high_g2, (void*)c->g1tab,
                     (void*)c->g2tab})
@@ -2243,41 +2247,44 @@ int zgr_coset_extend(zgr_ctx* c, const uint8_t* data_mont, int coset, uint8_t* o
 
 // Windowed MSM sum_i s_i P_i over 2^log_np device-resident points (param layout) with
 // the scalars (Montgomery) already in ln.y; the canonical affine result goes to out.
-// zgr_msm through the wide sort and bucket kernels: the FW_W windows of each scalar as
-// virtual scalars of one digit (k_fwp_count / k_fwp_scatter), one group per window,
-// bucket-segment sums waves (k_fb_sums), the groups' segmented reductions and combine,
-// then Horner over the windows (k_pm_horner).  dbases (np points, zero-padded) is
-// converted to the table form in place.  Returns 1 without launching anything when the
-// device cannot give the sort its LDS; the caller then runs the windowed kernels.
-int msm_plain(zgr_ctx* c, Lane& ln, uint32_t* dbases, int log_np, uint32_t* out) {
+// zgr_msm through the wide sort and bucket kernels: each scalar split by GLV into two
+// 127-bit halves over P_i and phi(P_i) (k_pm_split, k_pm_table), the PM_W windows of the
+// 2n halves as virtual scalars of one digit (k_fwp_count / k_fwp_scatter), one group per
+// window, bucket-segment sums waves (k_fb_sums), the groups' segmented reductions and
+// combine, then Horner over the windows (k_pm_horner).  Returns 1 without launching
+// anything when the device cannot give the sort its LDS; the caller then runs the
+// windowed kernels.
+int msm_plain(zgr_ctx* c, Lane& ln, const uint32_t* dbases, int log_np, uint32_t* out) {
     if (!c->pm.lds_ok) return 1;
-    const uint32_t np = 1u << log_np;
-    const uint32_t ns = std::min(np, 1u << 16);  // virtual scalars per sort block
-    const uint32_t bpw = np / ns;                 // sort blocks per window
+    const uint32_t np = 1u << log_np, nv = 2 * np;  // points, virtual points (P, phi(P))
+    const uint32_t ns = std::min(nv, 1u << 16);      // virtual points per sort block
+    const uint32_t bpw = nv / ns;                    // sort blocks per window
     int sh = 0;
     while ((1u << sh) < bpw) sh++;
-    const uint32_t nblk = FW_W * bpw;
-    const uint32_t nseg = std::max(1u, std::min(1024u, (uint32_t)((uint64_t)FW_W * np / 16384)));
+    const uint32_t nblk = PM_W * bpw;
+    const uint32_t nseg = std::max(1u, std::min(1024u, (uint32_t)((uint64_t)PM_W * nv / 16384)));
     auto& pm = c->pm;
     if (!pm.ent) {  // sized for the context's largest MSM
-        const size_t blk_max = (size_t)FW_W * std::max<size_t>(1, c->n >> 16);
-        DALLOC(pm.ent, (size_t)FW_W * c->n * sizeof(uint2));
+        const size_t blk_max = (size_t)PM_W * std::max<size_t>(1, (2 * c->n) >> 16);
+        DALLOC(pm.ent, (size_t)PM_W * 2 * c->n * sizeof(uint2));
         DALLOC(pm.cnt, blk_max * FW_BUCKETS * 4);
         DALLOC(pm.off, blk_max * FW_BUCKETS * 4);
         DALLOC(pm.tile, (2 * FW_TILES + 1) * 4);
         DALLOC(pm.seg, (1024 + 1) * 4);
-        DALLOC(pm.bkt, (size_t)FW_W * FW_BUCKETS * X29_WORDS * 4);
-        DALLOC(pm.part, (size_t)FW_W * FW_RSEGS * X29_WORDS * 4);
+        DALLOC(pm.bkt, (size_t)PM_W * FW_BUCKETS * X29_WORDS * 4);
+        DALLOC(pm.part, (size_t)PM_W * FW_RSEGS * X29_WORDS * 4);
         DALLOC(pm.hand, (size_t)1024 * 2 * FB_LANES * X29_WORDS * 4);
-        DALLOC(pm.heap, (size_t)(FW_W + 1) * X29_WORDS * 4);
-        DALLOC(pm.work, (1024 + FW_W * FW_RSEGS) * sizeof(uint2));
+        DALLOC(pm.heap, (size_t)(PM_W + 1) * X29_WORDS * 4);
+        DALLOC(pm.work, (1024 + PM_W * FW_RSEGS) * sizeof(uint2));
+        DALLOC(pm.halves, (size_t)PM_SPLIT_WORDS * c->n * 4);
+        DALLOC(pm.table, (size_t)2 * c->n * 64);
     }
     FbWide wd{};
     wd.ent = pm.ent;
     wd.off = pm.off;
     wd.tot = pm.tile + 2 * FW_TILES;
     wd.seg = pm.seg;
-    wd.table = dbases;
+    wd.table = pm.table;
     wd.bkt = pm.bkt;
     wd.hand = pm.hand;
     wd.part = pm.part;
@@ -2286,35 +2293,36 @@ int msm_plain(zgr_ctx* c, Lane& ln, uint32_t* dbases, int log_np, uint32_t* out)
     wd.segbase[0] = 0;
     wd.segbase[1] = nseg;
     wd.gbase[0] = 0;
-    wd.gbase[1] = FW_W;
+    wd.gbase[1] = PM_W;
     wd.line[0] = 0;
     wd.sh[0] = (uint32_t)sh;
     // work lists: the sums waves (wide line 0, segment s, hand-off slot s), then the
     // reduction items (window S, segment q)
     pm.host_work.clear();
     for (uint32_t sg = 0; sg < nseg; sg++) pm.host_work.push_back(make_uint2(FB_WIDE | (sg << 8), sg));
-    for (uint32_t S = 0; S < (uint32_t)FW_W; S++)
+    for (uint32_t S = 0; S < (uint32_t)PM_W; S++)
         for (uint32_t q = 0; q < (uint32_t)FW_RSEGS; q++) pm.host_work.push_back(make_uint2(FB_WIDE | (q << 4) | (S << 8), 0));
     HIPCHK(hipMemcpyAsync(pm.work, pm.host_work.data(), pm.host_work.size() * sizeof(uint2), hipMemcpyHostToDevice,
                           ln.st));
     const size_t shm = FW_BUCKETS * 4;
-    const uint32_t nred = (uint32_t)(FW_W * FW_RSEGS);
-    LAUNCH(c, ln.st, "msm_pm_table", k_pm_table, blocks(np, 256), 256, 0, dbases, np);
-    LAUNCH(c, ln.st, "msm_wide_sort", k_fwp_count, nblk, FW_SORT_T, shm, ln.y, sh, ns, pm.cnt);
+    const uint32_t nred = (uint32_t)(PM_W * FW_RSEGS);
+    LAUNCH(c, ln.st, "msm_pm_table", k_pm_split, blocks(np, 256), 256, 0, ln.y, np, pm.halves);
+    LAUNCH(c, ln.st, "msm_pm_table", k_pm_table, blocks(np, 256), 256, 0, dbases, np, pm.table);
+    LAUNCH(c, ln.st, "msm_wide_sort", k_fwp_count, nblk, FW_SORT_T, shm, pm.halves, np, sh, ns, pm.cnt);
     LAUNCH(c, ln.st, "msm_wide_sort", k_fw_tile_sums, FW_TILES, FW_TILE * FW_PARTS, 0, pm.cnt, nblk, pm.tile);
     LAUNCH(c, ln.st, "msm_wide_sort", k_fw_tile_scan, 1, FW_TILES, 0, pm.tile, pm.tile + FW_TILES,
            pm.tile + 2 * FW_TILES);
     LAUNCH(c, ln.st, "msm_wide_sort", k_fw_offsets, FW_TILES, FW_TILE * FW_PARTS, 0, pm.cnt, nblk,
            pm.tile + FW_TILES, pm.off);
-    LAUNCH(c, ln.st, "msm_wide_sort", k_fwp_scatter, nblk, FW_SORT_T, shm, ln.y, sh, ns, pm.off, pm.ent);
+    LAUNCH(c, ln.st, "msm_wide_sort", k_fwp_scatter, nblk, FW_SORT_T, shm, pm.halves, np, sh, ns, pm.off, pm.ent);
     LAUNCH(c, ln.st, "msm_wide_sort", k_fw_segments, blocks(nseg + 1, 64), 64, 0, wd, pm.seg);
-    LAUNCH(c, ln.st, "msm_accum", k_fb_sums, nseg, FB_LANES, 0, (const uint32_t*)dbases, FbLevels{}, wd, pm.work,
+    LAUNCH(c, ln.st, "msm_accum", k_fb_sums, nseg, FB_LANES, 0, (const uint32_t*)pm.table, FbLevels{}, wd, pm.work,
            ln.bkt, c->log_n, c->h, 1);
     // the groups' node level: heap node (2^h >> gl) + S with h = gl is node 1 + S
     const int gl = sh + FW_BLK_LOG;
     LAUNCH(c, ln.st, "msm_reduce", k_fb_reduce<64>, nred, FB_LANES, 0, FbLevels{}, wd, pm.work + nseg, nred, ln.bkt,
            pm.heap, gl, 1);
-    LAUNCH(c, ln.st, "msm_combine", k_fw_combine, (uint32_t)FW_W, FB_LANES, 0, wd, pm.heap, gl);
+    LAUNCH(c, ln.st, "msm_combine", k_fw_combine, (uint32_t)PM_W, FB_LANES, 0, wd, pm.heap, gl);
     LAUNCH(c, ln.st, "msm_root", k_pm_horner, 1, 64, 0, pm.heap, out);
     return ZGR_OK;
 }
@@ -2379,14 +2387,15 @@ int zgr_msm(zgr_ctx* c, const uint8_t* bases_mont, const uint8_t* scalars_mont, 
     const int log_bs = log_np < 10 ? log_np : 10;
     const int h = log_np - log_bs;
     Lane& ln = c->lanes[0];
-    uint32_t* dbases = nullptr;
-    CHECK(dmalloc((void**)&dbases, np * 64));
+    if (!c->pm.bases) CHECK(dmalloc((void**)&c->pm.bases, c->n * 64));  // kept for later calls
+    uint32_t* dbases = c->pm.bases;
     auto body = [&]() -> int {
         // zero-padded scalars: the padding points are never read by the bucket kernel
         HIPCHK(hipMemsetAsync(ln.y, 0, np * 32, ln.st));
         HIPCHK(hipMemsetAsync(dbases, 0, np * 64, ln.st));
         HIPCHK(hipMemcpyAsync(ln.y, scalars_mont, n * 32, hipMemcpyHostToDevice, ln.st));
         HIPCHK(hipMemcpyAsync(dbases, bases_mont, n * 64, hipMemcpyHostToDevice, ln.st));
+        LAUNCH(c, ln.st, "msm_inf", k_msm_clear_inf, blocks(n, 256), 256, 0, dbases, (uint32_t)n, ln.y);
         uint32_t* opts = c->slots[0].outpts;
         // ZGR_MSM=windowed keeps the per-chunk windowed kernels (same result)
         const char* mode = getenv("ZGR_MSM");
@@ -2399,7 +2408,6 @@ int zgr_msm(zgr_ctx* c, const uint8_t* bases_mont, const uint8_t* scalars_mont, 
     };
     int rc = body();
     (void)hipStreamSynchronize(ln.st);
-    (void)hipFree(dbases);
     return rc;
 }
 

@@ -394,19 +394,30 @@ def main():
     ntt["note"] = ("zgr_ntt_device (natural order in and out, forward), mean of 10 calls; the encode path runs "
                    "the same passes without the return to natural order (DIF iNTT, then DIT per coset)")
 
-    # MSM of 2^20 (zgr_msm: host bases and scalars in, the windowed kernels) and the
-    # 2^10-point EC-FFT (zgr_ecfft)
+    # MSM of 2^20 (zgr_msm: host bases and scalars in, GLV halves through the wide sort and
+    # bucket kernels) and the 2^10-point EC-FFT (zgr_ecfft)
     mbases = ctx.params_download(0).reshape(geo.h + 2, geo.n, 8)[geo.h].copy()
     mscal = roofline_scalars(geo)
     ctx.msm(mbases, mscal)
+    ctx.set_profiling(True)
+    ctx.kernel_time_reset()
     mt = []
     for _ in range(3):
         s0 = time.perf_counter()
         ctx.msm(mbases, mscal)
         mt.append(time.perf_counter() - s0)
-    msm = {"n": geo.n, "wall_ms": round(float(np.median(mt)) * 1e3, 2),
+    ctx.set_profiling(False)
+    mk = {}
+    for name in ("msm_inf", "msm_pm_table", "msm_wide_sort", "msm_accum", "msm_reduce", "msm_combine", "msm_root"):
+        a, cnt = ctx.kernel_time_ms(name)
+        if cnt:
+            mk[name] = round(a * cnt / 3, 3)
+    msm = {"n": geo.n, "wall_ms": round(float(np.median(mt)) * 1e3, 2), "kernel_ms": round(sum(mk.values()), 3),
+           "kernels": mk,
            "note": "zgr_msm, one MSM of 2^20 (basis line bases x dense scalars); the call uploads 64 MB of bases "
-                   "and 32 MB of scalars and runs the windowed bucket kernels; median of 3"}
+                   "and 32 MB of scalars, splits each scalar by GLV into two 127-bit halves over P and phi(P), and "
+                   "runs the wide sort and bucket kernels over 8 windows of 16 bits; wall: median of 3, kernel_ms: "
+                   "mean per call of the same 3 calls (HIP events)"}
     ec10 = ec_in[:(1 << 10) * 64]
     ctx.ecfft(ec10, 10, inverse=False)
     ctx.set_profiling(True)

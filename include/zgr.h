@@ -251,7 +251,11 @@ int zgr_ntt_device(zgr_ctx *ctx, void *data_dev, int log_n, int batch, int inver
 /* to_coset_blob (encode.rs:135-158) for coset 1..coset_n-1 of ctx's geometry:
  * data_mont (2^log_n, column-major) -> out_mont evaluations on that coset. */
 int zgr_coset_extend(zgr_ctx *ctx, const uint8_t *data_mont, int coset, uint8_t *out_mont);
-/* Plain MSM sum_i k_i * P_i: bases in param layout (mont), scalars mont. */
+/* Plain MSM sum_i k_i * P_i: VariableBaseMSM::msm as the CPU prover calls it per line
+ * (crates/amt/src/prove_params/prove.rs:17,31,55,77).  Bases in param layout (mont),
+ * scalars mont; the result is canonical affine.  Any n (zero-padded to a power of two).
+ * An all-zero base (never in a param file: fast_serde_bn254.rs:139-142 refuses to write
+ * it) is read as the point at infinity. */
 int zgr_msm(zgr_ctx *ctx, const uint8_t *bases_mont, const uint8_t *scalars_mont, size_t n,
             uint8_t *out_point);
 /* G1 EC-FFT: Pairing::fast_fft / fast_ifft (crates/amt/src/ec_algebra.rs:34-95) over

@@ -78,6 +78,25 @@ def test_msm_all_zero_and_single_point(ctx):
         assert np.array_equal(got, exp), k
 
 
+@pytest.mark.parametrize("n", [5, 300])
+def test_msm_bases_at_infinity(ctx, n):
+    """An all-zero base -- which the param files never hold (write_g1 refuses the point at
+    infinity, crates/amt/src/prove_params/fast_serde_bn254.rs:139-142) -- is read as the
+    point at infinity on either path: the result is the MSM over the other bases."""
+    rng = np.random.default_rng(1300 + n)
+    bases = _canon_to_mont_bases(_random_points(rng, n, with_inf=False))
+    bases[0::3] = 0
+    vals = edge_scalars(rng, n)
+    keep = [i for i in range(n) if i % 3]
+    exp = O.msm(bases[keep], np.array([O.int_to_limbs(vals[i]) for i in keep]))
+    got, win = run_both(ctx, bases, scalars_mont(vals))
+    assert np.array_equal(got, exp)
+    assert np.array_equal(win, exp)
+    # every base at infinity
+    got, win = run_both(ctx, np.zeros_like(bases), scalars_mont(vals))
+    assert not got.any() and not win.any()
+
+
 def test_msm_repeated_calls_and_sizes(ctx):
     """the path's buffers are reused across calls of different sizes"""
     for n in (2048, 3, 2048, 700):
