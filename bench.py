@@ -159,6 +159,8 @@ def main():
     ap.add_argument("--served-requests", type=int, default=48)
     ap.add_argument("--qps", type=float, default=0.0)
     ap.add_argument("--roofline-only", action="store_true")
+    ap.add_argument("--msm-only", action="store_true",
+                    help="only the isolated 2^20 MSM (zgr_msm), four calls, for a kernel trace")
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     args = ap.parse_args()
     if args.gpus != 1:
@@ -173,6 +175,20 @@ def main():
     ctx.params_generate(Z.tau_from_seed(TAU_SEED))
     t_params = time.time() - t0
     log("params (3 cosets x 12 x 2^20 bases) generated on device in %.2f s" % t_params)
+    if args.msm_only:
+        # one warm-up and three timed calls of zgr_msm over 2^20 basis-line bases and dense
+        # scalars (the same inputs as the msm_2pow20 field of the full run)
+        mbases = ctx.params_download(0).reshape(geo.h + 2, geo.n, 8)[geo.h].copy()
+        mscal = roofline_scalars(geo)
+        ctx.msm(mbases, mscal)
+        mt = []
+        for _ in range(3):
+            s0 = time.perf_counter()
+            ctx.msm(mbases, mscal)
+            mt.append(time.perf_counter() - s0)
+        ctx.close()
+        print(json.dumps({"msm_only": True, "n": geo.n, "calls": 4, "wall_ms": round(float(np.median(mt)) * 1e3, 2)}))
+        return
     if args.roofline_only:
         # workload for the counter passes: three isolated launches of the bucket kernels
         # (one coset of amt_prove each, the default plan: the tuner does not run, since
