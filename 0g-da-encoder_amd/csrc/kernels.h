@@ -417,7 +417,7 @@ constexpr int FW_MIN_GL = 4;                 // a line goes wide when its groups
 constexpr int FW_MAX_LINES = 16;
 constexpr int FW_RSEG = 2048;                // buckets per reduction wave
 constexpr int FW_RSEGS = FW_BUCKETS / FW_RSEG;
-constexpr int FW_SORT_T = 256;   // threads per block of the wide sort (one 4-chunk block each)
+constexpr int FW_SORT_T = 1024;  // threads per block of the wide sort (one 4-chunk block each)
 constexpr int FW_TILE = 128;     // buckets per block of the offset scan
 constexpr int FW_PARTS = 4;      // threads per bucket there (each sums a quarter of the blocks)
 constexpr int FW_TILES = FW_BUCKETS / FW_TILE;
@@ -1018,12 +1018,20 @@ __global__ __launch_bounds__(FB_LANES, 2) void k_fb_sums(const uint32_t* table, 
 // multiplications per bucket set, for dense scalars; RL = 64 halves an item's latency,
 // for sparse ones, whose few non-zero groups leave most of the device idle.  `items`
 // bounds the list (a last odd item leaves the second half idle).
-template <int RL>
+// RB < 2048 (wide items only): items of RB buckets, FW_BUCKETS / RB partials per group
+// (item FB_WIDE | w | q << 4 | S << 10), for a single MSM whose few groups would
+// otherwise give too few waves to fill the device.
+constexpr int ilog2c(int v) { return v <= 1 ? 0 : 1 + ilog2c(v >> 1); }
+
+template <int RL, int RB = FB_BUCKETS>
 __global__ __launch_bounds__(FB_LANES, 2) void k_fb_reduce(FbLevels lv, FbWide wd, const uint2* work,
                                                            uint32_t items, const uint32_t* bkt, uint32_t* heap,
                                                            int h, int lines) {
-    constexpr int FB_PER_LANE = FB_BUCKETS / RL;
-    constexpr int PL_LOG = RL == 32 ? 6 : 5;  // log2 FB_PER_LANE
+    constexpr int FB_PER_LANE = RB / RL;
+    constexpr int PL_LOG = ilog2c(FB_PER_LANE);
+    constexpr int RSEGS = FW_BUCKETS / RB;  // partials per wide group
+    constexpr int Q_BITS = ilog2c(RSEGS) > 4 ? ilog2c(RSEGS) : 4;
+    static_assert(FB_PER_LANE <= 64 && (1 << PL_LOG) == FB_PER_LANE, "lane share");
     __shared__ uint32_t red[FB_LANES * X29_WORDS];
     const uint32_t nb = 1u << h;
     const int half = (int)(threadIdx.x / RL), s = (int)(threadIdx.x % RL);
@@ -1036,15 +1044,16 @@ __global__ __launch_bounds__(FB_LANES, 2) void k_fb_reduce(FbLevels lv, FbWide w
     uint64_t filled = 0;       // bit i: bucket i of the lane's FB_PER_LANE has entries
     const bool wide = (it.x & FB_WIDE) != 0;
     if (live && wide) {
-        const uint32_t w = it.x & 0xfu, q = (it.x >> 4) & 0xfu, S = (it.x >> 8) & 0x7fffffu;
+        const uint32_t w = it.x & 0xfu, q = (it.x >> 4) & ((1u << Q_BITS) - 1u),
+                       S = (it.x >> (4 + Q_BITS)) & (0x7fffffffu >> (4 + Q_BITS));
         const uint32_t g = wd.gbase[w] + S, sh = wd.sh[w];
-        mul = q * (FW_RSEG / FB_PER_LANE) + (uint32_t)s;
+        mul = q * (RB / FB_PER_LANE) + (uint32_t)s;
         const uint32_t m0 = mul * FB_PER_LANE;  // bucket m0 + 1 is the lane's first
         bk = wd.bkt + ((size_t)g * FW_BUCKETS + m0) * X29_WORDS;
-        outp = wd.part + ((size_t)g * FW_RSEGS + q) * X29_WORDS;
+        outp = wd.part + ((size_t)g * RSEGS + q) * X29_WORDS;
         for (int i = 0; i < FB_PER_LANE; i++)
             filled |= (uint64_t)(fw_pos(wd, m0 + i, (S + 1) << sh) > fw_pos(wd, m0 + i, S << sh)) << i;
-    } else if (live) {
+    } else if (RB == FB_BUCKETS && live) {
         const uint32_t L = it.x & 0xffu, gl = (it.x >> 8) & 3u, S = it.x >> 10;
         outp = heap + ((size_t)L * 2 * nb + (nb >> gl) + S) * X29_WORDS;
         if (lv.cnt[gl][S]) {  // a zero group leaves `filled` empty: its sum is infinity
@@ -1065,9 +1074,9 @@ __global__ __launch_bounds__(FB_LANES, 2) void k_fb_reduce(FbLevels lv, FbWide w
             Rr = x29_add_lazy(Rr, B);
             T = x29_add_lazy(T, Rr);
         }
-        // T + P mul Rr = (mul Rr) doubled log2 P times, plus T (mul < 16 RL; the bit
+        // T + P mul Rr = (mul Rr) doubled log2 P times, plus T (mul < RSEGS RL; the bit
         // count is uniform over the wave, the multiplier per lane)
-        x29 off = x29_mul_small_uniform(Rr, mul, (RL == 32 ? 5 : 6) + (__any(wide) ? 4 : 0));
+        x29 off = x29_mul_small_uniform(Rr, mul, ilog2c(RL) + (__any(wide) ? ilog2c(RSEGS) : 0));
         for (int k = 0; k < PL_LOG; k++) off = x29_dbl(off);
         W = x29_add_lazy(T, off);
 
@@ -1085,18 +1094,20 @@ __global__ __launch_bounds__(FB_LANES, 2) void k_fb_reduce(FbLevels lv, FbWide w
 }
 
 // Group results of the wide lines: heap node (nb >> gl) + S of line L = the sum of the
-// group's FW_RSEGS reduction partials.  One wave per group, in group order.
+// group's RSEGS reduction partials.  One wave per group, in group order.
+template <int RSEGS = FW_RSEGS>
 __global__ __launch_bounds__(FB_LANES) void k_fw_combine(FbWide wd, uint32_t* heap, int h) {
-    __shared__ uint32_t red[FW_RSEGS * X29_WORDS];
+    static_assert(RSEGS <= FB_LANES, "one partial per lane");
+    __shared__ uint32_t red[RSEGS * X29_WORDS];
     const uint32_t g = blockIdx.x, s = threadIdx.x;
     int w = 0;
     while (w + 1 < wd.nw && g >= wd.gbase[w + 1]) w++;
     const uint32_t S = g - wd.gbase[w], gl = wd.sh[w] + FW_BLK_LOG, nb = 1u << h;
-    if (s < (uint32_t)FW_RSEGS)
-        x29_store(red + s * X29_WORDS, x29_load(wd.part + ((size_t)g * FW_RSEGS + s) * X29_WORDS));
+    if (s < (uint32_t)RSEGS)
+        x29_store(red + s * X29_WORDS, x29_load(wd.part + ((size_t)g * RSEGS + s) * X29_WORDS));
     __syncthreads();
-    for (uint32_t o = 1; o < (uint32_t)FW_RSEGS; o <<= 1) {
-        if (s < (uint32_t)FW_RSEGS && (s & (2 * o - 1)) == 0) {
+    for (uint32_t o = 1; o < (uint32_t)RSEGS; o <<= 1) {
+        if (s < (uint32_t)RSEGS && (s & (2 * o - 1)) == 0) {
             const x29 W = x29_add(x29_load(red + s * X29_WORDS), x29_load(red + (s + o) * X29_WORDS));
             x29_store(red + s * X29_WORDS, W);
         }
@@ -1627,15 +1638,82 @@ __global__ __launch_bounds__(256) void k_pm_table(const uint32_t* bases, uint32_
     fb_store29(table + 16 * ((size_t)n + i) + 8, y);
 }
 
+// The Horner below is one dependent chain: 112 doublings and 7 additions on one point,
+// whose cost is the instructions one wave issues, not the lanes it uses.  So the
+// independent multiplications of each formula level run side by side on lanes 0..3 of
+// the wave (operands picked per lane, the products broadcast back with readlane): an
+// XYZZ doubling is 3 multiplication steps instead of 9, an addition 4 instead of 14.
+
+// lane 0..3 picks a0..a3 (lanes above 3 repeat lane 3's pick)
+__device__ __forceinline__ fq29 q_pick4(int lane, const fq29& a0, const fq29& a1, const fq29& a2,
+                                        const fq29& a3) {
+    fq29 r;
+#pragma unroll
+    for (int k = 0; k < 9; k++) r.l[k] = lane == 0 ? a0.l[k] : lane == 1 ? a1.l[k] : lane == 2 ? a2.l[k] : a3.l[k];
+    return r;
+}
+
+// lane src's value, uniform over the wave
+__device__ __forceinline__ fq29 q_lane(const fq29& a, int src) {
+    fq29 r;
+#pragma unroll
+    for (int k = 0; k < 9; k++) r.l[k] = (uint32_t)__builtin_amdgcn_readlane((int)a.l[k], src);
+    return r;
+}
+
+// dbl-2008-s-1 (as x29_dbl) in three steps: V = U^2, X^2 | W = U V, S = X V, M^2, V ZZ |
+// M (S - X3), W Y, W ZZZ, with U = 2Y, M = 3X^2, X3 = M^2 - 2S, Y3 = M (S - X3) - W Y
+__device__ __forceinline__ x29 x29_dbl_lanes(const x29& p, int lane) {
+    const fq29 U = q_dbl(p.Y);
+    fq29 t = q_mul(q_pick4(lane, U, p.X, U, p.X), q_pick4(lane, U, p.X, U, p.X));
+    const fq29 V = q_lane(t, 0), X2 = q_lane(t, 1);
+    const fq29 M = q_add(q_dbl(X2), X2);
+    t = q_mul(q_pick4(lane, U, p.X, M, V), q_pick4(lane, V, V, M, p.ZZ));
+    const fq29 W = q_lane(t, 0), S = q_lane(t, 1);
+    x29 r;
+    r.X = q_sub(q_lane(t, 2), q_dbl(S));
+    r.ZZ = q_lane(t, 3);
+    const fq29 SX = q_sub(S, r.X);
+    t = q_mul(q_pick4(lane, M, W, W, W), q_pick4(lane, SX, p.Y, p.ZZZ, p.ZZZ));
+    r.Y = q_sub(q_lane(t, 0), q_lane(t, 1));
+    r.ZZZ = q_lane(t, 2);
+    return r;
+}
+
+// add-2008-s (as x29_add, every exceptional case handled) in four steps:
+// U1 = X1 ZZ2, U2 = X2 ZZ1, S1 = Y1 ZZZ2, S2 = Y2 ZZZ1 | PP = P^2, ZZ1 ZZ2, ZZZ1 ZZZ2, R^2 |
+// PPP = P PP, Q = U1 PP, ZZ3 = ZZ1 ZZ2 PP | ZZZ3 = ZZZ1 ZZZ2 PPP, S1 PPP, R (Q - X3)
+// with P = U2 - U1, R = S2 - S1, X3 = R^2 - PPP - 2Q, Y3 = R (Q - X3) - S1 PPP
+__device__ __forceinline__ x29 x29_add_lanes(const x29& p, const x29& q, int lane) {
+    if (x29_is_inf(p)) return q;
+    if (x29_is_inf(q)) return p;
+    fq29 t = q_mul(q_pick4(lane, p.X, q.X, p.Y, q.Y), q_pick4(lane, q.ZZ, p.ZZ, q.ZZZ, p.ZZZ));
+    const fq29 U1 = q_lane(t, 0), S1 = q_lane(t, 2);
+    const fq29 P = q_sub(q_lane(t, 1), U1), R = q_sub(q_lane(t, 3), S1);
+    if (q_is_zero(P)) return q_is_zero(R) ? x29_dbl_lanes(p, lane) : x29_inf();
+    t = q_mul(q_pick4(lane, P, p.ZZ, p.ZZZ, R), q_pick4(lane, P, q.ZZ, q.ZZZ, R));
+    const fq29 PP = q_lane(t, 0), ZZ12 = q_lane(t, 1), ZZZ12 = q_lane(t, 2), R2 = q_lane(t, 3);
+    t = q_mul(q_pick4(lane, P, U1, ZZ12, ZZ12), PP);
+    const fq29 PPP = q_lane(t, 0), Q = q_lane(t, 1);
+    x29 r;
+    r.ZZ = q_lane(t, 2);
+    r.X = q_sub(q_sub(R2, PPP), q_dbl(Q));
+    const fq29 QX = q_sub(Q, r.X);
+    t = q_mul(q_pick4(lane, ZZZ12, S1, R, R), q_pick4(lane, PPP, PPP, QX, QX));
+    r.ZZZ = q_lane(t, 0);
+    r.Y = q_sub(q_lane(t, 2), q_lane(t, 1));
+    return r;
+}
+
 // sum_w 2^(16 w) S_w from the window sums at heap nodes 1 .. PM_W, canonical affine
 __global__ __launch_bounds__(64) void k_pm_horner(const uint32_t* heap, uint32_t* out) {
-    if (threadIdx.x) return;
+    const int lane = (int)threadIdx.x;
     x29 r = x29_load(heap + PM_W * X29_WORDS);
     for (int w = PM_W - 2; w >= 0; w--) {
-        for (int k = 0; k < FW_C; k++) r = x29_dbl(r);
-        r = x29_add(r, x29_load(heap + (1 + w) * X29_WORDS));
+        for (int k = 0; k < FW_C; k++) r = x29_dbl_lanes(r, lane);
+        r = x29_add_lanes(r, x29_load(heap + (1 + w) * X29_WORDS), lane);
     }
-    x29_store_canon(out, r);
+    if (lane == 0) x29_store_canon(out, r);
 }
 
 }  // namespace zgr

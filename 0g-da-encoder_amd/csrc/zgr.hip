@@ -559,7 +559,7 @@ int msm_lines(zgr_ctx* c, Lane& ln, int coset, const uint32_t* src, int from_y, 
             LAUNCH(c, ln.st, "msm_reduce", k_fb_reduce<64>, rwaves, FB_LANES, 0, lv, wd, rwork, rwaves, ln.bkt,
                    ln.heap, c->h, lines);
         if (wd.nw)
-            LAUNCH(c, ln.st, "msm_combine", k_fw_combine, wd.gbase[wd.nw], FB_LANES, 0, wd, ln.heap, c->h);
+            LAUNCH(c, ln.st, "msm_combine", k_fw_combine<>, wd.gbase[wd.nw], FB_LANES, 0, wd, ln.heap, c->h);
         return ZGR_OK;
     }
     LAUNCH(c, ln.st, "msm_digits", k_msm_digits, blocks(c->n, 256), 256, 0, src, c->log_n, log_bs, ln.digits,
@@ -2254,6 +2254,10 @@ int zgr_coset_extend(zgr_ctx* c, const uint8_t* data_mont, int coset, uint8_t* o
 // combine, then Horner over the windows (k_pm_horner).  Returns 1 without launching
 // anything when the device cannot give the sort its LDS; the caller then runs the
 // windowed kernels.
+constexpr uint32_t PM_MAX_SEGS = 8192;
+constexpr int PM_RB = 512;                     // buckets per reduction item
+constexpr int PM_RSEGS = FW_BUCKETS / PM_RB;  // reduction partials per window
+
 int msm_plain(zgr_ctx* c, Lane& ln, const uint32_t* dbases, int log_np, uint32_t* out) {
     if (!c->pm.lds_ok) return 1;
     const uint32_t np = 1u << log_np, nv = 2 * np;  // points, virtual points (P, phi(P))
@@ -2262,7 +2266,13 @@ int msm_plain(zgr_ctx* c, Lane& ln, const uint32_t* dbases, int log_np, uint32_t
     int sh = 0;
     while ((1u << sh) < bpw) sh++;
     const uint32_t nblk = PM_W * bpw;
-    const uint32_t nseg = std::max(1u, std::min(1024u, (uint32_t)((uint64_t)PM_W * nv / 16384)));
+    // bucket segments (one sums wave each): one line of work, so more and shorter waves
+    // than an AMT line's 1024 to fill the device; ZGR_MSM_SEGS overrides
+    uint32_t nseg = std::max(1u, std::min(PM_MAX_SEGS, (uint32_t)((uint64_t)PM_W * nv / 8192)));
+    if (const char* e = getenv("ZGR_MSM_SEGS")) {
+        const long v = atol(e);
+        if (v >= 1) nseg = std::min(PM_MAX_SEGS, (uint32_t)v);
+    }
     auto& pm = c->pm;
     if (!pm.ent) {  // sized for the context's largest MSM
         const size_t blk_max = (size_t)PM_W * std::max<size_t>(1, (2 * c->n) >> 16);
@@ -2270,12 +2280,12 @@ int msm_plain(zgr_ctx* c, Lane& ln, const uint32_t* dbases, int log_np, uint32_t
         DALLOC(pm.cnt, blk_max * FW_BUCKETS * 4);
         DALLOC(pm.off, blk_max * FW_BUCKETS * 4);
         DALLOC(pm.tile, (2 * FW_TILES + 1) * 4);
-        DALLOC(pm.seg, (1024 + 1) * 4);
+        DALLOC(pm.seg, (PM_MAX_SEGS + 1) * 4);
         DALLOC(pm.bkt, (size_t)PM_W * FW_BUCKETS * X29_WORDS * 4);
-        DALLOC(pm.part, (size_t)PM_W * FW_RSEGS * X29_WORDS * 4);
-        DALLOC(pm.hand, (size_t)1024 * 2 * FB_LANES * X29_WORDS * 4);
+        DALLOC(pm.part, (size_t)PM_W * PM_RSEGS * X29_WORDS * 4);
+        DALLOC(pm.hand, (size_t)PM_MAX_SEGS * 2 * FB_LANES * X29_WORDS * 4);
         DALLOC(pm.heap, (size_t)(PM_W + 1) * X29_WORDS * 4);
-        DALLOC(pm.work, (1024 + PM_W * FW_RSEGS) * sizeof(uint2));
+        DALLOC(pm.work, (PM_MAX_SEGS + PM_W * PM_RSEGS) * sizeof(uint2));
         DALLOC(pm.halves, (size_t)PM_SPLIT_WORDS * c->n * 4);
         DALLOC(pm.table, (size_t)2 * c->n * 64);
     }
@@ -2301,11 +2311,12 @@ int msm_plain(zgr_ctx* c, Lane& ln, const uint32_t* dbases, int log_np, uint32_t
     pm.host_work.clear();
     for (uint32_t sg = 0; sg < nseg; sg++) pm.host_work.push_back(make_uint2(FB_WIDE | (sg << 8), sg));
     for (uint32_t S = 0; S < (uint32_t)PM_W; S++)
-        for (uint32_t q = 0; q < (uint32_t)FW_RSEGS; q++) pm.host_work.push_back(make_uint2(FB_WIDE | (q << 4) | (S << 8), 0));
+        for (uint32_t q = 0; q < (uint32_t)PM_RSEGS; q++)
+            pm.host_work.push_back(make_uint2(FB_WIDE | (q << 4) | (S << 10), 0));
     HIPCHK(hipMemcpyAsync(pm.work, pm.host_work.data(), pm.host_work.size() * sizeof(uint2), hipMemcpyHostToDevice,
                           ln.st));
     const size_t shm = FW_BUCKETS * 4;
-    const uint32_t nred = (uint32_t)(PM_W * FW_RSEGS);
+    const uint32_t nred = (uint32_t)(PM_W * PM_RSEGS);
     LAUNCH(c, ln.st, "msm_pm_table", k_pm_split, blocks(np, 256), 256, 0, ln.y, np, pm.halves);
     LAUNCH(c, ln.st, "msm_pm_table", k_pm_table, blocks(np, 256), 256, 0, dbases, np, pm.table);
     LAUNCH(c, ln.st, "msm_wide_sort", k_fwp_count, nblk, FW_SORT_T, shm, pm.halves, np, sh, ns, pm.cnt);
@@ -2320,9 +2331,9 @@ int msm_plain(zgr_ctx* c, Lane& ln, const uint32_t* dbases, int log_np, uint32_t
            ln.bkt, c->log_n, c->h, 1);
     // the groups' node level: heap node (2^h >> gl) + S with h = gl is node 1 + S
     const int gl = sh + FW_BLK_LOG;
-    LAUNCH(c, ln.st, "msm_reduce", k_fb_reduce<64>, nred, FB_LANES, 0, FbLevels{}, wd, pm.work + nseg, nred, ln.bkt,
-           pm.heap, gl, 1);
-    LAUNCH(c, ln.st, "msm_combine", k_fw_combine, (uint32_t)PM_W, FB_LANES, 0, wd, pm.heap, gl);
+    LAUNCH(c, ln.st, "msm_reduce", (k_fb_reduce<64, PM_RB>), nred, FB_LANES, 0, FbLevels{}, wd, pm.work + nseg, nred,
+           ln.bkt, pm.heap, gl, 1);
+    LAUNCH(c, ln.st, "msm_combine", k_fw_combine<PM_RSEGS>, (uint32_t)PM_W, FB_LANES, 0, wd, pm.heap, gl);
     LAUNCH(c, ln.st, "msm_root", k_pm_horner, 1, 64, 0, pm.heap, out);
     return ZGR_OK;
 }
