@@ -1,7 +1,8 @@
-"""zgr_msm (one MSM of arbitrary bases, BASELINE's isolated-primitive MSM) on both of its
-paths -- the wide sort and bucket kernels with one bucket group per 16-bit window (the
-default) and the per-chunk windowed kernels (ZGR_MSM=windowed) -- against the oracle's
-Pippenger (oracle/zg_oracle.c), on scalars chosen for the signed-digit edge cases."""
+"""zgr_msm (one MSM of arbitrary bases, the isolated-primitive MSM of 2^20 points at
+production size) on both of its paths -- GLV halves through the wide sort and bucket
+kernels with one bucket group per 16-bit window (the default) and the per-chunk windowed
+kernels (ZGR_MSM=windowed) -- against the oracle's Pippenger (oracle/zg_oracle.c), on
+scalars chosen for the signed-digit edge cases."""
 import os
 
 import numpy as np
@@ -95,6 +96,23 @@ def test_msm_bases_at_infinity(ctx, n):
     # every base at infinity
     got, win = run_both(ctx, np.zeros_like(bases), scalars_mont(vals))
     assert not got.any() and not win.any()
+
+
+@pytest.mark.parametrize("segs", ["1", "3", "8192"])
+def test_msm_segment_counts(ctx, segs):
+    """ZGR_MSM_SEGS: one sums wave for everything, an odd count, and more waves than
+    (bucket, window) runs (most segments empty) give the same point"""
+    n = 1500
+    rng = np.random.default_rng(1700)
+    bases = _canon_to_mont_bases(_random_points(rng, n, with_inf=False))
+    vals = edge_scalars(rng, n)
+    exp = O.msm(bases, np.array([O.int_to_limbs(v) for v in vals]))
+    os.environ["ZGR_MSM_SEGS"] = segs
+    try:
+        got = ctx.msm(bases, scalars_mont(vals))
+    finally:
+        del os.environ["ZGR_MSM_SEGS"]
+    assert np.array_equal(got, exp)
 
 
 def test_msm_repeated_calls_and_sizes(ctx):
