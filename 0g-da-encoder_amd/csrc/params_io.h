@@ -136,46 +136,68 @@ __device__ __forceinline__ g2_xyzz g2x_load(const uint32_t* p) {
 // coeff_0 = g2pp'[N - m], coeff_i = g2pp'[(i - 1) m] (i = 1..D-1), with from_pp's coset
 // scaling g2pp'[k] = w^k g2pp[k] (w = 1 / cf_c, generate.rs:152-168; scale = 0 for coset
 // 0), loaded bit-reversed (XYZZ) for the DIT stages.  g2pp canonical affine.
+// The h levels are independent transforms of 2^d points (d = 1..h), so every launch
+// below covers all of them at once: level d's points sit at work[2^d - 2, 2^(d+1) - 2),
+// the layout of the stored vanishes.  One launch per stage then costs one G2 scalar
+// multiplication's latency for all levels together (the levels' transforms are too small
+// to fill the device, so a launch per level and stage cost that latency each time).
+
+// point t of all levels -> (level d, index i in it): t + 2 in [2^d, 2^(d+1))
+__device__ __forceinline__ uint32_t van_level(uint32_t t, uint32_t* i) {
+    const uint32_t d = 31u - (uint32_t)__builtin_clz(t + 2u);
+    *i = t - ((1u << d) - 2u);
+    return d;
+}
+
 __global__ __launch_bounds__(64) void k_vanish_load(const uint32_t* g2pp, const uint32_t* w_mont, int scale, int log_n,
-                                                    int d, uint32_t* work) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >> d) return;
+                                                    int h, uint32_t* work) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (2u << h) - 2u) return;
+    uint32_t i;
+    const uint32_t d = van_level(t, &i);
     const uint32_t m = 1u << (log_n - d);
     const uint32_t src = i ? (i - 1) * m : (1u << log_n) - m;
     bool inf;
     const g2_aff a = g2_load_canon(g2pp + G2_WORDS * (size_t)src, &inf);
     g2_xyzz P = inf ? g2_inf() : g2_from_aff(a);
     if (scale) P = g2_mul(P, f_from_mont(f_pow_u64(f_load<FrP>(w_mont), src)));
-    g2x_store(work + G2X_WORDS * (size_t)bitrev_bits(i, d), P);
+    g2x_store(work + G2X_WORDS * (size_t)((1u << d) - 2u + bitrev_bits(i, (int)d)), P);
 }
 
-// one radix-2 DIT stage of the inverse-root DFT (fft_domain with group_gen and
-// group_gen_inv swapped, generate.rs:254-260): (u, v) -> (u + w v, u - w v), w =
-// tw_inv[j << (log_n - s)] = w_(2^s)^-j; one thread per butterfly
-__global__ __launch_bounds__(64) void k_vanish_stage(uint32_t* work, const uint32_t* tw_inv, int s, int log_n,
-                                                     uint32_t half_n) {
+// one radix-2 DIT stage s of the inverse-root DFT (fft_domain with group_gen and
+// group_gen_inv swapped, generate.rs:254-260) in every level d >= s: (u, v) ->
+// (u + w v, u - w v), w = tw_inv[j << (log_n - s)] = w_(2^s)^-j; one thread per
+// butterfly.  Level d's 2^(d-1) butterflies are threads [2^(d-1) - 2^(s-1), 2^d - 2^(s-1)).
+// Stage 1's twiddle is w^0 = 1: no multiplication.
+__global__ __launch_bounds__(64) void k_vanish_stage(uint32_t* work, const uint32_t* tw_inv, int s, int log_n, int h) {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= half_n) return;
-    const uint32_t half = 1u << (s - 1), j = t & (half - 1);
-    const uint32_t i0 = ((t >> (s - 1)) << s) | j, i1 = i0 + half;
-    const g2_xyzz u = g2x_load(work + G2X_WORDS * (size_t)i0);
-    const g2_xyzz v = g2_mul(g2x_load(work + G2X_WORDS * (size_t)i1),
-                             f_from_mont(f_load<FrP>(tw_inv + 8 * ((size_t)j << (log_n - s)))));
+    const uint32_t h0 = 1u << (s - 1);
+    if (t >= (1u << h) - h0) return;
+    const uint32_t d = 32u - (uint32_t)__builtin_clz(t + h0);  // floor(log2(t + h0)) + 1
+    const uint32_t td = t - ((1u << (d - 1)) - h0);
+    uint32_t* lv = work + G2X_WORDS * (size_t)((1u << d) - 2u);
+    const uint32_t j = td & (h0 - 1);
+    const uint32_t i0 = ((td >> (s - 1)) << s) | j, i1 = i0 + h0;
+    const g2_xyzz u = g2x_load(lv + G2X_WORDS * (size_t)i0);
+    g2_xyzz v = g2x_load(lv + G2X_WORDS * (size_t)i1);
+    if (s > 1) v = g2_mul(v, f_from_mont(f_load<FrP>(tw_inv + 8 * ((size_t)j << (log_n - s)))));
     g2_xyzz nv = v;
     nv.Y = q2_neg(v.Y);
-    g2x_store(work + G2X_WORDS * (size_t)i0, g2_add(u, v));
-    g2x_store(work + G2X_WORDS * (size_t)i1, g2_add(u, nv));
+    g2x_store(lv + G2X_WORDS * (size_t)i0, g2_add(u, v));
+    g2x_store(lv + G2X_WORDS * (size_t)i1, g2_add(u, nv));
 }
 
-// enact: DFT output t -> affine Montgomery at stored index bitrev_d(t) of level d; an
-// infinite output (a degenerate tau) sets *bad
-__global__ __launch_bounds__(64) void k_vanish_store(const uint32_t* work, int d, uint32_t* out, int* bad) {
+// enact: DFT output i of level d -> affine Montgomery at stored index bitrev_d(i) of
+// level d, for every level; an infinite output (a degenerate tau) sets *bad
+__global__ __launch_bounds__(64) void k_vanish_store(const uint32_t* work, int h, uint32_t* out, int* bad) {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >> d) return;
+    if (t >= (2u << h) - 2u) return;
+    uint32_t i;
+    const uint32_t d = van_level(t, &i);
     bool inf;
     g2_aff a = g2_to_aff(g2x_load(work + G2X_WORDS * (size_t)t), &inf);
     if (inf) *bad = 1;
-    g2_store(out + G2_WORDS * (size_t)((1u << d) - 2 + bitrev_bits(t, d)), a);
+    g2_store(out + G2_WORDS * (size_t)((1u << d) - 2u + bitrev_bits(i, (int)d)), a);
 }
 
 // ---------------------------------------------------------------------------

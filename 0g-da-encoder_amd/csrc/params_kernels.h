@@ -14,10 +14,10 @@ __global__ void k_power_tau(const uint32_t* tau_canon, uint64_t start, uint32_t 
                             const uint32_t* table2, uint32_t* out1, uint32_t* out2, int* bad);
 __global__ void k_vanish_closed(const uint32_t* pc, const uint32_t* consts, int log_n, int h, const uint32_t* table2,
                                 uint32_t* out, int* bad);
-__global__ void k_vanish_load(const uint32_t* g2pp, const uint32_t* w_mont, int scale, int log_n, int d,
+__global__ void k_vanish_load(const uint32_t* g2pp, const uint32_t* w_mont, int scale, int log_n, int h,
                               uint32_t* work);
-__global__ void k_vanish_stage(uint32_t* work, const uint32_t* tw_inv, int s, int log_n, uint32_t half_n);
-__global__ void k_vanish_store(const uint32_t* work, int d, uint32_t* out, int* bad);
+__global__ void k_vanish_stage(uint32_t* work, const uint32_t* tw_inv, int s, int log_n, int h);
+__global__ void k_vanish_store(const uint32_t* work, int h, uint32_t* out, int* bad);
 __global__ void k_g1_ark(const uint32_t* pts, uint32_t n, int mont, int compressed, uint32_t* out);
 __global__ void k_g2_ark(const uint32_t* pts, uint32_t n, int mont, int compressed, uint32_t* out);
 __global__ void k_fq_convert(const uint32_t* in, uint32_t* out, uint64_t count, int to_mont);

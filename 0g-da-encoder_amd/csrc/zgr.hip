@@ -1260,19 +1260,18 @@ int pp_vanishes(zgr_ctx* c, int coset, const uint32_t* g2pp, const uint32_t* hig
     hipStream_t st = c->lanes[0].st;
     uint32_t *work = nullptr, *w = nullptr;
     auto body = [&]() -> int {
-        DALLOC(work, ((size_t)G2X_WORDS * 4) << c->h);
+        DALLOC(work, ((size_t)G2X_WORDS * 4) << (c->h + 1));  // every level: 2^(h+1) - 2 points
         DALLOC(w, 32);
         LAUNCH(c, st, "params_vanish", k_rec_clear, 1, 64, 0, c->flag);
         LAUNCH(c, st, "params_vanish", k_pp_coset_w, 1, 64, 0, c->consts, coset, w);
-        for (int d = 1; d <= c->h; d++) {
-            LAUNCH(c, st, "params_vanish", k_vanish_load, blocks((size_t)1 << d, 64), 64, 0, g2pp, w, coset != 0,
-                   c->log_n, d, work);
-            for (int s = 1; s <= d; s++)
-                LAUNCH(c, st, "params_vanish", k_vanish_stage, blocks((size_t)1 << (d - 1), 64), 64, 0, work, c->tw_inv,
-                       s, c->log_n, 1u << (d - 1));
-            LAUNCH(c, st, "params_vanish", k_vanish_store, blocks((size_t)1 << d, 64), 64, 0, work, d, c->van[coset],
-                   c->flag);
-        }
+        // the h levels' transforms side by side: one launch per stage for all of them
+        const size_t pts = ((size_t)2 << c->h) - 2;
+        LAUNCH(c, st, "params_vanish", k_vanish_load, blocks(pts, 64), 64, 0, g2pp, w, coset != 0, c->log_n, c->h,
+               work);
+        for (int s = 1; s <= c->h; s++)
+            LAUNCH(c, st, "params_vanish", k_vanish_stage, blocks(((size_t)1 << c->h) - ((size_t)1 << (s - 1)), 64), 64,
+                   0, work, c->tw_inv, s, c->log_n, c->h);
+        LAUNCH(c, st, "params_vanish", k_vanish_store, blocks(pts, 64), 64, 0, work, c->h, c->van[coset], c->flag);
         HIPCHK(hipMemcpyAsync(c->vg2[coset], g2pp, G2_WORDS * 4, hipMemcpyDeviceToDevice, st));
         HIPCHK(hipMemcpyAsync(c->vg2[coset] + G2_WORDS, high_g2, G2_WORDS * 4, hipMemcpyDeviceToDevice, st));
         int bad = 0;
