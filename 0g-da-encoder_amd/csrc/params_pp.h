@@ -11,20 +11,26 @@
 //   enact             affine, index_reverse                            (:122-126)
 // The transforms run as radix-2 DIT stages over reduced-radix XYZZ points: a
 // bit-reversed load, log N stages of butterflies with one 254-bit scalar
-// multiplication each (2-bit windows), then the store.
+// multiplication each (signed 3-bit windows over the GLV halves), then the store.
+// Every transform ends with the factor 1 / N (the iDFT's size_inv, and gen_quotients'
+// explicit division); the transforms are linear, so the factor is applied once to the
+// two inputs (folded into the coset scaling) instead of to the 12 outputs.
 #pragma once
 
 namespace zgr {
 
-// x29 point of a canonical affine input (64 zero bytes = infinity) times w^i, w^i from
-// the Montgomery w by a 64-bit exponent (w = 1 for coset 0: no scaling).
+// x29 point of a canonical affine input (64 zero bytes = infinity) times w^i / N, w^i
+// from the Montgomery w by a 64-bit exponent (scale = 0, coset 0: w = 1, the factor
+// 1 / N alone).
 __global__ __launch_bounds__(EC_THREADS) void k_pp_scale(const uint32_t* in, const uint32_t* w_mont, int scale,
-                                                         uint32_t n, uint32_t* out) {
+                                                         const uint32_t* consts, uint32_t n, uint32_t* out) {
     __shared__ uint32_t tab[EC_LDS_WORDS];
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     x29 p = ec29_from_canon(in + 16 * (size_t)i);
-    if (scale) p = x29_mul_scalar(p, f_from_mont(f_pow_u64(f_load<FrP>(w_mont), i)), tab);
+    fr k = f_load<FrP>(consts + 8 * K_NINV);
+    if (scale) k = f_mul(k, f_pow_u64(f_load<FrP>(w_mont), i));
+    p = x29_mul_scalar(p, f_from_mont(k), tab);
     x29_store(out + X29_WORDS * (size_t)i, p);
 }
 
@@ -45,7 +51,8 @@ __global__ void k_pp_load(const uint32_t* src, uint32_t* pts, int log_n, uint32_
 }
 
 // Transform output p -> param layout (x_mont || y_mont) at index bitrev(p) (enact's
-// index_reverse), scaled by N^-1 when scale != 0; infinity is stored as zeros.
+// index_reverse), scaled by N^-1 when scale != 0 (from_pp's inputs already carry it, so
+// it passes 0); infinity is stored as zeros.
 __global__ __launch_bounds__(EC_THREADS) void k_pp_store(const uint32_t* pts, const uint32_t* consts, int scale,
                                                          int log_n, uint32_t* line) {
     __shared__ uint32_t tab[EC_LDS_WORDS];

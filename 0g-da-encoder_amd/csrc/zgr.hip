@@ -1223,17 +1223,21 @@ int pp_bases(zgr_ctx* c, int coset, const uint32_t* g1pp, const uint32_t* high_g
         DALLOC(pts, n * X29_WORDS * 4);
         DALLOC(w, 32);
         LAUNCH(c, st, "params_pp", k_pp_coset_w, 1, 64, 0, c->consts, coset, w);
-        // coset-scaled powers (generate.rs:152-170); coset 0 is unscaled
-        LAUNCH(c, st, "params_pp", k_pp_scale, blocks(n, 64), 64, 0, g1pp, w, coset != 0, (uint32_t)n, gp);
-        LAUNCH(c, st, "params_pp", k_pp_scale, blocks(n, 64), 64, 0, high_g1pp, w, coset != 0, (uint32_t)n, hp);
+        // coset-scaled powers (generate.rs:152-170; coset 0 is unscaled), times 1 / N: the
+        // size_inv that every transform below ends with, moved to its inputs
+        LAUNCH(c, st, "params_pp", k_pp_scale, blocks(n, 64), 64, 0, g1pp, w, coset != 0, c->consts, (uint32_t)n,
+               gp);
+        LAUNCH(c, st, "params_pp", k_pp_scale, blocks(n, 64), 64, 0, high_g1pp, w, coset != 0, c->consts,
+               (uint32_t)n, hp);
         const uint32_t half_n = (uint32_t)(n / 2);
         auto transform = [&](const uint32_t* src, uint32_t m, bool inverse, int line) -> int {
             LAUNCH(c, st, "params_pp", k_pp_load, blocks(n, 256), 256, 0, src, pts, log_n, m);
             const uint32_t* tw = inverse ? c->tw_inv : c->tw;
             for (int s = 1; s <= log_n; s++)
                 LAUNCH(c, st, "params_pp_fft", k_ec_dit_stage, blocks(half_n, 64), 64, 0, pts, tw, s, log_n - s, half_n);
-            // both directions end with size_inv: the iDFT's own, and gen_quotients' explicit one
-            LAUNCH(c, st, "params_pp", k_pp_store, blocks(n, 64), 64, 0, pts, c->consts, 1, log_n,
+            // both directions end with size_inv (the iDFT's own, gen_quotients' explicit
+            // one), already applied to gp and hp
+            LAUNCH(c, st, "params_pp", k_pp_store, blocks(n, 64), 64, 0, pts, c->consts, 0, log_n,
                    c->bases[coset] + (size_t)line * n * 16);
             return ZGR_OK;
         };
