@@ -1336,10 +1336,13 @@ __global__ __launch_bounds__(EC_THREADS) void k_ec_dit_stage(uint32_t* pts, cons
     uint32_t j = t & (half - 1);
     uint32_t i0 = ((t >> (s - 1)) << s) | j, i1 = i0 + half;
     x29 v = x29_load(pts + X29_WORDS * (size_t)i1);
-    // stage 1's twiddle is w^0 = 1, and a point at infinity stays there: no multiplication.
-    // (from_pp's quotient transforms take inputs that are infinity past the first N / 2^d
-    // + 1, so in their stages below d every v is infinity and whole waves skip it.)
-    if (s > 1 && !x29_is_inf(v))
+    // stage 1's twiddle is w^0 = 1, and a wave whose v are all infinity skips the
+    // multiplication (from_pp's quotient transforms take inputs that are infinity past the
+    // first N / 2^d + 1, so in their stages below d every v is infinity).  The test is per
+    // wave: a wave with some finite v multiplies on every lane (x29_mul_scalar returns
+    // infinity for infinity), since running the multiplication on part of the lanes
+    // measured twice as slow as on all of them.
+    if (s > 1 && __any(!x29_is_inf(v)))
         v = x29_mul_scalar(v, f_from_mont(f_load<FrP>(tw + 8 * (size_t)(j << tw_shift))), tab);
     const x29 u = x29_load(pts + X29_WORDS * (size_t)i0);
     x29 nv = v;

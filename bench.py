@@ -341,16 +341,11 @@ def main():
               "note": "zgr_verify_msm over 3 x 1024 rows (100 MB of host rows in: pageable bytes, and a "
                       "page-locked array), median of 3"}
 
-    # G1 EC-FFT (zgr_ecfft) over 2^20 points: 1024 distinct multiples kG, tiled
-    gx, gy, pm = 1, 2, Z.P_MOD
-    pts_small = []
-    x, y = gx, gy
-    for _ in range(1024):
-        pts_small.append(x.to_bytes(32, "little") + y.to_bytes(32, "little"))
-        lam = (y - gy) * pow(x - gx, pm - 2, pm) % pm if x != gx else 3 * x * x * pow(2 * y, pm - 2, pm) % pm
-        nx = (lam * lam - x - gx) % pm
-        x, y = nx, (lam * (x - nx) - y) % pm
-    ec_in = np.frombuffer(b"".join(pts_small) * (geo.n // 1024), dtype=np.uint8)
+    # G1 EC-FFT (zgr_ecfft) over 2^20 points: the powers [tau^i] G1 of the seeded tau,
+    # as a ceremony gives them (distinct points; a tiled input would make most
+    # butterflies' v infinity, which the stages skip by the wave)
+    ctx.power_tau_setup(Z.tau_from_seed(TAU_SEED))
+    ec_in = np.ascontiguousarray(ctx.power_tau_download()[0]).reshape(-1)
     ctx.set_profiling(True)
     ctx.kernel_time_reset()
     s0 = time.perf_counter()
