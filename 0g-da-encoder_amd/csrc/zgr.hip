@@ -2405,8 +2405,10 @@ int zgr_msm(zgr_ctx* c, const uint8_t* bases_mont, const uint8_t* scalars_mont, 
     uint32_t* dbases = c->pm.bases;
     auto body = [&]() -> int {
         // zero-padded scalars: the padding points are never read by the bucket kernel
-        HIPCHK(hipMemsetAsync(ln.y, 0, np * 32, ln.st));
-        HIPCHK(hipMemsetAsync(dbases, 0, np * 64, ln.st));
+        if (np > n) {
+            HIPCHK(hipMemsetAsync(ln.y + n * 8, 0, (np - n) * 32, ln.st));
+            HIPCHK(hipMemsetAsync(dbases + n * 16, 0, (np - n) * 64, ln.st));
+        }
         HIPCHK(hipMemcpyAsync(ln.y, scalars_mont, n * 32, hipMemcpyHostToDevice, ln.st));
         HIPCHK(hipMemcpyAsync(dbases, bases_mont, n * 64, hipMemcpyHostToDevice, ln.st));
         LAUNCH(c, ln.st, "msm_inf", k_msm_clear_inf, blocks(n, 256), 256, 0, dbases, (uint32_t)n, ln.y);
