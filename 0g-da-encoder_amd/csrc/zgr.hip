@@ -303,6 +303,9 @@ struct zgr_ctx {
     std::string tune_report;
 
     Lane lanes[2];
+    // a third lane for zgr_verify_msm's third coset, with one line's bucket slots
+    // (allocated on the first verification that needs it)
+    Lane vlane;
     Slot slots[2];
     std::vector<size_t> mlens;
     int next_lane = 0;
@@ -356,6 +359,7 @@ int timed(zgr_ctx* c, hipStream_t st, const char* name, F&& launch) {
 // wait for every stream of the context, then fold the finished timers
 int sync_all(zgr_ctx* c) {
     for (auto& l : c->lanes) HIPCHK(hipStreamSynchronize(l.st));
+    if (c->vlane.st) HIPCHK(hipStreamSynchronize(c->vlane.st));
     for (auto& s : c->slots) HIPCHK(hipStreamSynchronize(s.st));
     for (auto& t : c->pending) {
         float ms = 0;
@@ -1174,6 +1178,7 @@ void server_loop(zgr_server* sv) {
 void destroy_streams(zgr_ctx* c) {
     for (auto& l : c->lanes)
         if (l.st) (void)hipStreamDestroy(l.st);
+    if (c->vlane.st) (void)hipStreamDestroy(c->vlane.st);
     for (auto& s : c->slots) {
         if (s.st) (void)hipStreamDestroy(s.st);
         for (hipEvent_t e : {s.ev_pre, s.ev_done, s.ev_coset[0], s.ev_coset[1], s.ev_coset[2], s.ev_data[0],
@@ -1483,6 +1488,7 @@ void zgr_ctx_destroy(zgr_ctx* c) {
     (void)hipSetDevice(c->device);
     for (auto& l : c->lanes)
         if (l.st) (void)hipStreamSynchronize(l.st);
+    if (c->vlane.st) (void)hipStreamSynchronize(c->vlane.st);
     for (auto& s : c->slots)
         if (s.st) (void)hipStreamSynchronize(s.st);
     void* ptrs[] = {c->consts, c->tw,      c->tw_inv, c->pw_lo[0], c->pw_lo[1], c->pw_hi[0],
@@ -1509,6 +1515,9 @@ void zgr_ctx_destroy(zgr_ctx* c) {
         if (c->fbtab[i]) (void)hipFree(c->fbtab[i]);
         if (c->fwtab[i]) (void)hipFree(c->fwtab[i]);
     }
+    for (void* p : {(void*)c->vlane.y, (void*)c->vlane.entries, (void*)c->vlane.counts, (void*)c->vlane.heap,
+                    (void*)c->vlane.bstart, (void*)c->vlane.bkt})
+        if (p) (void)hipFree(p);
     for (auto& l : c->lanes) {
         void* lp[] = {l.y,      l.digits, l.entries, l.counts,  l.heap,    l.bstart,  l.bkt,
                       l.fw_cnt, l.fw_off, l.fw_tile, l.fw_ent, l.fw_seg, l.fw_bkt, l.fw_hand, l.fw_part};
@@ -2557,6 +2566,32 @@ int zgr_merkle_noalloc(zgr_ctx* c, const void* data_dev, size_t n_elems, int lev
     return sync_all(c);
 }
 
+// zgr_verify_msm's third lane (msm_fb_line on line h: one line's bucket regions, every
+// line's heap rows since the line index picks its tree); on failure it is released and
+// the caller keeps two lanes
+int ensure_vlane(zgr_ctx* c) {
+    Lane& v = c->vlane;
+    if (v.st) return ZGR_OK;
+    auto alloc = [&]() -> int {
+        HIPCHK(hipStreamCreateWithFlags(&v.st, hipStreamNonBlocking));
+        DALLOC(v.y, c->n * 32);
+        DALLOC(v.entries, c->n * FB_W * sizeof(uint2));
+        DALLOC(v.counts, c->nb * MSM_W * 4);
+        DALLOC(v.heap, (size_t)c->nlines * 2 * c->nb * X29_WORDS * 4);
+        DALLOC(v.bstart, c->nb * FB_BSTART * 4);
+        DALLOC(v.bkt, (size_t)c->nb * FB_REGION * X29_WORDS * 4);
+        return ZGR_OK;
+    };
+    const int rc = alloc();
+    if (rc != ZGR_OK) {
+        for (void* p : {(void*)v.y, (void*)v.entries, (void*)v.counts, (void*)v.heap, (void*)v.bstart, (void*)v.bkt})
+            if (p) (void)hipFree(p);
+        if (v.st) (void)hipStreamDestroy(v.st);
+        v = Lane{};
+    }
+    return rc;
+}
+
 int zgr_verify_msm(zgr_ctx* c, const uint32_t* row_ids, const uint8_t* rows, const uint8_t* alphas, size_t n_rows,
                    uint8_t* out_point) {
     Range range_("zgr_verify_msm");
@@ -2578,10 +2613,11 @@ int zgr_verify_msm(zgr_ctx* c, const uint32_t* row_ids, const uint8_t* rows, con
     uint8_t *d_rows = nullptr, *d_alpha = nullptr;
     uint32_t *d_ids = nullptr, *d_ks = nullptr, *d_pts = nullptr;
     int* flag = nullptr;
-    hipEvent_t ev_up = nullptr, ev_done = nullptr;
+    hipEvent_t ev_up = nullptr, ev_done = nullptr, ev_done2 = nullptr;
     auto body = [&]() -> int {
         HIPCHK(hipEventCreateWithFlags(&ev_up, hipEventDisableTiming));
         HIPCHK(hipEventCreateWithFlags(&ev_done, hipEventDisableTiming));
+        HIPCHK(hipEventCreateWithFlags(&ev_done2, hipEventDisableTiming));
         CHECK(dmalloc((void**)&d_rows, n_rows * COL * 32 + 32));
         CHECK(dmalloc((void**)&d_alpha, n_rows * 32 + 32));
         CHECK(dmalloc((void**)&d_ids, n_rows * 4 + 4));
@@ -2599,18 +2635,26 @@ int zgr_verify_msm(zgr_ctx* c, const uint32_t* row_ids, const uint8_t* rows, con
         std::vector<uint32_t> ks_all;
         for (int cs = 0; cs < c->coset_n; cs++) ks_all.insert(ks_all.end(), per[cs].begin(), per[cs].end());
         if (n_rows) HIPCHK(hipMemcpyAsync(d_ks, ks_all.data(), n_rows * 4, hipMemcpyHostToDevice, st));
-        // with the shifted-copy tables the cosets alternate between the two lanes (their
-        // streams wait for the uploads on lane 0's, and lane 0's for lane 1's results)
+        // with the shifted-copy tables the cosets run side by side, on lanes 0 and 1 and,
+        // for a third, the verification lane (their streams wait for the uploads on lane
+        // 0's, and lane 0's for their results); without it the third goes back to lane 0
         const bool fb = c->fb;
+        int ncos = 0;
+        for (int cs = 0; cs < c->coset_n; cs++) ncos += per[cs].empty() ? 0 : 1;
+        const bool three = fb && ncos >= 3 && ensure_vlane(c) == ZGR_OK;
         HIPCHK(hipEventRecord(ev_up, st));
-        bool used1 = false;
+        bool used1 = false, used2 = false;
         for (int cs = 0; cs < c->coset_n; cs++) {
             const uint32_t nk = (uint32_t)per[cs].size();
             if (!nk) continue;
-            Lane& lc = fb ? c->lanes[npts & 1] : ln;
-            if (&lc != &ln && !used1) {
+            Lane& lc = !fb ? ln : (three && npts == 2) ? c->vlane : c->lanes[npts & 1];
+            if (&lc == &c->lanes[1] && !used1) {
                 HIPCHK(hipStreamWaitEvent(lc.st, ev_up, 0));
                 used1 = true;
+            }
+            if (&lc == &c->vlane && !used2) {
+                HIPCHK(hipStreamWaitEvent(lc.st, ev_up, 0));
+                used2 = true;
             }
             HIPCHK(hipMemsetAsync(lc.y, 0, c->n * 32, lc.st));
             LAUNCH(c, lc.st, "verify", k_ver_scalars, blocks((size_t)nk * COL, 256), 256, 0, d_rows, d_alpha,
@@ -2628,6 +2672,10 @@ int zgr_verify_msm(zgr_ctx* c, const uint32_t* row_ids, const uint8_t* rows, con
             HIPCHK(hipEventRecord(ev_done, c->lanes[1].st));
             HIPCHK(hipStreamWaitEvent(st, ev_done, 0));
         }
+        if (used2) {
+            HIPCHK(hipEventRecord(ev_done2, c->vlane.st));
+            HIPCHK(hipStreamWaitEvent(st, ev_done2, 0));
+        }
         LAUNCH(c, st, "verify", k_ver_sum, 1, 64, 0, d_pts, npts, d_pts + 16 * c->coset_n);
         int hflag = 0;
         HIPCHK(hipMemcpyAsync(&hflag, flag, sizeof(int), hipMemcpyDeviceToHost, st));
@@ -2640,7 +2688,7 @@ int zgr_verify_msm(zgr_ctx* c, const uint32_t* row_ids, const uint8_t* rows, con
     (void)sync_all(c);
     for (void* p : {(void*)d_rows, (void*)d_alpha, (void*)d_ids, (void*)d_ks, (void*)d_pts, (void*)flag})
         if (p) (void)hipFree(p);
-    for (hipEvent_t e : {ev_up, ev_done})
+    for (hipEvent_t e : {ev_up, ev_done, ev_done2})
         if (e) (void)hipEventDestroy(e);
     return rc;
 }
