@@ -165,7 +165,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: the encoder is built and measured on a single "
-              "MI355X (blobs shard over GPUs with no exchange; see DESIGN.md). --gpus %d is not run."
+              "MI355X (blobs shard over GPUs with no exchange between them). --gpus %d is not run."
               % args.gpus, file=sys.stderr)
         sys.exit(2)
 
@@ -370,7 +370,8 @@ def main():
              "note": "one forward DFT of 2^20 G1 points (Pairing::fast_fft); 20 x 2^19 butterflies, those of "
                      "stages 2..20 with a scalar multiplication by a GLV-split twiddle (stage 1's are all 1)"}
 
-    # config 3 primitives, inputs resident in HBM where the entry point allows it.
+    # the isolated primitives at production shape, inputs resident in HBM where the entry
+    # point allows it.
     # NTT: the batched 1024 rows x 2^10 (one launch, each row in LDS) and one 2^20-point
     # transform (two passes over HBM + the return to natural order); k_ntt_pass is the
     # "ntt" timer, the bit-reversal copy "ntt_unrev"

@@ -137,7 +137,7 @@ def test_encode_bit_exact_vs_oracle(ctx, blob):
 
 
 def test_ecfft_2pow10(ctx):
-    # config 3: G1 EC-FFT of size 2^10 over seeded multiples of G1, both directions
+    # the isolated 2^10-point G1 EC-FFT over seeded multiples of G1, both directions
     rng = np.random.default_rng(1010)
     pts = np.zeros((1 << 10, 64), dtype=np.uint8)
     for i in range(1 << 10):
@@ -150,7 +150,7 @@ def test_ecfft_2pow10(ctx):
 
 
 def test_msm_2pow20(ctx):
-    # config 3: one MSM of 2^20 resident-style bases (the basis line of coset 0)
+    # the isolated MSM: 2^20 resident-style bases (the basis line of coset 0)
     # with uniform random scalars, bit-exact against the oracle's Pippenger
     rng = np.random.default_rng(2020)
     bases = ctx.params_download(0).reshape(GEO.h + 2, GEO.n, 8)[GEO.h].copy()
@@ -174,7 +174,7 @@ def _fr_rows(rng, rows, log_n):
 
 @pytest.mark.parametrize("inverse", [False, True])
 def test_ntt_batch_1024_rows_2pow10(ctx, inverse):
-    """config 3: the batched Fr NTT of 1024 rows x 2^10 (one launch, each row in LDS),
+    """the isolated batched Fr NTT of 1024 rows x 2^10 (one launch, each row in LDS),
     every row bit-exact against the oracle's radix-2 fft / ifft, through both the
     host-memory and the device-memory entry points."""
     rng = np.random.default_rng(3030 + inverse)
@@ -218,7 +218,7 @@ def test_coset_extend_production(ctx):
 
 
 def test_batch_equals_single(ctx, blob):
-    # config 4 shape: uneven full-geometry blobs in one batched call, each bit-equal
+    # the uneven-batch shape: uneven full-geometry blobs in one batched call, each bit-equal
     # to the same blob encoded alone
     raw0, out0 = blob
     rng = np.random.default_rng(44)
@@ -236,7 +236,7 @@ def test_batch_equals_single(ctx, blob):
 
 
 def test_uneven_mix_32(ctx):
-    """config 4: the seeded 32-blob mix (raw lengths log-uniform in [1 KB, 32.5 MB], the
+    """the uneven batch: the seeded 32-blob mix (raw lengths log-uniform in [1 KB, 32.5 MB], the
     same seeds bench.py times) in one batched call.  Every blob: commitment = [f(tau)] G1
     and high = [tau^(3N) f(tau)] G1 for the interpolant f of its column-major data,
     coset 0 equal to its raw blob, and every output equal to the same blob encoded alone."""
@@ -280,7 +280,7 @@ def test_autotune_production(ctx, blob):
 
 
 def test_served_equals_single(ctx, blob):
-    # config 5 shape: requests of mixed sizes submitted together to the serving
+    # the served shape: requests of mixed sizes submitted together to the serving
     # layer; each result equals the same blob encoded alone
     raw0, out0 = blob
     lengths = [GEO.max_raw_data_size, 5000, 3_000_000, 1, 20_000_000]
