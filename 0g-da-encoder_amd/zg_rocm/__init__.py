@@ -914,14 +914,23 @@ class DeferredVerifier:
             return True
         ids = sorted(self._rows)
         lhs = self.ctx.verify_msm(ids, [self._rows[i][0] for i in ids], [self._rows[i][1] for i in ids])
-        bases, scal = [], []
+        # sum_k alpha_k (C_k - sum_d P_(k,d)) with each distinct point once: the rows of a
+        # blob share its commitment and the AMT nodes above them (a depth-d proof point
+        # serves 2^(h-d) rows), so the scalars of equal points are summed first (the MSM
+        # is linear: a P + b P = (a + b) P) -- about 6,000 points for a blob's 3,072 rows
+        # instead of 33,800
+        merged = {}
         for i in ids:
             _, alpha, com, proof = self._rows[i]
             for pt, k in [(com, alpha)] + [(p, R_MOD - alpha) for p in proof]:
-                xy = _g1_from_wire(pt)
-                if xy is not None:
-                    bases.append(xy)
-                    scal.append(k % R_MOD)
+                key = bytes(pt)
+                merged[key] = merged.get(key, 0) + k
+        bases, scal = [], []
+        for key, k in merged.items():
+            xy = _g1_from_wire(key)
+            if xy is not None and k % R_MOD:
+                bases.append(xy)
+                scal.append(k % R_MOD)
         if not bases:
             return lhs == bytes(64)
         R256 = 1 << 256
