@@ -2286,20 +2286,22 @@ int msm_plain(zgr_ctx* c, Lane& ln, const uint32_t* dbases, int log_np, uint32_t
         if (v >= 1) nseg = std::min(PM_MAX_SEGS, (uint32_t)v);
     }
     auto& pm = c->pm;
-    if (!pm.ent) {  // sized for the context's largest MSM
+    {  // sized for the context's largest MSM, on the first call; each buffer allocated
+       // only if missing, so a call after a failed allocation completes the set
+        auto need = [](auto*& p, size_t bytes) { return p ? ZGR_OK : dmalloc((void**)&p, bytes); };
         const size_t blk_max = (size_t)PM_W * std::max<size_t>(1, (2 * c->n) >> 16);
-        DALLOC(pm.ent, (size_t)PM_W * 2 * c->n * sizeof(uint2));
-        DALLOC(pm.cnt, blk_max * FW_BUCKETS * 4);
-        DALLOC(pm.off, blk_max * FW_BUCKETS * 4);
-        DALLOC(pm.tile, (2 * FW_TILES + 1) * 4);
-        DALLOC(pm.seg, (PM_MAX_SEGS + 1) * 4);
-        DALLOC(pm.bkt, (size_t)PM_W * FW_BUCKETS * X29_WORDS * 4);
-        DALLOC(pm.part, (size_t)PM_W * PM_RSEGS * X29_WORDS * 4);
-        DALLOC(pm.hand, (size_t)PM_MAX_SEGS * 2 * FB_LANES * X29_WORDS * 4);
-        DALLOC(pm.heap, (size_t)(PM_W + 1) * X29_WORDS * 4);
-        DALLOC(pm.work, (PM_MAX_SEGS + PM_W * PM_RSEGS) * sizeof(uint2));
-        DALLOC(pm.halves, (size_t)PM_SPLIT_WORDS * c->n * 4);
-        DALLOC(pm.table, (size_t)2 * c->n * 64);
+        CHECK(need(pm.ent, (size_t)PM_W * 2 * c->n * sizeof(uint2)));
+        CHECK(need(pm.cnt, blk_max * FW_BUCKETS * 4));
+        CHECK(need(pm.off, blk_max * FW_BUCKETS * 4));
+        CHECK(need(pm.tile, (2 * FW_TILES + 1) * 4));
+        CHECK(need(pm.seg, (PM_MAX_SEGS + 1) * 4));
+        CHECK(need(pm.bkt, (size_t)PM_W * FW_BUCKETS * X29_WORDS * 4));
+        CHECK(need(pm.part, (size_t)PM_W * PM_RSEGS * X29_WORDS * 4));
+        CHECK(need(pm.hand, (size_t)PM_MAX_SEGS * 2 * FB_LANES * X29_WORDS * 4));
+        CHECK(need(pm.heap, (size_t)(PM_W + 1) * X29_WORDS * 4));
+        CHECK(need(pm.work, (PM_MAX_SEGS + PM_W * PM_RSEGS) * sizeof(uint2)));
+        CHECK(need(pm.halves, (size_t)PM_SPLIT_WORDS * c->n * 4));
+        CHECK(need(pm.table, (size_t)2 * c->n * 64));
     }
     FbWide wd{};
     wd.ent = pm.ent;
