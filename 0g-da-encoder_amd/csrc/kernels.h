@@ -1553,7 +1553,7 @@ __global__ __launch_bounds__(64) void k_params_gen(uint32_t* bases, const uint32
 // blk holds window blk >> sh over points [(blk mod 2^sh) ns, + ns), so a window's entries
 // form one group of the wide sort (bucket runs by (|digit|, window)) and an entry's point
 // index is its table row: no shifted copies.  The bucket kernels give one sum S_w per
-// window, and k_pm_horner returns sum_w 2^(16 w) S_w: 112 doublings on one lane, the
+// window, and k_pm_horner returns sum_w 2^(16 w) S_w: one chain of 112 doublings, the
 // latency that halving the windows by GLV halves.
 constexpr int PM_W = 8;           // 16-bit windows of a 127-bit half
 constexpr int PM_SPLIT_WORDS = 9;  // per scalar: |k1| (4 words), |k2| (4 words), signs
